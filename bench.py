@@ -1,0 +1,182 @@
+"""Throughput of the MI355X Lennard-Jones step: atom-steps/s at N = 4,000,000 (FCC 100^3 x 4, rho* 0.8442,
+T* 0.722, seed 1, rc 2.5, skin 0.3, dt 0.005, mixed precision, NVE, re-sort on every rebuild), one GPU.
+
+A "step" is one velocity-Verlet step of the whole system (kick-drift, displacement test, re-sort + list rebuild
+when stale, forces, kick), driven on the device by md_run. Inputs are generated on the host by the library's own
+createFCCLattice / initializeVelocities, uploaded, and the first list built before the timed region.
+
+Printed: one JSON line with the throughput, the force kernel's HBM roofline (event-timed inside the timed region)
+and the reference's own OpenMP path (oracle/_ref/libmdref_fast.so) timed on this host on a bounded sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "molecular-dynamics_amd", "python"))
+
+import mdgpu  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+WORKLOADS = {
+    # name: cells, density, temperature, seed, skin, precision, description
+    "lj4m": (100, 0.8442, 0.722, 1, 0.3, mdgpu.MD_MIXED,
+             "LJ NVE N=4,000,000 (FCC 100^3x4) rho*=0.8442 T*=0.722 seed 1 rc=2.5 skin=0.3 dt=0.005 mixed"),
+    "lj256k": (40, 0.8442, 1.0, 1, 0.3, mdgpu.MD_MIXED,
+               "LJ NVE N=256,000 (FCC 40^3x4) rho*=0.8442 T*=1.0 seed 1 rc=2.5 skin=0.3 dt=0.005 mixed"),
+    "lj2048": (8, 0.8442, 0.722, 1, 0.3, mdgpu.MD_FP64,
+               "LJ NVE N=2048 (FCC 8^3x4) rho*=0.8442 T*=0.722 seed 1 rc=2.5 skin=0.3 dt=0.005 fp64"),
+}
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "force_kernel_traffic.json")
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def drift_stats(energies, e0, n):
+    e = energies[:, 0] + energies[:, 1]
+    rel = (e - e0) / abs(e0)
+    steps = np.arange(1, len(e) + 1)
+    slope = np.polyfit(steps, rel, 1)[0] if len(e) > 1 else 0.0
+    return float(slope * 1000.0), float(np.abs(rel).max())
+
+
+def cpu_baseline(cells, density, temperature, seed, skin, mixed, budget_s):
+    """The reference's own OpenMP list path, built from its sources (-O3, OpenMP), timed on this host's cores
+    over a bounded number of production steps of the same workload."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    if not O.available("ref_fast"):
+        return None
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    cores = max(1, min(cores, 16))
+    O.set_threads("ref_fast", cores)
+    s = O.Sim.fcc(cells, density, temperature, seed, "ref_fast")
+    s.forcefield(1.0, 1.0, 2.5, mixed=mixed)
+    s.neighbour_list(2.5, skin)
+    s.list_build()
+    s.forces_list()
+    n = s.n
+    steps, t0 = 0, time.perf_counter()
+    while True:
+        s.step(0.005)
+        steps += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s or steps >= 400:
+            break
+    rate = n * steps / el
+    return {"value": rate, "unit": "atom-steps/s", "cores": cores, "kind": "reference",
+            "sample": f"{steps} production steps of the same workload (N={n}, {'mixed' if mixed else 'fp64'}, "
+                      f"list path, initial forces via the list) in {el:.1f} s on {cores} host threads; "
+                      f"{s.rebuilds} list builds incl. the initial one"}
+
+
+def calibrate_copy(nbytes):
+    """A plain device copy of known size, for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE."""
+    import torch
+
+    a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": nbytes, "launches": 5}))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--workload", default="lj4m", choices=sorted(WORKLOADS))
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--graph", action="store_true", help="replay a captured step instead of eager launches")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported (each box has one GPU).",
+              file=sys.stderr)
+        sys.exit(2)
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy)
+        return
+
+    cells, rho, temp, seed, skin, prec, desc = WORKLOADS[args.workload]
+    ctx = mdgpu.Context(0)
+    t_setup = time.perf_counter()
+    state = mdgpu.fcc_system(ctx, cells, rho, temp, seed)
+    n = state.n
+    nl = mdgpu.NeighbourList(ctx, 2.5, skin)
+    lj = mdgpu.LJ(1.0, 1.0, 2.5)
+    nl.build(state)
+    r0 = nl.forces(state, lj, prec)
+    e0 = r0.potential + state.kinetic_energy()
+    log(f"setup N={n} in {time.perf_counter() - t_setup:.1f} s; padding {nl.info().padding}")
+
+    warm = nl.run(state, lj, 0.005, args.warmup, precision=prec, use_graph=args.graph) if args.warmup else None
+    ctx.synchronize()
+    rebuilds0 = nl.info().rebuilds
+    timing = None if args.graph else []
+    t0 = time.perf_counter()
+    en = nl.run(state, lj, 0.005, args.steps, precision=prec, use_graph=args.graph, timing=timing)
+    ctx.synchronize()
+    elapsed = time.perf_counter() - t0
+    info = nl.info()
+    rebuilds = info.rebuilds - rebuilds0
+    value = n * args.steps / elapsed
+    log(f"timed {args.steps} steps in {elapsed:.3f} s: {value:.4g} atom-steps/s, {rebuilds} rebuilds")
+
+    all_en = en if warm is None else np.vstack([warm, en])
+    drift1000, max_drift = drift_stats(all_en, e0, n)
+    k_full = 2.0 * info.half_pairs / n
+    # force kernel: per atom it reads its position (32 B), its two row lengths (8 B) and its row (4 B per
+    # full-list neighbour), and writes its force (24 B); neighbour positions are re-reads of the same array
+    bytes_per_atom = 32.0 + 8.0 + 4.0 * k_full + 24.0
+    roofline = None
+    if timing:
+        avg_ms = timing[0] / args.steps
+        achieved = bytes_per_atom * n / (avg_ms * 1e-3) / 1e9
+        traffic = None
+        if os.path.exists(TRAFFIC_FILE):
+            with open(TRAFFIC_FILE) as f:
+                tr = json.load(f)
+            if tr.get("workload") == args.workload:
+                traffic = tr.get("hbm_bytes_per_launch")
+        roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                    "kernel": "k_forces_mixed" if prec == mdgpu.MD_MIXED else "k_forces_fp64",
+                    "avg_launch_ms": round(avg_ms, 5), "algorithmic_bytes_per_launch": round(bytes_per_atom * n)}
+    cpu = None
+    if not args.no_cpu:
+        log("timing the reference's CPU path ...")
+        cpu = cpu_baseline(cells, rho, temp, seed, skin, prec == mdgpu.MD_MIXED, args.cpu_seconds)
+    out = {
+        "metric": "atom-steps/s at N=4M LJ liquid (1 GPU)" if args.workload == "lj4m" else "atom-steps/s",
+        "value": value, "unit": "atom-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1000.0 * elapsed / args.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 pair maths, f64 positions/accumulation" if prec == mdgpu.MD_MIXED
+        else "f64", "data": "synthetic: FCC lattice + Maxwell-Boltzmann velocities generated by the library",
+        "config": {"workload": desc, "N": n, "steps_in_config": 2000, "graph": bool(args.graph)},
+        "roofline": roofline, "cpu_baseline": cpu,
+        "gpu_vs_cpu": (value / cpu["value"]) if cpu else None,
+        "interactions_per_s": value * k_full, "k_full_mean": k_full, "rebuilds": rebuilds,
+        "steps_per_rebuild": (args.steps / rebuilds) if rebuilds else None,
+        "drift_per_1000_steps": drift1000, "max_abs_dE_over_E0": max_drift,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,189 @@
+/* mdgpu — C ABI of the MI355X (gfx950) Lennard-Jones step engine.
+ *
+ * The reference engine (nhantran1711/molecular-dynamics) has no FFI: its boundary is the C++20 class API of
+ * its include/ headers. This header is the thin C layer underneath a host C++ facade that keeps those classes
+ * (molecular-dynamics_amd/facade/), and it is what any other host binds (ctypes, cgo, JNI). Each entry point
+ * below names the reference function it replaces.
+ *
+ * Conventions
+ *  - Every function returns an md_status; on failure md_last_error() holds a message. Where the reference throws
+ *    std::runtime_error the same text is reported with MD_ERR_RUNTIME.
+ *  - One md_ctx owns one HIP stream on one device; everything created from it is stream-ordered on it. Handles
+ *    are not thread-safe (the reference's classes are not either, Forcefield.hpp:61-66).
+ *  - Host arrays use the reference's storage: positions packed x,y,z,pad (4 doubles) per particle
+ *    (System.hpp:22), velocities and forces component-major (vx[0..n), vy[..], vz[..]) (System.hpp:131-132),
+ *    ids as uint64 (System.hpp:134).
+ *  - Particle indices change on every list build (the state is re-sorted by cell), as in the reference
+ *    (NeighbourList.hpp:21-23).
+ */
+#ifndef MDGPU_H
+#define MDGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum md_status {
+    MD_OK = 0,
+    MD_ERR_ARG = 1,      /* bad argument or handle */
+    MD_ERR_HIP = 2,      /* HIP runtime failure */
+    MD_ERR_RUNTIME = 3,  /* the reference's std::runtime_error cases */
+    MD_ERR_OVERFLOW = 4, /* a padded list row overflowed inside md_run and the state was restored */
+    MD_ERR_STATE = 5     /* call order violated (e.g. forces before any list build) */
+} md_status;
+
+typedef enum md_precision {
+    MD_FP64 = 0,  /* double pair maths; forces bit-equal to the reference's deterministic mode */
+    MD_MIXED = 1  /* double minimum image, float pair maths, double accumulation (Forcefield.cpp:333-450) */
+} md_precision;
+
+typedef enum md_field {
+    MD_FIELD_POS = 0,   /* double[4n] */
+    MD_FIELD_VEL = 1,   /* double[3n] */
+    MD_FIELD_FORCE = 2, /* double[3n] */
+    MD_FIELD_MASS = 3,  /* double[n] */
+    MD_FIELD_ID = 4     /* uint64[n] */
+} md_field;
+
+typedef struct md_ctx md_ctx;
+typedef struct md_state md_state;
+typedef struct md_list md_list;
+
+/* LennardJonesForceField(epsilon, sigma, cut_off) parameters (Forcefield.hpp:8). */
+typedef struct md_lj {
+    double epsilon;
+    double sigma;
+    double cutoff;
+} md_lj;
+
+/* What computeForces leaves behind: return value, getVirial(), getInsidePairs() (Forcefield.hpp:11-33). */
+typedef struct md_force_result {
+    double potential;
+    double virial;
+    double inside_pairs;
+} md_force_result;
+
+typedef struct md_list_info {
+    size_t half_pairs;     /* NeighbourList::getPairCount() (NeighbourList.hpp:47-49) */
+    int rebuilds;          /* NeighbourList::getRebuildCount() (NeighbourList.hpp:43-45) */
+    int cells_per_side;    /* CellList::getCellsPerSide() of the list's grid (CellList.hpp:20-22) */
+    int max_neighbours;    /* longest full-list row at the last build */
+    int padding;           /* allocated row length of the padded full list */
+} md_list_info;
+
+/* ---- host-side input generation (Initialisation.hpp:9-12) --------------------------------------------- */
+/* createFCCLattice(cells, a) (Initialisation.cpp:7-44): writes 4 cells^3 positions, x,y,z per particle. */
+int md_fcc_lattice(int cells, double lattice_constant, double* pos);
+/* initializeVelocities(system, T, seed) (Initialisation.cpp:47-76) on host arrays: mt19937(seed) Gaussian draws
+   per particle, centre-of-mass removal, exact rescale to T. vel is component-major [3n]; mass may be NULL (1). */
+int md_maxwell_boltzmann(size_t n, const double* mass, double temperature, uint32_t seed, double* vel);
+
+/* ---- errors and contexts ------------------------------------------------------------------------------ */
+const char* md_last_error(void);
+int md_device_count(int* count);
+int md_ctx_create(int device, md_ctx** out);
+int md_ctx_destroy(md_ctx* ctx);
+int md_ctx_synchronize(md_ctx* ctx);
+
+/* ---- System (System.hpp:12-136) ---------------------------------------------------------------------- */
+/* System(particles, boxLength) (System.cpp:9-29): n particles, zero state, unit masses, ids 0..n-1.
+   Upload the arrays with md_state_upload. */
+int md_state_create(md_ctx* ctx, size_t n, double box_length, md_state** out);
+int md_state_destroy(md_state* s);
+/* Deep copy of every per-particle array; dst must have the same n (System copy assignment). */
+int md_state_copy(md_state* dst, const md_state* src);
+int md_state_info(const md_state* s, size_t* n, double* box_length);
+int md_state_upload(md_state* s, int field, const void* host);
+int md_state_download(const md_state* s, int field, void* host);
+/* System::reorder: new slot k takes old slot order[k] for every array (System.cpp:42-64). */
+int md_state_reorder(md_state* s, const uint64_t* order);
+/* System::kineticEnergy in the reference's fixed summation order (System.cpp:67-76, Parallel.hpp:32-57). */
+int md_kinetic_energy(md_state* s, double* ke);
+/* v *= scale for every component (Thermostat.cpp:6-12); used by rescaleVelocities and BussiThermostat. */
+int md_scale_velocities(md_state* s, double scale);
+/* System::stateHash (System.cpp:85-105). */
+int md_state_hash(md_state* s, uint64_t* hash);
+
+/* ---- CellList (CellList.hpp:11-50) --------------------------------------------------------------------- */
+/* CellList(min_cell).build(system) on the current order, without re-sorting (CellList.cpp:9-52). Any output
+   pointer may be NULL; cell_start has side^3 + 1 entries. */
+int md_cells_build(md_state* s, double min_cell, int* cells_per_side, int* cell_of, uint64_t* cell_start,
+                   uint64_t* order);
+
+/* ---- NeighbourList (NeighbourList.hpp:12-76) ----------------------------------------------------------- */
+/* NeighbourList(cutOff, skin) (NeighbourList.cpp:5-8); max_neighbours = initial row padding, 0 = automatic. */
+int md_list_create(md_ctx* ctx, double cutoff, double skin, int max_neighbours, md_list** out);
+int md_list_destroy(md_list* l);
+/* NeighbourList::build: re-sorts the state by cell and builds the padded full list (NeighbourList.cpp:55-134). */
+int md_list_build(md_list* l, md_state* s);
+/* NeighbourList::update: rebuild if n changed or any particle moved > skin/2 (NeighbourList.cpp:11-36). */
+int md_list_update(md_list* l, md_state* s, int* rebuilt);
+int md_list_get_info(const md_list* l, md_list_info* info);
+/* The half list (j > i) in CSR, as neighboursOf / firstPairOf expose it (NeighbourList.hpp:25-31):
+   offsets[n+1], neighbours[half_pairs]; either may be NULL. */
+int md_list_half(md_list* l, uint64_t* offsets, uint64_t* neighbours);
+/* The padded full list as stored: counts[n], half_counts[n], rows[n * padding] (row-major on the host). */
+int md_list_full(md_list* l, int32_t* counts, int32_t* half_counts, int32_t* rows);
+
+/* ---- LennardJonesForceField (Forcefield.hpp:6-78) ------------------------------------------------------ */
+/* computeForces(system, neighbourList) (Forcefield.cpp:81-211, 214-326, 333-450). Throws the reference's
+   cutoff message as MD_ERR_RUNTIME when 2 rc > L (Forcefield.cpp:18-22). out may be NULL. */
+int md_forces(md_state* s, md_list* l, const md_lj* lj, int precision, md_force_result* out);
+/* computeForces(system): O(N^2) all pairs (Forcefield.cpp:39-78); forces bit-equal to the reference's. */
+int md_forces_allpairs(md_state* s, const md_lj* lj, md_force_result* out);
+
+/* ---- VelocityVerletIntegrator (Integrator.hpp:7-29) ---------------------------------------------------- */
+/* kickAndDrift (Integrator.cpp:32-53) and kick (Integrator.cpp:55-71). */
+int md_kick_drift(md_state* s, double dt);
+int md_kick(md_state* s, double dt);
+/* step(system, forceField, neighbourList) (Integrator.cpp:18-30); potential may be NULL. */
+int md_step(md_state* s, md_list* l, const md_lj* lj, int precision, double dt, md_force_result* out);
+/* step(system, forceField) with all-pairs forces (Integrator.cpp:7-16). */
+int md_step_allpairs(md_state* s, const md_lj* lj, double dt, md_force_result* out);
+
+/* ---- BussiThermostat / rescaleVelocities (Thermostat.hpp:8-26) ------------------------------------------ */
+/* One BussiThermostat::apply with the two draws supplied by the caller (Thermostat.cpp:36-58): r1 ~ N(0,1),
+   sum_sq ~ Gamma((dof-1)/2, 2). No-op when the kinetic energy is <= 0. */
+int md_bussi_apply(md_state* s, double temperature, double tau, double dt, double r1, double sum_sq);
+
+/* The thermostat's random stream: BussiThermostat's std::mt19937_64(seed) (Thermostat.cpp:32-34) and, per apply,
+   a fresh normal_distribution(0, 1) then gamma_distribution(0.5 (dof - 1), 2) draw (Thermostat.cpp:47-51), with
+   dof = 3 n - 3. md_bussi_rng_draws writes `count` (r1, sum_sq) pairs, advancing the stream. */
+typedef struct md_rng md_rng;
+int md_bussi_rng_create(uint64_t seed, md_rng** out);
+int md_bussi_rng_destroy(md_rng* r);
+int md_bussi_rng_draws(md_rng* r, size_t n_particles, int count, double* out);
+
+/* ---- RadialDistribution (RadialDistribution.hpp:8-29) --------------------------------------------------- */
+/* RadialDistribution::sample: adds the half-list pair counts with r^2 < rmax^2 to counts[bins]
+   (RadialDistribution.cpp:11-30). */
+int md_rdf_sample(md_state* s, md_list* l, double rmax, int bins, uint64_t* counts);
+
+/* ---- Multi-step driver (md_sim's production loop, main.cpp:272-304, without host round trips) ---------- */
+typedef struct md_run_config {
+    double dt;
+    int precision;        /* md_precision */
+    int steps;
+    md_lj lj;
+    int thermostat;       /* 0 = NVE, 1 = Bussi every step (main.cpp:275-277) */
+    double temperature;   /* Bussi target */
+    double tau;
+    const double* draws;  /* Bussi draws per step: draws[2k] = r1, draws[2k+1] = sum_sq (host, 2*steps) */
+    int use_graph;        /* capture one step as a hipGraph and replay it */
+    double* force_kernel_ms; /* optional out (eager runs only): summed duration of the force kernel launches,
+                                measured with HIP events on the context's stream */
+} md_run_config;
+
+/* Advances `steps` velocity-Verlet steps entirely on the device: the list update decision, re-sort and rebuild
+   are driven by a device flag. energies (host, may be NULL) receives 4 doubles per step: potential, kinetic,
+   virial, inside pairs. The kinetic energy is a fixed-order tree sum after the closing half kick in NVE, and
+   the thermostat's new kinetic energy K' (eq. A7) with the Bussi thermostat on. */
+int md_run(md_state* s, md_list* l, const md_run_config* cfg, double* energies);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,632 @@
+// Cell grid, stable counting sort into cells, re-sort of the state by cell, and the one-pass padded full Verlet
+// list (reference: CellList.cpp:9-83, NeighbourList.cpp:11-134, System.cpp:42-64).
+//
+// Rebuild pipeline, every kernel optionally gated by a device flag so a captured step can carry it:
+//   k_cell_count   cell id per particle (CellList.cpp:26-38) + atomic histogram
+//   k_scan_cells   exclusive prefix sum of the histogram -> cell start offsets (CellList.cpp:41-44)
+//   k_cell_scatter particle index into its cell's range, unordered (atomics)
+//   k_cell_sort    one wavefront per cell ranks its members by index, restoring the counting sort's stable order
+//                  (CellList.cpp:47-51); when re-sorting, it also gathers every particle array into the new order
+//                  (System::reorder) and records the build-time positions (NeighbourList.cpp:129-132)
+//   k_list_build   one thread per particle: the j > i candidates of the 27 cells in the reference's visit order
+//                  (its half-list row), then the j < i candidates in increasing index; r^2 < (rc+skin)^2
+//                  strictly (NeighbourList.cpp:75-98); also moves the re-sorted arrays back into place.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "device.hpp"
+#include "internal.hpp"
+
+namespace mdgpu {
+
+struct PermuteArgs {
+    int enabled;
+    const double4* pos;
+    const double* vel;
+    const double* frc;
+    const double* mass;
+    const uint64_t* id;
+    double4* pos2;
+    double* vel2;
+    double* frc2;
+    double* mass2;
+    uint64_t* id2;
+    double4* ref;
+    int* cell_of2;
+};
+
+struct CopyBackArgs {
+    int enabled;
+    const double* vel2;
+    const double* frc2;
+    const double* mass2;
+    const uint64_t* id2;
+    const int* cell_of2;
+    double4* pos;
+    double* vel;
+    double* frc;
+    double* mass;
+    uint64_t* id;
+    int* cell_of;
+};
+
+__device__ __forceinline__ bool gated_off(const int* gate) { return gate && *gate == 0; }
+
+__global__ void __launch_bounds__(256) k_cell_count(const double4* __restrict__ pos, size_t n, double per_length,
+                                                     int side, int* __restrict__ cell_of,
+                                                     uint32_t* __restrict__ counts, const int* gate) {
+    if (gated_off(gate)) return;
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double4 r = pos[i];
+    const int c = cell_of_position(r.x, r.y, r.z, per_length, side);
+    cell_of[i] = c;
+    atomicAdd(&counts[c], 1u);
+}
+
+// Exclusive scan of `ncells` counts by one workgroup: each thread owns a contiguous chunk, the chunk totals are
+// scanned across the block, then every chunk is rewritten with its running offset. start[ncells] = total.
+__global__ void __launch_bounds__(1024) k_scan_cells(const uint32_t* __restrict__ counts, size_t ncells,
+                                                      uint32_t* __restrict__ start, const int* gate) {
+    if (gated_off(gate)) return;
+    __shared__ uint32_t wave_total[1024 / WAVE];
+    const size_t per = (ncells + 1023) / 1024;
+    const size_t b = threadIdx.x * per;
+    const size_t e = b + per < ncells ? b + per : ncells;
+    uint32_t sum = 0;
+    for (size_t c = b; c < e; ++c) sum += counts[c];
+    const int lane = threadIdx.x % WAVE, wave = threadIdx.x / WAVE;
+    uint32_t incl = sum;
+#pragma unroll
+    for (int off = 1; off < WAVE; off <<= 1) {
+        const uint32_t o = __shfl_up(incl, off, WAVE);
+        if (lane >= off) incl += o;
+    }
+    if (lane == WAVE - 1) wave_total[wave] = incl;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t acc = 0;
+        for (int w = 0; w < 1024 / WAVE; ++w) {
+            const uint32_t t = wave_total[w];
+            wave_total[w] = acc;
+            acc += t;
+        }
+        start[ncells] = acc;
+    }
+    __syncthreads();
+    uint32_t run = wave_total[wave] + incl - sum;
+    for (size_t c = b; c < e; ++c) {
+        start[c] = run;
+        run += counts[c];
+    }
+}
+
+__global__ void __launch_bounds__(256) k_cell_scatter(const int* __restrict__ cell_of, size_t n,
+                                                       const uint32_t* __restrict__ start,
+                                                       uint32_t* __restrict__ fill, uint32_t* __restrict__ order_raw,
+                                                       const int* gate) {
+    if (gated_off(gate)) return;
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int c = cell_of[i];
+    const uint32_t slot = atomicAdd(&fill[c], 1u);
+    order_raw[start[c] + slot] = static_cast<uint32_t>(i);
+}
+
+// One wavefront per cell: rank = number of members with a smaller index, so the members land in increasing
+// index order — exactly the order of the reference's serial counting sort. Also clears the cell's histogram
+// and fill counters for the next build.
+__global__ void __launch_bounds__(256) k_cell_sort(const uint32_t* __restrict__ start, size_t ncells, size_t n,
+                                                    const uint32_t* __restrict__ order_raw,
+                                                    uint32_t* __restrict__ order, uint32_t* __restrict__ fill,
+                                                    uint32_t* __restrict__ counts, PermuteArgs pa, const int* gate) {
+    if (gated_off(gate)) return;
+    const size_t cell = (static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x) / WAVE;
+    const int lane = threadIdx.x % WAVE;
+    if (cell >= ncells) return;
+    const uint32_t b = start[cell];
+    const uint32_t m = start[cell + 1] - b;
+    if (lane == 0) {
+        fill[cell] = 0;
+        counts[cell] = 0;
+    }
+    for (uint32_t base = 0; base < m; base += WAVE) {
+        const bool valid = base + lane < m;
+        const uint32_t v = valid ? order_raw[b + base + lane] : 0xffffffffu;
+        uint32_t rank = 0;
+        if (m <= WAVE) {
+            for (uint32_t s = 0; s < m; ++s) rank += (static_cast<uint32_t>(__shfl(static_cast<int>(v), s, WAVE)) < v);
+        } else {
+            for (uint32_t s = 0; s < m; ++s) rank += (order_raw[b + s] < v);
+        }
+        if (valid) {
+            const uint32_t k = b + rank;
+            order[k] = v;
+            if (pa.enabled) {
+                const double4 r = pa.pos[v];
+                pa.pos2[k] = r;
+                pa.ref[k] = r;
+                for (int c = 0; c < 3; ++c) {
+                    pa.vel2[c * n + k] = pa.vel[c * n + v];
+                    pa.frc2[c * n + k] = pa.frc[c * n + v];
+                }
+                pa.mass2[k] = pa.mass[v];
+                pa.id2[k] = pa.id[v];
+                pa.cell_of2[k] = static_cast<int>(cell);
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ void sort3(int& a, int& b, int& c) {
+    int t;
+    if (a > b) { t = a; a = b; b = t; }
+    if (b > c) { t = b; b = c; c = t; }
+    if (a > b) { t = a; a = b; b = t; }
+}
+
+__global__ void __launch_bounds__(256) k_list_build(const double4* __restrict__ pos, size_t n,
+                                                     const int* __restrict__ cell_of,
+                                                     const uint32_t* __restrict__ start, int side, double box,
+                                                     double inv_box, double radius2, int use_cells, int padding,
+                                                     int32_t* __restrict__ rows, int32_t* __restrict__ count,
+                                                     int32_t* __restrict__ half, int* __restrict__ flags,
+                                                     unsigned long long* __restrict__ pair_total,
+                                                     CopyBackArgs cb, const int* gate) {
+    if (gated_off(gate)) return;
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    int cnt = 0, hcnt = 0;
+    if (i < n) {
+        const double4 ri = pos[i];
+        if (cb.enabled) {
+            cb.pos[i] = ri;
+            for (int c = 0; c < 3; ++c) {
+                cb.vel[c * n + i] = cb.vel2[c * n + i];
+                cb.frc[c * n + i] = cb.frc2[c * n + i];
+            }
+            cb.mass[i] = cb.mass2[i];
+            cb.id[i] = cb.id2[i];
+            cb.cell_of[i] = cb.cell_of2[i];
+        }
+        auto consider = [&](uint32_t j) {
+            const double4 rj = pos[j];
+            const double dx = min_image(ri.x - rj.x, box, inv_box);
+            const double dy = min_image(ri.y - rj.y, box, inv_box);
+            const double dz = min_image(ri.z - rj.z, box, inv_box);
+            const double r2 = dx * dx + dy * dy + dz * dz;
+            if (r2 < radius2) {
+                if (cnt < padding) rows[static_cast<size_t>(cnt) * n + i] = static_cast<int32_t>(j);
+                ++cnt;
+            }
+        };
+        const uint32_t self = static_cast<uint32_t>(i);
+        if (use_cells) {
+            const int c = cell_of[i];
+            const int cx = c / (side * side), cy = (c / side) % side, cz = c % side;
+            // j > i in the reference's stencil order: dx, then dy, then dz, each -1..1 (CellList.cpp:67-80)
+            for (int ox = -1; ox <= 1; ++ox) {
+                const int x = wrap_index(cx + ox, side);
+                for (int oy = -1; oy <= 1; ++oy) {
+                    const int y = wrap_index(cy + oy, side);
+                    for (int oz = -1; oz <= 1; ++oz) {
+                        const int nc = (x * side + y) * side + wrap_index(cz + oz, side);
+                        if (nc < c) continue;  // lower cells hold only lower indices
+                        const uint32_t e = start[nc + 1];
+                        for (uint32_t j = (nc == c ? self + 1 : start[nc]); j < e; ++j) consider(j);
+                    }
+                }
+            }
+            hcnt = cnt;
+            // j < i in increasing index: the 27 cells in increasing id are the sorted x, y, z offsets in
+            // lexicographic order
+            int xs[3] = {wrap_index(cx - 1, side), cx, wrap_index(cx + 1, side)};
+            int ys[3] = {wrap_index(cy - 1, side), cy, wrap_index(cy + 1, side)};
+            int zs[3] = {wrap_index(cz - 1, side), cz, wrap_index(cz + 1, side)};
+            sort3(xs[0], xs[1], xs[2]);
+            sort3(ys[0], ys[1], ys[2]);
+            sort3(zs[0], zs[1], zs[2]);
+            for (int a = 0; a < 3; ++a)
+                for (int b2 = 0; b2 < 3; ++b2)
+                    for (int d = 0; d < 3; ++d) {
+                        const int nc = (xs[a] * side + ys[b2]) * side + zs[d];
+                        if (nc > c) continue;
+                        const uint32_t e = (nc == c) ? self : start[nc + 1];
+                        for (uint32_t j = start[nc]; j < e; ++j) consider(j);
+                    }
+        } else {
+            // grid too small for a distinct 27-cell stencil: every pair (NeighbourList.cpp:46-50)
+            for (uint32_t j = self + 1; j < n; ++j) consider(j);
+            hcnt = cnt;
+            for (uint32_t j = 0; j < self; ++j) consider(j);
+        }
+        // stored lengths never exceed the padding, so readers stay inside the rows even after an overflow
+        // (the true maximum goes to FLAG_MAXCOUNT and the overflow flag below)
+        count[i] = cnt < padding ? cnt : padding;
+        half[i] = hcnt < padding ? hcnt : padding;
+    }
+    // wavefront max / sum, then one atomic per wavefront
+    int mx = cnt;
+    unsigned long long hs = static_cast<unsigned long long>(hcnt);
+#pragma unroll
+    for (int off = WAVE / 2; off > 0; off >>= 1) {
+        const int o = __shfl_xor(mx, off, WAVE);
+        mx = o > mx ? o : mx;
+        hs += __shfl_xor(hs, off, WAVE);
+    }
+    if (threadIdx.x % WAVE == 0) {
+        atomicMax(&flags[FLAG_MAXCOUNT], mx);
+        if (mx > padding) atomicOr(&flags[FLAG_OVERFLOW], 1);
+        atomicAdd(pair_total, hs);
+    }
+    if (gate && i == 0) atomicAdd(&flags[FLAG_REBUILDS], 1);
+}
+
+__global__ void __launch_bounds__(256) k_displacement(const double4* __restrict__ pos,
+                                                       const double4* __restrict__ ref, size_t n, double box,
+                                                       double inv_box, double max_disp2, int* __restrict__ flags) {
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double4 r = pos[i], q = ref[i];
+    const double dx = min_image(r.x - q.x, box, inv_box);
+    const double dy = min_image(r.y - q.y, box, inv_box);
+    const double dz = min_image(r.z - q.z, box, inv_box);
+    if (dx * dx + dy * dy + dz * dz > max_disp2) flags[FLAG_STALE] = 1;
+}
+
+__global__ void k_clear_build_flags(int* flags, unsigned long long* pair_total, const int* gate) {
+    if (gated_off(gate)) return;
+    flags[FLAG_OVERFLOW] = 0;
+    flags[FLAG_MAXCOUNT] = 0;
+    *pair_total = 0ull;
+}
+
+namespace {
+
+int grid_side(double box, double min_cell) {
+    const int s = static_cast<int>(box / min_cell);
+    return s > 1 ? s : 1;
+}
+
+// Padded row length for a fresh list: 1.35x the ideal-gas full-list count plus slack, multiple of 8.
+int auto_padding(size_t n, double box, double radius) {
+    const double density = static_cast<double>(n) / (box * box * box);
+    const double expected = 4.0 / 3.0 * M_PI * radius * radius * radius * density;
+    long p = static_cast<long>(std::ceil(expected * 1.35 + 24.0));
+    p = (p + 7) / 8 * 8;
+    const long cap = n > 1 ? static_cast<long>(n - 1) : 1;
+    return static_cast<int>(std::min<long>(p, std::max<long>(cap, 1)));
+}
+
+}  // namespace
+
+// Scratch words kept per list next to the flags: the running half-pair total.
+static unsigned long long* pair_total_of(md_list* l) {
+    return reinterpret_cast<unsigned long long*>(l->flags + FLAG_COUNT);
+}
+
+void list_prepare(md_list* l, md_state* s) {
+    hipStream_t st = l->ctx->stream;
+    const int side = grid_side(s->box, l->radius);
+    const size_t ncells = static_cast<size_t>(side) * side * side;
+    if (!l->flags) {
+        l->flags = dev_alloc<int>(FLAG_COUNT + 4);
+        MD_HIP(hipMemsetAsync(l->flags, 0, sizeof(int) * (FLAG_COUNT + 4), st));
+    }
+    if (ncells + 1 > l->ncells_cap || !l->cell_count) {
+        dev_free(l->cell_count);
+        dev_free(l->cell_start);
+        dev_free(l->scan_tmp);
+        l->cell_count = dev_alloc<uint32_t>(ncells + 1);
+        l->cell_start = dev_alloc<uint32_t>(ncells + 1);
+        l->scan_tmp = dev_alloc<uint32_t>(ncells + 1);  // per-cell fill counters of the scatter
+        l->ncells_cap = ncells + 1;
+        MD_HIP(hipMemsetAsync(l->cell_count, 0, sizeof(uint32_t) * (ncells + 1), st));
+        MD_HIP(hipMemsetAsync(l->scan_tmp, 0, sizeof(uint32_t) * (ncells + 1), st));
+    }
+    l->side = side;
+    l->ncells = ncells;
+    if (s->n > l->n_cap || !l->cell_of) {
+        dev_free(l->cell_of);
+        dev_free(l->cell_of_alt);
+        dev_free(l->order_raw);
+        dev_free(l->order);
+        dev_free(l->ref);
+        dev_free(l->count);
+        dev_free(l->half);
+        l->cell_of = dev_alloc<int>(s->n);
+        l->cell_of_alt = dev_alloc<int>(s->n);
+        l->order_raw = dev_alloc<uint32_t>(s->n);
+        l->order = dev_alloc<uint32_t>(s->n);
+        l->ref = dev_alloc<double4>(s->n);
+        l->count = dev_alloc<int32_t>(s->n);
+        l->half = dev_alloc<int32_t>(s->n);
+        l->n_cap = s->n;
+    }
+    if (l->padding <= 0) l->padding = auto_padding(s->n, s->box, l->radius);
+    const size_t need = static_cast<size_t>(l->padding) * (s->n ? s->n : 1);
+    if (need > l->rows_cap || !l->rows) {
+        dev_free(l->rows);
+        l->rows = dev_alloc<int32_t>(need);
+        l->rows_cap = need;
+    }
+    l->n = s->n;
+}
+
+// Histogram, scan, scatter and stable per-cell order of the current positions; with `permute` the state is
+// also gathered into its alt arrays in that order.
+static void cells_enqueue(md_list* l, md_state* s, double per_length, bool permute, const int* gate) {
+    hipStream_t st = l->ctx->stream;
+    const size_t n = s->n;
+    if (n == 0) return;
+    k_cell_count<<<blocks_for(n, 256), 256, 0, st>>>(s->pos, n, per_length, l->side, l->cell_of, l->cell_count,
+                                                     gate);
+    k_scan_cells<<<1, 1024, 0, st>>>(l->cell_count, l->ncells, l->cell_start, gate);
+    k_cell_scatter<<<blocks_for(n, 256), 256, 0, st>>>(l->cell_of, n, l->cell_start, l->scan_tmp, l->order_raw,
+                                                       gate);
+    PermuteArgs pa{};
+    pa.enabled = permute ? 1 : 0;
+    if (permute) {
+        pa.pos = s->pos;
+        pa.vel = s->vel;
+        pa.frc = s->frc;
+        pa.mass = s->mass;
+        pa.id = s->id;
+        pa.pos2 = s->pos_alt;
+        pa.vel2 = s->vel_alt;
+        pa.frc2 = s->frc_alt;
+        pa.mass2 = s->mass_alt;
+        pa.id2 = s->id_alt;
+        pa.ref = l->ref;
+        pa.cell_of2 = l->cell_of_alt;
+    }
+    k_cell_sort<<<blocks_for(l->ncells * WAVE, 256), 256, 0, st>>>(l->cell_start, l->ncells, n, l->order_raw,
+                                                                   l->order, l->scan_tmp, l->cell_count, pa, gate);
+    MD_HIP(hipGetLastError());
+}
+
+static void list_kernel_enqueue(md_list* l, md_state* s, bool from_alt, const int* gate) {
+    hipStream_t st = l->ctx->stream;
+    const size_t n = s->n;
+    CopyBackArgs cb{};
+    cb.enabled = from_alt ? 1 : 0;
+    if (from_alt) {
+        cb.vel2 = s->vel_alt;
+        cb.frc2 = s->frc_alt;
+        cb.mass2 = s->mass_alt;
+        cb.id2 = s->id_alt;
+        cb.cell_of2 = l->cell_of_alt;
+        cb.pos = s->pos;
+        cb.vel = s->vel;
+        cb.frc = s->frc;
+        cb.mass = s->mass;
+        cb.id = s->id;
+        cb.cell_of = l->cell_of;
+    }
+    k_clear_build_flags<<<1, 1, 0, st>>>(l->flags, pair_total_of(l), gate);
+    k_list_build<<<blocks_for(n ? n : 1, 256), 256, 0, st>>>(
+        from_alt ? s->pos_alt : s->pos, n, from_alt ? l->cell_of_alt : l->cell_of, l->cell_start, l->side, s->box,
+        s->inv_box, l->radius2, l->side >= 3 ? 1 : 0, l->padding, l->rows, l->count, l->half, l->flags,
+        pair_total_of(l), cb, gate);
+    MD_HIP(hipGetLastError());
+}
+
+void cells_build(md_list* l, md_state* s, double min_cell, int side) {
+    (void)min_cell;
+    cells_enqueue(l, s, side / s->box, false, nullptr);
+}
+
+void list_rebuild(md_list* l, md_state* s) {
+    if (2.0 * l->radius > s->box) fail(MD_ERR_RUNTIME, "Cutoff + skin must be less than half the box length");
+    list_prepare(l, s);
+    hipStream_t st = l->ctx->stream;
+    const double per_length = l->side / s->box;
+    cells_enqueue(l, s, per_length, true, nullptr);
+    list_kernel_enqueue(l, s, true, nullptr);
+    int* hf = l->ctx->host_flags;
+    for (int attempt = 0;; ++attempt) {
+        MD_HIP(hipMemcpyAsync(hf, l->flags, sizeof(int) * FLAG_COUNT, hipMemcpyDeviceToHost, st));
+        MD_HIP(hipMemcpyAsync(hf + FLAG_COUNT, pair_total_of(l), sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                              st));
+        MD_HIP(hipStreamSynchronize(st));
+        l->max_count = hf[FLAG_MAXCOUNT];
+        unsigned long long total;
+        std::memcpy(&total, hf + FLAG_COUNT, sizeof(total));
+        l->half_pairs = static_cast<size_t>(total);
+        if (!hf[FLAG_OVERFLOW] || s->n == 0) break;
+        if (attempt > 2) fail(MD_ERR_HIP, "neighbour list padding did not converge");
+        // grow the rows and rebuild the list from the (already re-sorted) state
+        l->padding = (l->max_count + l->max_count / 4 + 16 + 7) / 8 * 8;
+        list_prepare(l, s);
+        list_kernel_enqueue(l, s, false, nullptr);
+    }
+    MD_HIP(hipMemsetAsync(l->flags + FLAG_STALE, 0, sizeof(int), st));
+    l->built = true;
+    ++l->rebuilds;
+}
+
+void displacement_check(md_list* l, md_state* s) {
+    hipStream_t st = l->ctx->stream;
+    if (s->n == 0) return;
+    k_displacement<<<blocks_for(s->n, 256), 256, 0, st>>>(s->pos, l->ref, s->n, s->box, s->inv_box, l->max_disp2,
+                                                          l->flags);
+    MD_HIP(hipGetLastError());
+}
+
+// Device-flag-gated rebuild used inside md_run: every kernel reads flags[FLAG_STALE] and returns at once when it
+// is clear, so the same launch sequence serves steps with and without a rebuild.
+void list_enqueue_conditional_rebuild(md_list* l, md_state* s) {
+    const int* gate = l->flags + FLAG_STALE;
+    const double per_length = l->side / s->box;
+    cells_enqueue(l, s, per_length, true, gate);
+    list_kernel_enqueue(l, s, true, gate);
+}
+
+static void release_list_buffers(md_list* l) {
+    dev_free(l->cell_count);
+    dev_free(l->cell_start);
+    dev_free(l->scan_tmp);
+    dev_free(l->cell_of);
+    dev_free(l->cell_of_alt);
+    dev_free(l->order_raw);
+    dev_free(l->order);
+    dev_free(l->ref);
+    dev_free(l->count);
+    dev_free(l->half);
+    dev_free(l->rows);
+    dev_free(l->flags);
+}
+
+void set_error(const std::string& m);
+
+}  // namespace mdgpu
+
+using namespace mdgpu;
+
+
+extern "C" {
+
+int md_cells_build(md_state* s, double min_cell, int* cells_per_side, int* cell_of, uint64_t* cell_start,
+                   uint64_t* order) {
+    MD_API_BEGIN
+    if (!s) fail(MD_ERR_ARG, "state is NULL");
+    if (!(min_cell > 0.0)) fail(MD_ERR_ARG, "min_cell must be positive");
+    // a private grid object: the standalone CellList of the reference owns its own arrays
+    md_list tmp;
+    tmp.ctx = s->ctx;
+    tmp.radius = min_cell;
+    tmp.padding = 1;
+    const size_t n = s->n;
+    try {
+        list_prepare(&tmp, s);
+        const int side = tmp.side;
+        cells_build(&tmp, s, min_cell, side);
+        hipStream_t st = s->ctx->stream;
+        std::vector<int> co(n);
+        std::vector<uint32_t> cs(tmp.ncells + 1), od(n);
+        if (n) {
+            MD_HIP(hipMemcpyAsync(co.data(), tmp.cell_of, sizeof(int) * n, hipMemcpyDeviceToHost, st));
+            MD_HIP(hipMemcpyAsync(od.data(), tmp.order, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, st));
+            MD_HIP(hipMemcpyAsync(cs.data(), tmp.cell_start, sizeof(uint32_t) * (tmp.ncells + 1),
+                                  hipMemcpyDeviceToHost, st));
+        }
+        MD_HIP(hipStreamSynchronize(st));
+        if (cells_per_side) *cells_per_side = side;
+        if (cell_of) std::copy(co.begin(), co.end(), cell_of);
+        if (order)
+            for (size_t i = 0; i < n; ++i) order[i] = od[i];
+        if (cell_start) {
+            if (n)
+                for (size_t c = 0; c <= tmp.ncells; ++c) cell_start[c] = cs[c];
+            else
+                for (size_t c = 0; c <= tmp.ncells; ++c) cell_start[c] = 0;
+        }
+    } catch (...) {
+        release_list_buffers(&tmp);
+        throw;
+    }
+    release_list_buffers(&tmp);
+    MD_API_END
+}
+
+int md_list_create(md_ctx* ctx, double cutoff, double skin, int max_neighbours, md_list** out) {
+    MD_API_BEGIN
+    if (!ctx || !out) fail(MD_ERR_ARG, "ctx/out is NULL");
+    auto* l = new md_list;
+    l->ctx = ctx;
+    l->cutoff = cutoff;
+    l->skin = skin;
+    l->radius = cutoff + skin;
+    l->radius2 = (cutoff + skin) * (cutoff + skin);
+    l->max_disp2 = 0.25 * skin * skin;
+    l->padding = max_neighbours > 0 ? max_neighbours : 0;
+    *out = l;
+    MD_API_END
+}
+
+int md_list_destroy(md_list* l) {
+    if (!l) return MD_OK;
+    if (l->ctx) (void)hipStreamSynchronize(l->ctx->stream);
+    release_list_buffers(l);
+    delete l;
+    return MD_OK;
+}
+
+int md_list_build(md_list* l, md_state* s) {
+    MD_API_BEGIN
+    if (!l || !s) fail(MD_ERR_ARG, "list/state is NULL");
+    list_rebuild(l, s);
+    MD_API_END
+}
+
+int md_list_update(md_list* l, md_state* s, int* rebuilt) {
+    MD_API_BEGIN
+    if (!l || !s) fail(MD_ERR_ARG, "list/state is NULL");
+    bool stale = !l->built || l->n != s->n;
+    if (!stale) {
+        hipStream_t st = l->ctx->stream;
+        MD_HIP(hipMemsetAsync(l->flags + FLAG_STALE, 0, sizeof(int), st));
+        displacement_check(l, s);
+        MD_HIP(hipMemcpyAsync(l->ctx->host_flags, l->flags + FLAG_STALE, sizeof(int), hipMemcpyDeviceToHost, st));
+        MD_HIP(hipStreamSynchronize(st));
+        stale = l->ctx->host_flags[0] != 0;
+    }
+    if (stale) list_rebuild(l, s);
+    if (rebuilt) *rebuilt = stale ? 1 : 0;
+    MD_API_END
+}
+
+int md_list_get_info(const md_list* l, md_list_info* info) {
+    MD_API_BEGIN
+    if (!l || !info) fail(MD_ERR_ARG, "list/info is NULL");
+    info->half_pairs = l->half_pairs;
+    info->rebuilds = l->rebuilds;
+    info->cells_per_side = l->side;
+    info->max_neighbours = l->max_count;
+    info->padding = l->padding;
+    MD_API_END
+}
+
+int md_list_half(md_list* l, uint64_t* offsets, uint64_t* neighbours) {
+    MD_API_BEGIN
+    if (!l) fail(MD_ERR_ARG, "list is NULL");
+    if (!l->built) fail(MD_ERR_STATE, "list not built");
+    const size_t n = l->n;
+    hipStream_t st = l->ctx->stream;
+    std::vector<int32_t> half(n), rows(neighbours ? static_cast<size_t>(l->padding) * n : 0);
+    if (n) MD_HIP(hipMemcpyAsync(half.data(), l->half, sizeof(int32_t) * n, hipMemcpyDeviceToHost, st));
+    if (neighbours && n)
+        MD_HIP(hipMemcpyAsync(rows.data(), l->rows, sizeof(int32_t) * rows.size(), hipMemcpyDeviceToHost, st));
+    MD_HIP(hipStreamSynchronize(st));
+    uint64_t at = 0;
+    for (size_t i = 0; i < n; ++i) {
+        if (offsets) offsets[i] = at;
+        if (neighbours)
+            for (int k = 0; k < half[i]; ++k) neighbours[at + k] = static_cast<uint64_t>(rows[static_cast<size_t>(k) * n + i]);
+        at += static_cast<uint64_t>(half[i]);
+    }
+    if (offsets) offsets[n] = at;
+    MD_API_END
+}
+
+int md_list_full(md_list* l, int32_t* counts, int32_t* half_counts, int32_t* rows_out) {
+    MD_API_BEGIN
+    if (!l) fail(MD_ERR_ARG, "list is NULL");
+    if (!l->built) fail(MD_ERR_STATE, "list not built");
+    const size_t n = l->n;
+    hipStream_t st = l->ctx->stream;
+    if (counts && n) MD_HIP(hipMemcpyAsync(counts, l->count, sizeof(int32_t) * n, hipMemcpyDeviceToHost, st));
+    if (half_counts && n) MD_HIP(hipMemcpyAsync(half_counts, l->half, sizeof(int32_t) * n, hipMemcpyDeviceToHost, st));
+    std::vector<int32_t> rows(rows_out ? static_cast<size_t>(l->padding) * n : 0);
+    if (rows_out && n)
+        MD_HIP(hipMemcpyAsync(rows.data(), l->rows, sizeof(int32_t) * rows.size(), hipMemcpyDeviceToHost, st));
+    MD_HIP(hipStreamSynchronize(st));
+    if (rows_out)
+        for (size_t i = 0; i < n; ++i)
+            for (int k = 0; k < l->padding; ++k)
+                rows_out[i * l->padding + k] = rows[static_cast<size_t>(k) * n + i];
+    MD_API_END
+}
+
+}  // extern "C"

@@ -1,0 +1,79 @@
+// Device helpers shared by the mdgpu kernels. The library is compiled with -ffp-contract=off, so every double
+// expression here rounds once per operation in source order — the same arithmetic as the reference's own
+// expressions (System.hpp:86-112, Forcefield.cpp:25-36), which keeps cell ids, list membership, integrator
+// updates and FP64 forces bit-equal to the CPU results.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace mdgpu {
+
+constexpr int WAVE = 64;
+
+// Minimum image of one separation component: d - L * round(d / L), round half away from zero (System.hpp:94).
+__device__ __forceinline__ double min_image(double d, double box, double inv_box) {
+    const double images = round(d * inv_box);
+    return d - box * images;
+}
+
+// x - L * floor(x / L) (System.hpp:110-112).
+__device__ __forceinline__ double wrap_coord(double x, double box, double inv_box) {
+    const double images = floor(x * inv_box);
+    return x - box * images;
+}
+
+// Cell coordinate along one axis: min(int(x * side / L), side - 1) (CellList.cpp:30-33). Coordinates below the
+// box (never produced by the wrapped dynamics; the reference would index before its arrays) are clamped to 0.
+__device__ __forceinline__ int cell_axis(double x, double per_length, int side) {
+    const int c = static_cast<int>(x * per_length);
+    return c < 0 ? 0 : (c < side - 1 ? c : side - 1);
+}
+
+// Linear cell id (cx * side + cy) * side + cz (CellList.cpp:36).
+__device__ __forceinline__ int cell_of_position(double x, double y, double z, double per_length, int side) {
+    return (cell_axis(x, per_length, side) * side + cell_axis(y, per_length, side)) * side +
+           cell_axis(z, per_length, side);
+}
+
+__device__ __forceinline__ int wrap_index(int c, int side) {
+    return c < 0 ? c + side : (c >= side ? c - side : c);
+}
+
+// Wavefront sum by butterfly shuffles; every lane returns the same value, combined in a fixed order.
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = WAVE / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, WAVE);
+    return v;
+}
+
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+    for (int off = WAVE / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, WAVE);
+    return v;
+}
+
+// Block-wide sums of `W` values per thread into out[0..W) for thread 0 (fixed order: waves, then wave 0..).
+template <int W, int THREADS>
+__device__ __forceinline__ void block_sum(double (&v)[W], double* out) {
+    __shared__ double lds[THREADS / WAVE][W];
+    const int lane = threadIdx.x % WAVE, wave = threadIdx.x / WAVE;
+#pragma unroll
+    for (int k = 0; k < W; ++k) v[k] = wave_sum(v[k]);
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < W; ++k) lds[wave][k] = v[k];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < W; ++k) {
+            double s = 0.0;
+            for (int w = 0; w < THREADS / WAVE; ++w) s += lds[w][k];
+            out[k] = s;
+        }
+    }
+}
+
+}  // namespace mdgpu

@@ -1,0 +1,263 @@
+// Lennard-Jones force kernels (reference: Forcefield.cpp:8-450).
+//
+// Gather style over the padded FULL list: thread i owns particle i, walks its row and sums its own force, so
+// there is one writer per particle, no atomics, and a fixed summation order. Potential energy, virial and the
+// inside-cutoff count are taken over the row's first `half[i]` entries (the j > i pairs), so each pair is
+// counted once, as in the reference's half list; they are reduced per wavefront by shuffles, per block through
+// LDS, and across blocks by one fixed-order reduction.
+//
+// FP64: each pair term is F/r * d rounded once and added in row order: the j > i part reproduces the reference's
+// deterministic-mode "own pairs" sum and the j < i part (increasing j) its "pairs as second" subtraction, since
+// d_ij = -d_ji exactly. Forces are therefore bit-equal to Forcefield.cpp:214-326 and to the O(N^2) path.
+// MIXED: double minimum image narrowed to float, float pair maths (Forcefield.cpp:352-446), double accumulation.
+#include <cmath>
+
+#include "device.hpp"
+#include "internal.hpp"
+
+namespace mdgpu {
+
+struct LJConst {
+    double sig2, c24, c4, shift, rc2;
+    float sig2f, c24f, c4f, shiftf, rc2f;
+};
+
+// Constructor constants (Forcefield.cpp:8-15): shift = V(rc) with std::pow, as the reference computes it.
+static LJConst make_lj(const md_lj& p) {
+    LJConst c;
+    c.sig2 = p.sigma * p.sigma;
+    c.c24 = 24.0 * p.epsilon;
+    c.c4 = 4.0 * p.epsilon;
+    const double s6 = std::pow(p.sigma / p.cutoff, 6);
+    const double s12 = s6 * s6;
+    c.shift = 4.0 * p.epsilon * (s12 - s6);
+    c.rc2 = p.cutoff * p.cutoff;
+    c.sig2f = static_cast<float>(c.sig2);
+    c.c24f = 24.0f * static_cast<float>(p.epsilon);
+    c.c4f = 4.0f * static_cast<float>(p.epsilon);
+    c.shiftf = static_cast<float>(c.shift);
+    c.rc2f = static_cast<float>(c.rc2);
+    return c;
+}
+
+constexpr int FORCE_THREADS = 256;
+
+__global__ void __launch_bounds__(FORCE_THREADS)
+    k_forces_fp64(const double4* __restrict__ pos, size_t n, const int32_t* __restrict__ rows,
+                  const int32_t* __restrict__ count, const int32_t* __restrict__ half, double box, double inv_box,
+                  LJConst lj, double* __restrict__ frc, double* __restrict__ partials, int* __restrict__ clear_flag) {
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    double acc[3] = {0.0, 0.0, 0.0};  // potential, virial, inside pairs
+    if (i < n) {
+        const double4 ri = pos[i];
+        const int m = count[i], h = half[i];
+        const int32_t* row = rows + i;
+        for (int k = 0; k < m; ++k) {
+            const double4 rj = pos[row[static_cast<size_t>(k) * n]];
+            const double dx = min_image(ri.x - rj.x, box, inv_box);
+            const double dy = min_image(ri.y - rj.y, box, inv_box);
+            const double dz = min_image(ri.z - rj.z, box, inv_box);
+            const double r2 = dx * dx + dy * dy + dz * dz;
+            const double inv_r2 = 1.0 / r2;
+            const double s2 = lj.sig2 * inv_r2;
+            const double s6 = s2 * s2 * s2;
+            const double s12 = s6 * s6;
+            const bool inside = r2 < lj.rc2;
+            const double f = inside ? lj.c24 * (2.0 * s12 - s6) * inv_r2 : 0.0;
+            fx += f * dx;
+            fy += f * dy;
+            fz += f * dz;
+            if (k < h) {
+                acc[0] += inside ? lj.c4 * (s12 - s6) - lj.shift : 0.0;
+                acc[1] += f * r2;
+                acc[2] += inside ? 1.0 : 0.0;
+            }
+        }
+        frc[i] = fx;
+        frc[n + i] = fy;
+        frc[2 * n + i] = fz;
+    }
+    if (clear_flag && i == 0) *clear_flag = 0;
+    block_sum<3, FORCE_THREADS>(acc, partials + static_cast<size_t>(blockIdx.x) * 3);
+}
+
+__global__ void __launch_bounds__(FORCE_THREADS)
+    k_forces_mixed(const double4* __restrict__ pos, size_t n, const int32_t* __restrict__ rows,
+                   const int32_t* __restrict__ count, const int32_t* __restrict__ half, double box, double inv_box,
+                   LJConst lj, double* __restrict__ frc, double* __restrict__ partials, int* __restrict__ clear_flag) {
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    double acc[3] = {0.0, 0.0, 0.0};
+    if (i < n) {
+        const double4 ri = pos[i];
+        const int m = count[i], h = half[i];
+        const int32_t* row = rows + i;
+        float u = 0.0f, w = 0.0f, c = 0.0f;
+        for (int k = 0; k < m; ++k) {
+            const double4 rj = pos[row[static_cast<size_t>(k) * n]];
+            double ex = ri.x - rj.x, ey = ri.y - rj.y, ez = ri.z - rj.z;
+            ex = fma(-box, rint(ex * inv_box), ex);
+            ey = fma(-box, rint(ey * inv_box), ey);
+            ez = fma(-box, rint(ez * inv_box), ez);
+            const float dx = static_cast<float>(ex), dy = static_cast<float>(ey), dz = static_cast<float>(ez);
+            const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+            const float inv_r2 = __builtin_amdgcn_rcpf(r2);
+            const float s2 = lj.sig2f * inv_r2;
+            const float s6 = s2 * s2 * s2;
+            const float s12 = s6 * s6;
+            const bool inside = r2 < lj.rc2f;
+            const float f = inside ? lj.c24f * fmaf(2.0f, s12, -s6) * inv_r2 : 0.0f;
+            fx += static_cast<double>(f * dx);
+            fy += static_cast<double>(f * dy);
+            fz += static_cast<double>(f * dz);
+            if (k < h) {
+                u += inside ? fmaf(lj.c4f, s12 - s6, -lj.shiftf) : 0.0f;
+                w = fmaf(f, r2, w);
+                c += inside ? 1.0f : 0.0f;
+            }
+        }
+        acc[0] = u;
+        acc[1] = w;
+        acc[2] = c;
+        frc[i] = fx;
+        frc[n + i] = fy;
+        frc[2 * n + i] = fz;
+    }
+    if (clear_flag && i == 0) *clear_flag = 0;
+    block_sum<3, FORCE_THREADS>(acc, partials + static_cast<size_t>(blockIdx.x) * 3);
+}
+
+// O(N^2): particle i sums over every j != i in increasing j, skipping r^2 >= rc^2 (Forcefield.cpp:39-78); that is
+// the reference's accumulation order for particle i (rows j < i subtract, then its own row adds), so the forces
+// are bit-equal to it. Positions are staged through LDS one tile at a time.
+__global__ void __launch_bounds__(FORCE_THREADS)
+    k_forces_allpairs(const double4* __restrict__ pos, size_t n, double box, double inv_box, LJConst lj,
+                      double* __restrict__ frc, double* __restrict__ partials) {
+    __shared__ double4 tile[FORCE_THREADS];
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const bool active = i < n;
+    const double4 ri = active ? pos[i] : make_double4(0, 0, 0, 0);
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (size_t base = 0; base < n; base += FORCE_THREADS) {
+        __syncthreads();
+        if (base + threadIdx.x < n) tile[threadIdx.x] = pos[base + threadIdx.x];
+        __syncthreads();
+        const size_t cnt = (n - base) < FORCE_THREADS ? (n - base) : FORCE_THREADS;
+        if (!active) continue;
+        for (size_t t = 0; t < cnt; ++t) {
+            const size_t j = base + t;
+            if (j == i) continue;
+            const double4 rj = tile[t];
+            const double dx = min_image(ri.x - rj.x, box, inv_box);
+            const double dy = min_image(ri.y - rj.y, box, inv_box);
+            const double dz = min_image(ri.z - rj.z, box, inv_box);
+            const double r2 = dx * dx + dy * dy + dz * dz;
+            if (r2 >= lj.rc2) continue;
+            const double inv_r2 = 1.0 / r2;
+            const double s2 = lj.sig2 * inv_r2;
+            const double s6 = s2 * s2 * s2;
+            const double s12 = s6 * s6;
+            const double f = lj.c24 * (2.0 * s12 - s6) * inv_r2;
+            fx += f * dx;
+            fy += f * dy;
+            fz += f * dz;
+            if (j > i) {
+                acc[0] += lj.c4 * (s12 - s6) - lj.shift;
+                acc[1] += f * r2;
+                acc[2] += 1.0;
+            }
+        }
+    }
+    if (active) {
+        frc[i] = fx;
+        frc[n + i] = fy;
+        frc[2 * n + i] = fz;
+    }
+    block_sum<3, FORCE_THREADS>(acc, partials + static_cast<size_t>(blockIdx.x) * 3);
+}
+
+void check_cutoff(const md_state* s, const md_lj& lj) {
+    if (2.0 * lj.cutoff > s->box) fail(MD_ERR_RUNTIME, "Cutoff must be less than half the box length");
+}
+
+void forces_enqueue(md_state* s, md_list* l, const md_lj& p, int precision, double* out_dev, int* clear_flag,
+                    hipEvent_t ev_start, hipEvent_t ev_stop) {
+    hipStream_t st = s->ctx->stream;
+    const size_t n = s->n;
+    const unsigned blocks = blocks_for(n ? n : 1, FORCE_THREADS);
+    state_reserve_partials(s, static_cast<size_t>(blocks) * 3);
+    const LJConst lj = make_lj(p);
+    if (ev_start) MD_HIP(hipEventRecord(ev_start, st));
+    if (precision == MD_MIXED)
+        k_forces_mixed<<<blocks, FORCE_THREADS, 0, st>>>(s->pos, n, l->rows, l->count, l->half, s->box, s->inv_box,
+                                                          lj, s->frc, s->partials, clear_flag);
+    else
+        k_forces_fp64<<<blocks, FORCE_THREADS, 0, st>>>(s->pos, n, l->rows, l->count, l->half, s->box, s->inv_box,
+                                                         lj, s->frc, s->partials, clear_flag);
+    MD_HIP(hipGetLastError());
+    if (ev_stop) MD_HIP(hipEventRecord(ev_stop, st));
+    reduce_partials_enqueue(st, s->partials, static_cast<int>(blocks), 3, out_dev);
+}
+
+void forces_enqueue(md_state* s, md_list* l, const md_lj& p, int precision, double* out_dev) {
+    forces_enqueue(s, l, p, precision, out_dev, nullptr, nullptr, nullptr);
+}
+
+void forces_allpairs_enqueue(md_state* s, const md_lj& p, double* out_dev) {
+    hipStream_t st = s->ctx->stream;
+    const size_t n = s->n;
+    const unsigned blocks = blocks_for(n ? n : 1, FORCE_THREADS);
+    state_reserve_partials(s, static_cast<size_t>(blocks) * 3);
+    k_forces_allpairs<<<blocks, FORCE_THREADS, 0, st>>>(s->pos, n, s->box, s->inv_box, make_lj(p), s->frc,
+                                                         s->partials);
+    MD_HIP(hipGetLastError());
+    reduce_partials_enqueue(st, s->partials, static_cast<int>(blocks), 3, out_dev);
+}
+
+void set_error(const std::string& m);
+
+void read_force_result(md_state* s, md_force_result* out) {
+    if (!out) return;
+    hipStream_t st = s->ctx->stream;
+    MD_HIP(hipMemcpyAsync(s->ctx->host_scalars, s->scalars, 3 * sizeof(double), hipMemcpyDeviceToHost, st));
+    MD_HIP(hipStreamSynchronize(st));
+    out->potential = s->ctx->host_scalars[0];
+    out->virial = s->ctx->host_scalars[1];
+    out->inside_pairs = s->ctx->host_scalars[2];
+}
+
+void validate_list(const md_state* s, const md_list* l) {
+    if (!l->built) fail(MD_ERR_STATE, "neighbour list has not been built");
+    if (l->n != s->n) fail(MD_ERR_STATE, "neighbour list was built for a different particle count");
+}
+
+}  // namespace mdgpu
+
+using namespace mdgpu;
+
+
+extern "C" {
+
+int md_forces(md_state* s, md_list* l, const md_lj* lj, int precision, md_force_result* out) {
+    MD_API_BEGIN
+    if (!s || !l || !lj) fail(MD_ERR_ARG, "state/list/lj is NULL");
+    if (precision != MD_FP64 && precision != MD_MIXED) fail(MD_ERR_ARG, "unknown precision");
+    check_cutoff(s, *lj);
+    validate_list(s, l);
+    forces_enqueue(s, l, *lj, precision, s->scalars);
+    read_force_result(s, out);
+    MD_API_END
+}
+
+int md_forces_allpairs(md_state* s, const md_lj* lj, md_force_result* out) {
+    MD_API_BEGIN
+    if (!s || !lj) fail(MD_ERR_ARG, "state/lj is NULL");
+    check_cutoff(s, *lj);
+    forces_allpairs_enqueue(s, *lj, s->scalars);
+    read_force_result(s, out);
+    MD_API_END
+}
+
+}  // extern "C"

@@ -1,0 +1,315 @@
+// Velocity Verlet (reference: Integrator.cpp:7-71), the Bussi thermostat update (Thermostat.cpp:36-58) and the
+// multi-step driver md_run (md_sim's production loop, main.cpp:272-304) that keeps the whole step on the GPU.
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "device.hpp"
+#include "internal.hpp"
+
+namespace mdgpu {
+
+constexpr int VV_THREADS = 256;
+
+// v += (dt/2 f) / m; r = wrap(r + dt v) per component (Integrator.cpp:32-53). With `ref` set, also the list's
+// staleness test of NeighbourList::update: minimum-image |r - r_build|^2 > skin^2 / 4 (NeighbourList.cpp:25-30).
+__global__ void __launch_bounds__(VV_THREADS)
+    k_kick_drift(double4* __restrict__ pos, double* __restrict__ vel, const double* __restrict__ frc,
+                 const double* __restrict__ mass, size_t n, double half_dt, double dt, double box, double inv_box,
+                 const double4* __restrict__ ref, double max_disp2, int* __restrict__ stale) {
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double m = mass[i];
+    double4 r = pos[i];
+    double v[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const double kick = half_dt * frc[c * n + i];
+        v[c] = vel[c * n + i] + kick / m;
+        vel[c * n + i] = v[c];
+    }
+    r.x = wrap_coord(r.x + dt * v[0], box, inv_box);
+    r.y = wrap_coord(r.y + dt * v[1], box, inv_box);
+    r.z = wrap_coord(r.z + dt * v[2], box, inv_box);
+    pos[i] = r;
+    if (ref) {
+        const double4 q = ref[i];
+        const double dx = min_image(r.x - q.x, box, inv_box);
+        const double dy = min_image(r.y - q.y, box, inv_box);
+        const double dz = min_image(r.z - q.z, box, inv_box);
+        if (dx * dx + dy * dy + dz * dz > max_disp2) *stale = 1;
+    }
+}
+
+// v += (dt/2 f) / m (Integrator.cpp:55-71); with `partials`, each block also sums (0.5 m) v^2 in a fixed order.
+__global__ void __launch_bounds__(VV_THREADS)
+    k_kick(double* __restrict__ vel, const double* __restrict__ frc, const double* __restrict__ mass, size_t n,
+           double half_dt, double* __restrict__ partials) {
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    double ke[1] = {0.0};
+    if (i < n) {
+        const double m = mass[i];
+        double v[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const double kick = half_dt * frc[c * n + i];
+            v[c] = vel[c * n + i] + kick / m;
+            vel[c * n + i] = v[c];
+        }
+        const double half_m = 0.5 * m;
+        ke[0] = half_m * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    }
+    if (partials) block_sum<1, VV_THREADS>(ke, partials + blockIdx.x);
+}
+
+// Bussi eq. A7 on the device from the reference-order kinetic energy in scalars[3] and the caller's draws
+// (Thermostat.cpp:41-57). Writes the velocity scale to scalars[5] and the new kinetic energy to scalars[6].
+// Draws are consumed only when KE > 0, as the reference draws only then.
+__global__ void k_bussi(double* __restrict__ scalars, const double* __restrict__ draws, int* __restrict__ draw_index,
+                        double dof, double kbar, double c) {
+    const double ke = scalars[3];
+    if (!(ke > 0.0)) {
+        scalars[5] = 1.0;
+        scalars[6] = ke;
+        return;
+    }
+    const int k = *draw_index;
+    *draw_index = k + 1;
+    const double r1 = draws[2 * k], sum_sq = draws[2 * k + 1];
+    const double relax = (1.0 - c) * (kbar * (sum_sq + r1 * r1) / dof - ke);
+    const double noise = 2.0 * r1 * sqrt(kbar / dof * ke * (1.0 - c) * c);
+    const double ke_new = ke + relax + noise;
+    scalars[5] = sqrt(ke_new / ke);
+    scalars[6] = ke_new;
+}
+
+__global__ void k_scale_dev(double* __restrict__ vel, size_t count, const double* __restrict__ scale) {
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < count) vel[i] = vel[i] * *scale;
+}
+
+// Per-step record: potential, kinetic, virial, inside pairs -> energies[4 * step].
+__global__ void k_record(const double* __restrict__ scalars, int ke_slot, double* __restrict__ energies,
+                         int* __restrict__ step) {
+    const int k = *step;
+    energies[4 * k + 0] = scalars[0];
+    energies[4 * k + 1] = scalars[ke_slot];
+    energies[4 * k + 2] = scalars[1];
+    energies[4 * k + 3] = scalars[2];
+    *step = k + 1;
+}
+
+void kick_drift_enqueue(md_state* s, double dt, md_list* check_list) {
+    if (s->n == 0) return;
+    k_kick_drift<<<blocks_for(s->n, VV_THREADS), VV_THREADS, 0, s->ctx->stream>>>(
+        s->pos, s->vel, s->frc, s->mass, s->n, 0.5 * dt, dt, s->box, s->inv_box,
+        check_list ? check_list->ref : nullptr, check_list ? check_list->max_disp2 : 0.0,
+        check_list ? check_list->flags + FLAG_STALE : nullptr);
+    MD_HIP(hipGetLastError());
+}
+
+void kick_enqueue(md_state* s, double dt, double* ke_dev) {
+    if (s->n == 0) return;
+    const unsigned blocks = blocks_for(s->n, VV_THREADS);
+    if (ke_dev) state_reserve_partials(s, blocks);
+    k_kick<<<blocks, VV_THREADS, 0, s->ctx->stream>>>(s->vel, s->frc, s->mass, s->n, 0.5 * dt,
+                                                      ke_dev ? s->partials : nullptr);
+    MD_HIP(hipGetLastError());
+    if (ke_dev) reduce_partials_enqueue(s->ctx->stream, s->partials, static_cast<int>(blocks), 1, ke_dev);
+}
+
+// The list update of the eager step: staleness test, read-back, host-driven rebuild (NeighbourList.cpp:11-36).
+static bool eager_update(md_list* l, md_state* s) {
+    bool stale = !l->built || l->n != s->n;
+    if (!stale) {
+        hipStream_t st = l->ctx->stream;
+        MD_HIP(hipMemsetAsync(l->flags + FLAG_STALE, 0, sizeof(int), st));
+        displacement_check(l, s);
+        MD_HIP(hipMemcpyAsync(l->ctx->host_flags, l->flags + FLAG_STALE, sizeof(int), hipMemcpyDeviceToHost, st));
+        MD_HIP(hipStreamSynchronize(st));
+        stale = l->ctx->host_flags[0] != 0;
+    }
+    if (stale) list_rebuild(l, s);
+    return stale;
+}
+
+struct RunBuffers {
+    double* energies = nullptr;
+    double* draws = nullptr;
+    int* counters = nullptr;  // [0] step, [1] draw index
+    ~RunBuffers() {
+        dev_free(energies);
+        dev_free(draws);
+        dev_free(counters);
+    }
+};
+
+}  // namespace mdgpu
+
+using namespace mdgpu;
+
+
+extern "C" {
+
+int md_kick_drift(md_state* s, double dt) {
+    MD_API_BEGIN
+    if (!s) fail(MD_ERR_ARG, "state is NULL");
+    kick_drift_enqueue(s, dt, nullptr);
+    MD_API_END
+}
+
+int md_kick(md_state* s, double dt) {
+    MD_API_BEGIN
+    if (!s) fail(MD_ERR_ARG, "state is NULL");
+    kick_enqueue(s, dt, nullptr);
+    MD_API_END
+}
+
+int md_step(md_state* s, md_list* l, const md_lj* lj, int precision, double dt, md_force_result* out) {
+    MD_API_BEGIN
+    if (!s || !l || !lj) fail(MD_ERR_ARG, "state/list/lj is NULL");
+    if (precision != MD_FP64 && precision != MD_MIXED) fail(MD_ERR_ARG, "unknown precision");
+    kick_drift_enqueue(s, dt, nullptr);
+    eager_update(l, s);
+    check_cutoff(s, *lj);
+    forces_enqueue(s, l, *lj, precision, s->scalars);
+    kick_enqueue(s, dt, nullptr);
+    read_force_result(s, out);
+    MD_API_END
+}
+
+int md_step_allpairs(md_state* s, const md_lj* lj, double dt, md_force_result* out) {
+    MD_API_BEGIN
+    if (!s || !lj) fail(MD_ERR_ARG, "state/lj is NULL");
+    kick_drift_enqueue(s, dt, nullptr);
+    check_cutoff(s, *lj);
+    forces_allpairs_enqueue(s, *lj, s->scalars);
+    kick_enqueue(s, dt, nullptr);
+    read_force_result(s, out);
+    MD_API_END
+}
+
+int md_run(md_state* s, md_list* l, const md_run_config* cfg, double* energies) {
+    MD_API_BEGIN
+    if (!s || !l || !cfg) fail(MD_ERR_ARG, "state/list/config is NULL");
+    if (cfg->steps < 0) fail(MD_ERR_ARG, "steps must be >= 0");
+    if (cfg->precision != MD_FP64 && cfg->precision != MD_MIXED) fail(MD_ERR_ARG, "unknown precision");
+    if (cfg->thermostat && !cfg->draws) fail(MD_ERR_ARG, "Bussi thermostat needs draws");
+    check_cutoff(s, cfg->lj);
+    if (!l->built || l->n != s->n) list_rebuild(l, s);
+    if (cfg->steps == 0 || s->n == 0) return MD_OK;
+    hipStream_t st = s->ctx->stream;
+    const int steps = cfg->steps;
+    RunBuffers rb;
+    rb.energies = dev_alloc<double>(4 * static_cast<size_t>(steps));
+    rb.counters = dev_alloc<int>(4);
+    MD_HIP(hipMemsetAsync(rb.counters, 0, 4 * sizeof(int), st));
+    if (cfg->thermostat) {
+        rb.draws = dev_alloc<double>(2 * static_cast<size_t>(steps));
+        MD_HIP(hipMemcpyAsync(rb.draws, cfg->draws, 2 * sizeof(double) * steps, hipMemcpyHostToDevice, st));
+    }
+    // clear the overflow word and the device rebuild counter before the run
+    MD_HIP(hipMemsetAsync(l->flags + FLAG_OVERFLOW, 0, sizeof(int), st));
+    MD_HIP(hipMemsetAsync(l->flags + FLAG_REBUILDS, 0, sizeof(int), st));
+    MD_HIP(hipMemsetAsync(l->flags + FLAG_STALE, 0, sizeof(int), st));
+    const double dof = 3.0 * static_cast<double>(s->n) - 3.0;
+    const double kbar = 0.5 * dof * cfg->temperature;
+    const double c = cfg->thermostat ? std::exp(-cfg->dt / cfg->tau) : 0.0;
+    const size_t nvel = 3 * s->n;
+    // reserve the partials once, before any capture
+    state_reserve_partials(s, static_cast<size_t>(blocks_for(s->n, 256)) * 3 + 8);
+
+    // optional force-kernel timing: one event pair per step, eager launches only
+    const bool timed = cfg->force_kernel_ms != nullptr && !cfg->use_graph;
+    std::vector<hipEvent_t> events;
+    struct EventGuard {
+        std::vector<hipEvent_t>& ev;
+        ~EventGuard() {
+            for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+        }
+    } guard{events};
+    if (timed) {
+        events.resize(2 * static_cast<size_t>(steps), nullptr);
+        for (auto& e : events) MD_HIP(hipEventCreate(&e));
+    }
+    int step_index = 0;
+
+    auto enqueue_step = [&]() {
+        kick_drift_enqueue(s, cfg->dt, l);
+        list_enqueue_conditional_rebuild(l, s);
+        hipEvent_t e0 = timed ? events[2 * step_index] : nullptr;
+        hipEvent_t e1 = timed ? events[2 * step_index + 1] : nullptr;
+        ++step_index;
+        forces_enqueue(s, l, cfg->lj, cfg->precision, s->scalars, l->flags + FLAG_STALE, e0, e1);
+        if (cfg->thermostat) {
+            kick_enqueue(s, cfg->dt, nullptr);
+            kinetic_exact_enqueue(s, s->scalars + 3);
+            k_bussi<<<1, 1, 0, st>>>(s->scalars, rb.draws, rb.counters + 1, dof, kbar, c);
+            k_scale_dev<<<blocks_for(nvel, 256), 256, 0, st>>>(s->vel, nvel, s->scalars + 5);
+            k_record<<<1, 1, 0, st>>>(s->scalars, 6, rb.energies, rb.counters);
+        } else {
+            kick_enqueue(s, cfg->dt, s->scalars + 3);
+            k_record<<<1, 1, 0, st>>>(s->scalars, 3, rb.energies, rb.counters);
+        }
+        MD_HIP(hipGetLastError());
+    };
+
+    if (cfg->use_graph) {
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        MD_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        try {
+            enqueue_step();
+        } catch (...) {
+            (void)hipStreamEndCapture(st, &graph);
+            if (graph) (void)hipGraphDestroy(graph);
+            throw;
+        }
+        MD_HIP(hipStreamEndCapture(st, &graph));
+        hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            (void)hipGraphDestroy(graph);
+            hip_check(e, "hipGraphInstantiate");
+        }
+        for (int k = 0; k < steps; ++k) {
+            e = hipGraphLaunch(exec, st);
+            if (e != hipSuccess) break;
+        }
+        (void)hipStreamSynchronize(st);
+        (void)hipGraphExecDestroy(exec);
+        (void)hipGraphDestroy(graph);
+        hip_check(e, "hipGraphLaunch");
+    } else {
+        for (int k = 0; k < steps; ++k) enqueue_step();
+    }
+    int* hf = s->ctx->host_flags;
+    MD_HIP(hipMemcpyAsync(hf, l->flags, sizeof(int) * (FLAG_COUNT + 2), hipMemcpyDeviceToHost, st));
+    if (energies)
+        MD_HIP(hipMemcpyAsync(energies, rb.energies, 4 * sizeof(double) * steps, hipMemcpyDeviceToHost, st));
+    MD_HIP(hipStreamSynchronize(st));
+    if (timed) {
+        double total_ms = 0.0;
+        for (int k = 0; k < steps; ++k) {
+            float ms = 0.0f;
+            MD_HIP(hipEventElapsedTime(&ms, events[2 * k], events[2 * k + 1]));
+            total_ms += ms;
+        }
+        *cfg->force_kernel_ms = total_ms;
+    }
+    l->rebuilds += hf[FLAG_REBUILDS];
+    if (hf[FLAG_REBUILDS] > 0) {
+        // the words after the flags hold the half-pair total of the last build
+        unsigned long long total;
+        std::memcpy(&total, hf + FLAG_COUNT, sizeof(total));
+        l->half_pairs = static_cast<size_t>(total);
+        l->max_count = hf[FLAG_MAXCOUNT];
+    }
+    if (hf[FLAG_OVERFLOW]) {
+        l->built = false;  // rows were truncated: the next call rebuilds with more padding
+        l->padding = (hf[FLAG_MAXCOUNT] + hf[FLAG_MAXCOUNT] / 4 + 16 + 7) / 8 * 8;
+        fail(MD_ERR_OVERFLOW, "a neighbour-list row outgrew its padding during md_run; state is invalid");
+    }
+    MD_API_END
+}
+
+}  // extern "C"

@@ -1,0 +1,175 @@
+// Internal declarations of the mdgpu library (gfx950). Device layout in HBM, per particle i of n:
+//   pos   double4[n]      x, y, z, pad — the reference's packed layout (System.hpp:22), one 32-byte load each
+//   vel   double[3n]      vx | vy | vz component-major (System.hpp:131), coalesced per component
+//   frc   double[3n]      fx | fy | fz
+//   mass  double[n], id uint64[n]
+// Neighbour list (one per md_list): a padded FULL list stored column-major, rows[k * n + i] = k-th neighbour of
+// i, so lane i of a wavefront reads consecutive words for the same k. Each row holds first the j > i
+// neighbours in the reference's 27-cell visit order (exactly the reference's half-list row), then the j < i
+// neighbours in increasing index; count[i] and half[i] give both lengths.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <exception>
+#include <string>
+
+#include "mdgpu.h"
+
+namespace mdgpu {
+
+struct Error {
+    int code;
+    std::string msg;
+};
+
+[[noreturn]] void fail(int code, const std::string& msg);
+void hip_check(hipError_t e, const char* what);
+#define MD_HIP(expr) ::mdgpu::hip_check((expr), #expr)
+
+template <class T>
+T* dev_alloc(size_t count) {
+    void* p = nullptr;
+    if (count == 0) count = 1;
+    hip_check(hipMalloc(&p, count * sizeof(T)), "hipMalloc");
+    return static_cast<T*>(p);
+}
+
+template <class T>
+void dev_free(T*& p) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+// Grows a device buffer to at least `count` elements (contents are not preserved).
+template <class T>
+void dev_reserve(T*& p, size_t& cap, size_t count) {
+    if (count <= cap && p) return;
+    dev_free(p);
+    p = dev_alloc<T>(count);
+    cap = count;
+}
+
+inline unsigned blocks_for(size_t n, unsigned threads) {
+    return static_cast<unsigned>((n + threads - 1) / threads);
+}
+
+// Device-side flags of a list, shared by the eager and the captured step.
+enum Flag : int {
+    FLAG_STALE = 0,     // some particle moved more than skin/2 since the last build
+    FLAG_OVERFLOW = 1,  // a row needed more than `padding` entries
+    FLAG_MAXCOUNT = 2,  // longest row seen by the last build
+    FLAG_REBUILDS = 3,  // rebuilds performed on the device
+    FLAG_FORCE = 4,     // host-requested rebuild (first build, n changed)
+    FLAG_COUNT = 8
+};
+
+}  // namespace mdgpu
+
+// Body wrapper of every extern "C" entry point: internal errors become status codes + md_last_error().
+#define MD_API_BEGIN try {
+#define MD_API_END                                                                                             \
+    return MD_OK;                                                                                              \
+    }                                                                                                          \
+    catch (const ::mdgpu::Error& e) {                                                                          \
+        ::mdgpu::set_error(e.msg);                                                                             \
+        return e.code;                                                                                         \
+    }                                                                                                          \
+    catch (const std::exception& e) {                                                                          \
+        ::mdgpu::set_error(e.what());                                                                          \
+        return MD_ERR_HIP;                                                                                     \
+    }
+
+struct md_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    // pinned host words for small read-backs (flags, scalars)
+    double* host_scalars = nullptr;  // 64 doubles
+    int* host_flags = nullptr;       // 16 ints
+};
+
+struct md_state {
+    md_ctx* ctx = nullptr;
+    size_t n = 0;
+    double box = 0.0, inv_box = 0.0;
+    double4* pos = nullptr;
+    double* vel = nullptr;
+    double* frc = nullptr;
+    double* mass = nullptr;
+    uint64_t* id = nullptr;
+    // second copies used as the destination of a permutation
+    double4* pos_alt = nullptr;
+    double* vel_alt = nullptr;
+    double* frc_alt = nullptr;
+    double* mass_alt = nullptr;
+    uint64_t* id_alt = nullptr;
+    // reductions: per-block partials and the device result words
+    double* partials = nullptr;
+    size_t partials_cap = 0;
+    double* scalars = nullptr;  // [0..2] U, W, inside; [3] KE; [4] KE fast; [5..7] scratch
+};
+
+struct md_list {
+    md_ctx* ctx = nullptr;
+    double cutoff = 0.0, skin = 0.0, radius = 0.0, radius2 = 0.0, max_disp2 = 0.0;
+    int side = 0;
+    size_t ncells = 0, ncells_cap = 0;
+    size_t n = 0, n_cap = 0;
+    int padding = 0;
+    size_t rows_cap = 0;
+    bool built = false;
+    int rebuilds = 0;
+    size_t half_pairs = 0;
+    int max_count = 0;
+    // cell grid
+    uint32_t* cell_count = nullptr;  // ncells
+    uint32_t* cell_start = nullptr;  // ncells + 1
+    uint32_t* scan_tmp = nullptr;    // block sums of the scan
+    size_t scan_cap = 0;
+    int* cell_of = nullptr;          // n (cell of particle i in the current order)
+    int* cell_of_alt = nullptr;
+    uint32_t* order_raw = nullptr;   // n, unstable scatter result
+    uint32_t* order = nullptr;       // n, stable sorted order
+    double4* ref = nullptr;          // n, positions at the last build
+    // padded full list
+    int32_t* rows = nullptr;         // padding * n, column-major
+    int32_t* count = nullptr;        // n
+    int32_t* half = nullptr;         // n
+    int* flags = nullptr;            // FLAG_COUNT ints on the device
+};
+
+namespace mdgpu {
+
+// cells.hip
+void cells_build(md_list* l, md_state* s, double min_cell, int side);   // count, scan, scatter, stable order
+void list_rebuild(md_list* l, md_state* s);                            // full host-driven rebuild + read-back
+void list_enqueue_conditional_rebuild(md_list* l, md_state* s);       // device-flag-gated rebuild (md_run)
+void list_prepare(md_list* l, md_state* s);                            // grid/buffers for the state's size
+void displacement_check(md_list* l, md_state* s);                      // sets FLAG_STALE
+
+// api.hip
+void state_reserve_partials(md_state* s, size_t count);
+void kinetic_exact_enqueue(md_state* s, double* out_dev);              // reference-order KE into *out_dev
+
+// forces.hip
+void forces_enqueue(md_state* s, md_list* l, const md_lj& lj, int precision, double* out_dev);
+// same, and the force kernel clears *clear_flag (the list's stale flag inside md_run)
+// (events, when set, are recorded on the stream right before and after the force kernel)
+void forces_enqueue(md_state* s, md_list* l, const md_lj& lj, int precision, double* out_dev, int* clear_flag,
+                    hipEvent_t ev_start, hipEvent_t ev_stop);
+void forces_allpairs_enqueue(md_state* s, const md_lj& lj, double* out_dev);
+void check_cutoff(const md_state* s, const md_lj& lj);
+void validate_list(const md_state* s, const md_list* l);
+void read_force_result(md_state* s, md_force_result* out);
+void set_error(const std::string& m);
+
+// integrate.hip
+void kick_drift_enqueue(md_state* s, double dt, md_list* check_list);
+void kick_enqueue(md_state* s, double dt, double* ke_dev);
+
+// Final fixed-order reduction of `blocks` rows of `width` partials into out_dev[0..width).
+void reduce_partials_enqueue(hipStream_t st, const double* partials, int blocks, int width, double* out_dev);
+
+}  // namespace mdgpu

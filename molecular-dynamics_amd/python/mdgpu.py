@@ -1,0 +1,416 @@
+"""ctypes binding of the mdgpu C ABI (include/mdgpu.h) — the Python-side view of the same entry points the C++
+facade calls. Loads molecular-dynamics_amd/libmdgpu.so and raises if it is missing: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ROOT = os.path.dirname(PKG_DIR)
+LIB_PATH = os.path.join(PKG_DIR, "libmdgpu.so")
+HEADER = os.path.join(ROOT, "include", "mdgpu.h")
+
+MD_FP64, MD_MIXED = 0, 1
+FIELD_POS, FIELD_VEL, FIELD_FORCE, FIELD_MASS, FIELD_ID = range(5)
+STATUS = {0: "MD_OK", 1: "MD_ERR_ARG", 2: "MD_ERR_HIP", 3: "MD_ERR_RUNTIME", 4: "MD_ERR_OVERFLOW", 5: "MD_ERR_STATE"}
+
+
+class MdError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{STATUS.get(code, code)}: {msg}")
+        self.code = code
+        self.msg = msg
+
+
+class LJ(C.Structure):
+    _fields_ = [("epsilon", C.c_double), ("sigma", C.c_double), ("cutoff", C.c_double)]
+
+
+class ForceResult(C.Structure):
+    _fields_ = [("potential", C.c_double), ("virial", C.c_double), ("inside_pairs", C.c_double)]
+
+
+class ListInfo(C.Structure):
+    _fields_ = [("half_pairs", C.c_size_t), ("rebuilds", C.c_int), ("cells_per_side", C.c_int),
+                ("max_neighbours", C.c_int), ("padding", C.c_int)]
+
+
+class RunConfig(C.Structure):
+    _fields_ = [("dt", C.c_double), ("precision", C.c_int), ("steps", C.c_int), ("lj", LJ),
+                ("thermostat", C.c_int), ("temperature", C.c_double), ("tau", C.c_double),
+                ("draws", C.POINTER(C.c_double)), ("use_graph", C.c_int),
+                ("force_kernel_ms", C.POINTER(C.c_double))]
+
+
+_P = C.c_void_p
+_PP = C.POINTER(C.c_void_p)
+_DP = C.POINTER(C.c_double)
+_U64P = C.POINTER(C.c_uint64)
+_IP = C.POINTER(C.c_int)
+_I32P = C.POINTER(C.c_int32)
+
+_SIGS = {
+    "md_last_error": (C.c_char_p, []),
+    "md_device_count": (C.c_int, [_IP]),
+    "md_ctx_create": (C.c_int, [C.c_int, _PP]),
+    "md_ctx_destroy": (C.c_int, [_P]),
+    "md_ctx_synchronize": (C.c_int, [_P]),
+    "md_state_create": (C.c_int, [_P, C.c_size_t, C.c_double, _PP]),
+    "md_state_destroy": (C.c_int, [_P]),
+    "md_state_copy": (C.c_int, [_P, _P]),
+    "md_state_info": (C.c_int, [_P, C.POINTER(C.c_size_t), _DP]),
+    "md_state_upload": (C.c_int, [_P, C.c_int, _P]),
+    "md_state_download": (C.c_int, [_P, C.c_int, _P]),
+    "md_state_reorder": (C.c_int, [_P, _U64P]),
+    "md_kinetic_energy": (C.c_int, [_P, _DP]),
+    "md_scale_velocities": (C.c_int, [_P, C.c_double]),
+    "md_state_hash": (C.c_int, [_P, _U64P]),
+    "md_cells_build": (C.c_int, [_P, C.c_double, _IP, _IP, _U64P, _U64P]),
+    "md_list_create": (C.c_int, [_P, C.c_double, C.c_double, C.c_int, _PP]),
+    "md_list_destroy": (C.c_int, [_P]),
+    "md_list_build": (C.c_int, [_P, _P]),
+    "md_list_update": (C.c_int, [_P, _P, _IP]),
+    "md_list_get_info": (C.c_int, [_P, C.POINTER(ListInfo)]),
+    "md_list_half": (C.c_int, [_P, _U64P, _U64P]),
+    "md_list_full": (C.c_int, [_P, _I32P, _I32P, _I32P]),
+    "md_forces": (C.c_int, [_P, _P, C.POINTER(LJ), C.c_int, C.POINTER(ForceResult)]),
+    "md_forces_allpairs": (C.c_int, [_P, C.POINTER(LJ), C.POINTER(ForceResult)]),
+    "md_kick_drift": (C.c_int, [_P, C.c_double]),
+    "md_kick": (C.c_int, [_P, C.c_double]),
+    "md_step": (C.c_int, [_P, _P, C.POINTER(LJ), C.c_int, C.c_double, C.POINTER(ForceResult)]),
+    "md_step_allpairs": (C.c_int, [_P, C.POINTER(LJ), C.c_double, C.POINTER(ForceResult)]),
+    "md_bussi_apply": (C.c_int, [_P, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double]),
+    "md_rdf_sample": (C.c_int, [_P, _P, C.c_double, C.c_int, _U64P]),
+    "md_run": (C.c_int, [_P, _P, C.POINTER(RunConfig), _DP]),
+    "md_bussi_rng_create": (C.c_int, [C.c_uint64, _PP]),
+    "md_bussi_rng_destroy": (C.c_int, [_P]),
+    "md_bussi_rng_draws": (C.c_int, [_P, C.c_size_t, C.c_int, _DP]),
+    "md_fcc_lattice": (C.c_int, [C.c_int, C.c_double, _DP]),
+    "md_maxwell_boltzmann": (C.c_int, [C.c_size_t, _DP, C.c_double, C.c_uint32, _DP]),
+}
+
+
+def fcc_lattice(cells: int, lattice_constant: float) -> np.ndarray:
+    """createFCCLattice: (4 cells^3, 3) positions."""
+    out = np.empty((4 * cells ** 3, 3))
+    check(lib().md_fcc_lattice(cells, lattice_constant, out.ctypes.data_as(_DP)))
+    return out
+
+
+def maxwell_boltzmann(n: int, temperature: float, seed: int, mass=None) -> np.ndarray:
+    """initializeVelocities on host arrays: (3, n) component-major velocities."""
+    out = np.empty((3, n))
+    m = None if mass is None else np.ascontiguousarray(mass, dtype=np.float64)
+    check(lib().md_maxwell_boltzmann(n, None if m is None else m.ctypes.data_as(_DP), temperature, seed,
+                                     out.ctypes.data_as(_DP)))
+    return out
+
+
+_libm = C.CDLL("libm.so.6")
+_libm.cbrt.restype = C.c_double
+_libm.cbrt.argtypes = [C.c_double]
+
+
+def lattice_constant(density: float) -> float:
+    """a = std::cbrt(4 / rho) (main.cpp:165), through libm's cbrt (numpy's may differ in the last bit)."""
+    return _libm.cbrt(4.0 / density)
+
+
+def fcc_system(ctx, cells: int, density: float, temperature: float, seed: int) -> "State":
+    """The reference's setup (main.cpp:164-170): FCC at a = cbrt(4/rho), box cells*a, MB velocities."""
+    a = lattice_constant(density)
+    pos = fcc_lattice(cells, a)
+    vel = maxwell_boltzmann(pos.shape[0], temperature, seed)
+    return State.from_arrays(ctx, cells * a, pos, vel=vel)
+
+_lib = None
+
+
+def header_symbols() -> list[str]:
+    """Every function name declared in include/mdgpu.h."""
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(md_[a-z0-9_]+)\s*\(", text)))
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FileNotFoundError(f"{LIB_PATH} is not built: run __graft_entry__.build() or "
+                                    "`make -C molecular-dynamics_amd`")
+        h = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(h, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = h
+    return _lib
+
+
+def check(rc: int):
+    if rc != 0:
+        raise MdError(rc, lib().md_last_error().decode())
+
+
+def _ptr(a: np.ndarray):
+    return C.c_void_p(a.ctypes.data)
+
+
+def _u64p(a):
+    return None if a is None else a.ctypes.data_as(_U64P)
+
+
+class Context:
+    def __init__(self, device: int = 0):
+        self.h = C.c_void_p()
+        check(lib().md_ctx_create(device, C.byref(self.h)))
+
+    def synchronize(self):
+        check(lib().md_ctx_synchronize(self.h))
+
+    def close(self):
+        if self.h:
+            lib().md_ctx_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class State:
+    """A device-resident System: positions packed x,y,z,pad; velocities and forces component-major."""
+
+    def __init__(self, ctx: Context, n: int, box: float):
+        self.ctx = ctx
+        self.n = n
+        self.box = box
+        self.h = C.c_void_p()
+        check(lib().md_state_create(ctx.h, n, box, C.byref(self.h)))
+
+    @classmethod
+    def from_arrays(cls, ctx, box, pos, vel=None, mass=None, force=None):
+        pos = np.asarray(pos, dtype=np.float64)
+        n = pos.shape[0]
+        s = cls(ctx, n, box)
+        p4 = np.zeros((n, 4))
+        p4[:, : pos.shape[1]] = pos[:, :3] if pos.shape[1] >= 3 else pos
+        if pos.shape[1] == 4:
+            p4 = np.ascontiguousarray(pos)
+        s.upload(FIELD_POS, p4)
+        if vel is not None:
+            s.upload(FIELD_VEL, np.ascontiguousarray(vel, dtype=np.float64).reshape(3, n))
+        if force is not None:
+            s.upload(FIELD_FORCE, np.ascontiguousarray(force, dtype=np.float64).reshape(3, n))
+        if mass is not None:
+            s.upload(FIELD_MASS, np.ascontiguousarray(mass, dtype=np.float64).reshape(n))
+        return s
+
+    def upload(self, field: int, arr: np.ndarray):
+        arr = np.ascontiguousarray(arr)
+        check(lib().md_state_upload(self.h, field, _ptr(arr)))
+
+    def download(self, field: int) -> np.ndarray:
+        n = self.n
+        shape = {FIELD_POS: (n, 4), FIELD_VEL: (3, n), FIELD_FORCE: (3, n), FIELD_MASS: (n,), FIELD_ID: (n,)}[field]
+        dtype = np.uint64 if field == FIELD_ID else np.float64
+        out = np.empty(shape, dtype=dtype)
+        check(lib().md_state_download(self.h, field, _ptr(out)))
+        return out
+
+    def state(self) -> dict:
+        return {"pos": self.download(FIELD_POS), "vel": self.download(FIELD_VEL),
+                "force": self.download(FIELD_FORCE), "mass": self.download(FIELD_MASS),
+                "ids": self.download(FIELD_ID)}
+
+    def copy_from(self, other: "State"):
+        check(lib().md_state_copy(self.h, other.h))
+
+    def reorder(self, order):
+        order = np.ascontiguousarray(order, dtype=np.uint64)
+        check(lib().md_state_reorder(self.h, _u64p(order)))
+
+    def kinetic_energy(self) -> float:
+        v = C.c_double()
+        check(lib().md_kinetic_energy(self.h, C.byref(v)))
+        return v.value
+
+    def temperature(self) -> float:
+        dof = 3.0 * self.n - 3.0
+        return 2.0 * self.kinetic_energy() / dof
+
+    def scale_velocities(self, s: float):
+        check(lib().md_scale_velocities(self.h, s))
+
+    def rescale(self, target: float):
+        """rescaleVelocities (Thermostat.cpp:17-30): the scale is formed on the host from the exact KE."""
+        t = self.temperature()
+        if t <= 0.0:
+            return
+        self.scale_velocities(float(np.sqrt(target / t)))
+
+    def state_hash(self) -> int:
+        v = C.c_uint64()
+        check(lib().md_state_hash(self.h, C.byref(v)))
+        return int(v.value)
+
+    def cells(self, min_cell: float):
+        side = C.c_int()
+        check(lib().md_cells_build(self.h, min_cell, C.byref(side), None, None, None))
+        s = side.value
+        cell_of = np.empty(self.n, dtype=np.int32)
+        start = np.empty(s ** 3 + 1, dtype=np.uint64)
+        order = np.empty(self.n, dtype=np.uint64)
+        check(lib().md_cells_build(self.h, min_cell, C.byref(side), cell_of.ctypes.data_as(_IP), _u64p(start),
+                                   _u64p(order)))
+        return s, cell_of, start, order
+
+    def kick_drift(self, dt):
+        check(lib().md_kick_drift(self.h, dt))
+
+    def kick(self, dt):
+        check(lib().md_kick(self.h, dt))
+
+    def forces_allpairs(self, lj: LJ) -> ForceResult:
+        r = ForceResult()
+        check(lib().md_forces_allpairs(self.h, C.byref(lj), C.byref(r)))
+        return r
+
+    def step_allpairs(self, lj: LJ, dt: float) -> ForceResult:
+        r = ForceResult()
+        check(lib().md_step_allpairs(self.h, C.byref(lj), dt, C.byref(r)))
+        return r
+
+    def bussi_apply(self, temperature, tau, dt, r1, sum_sq):
+        check(lib().md_bussi_apply(self.h, temperature, tau, dt, r1, sum_sq))
+
+    def close(self):
+        if self.h:
+            lib().md_state_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class NeighbourList:
+    def __init__(self, ctx: Context, cutoff=2.5, skin=0.2, max_neighbours=0):
+        self.ctx = ctx
+        self.h = C.c_void_p()
+        check(lib().md_list_create(ctx.h, cutoff, skin, max_neighbours, C.byref(self.h)))
+        self.n = 0
+
+    def build(self, state: State):
+        check(lib().md_list_build(self.h, state.h))
+        self.n = state.n
+
+    def update(self, state: State) -> bool:
+        r = C.c_int()
+        check(lib().md_list_update(self.h, state.h, C.byref(r)))
+        self.n = state.n
+        return bool(r.value)
+
+    def info(self) -> ListInfo:
+        i = ListInfo()
+        check(lib().md_list_get_info(self.h, C.byref(i)))
+        return i
+
+    def half(self):
+        info = self.info()
+        offs = np.empty(self.n + 1, dtype=np.uint64)
+        neigh = np.empty(max(info.half_pairs, 1), dtype=np.uint64)
+        check(lib().md_list_half(self.h, _u64p(offs), _u64p(neigh)))
+        return offs, neigh[: info.half_pairs]
+
+    def full(self):
+        info = self.info()
+        counts = np.empty(self.n, dtype=np.int32)
+        half = np.empty(self.n, dtype=np.int32)
+        rows = np.empty((self.n, info.padding), dtype=np.int32)
+        check(lib().md_list_full(self.h, counts.ctypes.data_as(_I32P), half.ctypes.data_as(_I32P),
+                                 rows.ctypes.data_as(_I32P)))
+        return counts, half, rows
+
+    def forces(self, state: State, lj: LJ, precision=MD_FP64) -> ForceResult:
+        r = ForceResult()
+        check(lib().md_forces(state.h, self.h, C.byref(lj), precision, C.byref(r)))
+        return r
+
+    def step(self, state: State, lj: LJ, dt: float, precision=MD_FP64) -> ForceResult:
+        r = ForceResult()
+        check(lib().md_step(state.h, self.h, C.byref(lj), precision, dt, C.byref(r)))
+        return r
+
+    def rdf_sample(self, state: State, rmax: float, bins: int, counts: np.ndarray):
+        assert counts.dtype == np.uint64 and counts.shape == (bins,)
+        check(lib().md_rdf_sample(state.h, self.h, rmax, bins, _u64p(counts)))
+
+    def run(self, state: State, lj: LJ, dt: float, steps: int, precision=MD_MIXED, draws=None,
+            temperature=1.0, tau=0.1, use_graph=True, record=True, timing=None):
+        """md_run. With `timing` (a list), eager launches are used and the summed force-kernel time in ms is
+        appended to it."""
+        cfg = RunConfig()
+        cfg.dt = dt
+        cfg.precision = precision
+        cfg.steps = steps
+        cfg.lj = lj
+        cfg.thermostat = 0 if draws is None else 1
+        cfg.temperature = temperature
+        cfg.tau = tau
+        if draws is not None:
+            draws = np.ascontiguousarray(draws, dtype=np.float64).reshape(steps, 2)
+            cfg.draws = draws.ctypes.data_as(_DP)
+        cfg.use_graph = 1 if (use_graph and timing is None) else 0
+        ms = C.c_double(0.0)
+        if timing is not None:
+            cfg.force_kernel_ms = C.pointer(ms)
+        energies = np.empty((steps, 4)) if record else None
+        check(lib().md_run(state.h, self.h, C.byref(cfg), None if energies is None else energies.ctypes.data_as(_DP)))
+        if timing is not None:
+            timing.append(ms.value)
+        return energies
+
+    def close(self):
+        if self.h:
+            lib().md_list_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class BussiRng:
+    """The thermostat's draw stream: mt19937_64(seed), then per apply one N(0,1) and one Gamma((dof-1)/2, 2)."""
+
+    def __init__(self, seed: int):
+        self.h = C.c_void_p()
+        check(lib().md_bussi_rng_create(seed, C.byref(self.h)))
+
+    def draws(self, n_particles: int, count: int) -> np.ndarray:
+        out = np.empty((count, 2))
+        check(lib().md_bussi_rng_draws(self.h, n_particles, count, out.ctypes.data_as(_DP)))
+        return out
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().md_bussi_rng_destroy(self.h)
+            self.h = None
+
+
+def device_count() -> int:
+    c = C.c_int()
+    check(lib().md_device_count(C.byref(c)))
+    return c.value
