@@ -186,6 +186,7 @@ int md_ctx_create(int device, md_ctx** out) {
 int md_ctx_destroy(md_ctx* c) {
     MD_API_BEGIN
     if (!c) return MD_OK;
+    if (c->live > 0) fail(MD_ERR_STATE, "context still owns states or lists; destroy them first");
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -210,6 +211,7 @@ int md_state_create(md_ctx* ctx, size_t n, double box_length, md_state** out) {
     auto* s = new md_state;
     try {
         s->ctx = ctx;
+        ++ctx->live;
         s->n = n;
         s->box = box_length;
         s->inv_box = 1.0 / box_length;
@@ -246,6 +248,7 @@ int md_state_destroy(md_state* s) {
     if (s->ctx) {
         (void)hipSetDevice(s->ctx->device);
         (void)hipStreamSynchronize(s->ctx->stream);
+        --s->ctx->live;
     }
     dev_free(s->pos);
     dev_free(s->vel);

@@ -535,6 +535,7 @@ int md_list_create(md_ctx* ctx, double cutoff, double skin, int max_neighbours, 
     if (!ctx || !out) fail(MD_ERR_ARG, "ctx/out is NULL");
     auto* l = new md_list;
     l->ctx = ctx;
+    ++ctx->live;
     l->cutoff = cutoff;
     l->skin = skin;
     l->radius = cutoff + skin;
@@ -547,7 +548,10 @@ int md_list_create(md_ctx* ctx, double cutoff, double skin, int max_neighbours, 
 
 int md_list_destroy(md_list* l) {
     if (!l) return MD_OK;
-    if (l->ctx) (void)hipStreamSynchronize(l->ctx->stream);
+    if (l->ctx) {
+        (void)hipStreamSynchronize(l->ctx->stream);
+        --l->ctx->live;
+    }
     release_list_buffers(l);
     delete l;
     return MD_OK;

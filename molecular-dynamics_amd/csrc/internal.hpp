@@ -88,6 +88,7 @@ struct md_ctx {
     // pinned host words for small read-backs (flags, scalars)
     double* host_scalars = nullptr;  // 64 doubles
     int* host_flags = nullptr;       // 16 ints
+    int live = 0;                    // states and lists created from this context and not yet destroyed
 };
 
 struct md_state {

@@ -165,7 +165,20 @@ def _u64p(a):
     return None if a is None else a.ctypes.data_as(_U64P)
 
 
+def packed_positions(pos) -> np.ndarray:
+    """(n, 3) or (n, 4) positions -> contiguous (n, 4) x, y, z, pad (System.hpp:22)."""
+    pos = np.asarray(pos, dtype=np.float64)
+    if pos.ndim == 2 and pos.shape[1] == 4:
+        return np.ascontiguousarray(pos)
+    pos = pos.reshape(-1, 3)
+    p4 = np.zeros((pos.shape[0], 4))
+    p4[:, :3] = pos
+    return p4
+
+
 class Context:
+    """Owns the HIP stream. States and lists keep a reference to it, so it outlives them."""
+
     def __init__(self, device: int = 0):
         self.h = C.c_void_p()
         check(lib().md_ctx_create(device, C.byref(self.h)))
@@ -174,8 +187,9 @@ class Context:
         check(lib().md_ctx_synchronize(self.h))
 
     def close(self):
+        """Destroys the context; fails (MD_ERR_STATE) while states or lists created from it are alive."""
         if self.h:
-            lib().md_ctx_destroy(self.h)
+            check(lib().md_ctx_destroy(self.h))
             self.h = C.c_void_p()
 
     def __del__(self):
@@ -197,13 +211,9 @@ class State:
 
     @classmethod
     def from_arrays(cls, ctx, box, pos, vel=None, mass=None, force=None):
-        pos = np.asarray(pos, dtype=np.float64)
-        n = pos.shape[0]
+        p4 = packed_positions(pos)
+        n = p4.shape[0]
         s = cls(ctx, n, box)
-        p4 = np.zeros((n, 4))
-        p4[:, : pos.shape[1]] = pos[:, :3] if pos.shape[1] >= 3 else pos
-        if pos.shape[1] == 4:
-            p4 = np.ascontiguousarray(pos)
         s.upload(FIELD_POS, p4)
         if vel is not None:
             s.upload(FIELD_VEL, np.ascontiguousarray(vel, dtype=np.float64).reshape(3, n))

@@ -28,6 +28,5 @@ def ctx():
 
     if mdgpu.device_count() < 1:
         pytest.fail("no GPU visible to the HIP runtime")
-    c = mdgpu.Context(0)
-    yield c
-    c.close()
+    # released by garbage collection after the last state / list that references it
+    return mdgpu.Context(0)

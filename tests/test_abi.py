@@ -42,6 +42,17 @@ def test_host_input_generation_matches_oracle():
         assert np.array_equal(vel, ref["vel"])
 
 
+def test_packed_positions_shapes():
+    import numpy as np
+
+    p3 = np.arange(12.0).reshape(4, 3)
+    p4 = mdgpu.packed_positions(p3)
+    assert p4.shape == (4, 4) and np.array_equal(p4[:, :3], p3) and not p4[:, 3].any()
+    q = np.arange(16.0).reshape(4, 4)
+    assert np.array_equal(mdgpu.packed_positions(q), q)
+    assert mdgpu.packed_positions(np.zeros((0, 3))).shape == (0, 4)
+
+
 def test_errors_without_touching_the_device():
     lib = mdgpu.lib()
     assert lib.md_state_create(None, 10, 1.0, ctypes.byref(ctypes.c_void_p())) == 1  # MD_ERR_ARG
