@@ -72,6 +72,7 @@ typedef struct md_list_info {
     int cells_per_side;    /* CellList::getCellsPerSide() of the list's grid (CellList.hpp:20-22) */
     int max_neighbours;    /* longest full-list row at the last build */
     int padding;           /* allocated row length of the padded full list */
+    size_t particles;      /* particle count of the last build (rows of the list) */
 } md_list_info;
 
 /* ---- host-side input generation (Initialisation.hpp:9-12) --------------------------------------------- */

@@ -589,6 +589,7 @@ int md_list_get_info(const md_list* l, md_list_info* info) {
     info->cells_per_side = l->side;
     info->max_neighbours = l->max_count;
     info->padding = l->padding;
+    info->particles = l->built ? l->n : 0;
     MD_API_END
 }
 

@@ -1,0 +1,661 @@
+// The reference's unit-test suite (tests/test_*.cpp, 37 tests), restated against the GPU-backed facade classes.
+// Each test names the reference test it restates; thread-count loops of the reference collapse to one run (the
+// GPU path has no thread count). Also: the reference-produced deterministic-mode state hash.
+#include <algorithm>
+#include <cstdio>
+#include <limits>
+#include <memory>
+#include <random>
+#include <set>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "CellList.hpp"
+#include "Forcefield.hpp"
+#include "Initialisation.hpp"
+#include "Integrator.hpp"
+#include "NeighbourList.hpp"
+#include "RadialDistribution.hpp"
+#include "System.hpp"
+#include "Thermostat.hpp"
+#include "Trajectory.hpp"
+#include "mdtest.hpp"
+
+namespace {
+
+// The reference tests' state point (tests/TestHelpers.hpp:18-27)
+constexpr double kDensity = 0.8442;
+constexpr double kTemperature = 0.728;
+constexpr double kCutOff = 2.5;
+constexpr double kSkin = 0.2;
+constexpr double kDt = 0.005;
+
+double latticeConstant() {
+    return std::cbrt(4.0 / kDensity);
+}
+
+System lattice(int cells, unsigned seed = 42) {
+    System s(createFCCLattice(cells, latticeConstant()), cells * latticeConstant());
+    initializeVelocities(s, kTemperature, seed);
+    return s;
+}
+
+// melt the lattice with a rescale every 10 steps (TestHelpers.hpp:39-58)
+System liquid(int cells, int steps = 300) {
+    System s = lattice(cells);
+    LennardJonesForceField ff(1.0, 1.0, kCutOff);
+    VelocityVerletIntegrator vv(kDt);
+    NeighbourList list(kCutOff, kSkin);
+    list.build(s);
+    ff.computeForces(s, list);
+    for (int k = 1; k <= steps; ++k) {
+        vv.step(s, ff, list);
+        if (k % 10 == 0) rescaleVelocities(s, kTemperature);
+    }
+    return s;
+}
+
+std::array<double, 3> momentum(const System& s) {
+    std::array<double, 3> p {0.0, 0.0, 0.0};
+    for (int k = 0; k < 3; ++k)
+        for (std::size_t i = 0; i < s.size(); ++i) p[k] += s.mass()[i] * s.velocity(k)[i];
+    return p;
+}
+
+double ljForce(double r) {
+    return 24.0 * (2.0 * std::pow(r, -13) - std::pow(r, -7));
+}
+
+double ljEnergy(double r, double rc) {
+    auto v = [](double x) { return 4.0 * (std::pow(x, -12) - std::pow(x, -6)); };
+    return v(r) - v(rc);
+}
+
+System twoAlongX(double x0, double x1, double box) {
+    Particle a, b;
+    a.position = {x0, 5.0, 5.0};
+    b.position = {x1, 5.0, 5.0};
+    return System({a, b}, box);
+}
+
+using Pairs = std::set<std::pair<std::size_t, std::size_t>>;
+
+Pairs listed(const NeighbourList& list, std::size_t n) {
+    Pairs p;
+    for (std::size_t i = 0; i < n; ++i)
+        for (std::size_t j : list.neighboursOf(i)) {
+            EXPECT_GT(j, i);
+            p.insert({i, j});
+        }
+    return p;
+}
+
+Pairs within(const System& s, double radius) {
+    Pairs p;
+    for (std::size_t i = 0; i < s.size(); ++i)
+        for (std::size_t j = i + 1; j < s.size(); ++j) {
+            const auto d = s.minimumImageDistance(i, j);
+            if (d[0] * d[0] + d[1] * d[1] + d[2] * d[2] < radius * radius) p.insert({i, j});
+        }
+    return p;
+}
+
+std::vector<std::array<double, 3>> forcesOf(const System& s) {
+    std::vector<std::array<double, 3>> f(s.size());
+    for (std::size_t i = 0; i < s.size(); ++i) f[i] = s.forceOf(i);
+    return f;
+}
+
+double maxDiff(const std::vector<std::array<double, 3>>& a, const System& s) {
+    double m = 0.0;
+    for (std::size_t i = 0; i < a.size(); ++i)
+        for (int k = 0; k < 3; ++k) m = std::max(m, std::abs(a[i][k] - s.forceOf(i)[k]));
+    return m;
+}
+
+}  // namespace
+
+// ---- test_system.cpp ---------------------------------------------------------------------------------------
+TEST(System, MinimumImagePicksNearestPeriodicCopy) {
+    System s({}, 10.0);
+    for (double d : s.minimumImageDistance({0.5, 0.5, 0.5}, {9.5, 9.5, 9.5})) EXPECT_NEAR(d, 1.0, 1e-12);
+    const auto d = s.minimumImageDistance({3.0, 4.0, 5.0}, {1.0, 6.0, 5.0});
+    EXPECT_NEAR(d[0], 2.0, 1e-12);
+    EXPECT_NEAR(d[1], -2.0, 1e-12);
+    EXPECT_NEAR(d[2], 0.0, 1e-12);
+}
+
+TEST(System, MinimumImageComponentsNeverExceedHalfBox) {
+    System s({}, 7.0);
+    for (int k = 0; k <= 108; ++k) {
+        const double x = -20.0 + 0.37 * k;
+        for (double d : s.minimumImageDistance({x, 2.0 * x, -x}, {0.0, 0.0, 0.0})) EXPECT_LE(std::abs(d), 3.5 + 1e-12);
+    }
+}
+
+TEST(System, WrapPositionMapsIntoBox) {
+    System s({}, 10.0);
+    std::array<double, 3> r {-0.5, 10.0, 23.0};
+    s.wrapPosition(r);
+    EXPECT_NEAR(r[0], 9.5, 1e-12);
+    EXPECT_NEAR(r[1], 0.0, 1e-12);
+    EXPECT_NEAR(r[2], 3.0, 1e-12);
+    for (int k = 0; k <= 95; ++k) {
+        const double x = -35.0 + 0.73 * k;
+        std::array<double, 3> p {x, -x, 0.5 * x};
+        s.wrapPosition(p);
+        for (double c : p) {
+            EXPECT_GE(c, 0.0);
+            EXPECT_LT(c, 10.0);
+        }
+    }
+}
+
+TEST(System, KineticEnergyAndTemperature) {
+    Particle a, b;
+    a.velocity = {1.0, 0.0, 0.0};
+    b.velocity = {-1.0, 0.0, 0.0};
+    b.mass = 2.0;
+    System s({a, b}, 10.0);
+    EXPECT_DOUBLE_EQ(s.kineticEnergy(), 1.5);
+    EXPECT_DOUBLE_EQ(s.temperature(), 2.0 * 1.5 / 3.0);
+}
+
+TEST(System, ReorderPermutesParticles) {
+    std::vector<Particle> ps(4);
+    for (int i = 0; i < 4; ++i) ps[i].position = {static_cast<double>(i), 0.0, 0.0};
+    System s(ps, 10.0);
+    const std::vector<std::size_t> order {2, 0, 3, 1};
+    s.reorder(order);
+    for (std::size_t k = 0; k < order.size(); ++k) EXPECT_DOUBLE_EQ(s.positionOf(k)[0], static_cast<double>(order[k]));
+}
+
+// ---- test_forcefield.cpp -----------------------------------------------------------------------------------
+TEST(ForceField, TwoParticlesMatchAnalyticForceAndEnergy) {
+    LennardJonesForceField ff(1.0, 1.0, kCutOff);
+    for (double r : {0.95, 1.12, 1.5, 2.0, 2.45}) {
+        System s = twoAlongX(5.0, 5.0 + r, 20.0);
+        const double u = ff.computeForces(s);
+        EXPECT_NEAR(s.forceOf(0)[0], -ljForce(r), 1e-10);
+        EXPECT_NEAR(s.forceOf(1)[0], ljForce(r), 1e-10);
+        EXPECT_NEAR(s.forceOf(0)[1], 0.0, 1e-12);
+        EXPECT_NEAR(u, ljEnergy(r, kCutOff), 1e-12);
+    }
+}
+
+TEST(ForceField, ForceVanishesAtPotentialMinimum) {
+    LennardJonesForceField ff(1.0, 1.0, kCutOff);
+    System s = twoAlongX(5.0, 5.0 + std::pow(2.0, 1.0 / 6.0), 20.0);
+    ff.computeForces(s);
+    EXPECT_NEAR(s.forceOf(0)[0], 0.0, 1e-12);
+}
+
+TEST(ForceField, NothingBeyondCutoff) {
+    LennardJonesForceField ff(1.0, 1.0, kCutOff);
+    System s = twoAlongX(5.0, 5.0 + kCutOff + 1e-9, 20.0);
+    EXPECT_DOUBLE_EQ(ff.computeForces(s), 0.0);
+    EXPECT_DOUBLE_EQ(s.forceOf(0)[0], 0.0);
+}
+
+TEST(ForceField, InteractsAcrossPeriodicBoundary) {
+    LennardJonesForceField ff(1.0, 1.0, kCutOff);
+    System s = twoAlongX(0.5, 19.5, 20.0);
+    const double u = ff.computeForces(s);
+    EXPECT_NEAR(s.forceOf(0)[0], ljForce(1.0), 1e-10);
+    EXPECT_NEAR(u, ljEnergy(1.0, kCutOff), 1e-12);
+}
+
+TEST(ForceField, RejectsCutoffBeyondHalfBox) {
+    LennardJonesForceField ff(1.0, 1.0, 2.5);
+    System s = twoAlongX(1.0, 2.0, 4.0);
+    EXPECT_THROW(ff.computeForces(s), std::runtime_error);
+}
+
+TEST(ForceField, NeighbourListMatchesAllPairs) {
+    System s = liquid(8);
+    LennardJonesForceField ff(1.0, 1.0, kCutOff);
+    NeighbourList list(kCutOff, kSkin);
+    list.build(s);
+    const double uList = ff.computeForces(s, list);
+    const auto fList = forcesOf(s);
+    const double uRef = ff.computeForces(s);
+    EXPECT_NEAR(uList, uRef, 1e-9 * std::abs(uRef));
+    EXPECT_LT(maxDiff(fList, s), 1e-11);
+}
+
+TEST(ForceField, TotalForceIsZero) {
+    System s = liquid(6);
+    LennardJonesForceField ff(1.0, 1.0, kCutOff);
+    NeighbourList list(kCutOff, kSkin);
+    list.build(s);
+    ff.computeForces(s, list);
+    for (int k = 0; k < 3; ++k) {
+        double t = 0.0;
+        for (double f : s.force(k)) t += f;
+        EXPECT_NEAR(t, 0.0, 1e-10);
+    }
+}
+
+TEST(ForceField, MixedPrecisionMatchesDouble) {
+    System s = liquid(8);
+    LennardJonesForceField ff(1.0, 1.0, kCutOff);
+    NeighbourList list(kCutOff, kSkin);
+    list.build(s);
+    const double u = ff.computeForces(s, list);
+    const auto ref = forcesOf(s);
+    ff.setMixedPrecision(true);
+    const double um = ff.computeForces(s, list);
+    EXPECT_NEAR(um, u, 1e-5 * std::abs(u));
+    double maxF = 0.0;
+    for (const auto& f : ref)
+        for (double c : f) maxF = std::max(maxF, std::abs(c));
+    EXPECT_LT(maxDiff(ref, s), 1e-5 * maxF);
+}
+
+TEST(ForceField, MixedPrecisionConservesEnergy) {
+    System s = liquid(5, 500);
+    LennardJonesForceField ff(1.0, 1.0, kCutOff);
+    ff.setMixedPrecision(true);
+    VelocityVerletIntegrator vv(kDt);
+    NeighbourList list(kCutOff, kSkin);
+    list.build(s);
+    double u = ff.computeForces(s, list);
+    const double e0 = s.kineticEnergy() + u;
+    double drift = 0.0;
+    for (int k = 0; k < 2000; ++k) {
+        u = vv.step(s, ff, list);
+        drift = std::max(drift, std::abs((s.kineticEnergy() + u - e0) / e0));
+    }
+    EXPECT_LT(drift, 3e-4);
+}
+
+// ---- test_neighbourlist.cpp --------------------------------------------------------------------------------
+TEST(NeighbourList, ContainsExactlyThePairsWithinListRadius) {
+    System s = liquid(8);
+    NeighbourList list(kCutOff, kSkin);
+    list.build(s);
+    const auto want = within(s, kCutOff + kSkin);
+    EXPECT_TRUE(listed(list, s.size()) == want);
+    EXPECT_EQ(list.getPairCount(), want.size());
+}
+
+TEST(NeighbourList, SmallBoxFallbackIsCorrect) {
+    System s = liquid(4);
+    NeighbourList list(kCutOff, kSkin);
+    list.build(s);
+    EXPECT_TRUE(listed(list, s.size()) == within(s, kCutOff + kSkin));
+}
+
+TEST(NeighbourList, RebuildsOnlyAfterHalfSkinDisplacement) {
+    System s = liquid(6);
+    NeighbourList list(kCutOff, kSkin);
+    list.build(s);
+    const int builds = list.getRebuildCount();
+    auto r = s.positionOf(0);
+    r[0] += 0.4 * kSkin;
+    s.wrapPosition(r);
+    s.setPosition(0, r);
+    EXPECT_FALSE(list.update(s));
+    EXPECT_EQ(list.getRebuildCount(), builds);
+    r[0] += 0.2 * kSkin;
+    s.wrapPosition(r);
+    s.setPosition(0, r);
+    EXPECT_TRUE(list.update(s));
+    EXPECT_EQ(list.getRebuildCount(), builds + 1);
+}
+
+TEST(NeighbourList, RejectsListRadiusBeyondHalfBox) {
+    System s = lattice(3);
+    NeighbourList list(kCutOff, kSkin);
+    EXPECT_THROW(list.build(s), std::runtime_error);
+}
+
+TEST(CellList, EveryParticleInExactlyOneCellMatchingItsPosition) {
+    System s = liquid(8);
+    CellList cells(kCutOff + kSkin);
+    cells.build(s);
+    const int n = cells.getCellsPerSide();
+    ASSERT_GE(n, 3);
+    const double size = s.getBoxLength() / n;
+    std::vector<int> seen(s.size(), 0);
+    for (int c = 0; c < n * n * n; ++c)
+        for (std::size_t i : cells.particlesIn(c)) {
+            ++seen[i];
+            EXPECT_EQ(cells.cellOf(i), c);
+            const auto r = s.positionOf(i);
+            int want = 0;
+            for (int k = 0; k < 3; ++k) want = want * n + std::min(static_cast<int>(r[k] / size), n - 1);
+            EXPECT_EQ(want, c);
+        }
+    for (int count : seen) EXPECT_EQ(count, 1);
+}
+
+TEST(CellList, NeighbourCellsAreDistinctAndWrap) {
+    System s = lattice(8);
+    CellList cells(kCutOff + kSkin);
+    cells.build(s);
+    const int n = cells.getCellsPerSide();
+    ASSERT_GE(n, 3);
+    for (int c : {0, n * n * n - 1, n * n + n + 1}) {
+        const auto nb = cells.neighbourCells(c);
+        std::set<int> unique(nb.begin(), nb.end());
+        EXPECT_EQ(unique.size(), 27u);
+        EXPECT_TRUE(unique.count(c) == 1);
+        for (int x : nb) {
+            EXPECT_GE(x, 0);
+            EXPECT_LT(x, n * n * n);
+        }
+    }
+}
+
+// ---- test_dynamics.cpp -------------------------------------------------------------------------------------
+TEST(Initialisation, FccLatticeHasExpectedCountAndSpacing) {
+    const int cells = 3;
+    const double a = 1.5;
+    const auto ps = createFCCLattice(cells, a);
+    ASSERT_EQ(ps.size(), 4u * cells * cells * cells);
+    System s(ps, cells * a);
+    double dmin = std::numeric_limits<double>::max();
+    for (std::size_t i = 0; i < ps.size(); ++i)
+        for (std::size_t j = i + 1; j < ps.size(); ++j) {
+            const auto d = s.minimumImageDistance(ps[i].position, ps[j].position);
+            dmin = std::min(dmin, std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]));
+        }
+    EXPECT_NEAR(dmin, a / std::sqrt(2.0), 1e-12);
+}
+
+TEST(Initialisation, VelocitiesHitTargetTemperatureWithZeroMomentum) {
+    System s = lattice(5);
+    EXPECT_NEAR(s.temperature(), kTemperature, 1e-12);
+    for (double p : momentum(s)) EXPECT_NEAR(p, 0.0, 1e-12);
+}
+
+TEST(Thermostat, RescalesToTarget) {
+    System s = lattice(4);
+    rescaleVelocities(s, 1.7);
+    EXPECT_NEAR(s.temperature(), 1.7, 1e-12);
+}
+
+TEST(Dynamics, ConservesEnergy) {
+    System s = liquid(5, 500);
+    LennardJonesForceField ff(1.0, 1.0, kCutOff);
+    VelocityVerletIntegrator vv(kDt);
+    NeighbourList list(kCutOff, kSkin);
+    list.build(s);
+    double u = ff.computeForces(s, list);
+    const double e0 = s.kineticEnergy() + u;
+    double drift = 0.0;
+    for (int k = 0; k < 2000; ++k) {
+        u = vv.step(s, ff, list);
+        drift = std::max(drift, std::abs((s.kineticEnergy() + u - e0) / e0));
+    }
+    EXPECT_LT(drift, 3e-4);
+    EXPECT_GT(list.getRebuildCount(), 1);
+}
+
+TEST(Dynamics, ConservesMomentum) {
+    System s = liquid(5, 0);
+    LennardJonesForceField ff(1.0, 1.0, kCutOff);
+    VelocityVerletIntegrator vv(kDt);
+    NeighbourList list(kCutOff, kSkin);
+    list.build(s);
+    ff.computeForces(s, list);
+    for (int k = 0; k < 1000; ++k) vv.step(s, ff, list);
+    for (double p : momentum(s)) EXPECT_NEAR(p, 0.0, 1e-10);
+}
+
+TEST(Dynamics, IsTimeReversible) {
+    System s = liquid(4, 200);
+    LennardJonesForceField ff(1.0, 1.0, kCutOff);
+    VelocityVerletIntegrator vv(kDt);
+    ff.computeForces(s);
+    std::vector<std::array<double, 3>> start(s.size());
+    for (std::size_t i = 0; i < s.size(); ++i) start[i] = s.positionOf(i);
+    for (int k = 0; k < 100; ++k) vv.step(s, ff);
+    for (int k = 0; k < 3; ++k)
+        for (double& v : s.velocity(k)) v = -v;
+    for (int k = 0; k < 100; ++k) vv.step(s, ff);
+    double err = 0.0;
+    for (std::size_t i = 0; i < s.size(); ++i) {
+        const auto d = s.minimumImageDistance(s.positionOf(i), start[i]);
+        err = std::max({err, std::abs(d[0]), std::abs(d[1]), std::abs(d[2])});
+    }
+    EXPECT_LT(err, 1e-8);
+}
+
+// ---- test_observables.cpp ----------------------------------------------------------------------------------
+TEST(Observables, TwoParticleVirialAndInsideCount) {
+    LennardJonesForceField ff(1.0, 1.0, kCutOff);
+    Particle a, b;
+    a.position = {5.0, 5.0, 5.0};
+    b.position = {6.3, 5.0, 5.0};
+    System s({a, b}, 20.0);
+    ff.computeForces(s);
+    EXPECT_NEAR(ff.getVirial(), 1.3 * ljForce(1.3), 1e-12);
+    EXPECT_DOUBLE_EQ(ff.getInsidePairs(), 1.0);
+}
+
+TEST(Observables, VirialAndInsideCountMatchAcrossForcePaths) {
+    System s = liquid(8);
+    LennardJonesForceField ff(1.0, 1.0, kCutOff);
+    NeighbourList list(kCutOff, kSkin, true);
+    list.build(s);
+    ff.computeForces(s);
+    const double w = ff.getVirial(), c = ff.getInsidePairs();
+    for (bool det : {false, true}) {
+        ff.setDeterministic(det);
+        ff.computeForces(s, list);
+        EXPECT_NEAR(ff.getVirial(), w, 1e-9 * std::abs(w));
+        EXPECT_DOUBLE_EQ(ff.getInsidePairs(), c);
+    }
+}
+
+TEST(Observables, BussiThermostatSamplesCanonicalKineticEnergy) {
+    System s = lattice(5);
+    const double target = 0.85;
+    const double dof = 3.0 * static_cast<double>(s.size()) - 3.0;
+    BussiThermostat th(target, 0.05, 7);
+    double sk = 0.0, sk2 = 0.0;
+    int samples = 0;
+    for (int k = 0; k < 100000; ++k) {
+        th.apply(s, 0.005);
+        if (k >= 1000) {
+            const double ke = s.kineticEnergy();
+            sk += ke;
+            sk2 += ke * ke;
+            ++samples;
+        }
+    }
+    const double mean = sk / samples, var = sk2 / samples - mean * mean;
+    EXPECT_NEAR(mean / (0.5 * dof), target, 0.005 * target);
+    EXPECT_NEAR(var / (0.5 * dof * target * target), 1.0, 0.05);
+}
+
+TEST(Observables, RadialDistributionOfIdealGasIsOne) {
+    const double box = 12.0;
+    std::mt19937 rng(3);
+    std::uniform_real_distribution<double> uniform(0.0, box);
+    RadialDistribution rdf(kCutOff, 25);
+    for (int k = 0; k < 40; ++k) {
+        std::vector<Particle> ps(2000);
+        for (auto& p : ps) p.position = {uniform(rng), uniform(rng), uniform(rng)};
+        System s(ps, box);
+        NeighbourList list(kCutOff, kSkin);
+        list.build(s);
+        rdf.sample(s, list);
+    }
+    const auto r = rdf.radii();
+    const auto g = rdf.values();
+    for (std::size_t b = 0; b < g.size(); ++b)
+        if (r[b] > 1.0) EXPECT_NEAR(g[b], 1.0, 0.06);
+}
+
+// ---- test_determinism.cpp ----------------------------------------------------------------------------------
+namespace {
+
+std::uint64_t runAndHash(bool deterministic, int* rebuilds = nullptr) {
+    System s = lattice(6);
+    LennardJonesForceField ff(1.0, 1.0, kCutOff);
+    ff.setDeterministic(deterministic);
+    VelocityVerletIntegrator vv(kDt);
+    NeighbourList list(kCutOff, kSkin, deterministic);
+    list.build(s);
+    ff.computeForces(s, list);
+    for (int k = 1; k <= 400; ++k) {
+        vv.step(s, ff, list);
+        if (k <= 200 && k % 10 == 0) rescaleVelocities(s, kTemperature);
+    }
+    if (rebuilds) *rebuilds = list.getRebuildCount();
+    return s.stateHash();
+}
+
+}  // namespace
+
+TEST(Determinism, DefaultModeIsReproducibleAtFixedThreadCount) {
+    const std::uint64_t first = runAndHash(false);
+    for (int run = 0; run < 3; ++run) EXPECT_EQ(runAndHash(false), first);
+}
+
+TEST(Determinism, DeterministicModeIsIndependentOfThreadCount) {
+    // the GPU path has one fixed order; the reference's own build of this scenario hashes to this value
+    int rebuilds = 0;
+    const std::uint64_t h = runAndHash(true, &rebuilds);
+    EXPECT_EQ(h, 0x7833f91c741e992full);
+    EXPECT_EQ(rebuilds, 67);
+    EXPECT_EQ(runAndHash(false), h);  // default mode runs the same deterministic kernel
+}
+
+TEST(Determinism, DeterministicForcesMatchAllPairs) {
+    System s = liquid(8);
+    LennardJonesForceField ff(1.0, 1.0, kCutOff);
+    NeighbourList list(kCutOff, kSkin, true);
+    list.build(s);
+    ff.setDeterministic(true);
+    const double u = ff.computeForces(s, list);
+    const auto f = forcesOf(s);
+    const double ref = ff.computeForces(s);
+    EXPECT_NEAR(u, ref, 1e-9 * std::abs(ref));
+    EXPECT_LT(maxDiff(f, s), 1e-11);
+}
+
+TEST(Determinism, RequiresReverseIndex) {
+    System s = lattice(6);
+    LennardJonesForceField ff(1.0, 1.0, kCutOff);
+    ff.setDeterministic(true);
+    NeighbourList list(kCutOff, kSkin);
+    list.build(s);
+    EXPECT_THROW(ff.computeForces(s, list), std::runtime_error);
+}
+
+TEST(Determinism, ReverseIndexListsEveryPairOnce) {
+    System s = liquid(6);
+    NeighbourList list(kCutOff, kSkin, true);
+    list.build(s);
+    std::vector<int> seen(list.getPairCount(), 0);
+    for (std::size_t j = 0; j < s.size(); ++j) {
+        std::size_t prev = 0;
+        bool first = true;
+        for (std::size_t k : list.pairsAsSecond(j)) {
+            ++seen[k];
+            EXPECT_TRUE(first || k > prev);
+            prev = k;
+            first = false;
+        }
+    }
+    for (int c : seen) EXPECT_EQ(c, 1);
+}
+
+// ---- test_trajectory.cpp -----------------------------------------------------------------------------------
+namespace {
+
+std::vector<std::array<double, 3>> readFrame(std::FILE* f, std::size_t& n, std::vector<std::size_t>& ids) {
+    char line[512];
+    if (!std::fgets(line, sizeof line, f)) return {};
+    n = std::stoul(line);
+    if (!std::fgets(line, sizeof line, f)) return {};
+    std::vector<std::array<double, 3>> pos;
+    for (std::size_t k = 0; k < n; ++k) {
+        if (!std::fgets(line, sizeof line, f)) break;
+        std::istringstream in(line);
+        std::string sp;
+        std::array<double, 3> r {}, v {};
+        std::size_t id = 0;
+        in >> sp >> r[0] >> r[1] >> r[2] >> v[0] >> v[1] >> v[2] >> id;
+        pos.push_back(r);
+        ids.push_back(id);
+    }
+    return pos;
+}
+
+}  // namespace
+
+TEST(Trajectory, IdsFollowParticlesThroughReordering) {
+    System s = liquid(6, 0);
+    std::vector<std::array<double, 3>> byId(s.size());
+    for (std::size_t i = 0; i < s.size(); ++i) byId[s.ids()[i]] = s.positionOf(i);
+    NeighbourList list(kCutOff, kSkin);
+    list.build(s);
+    std::vector<std::size_t> ids(s.ids().begin(), s.ids().end());
+    EXPECT_FALSE(std::is_sorted(ids.begin(), ids.end()));
+    std::sort(ids.begin(), ids.end());
+    for (std::size_t k = 0; k < ids.size(); ++k) EXPECT_EQ(ids[k], k);
+    for (std::size_t i = 0; i < s.size(); ++i) EXPECT_TRUE(s.positionOf(i) == byId[s.ids()[i]]);
+}
+
+TEST(Trajectory, FramesAreConsistentAcrossRebuilds) {
+    System s = liquid(6);
+    LennardJonesForceField ff(1.0, 1.0, kCutOff);
+    VelocityVerletIntegrator vv(kDt);
+    NeighbourList list(kCutOff, kSkin);
+    list.build(s);
+    ff.computeForces(s, list);
+    std::unique_ptr<std::FILE, int (*)(std::FILE*)> owner(std::tmpfile(), &std::fclose);
+    std::FILE* f = owner.get();
+    ASSERT_NE(f, nullptr);
+    writeXyzFrame(f, s, 0.0);
+    vv.step(s, ff, list);
+    list.build(s);
+    writeXyzFrame(f, s, kDt);
+    ASSERT_EQ(std::fseek(f, 0, SEEK_SET), 0);
+    std::size_t n1 = 0, n2 = 0;
+    std::vector<std::size_t> ids1, ids2;
+    const auto a = readFrame(f, n1, ids1);
+    const auto b = readFrame(f, n2, ids2);
+    ASSERT_EQ(n1, s.size());
+    ASSERT_EQ(n2, s.size());
+    double moved = 0.0;
+    for (std::size_t k = 0; k < n1; ++k) {
+        EXPECT_EQ(ids1[k], k);
+        EXPECT_EQ(ids2[k], k);
+        const auto d = s.minimumImageDistance(a[k], b[k]);
+        moved = std::max(moved, std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]));
+    }
+    EXPECT_LT(moved, 0.1);
+}
+
+int main(int argc, char** argv) {
+    const std::string filter = argc > 1 ? argv[1] : "";
+    int failed = 0, ran = 0;
+    for (const auto& c : mdtest::registry()) {
+        if (!filter.empty() && c.name.find(filter) == std::string::npos) continue;
+        std::printf("[ RUN  ] %s\n", c.name.c_str());
+        std::fflush(stdout);
+        const int before = mdtest::failures();
+        try {
+            c.fn();
+        } catch (const mdtest::AssertAbort&) {
+        } catch (const std::exception& e) {
+            mdtest::report(__FILE__, __LINE__, std::string("unexpected exception: ") + e.what());
+        }
+        const bool ok = mdtest::failures() == before;
+        failed += ok ? 0 : 1;
+        ++ran;
+        std::printf("[ %s ] %s\n", ok ? " OK " : "FAIL", c.name.c_str());
+        std::fflush(stdout);
+    }
+    std::printf("%d tests, %d failed\n", ran, failed);
+    return failed;
+}
