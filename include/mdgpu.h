@@ -184,6 +184,34 @@ typedef struct md_run_config {
    the thermostat's new kinetic energy K' (eq. A7) with the Bussi thermostat on. */
 int md_run(md_state* s, md_list* l, const md_run_config* cfg, double* energies);
 
+/* ---- Adjoint (no counterpart in the reference) ----------------------------------------------------------- */
+/* Gradients of L = mean_k U(r_k) + (mean_k P_k - pressure_target)^2 over k = 1..steps, P = (2 KE + W) / (3V)
+   (the reference's virial pressure, main.cpp:209-211), with respect to epsilon, sigma and the initial positions and
+   velocities. FP64 velocity-Verlet forward over a fixed-order list (the state is not re-sorted; `s` itself is not
+   modified), (r, v, f) checkpoints every `checkpoint_every` steps, and a reverse sweep that recomputes each
+   segment and runs the Hessian-vector-product adjoint kernels through it. */
+typedef struct md_adjoint_config {
+    double dt;
+    int steps;
+    int checkpoint_every;
+    md_lj lj;
+    double pressure_target;
+} md_adjoint_config;
+
+typedef struct md_adjoint_result {
+    double loss;
+    double mean_potential;
+    double mean_pressure;
+    double d_epsilon;
+    double d_sigma;
+    double forward_ms; /* forward pass, HIP events on the context stream */
+    double reverse_ms; /* recomputation + adjoint sweep */
+} md_adjoint_result;
+
+/* grad_positions / grad_velocities (may be NULL): double[3n] component-major, in the state's particle order. */
+int md_adjoint_run(md_state* s, md_list* l, const md_adjoint_config* cfg, md_adjoint_result* out,
+                   double* grad_positions, double* grad_velocities);
+
 #ifdef __cplusplus
 }
 #endif

@@ -463,6 +463,112 @@ void list_enqueue_conditional_rebuild(md_list* l, md_state* s) {
     list_kernel_enqueue(l, s, true, gate);
 }
 
+// Fixed-order list (no re-sort): candidates are the members of the 27 cells taken through the stable cell order,
+// so particle indices never change. Rows hold the j > i neighbours first, then the j < i ones. Also records the
+// build-time positions. Used by the adjoint run, whose reverse sweep needs the same indices at every step.
+__global__ void __launch_bounds__(256) k_list_build_fixed(const double4* __restrict__ pos, size_t n,
+                                                           const int* __restrict__ cell_of,
+                                                           const uint32_t* __restrict__ start,
+                                                           const uint32_t* __restrict__ order, int side, double box,
+                                                           double inv_box, double radius2, int use_cells, int padding,
+                                                           int32_t* __restrict__ rows, int32_t* __restrict__ count,
+                                                           int32_t* __restrict__ half, int* __restrict__ flags,
+                                                           unsigned long long* __restrict__ pair_total,
+                                                           double4* __restrict__ ref, const int* gate) {
+    if (gated_off(gate)) return;
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    int cnt = 0, hcnt = 0;
+    if (i < n) {
+        const double4 ri = pos[i];
+        ref[i] = ri;
+        const uint32_t self = static_cast<uint32_t>(i);
+        auto consider = [&](uint32_t j) {
+            const double4 rj = pos[j];
+            const double dx = min_image(ri.x - rj.x, box, inv_box);
+            const double dy = min_image(ri.y - rj.y, box, inv_box);
+            const double dz = min_image(ri.z - rj.z, box, inv_box);
+            if (dx * dx + dy * dy + dz * dz < radius2) {
+                if (cnt < padding) rows[static_cast<size_t>(cnt) * n + i] = static_cast<int32_t>(j);
+                ++cnt;
+            }
+        };
+        for (int pass = 0; pass < 2; ++pass) {
+            const bool upper = pass == 0;
+            if (use_cells) {
+                const int c = cell_of[i];
+                const int cx = c / (side * side), cy = (c / side) % side, cz = c % side;
+                for (int ox = -1; ox <= 1; ++ox)
+                    for (int oy = -1; oy <= 1; ++oy)
+                        for (int oz = -1; oz <= 1; ++oz) {
+                            const int nc = (wrap_index(cx + ox, side) * side + wrap_index(cy + oy, side)) * side +
+                                           wrap_index(cz + oz, side);
+                            for (uint32_t k = start[nc]; k < start[nc + 1]; ++k) {
+                                const uint32_t j = order[k];
+                                if (upper ? j > self : j < self) consider(j);
+                            }
+                        }
+            } else {
+                for (uint32_t j = upper ? self + 1 : 0; j < (upper ? n : self); ++j) consider(j);
+            }
+            if (upper) hcnt = cnt;
+        }
+        count[i] = cnt < padding ? cnt : padding;
+        half[i] = hcnt < padding ? hcnt : padding;
+    }
+    int mx = cnt;
+    unsigned long long hs = static_cast<unsigned long long>(hcnt);
+#pragma unroll
+    for (int off = WAVE / 2; off > 0; off >>= 1) {
+        const int o = __shfl_xor(mx, off, WAVE);
+        mx = o > mx ? o : mx;
+        hs += __shfl_xor(hs, off, WAVE);
+    }
+    if (threadIdx.x % WAVE == 0) {
+        atomicMax(&flags[FLAG_MAXCOUNT], mx);
+        if (mx > padding) atomicOr(&flags[FLAG_OVERFLOW], 1);
+        atomicAdd(pair_total, hs);
+    }
+    if (gate && i == 0) atomicAdd(&flags[FLAG_REBUILDS], 1);
+}
+
+static void list_fixed_enqueue(md_list* l, md_state* s, const int* gate) {
+    hipStream_t st = l->ctx->stream;
+    const size_t n = s->n;
+    cells_enqueue(l, s, l->side / s->box, false, gate);
+    k_clear_build_flags<<<1, 1, 0, st>>>(l->flags, pair_total_of(l), gate);
+    k_list_build_fixed<<<blocks_for(n ? n : 1, 256), 256, 0, st>>>(
+        s->pos, n, l->cell_of, l->cell_start, l->order, l->side, s->box, s->inv_box, l->radius2,
+        l->side >= 3 ? 1 : 0, l->padding, l->rows, l->count, l->half, l->flags, pair_total_of(l), l->ref, gate);
+    MD_HIP(hipGetLastError());
+}
+
+void list_rebuild_fixed(md_list* l, md_state* s) {
+    if (2.0 * l->radius > s->box) fail(MD_ERR_RUNTIME, "Cutoff + skin must be less than half the box length");
+    list_prepare(l, s);
+    hipStream_t st = l->ctx->stream;
+    int* hf = l->ctx->host_flags;
+    for (int attempt = 0;; ++attempt) {
+        list_fixed_enqueue(l, s, nullptr);
+        MD_HIP(hipMemcpyAsync(hf, l->flags, sizeof(int) * (FLAG_COUNT + 2), hipMemcpyDeviceToHost, st));
+        MD_HIP(hipStreamSynchronize(st));
+        l->max_count = hf[FLAG_MAXCOUNT];
+        unsigned long long total;
+        std::memcpy(&total, hf + FLAG_COUNT, sizeof(total));
+        l->half_pairs = static_cast<size_t>(total);
+        if (!hf[FLAG_OVERFLOW] || s->n == 0) break;
+        if (attempt > 2) fail(MD_ERR_HIP, "neighbour list padding did not converge");
+        l->padding = (l->max_count + l->max_count / 4 + 16 + 7) / 8 * 8;
+        list_prepare(l, s);
+    }
+    MD_HIP(hipMemsetAsync(l->flags + FLAG_STALE, 0, sizeof(int), st));
+    l->built = true;
+    ++l->rebuilds;
+}
+
+void list_enqueue_conditional_rebuild_fixed(md_list* l, md_state* s) {
+    list_fixed_enqueue(l, s, l->flags + FLAG_STALE);
+}
+
 static void release_list_buffers(md_list* l) {
     dev_free(l->cell_count);
     dev_free(l->cell_start);

@@ -93,7 +93,7 @@ __global__ void __launch_bounds__(FORCE_THREADS)
         const double4 ri = pos[i];
         const int m = count[i], h = half[i];
         const int32_t* row = rows + i;
-        float u = 0.0f, w = 0.0f, c = 0.0f;
+        double u = 0.0, w = 0.0, c = 0.0;
         for (int k = 0; k < m; ++k) {
             const double4 rj = pos[row[static_cast<size_t>(k) * n]];
             double ex = ri.x - rj.x, ey = ri.y - rj.y, ez = ri.z - rj.z;
@@ -102,7 +102,9 @@ __global__ void __launch_bounds__(FORCE_THREADS)
             ez = fma(-box, rint(ez * inv_box), ez);
             const float dx = static_cast<float>(ex), dy = static_cast<float>(ey), dz = static_cast<float>(ez);
             const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-            const float inv_r2 = __builtin_amdgcn_rcpf(r2);
+            // hardware reciprocal (1 ulp) plus one Newton step
+            const float q = __builtin_amdgcn_rcpf(r2);
+            const float inv_r2 = fmaf(q, fmaf(-r2, q, 1.0f), q);
             const float s2 = lj.sig2f * inv_r2;
             const float s6 = s2 * s2 * s2;
             const float s12 = s6 * s6;
@@ -112,9 +114,9 @@ __global__ void __launch_bounds__(FORCE_THREADS)
             fy += static_cast<double>(f * dy);
             fz += static_cast<double>(f * dz);
             if (k < h) {
-                u += inside ? fmaf(lj.c4f, s12 - s6, -lj.shiftf) : 0.0f;
-                w = fmaf(f, r2, w);
-                c += inside ? 1.0f : 0.0f;
+                u += inside ? static_cast<double>(fmaf(lj.c4f, s12 - s6, -lj.shiftf)) : 0.0;
+                w += static_cast<double>(f * r2);
+                c += inside ? 1.0 : 0.0;
             }
         }
         acc[0] = u;

@@ -149,6 +149,8 @@ void list_rebuild(md_list* l, md_state* s);                            // full h
 void list_enqueue_conditional_rebuild(md_list* l, md_state* s);       // device-flag-gated rebuild (md_run)
 void list_prepare(md_list* l, md_state* s);                            // grid/buffers for the state's size
 void displacement_check(md_list* l, md_state* s);                      // sets FLAG_STALE
+void list_rebuild_fixed(md_list* l, md_state* s);                      // list without re-sorting (host-driven)
+void list_enqueue_conditional_rebuild_fixed(md_list* l, md_state* s); // same, gated by FLAG_STALE
 
 // api.hip
 void state_reserve_partials(md_state* s, size_t count);

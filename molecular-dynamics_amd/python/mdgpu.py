@@ -46,6 +46,17 @@ class RunConfig(C.Structure):
                 ("force_kernel_ms", C.POINTER(C.c_double))]
 
 
+class AdjointConfig(C.Structure):
+    _fields_ = [("dt", C.c_double), ("steps", C.c_int), ("checkpoint_every", C.c_int), ("lj", LJ),
+                ("pressure_target", C.c_double)]
+
+
+class AdjointResult(C.Structure):
+    _fields_ = [("loss", C.c_double), ("mean_potential", C.c_double), ("mean_pressure", C.c_double),
+                ("d_epsilon", C.c_double), ("d_sigma", C.c_double), ("forward_ms", C.c_double),
+                ("reverse_ms", C.c_double)]
+
+
 _P = C.c_void_p
 _PP = C.POINTER(C.c_void_p)
 _DP = C.POINTER(C.c_double)
@@ -91,7 +102,19 @@ _SIGS = {
     "md_bussi_rng_draws": (C.c_int, [_P, C.c_size_t, C.c_int, _DP]),
     "md_fcc_lattice": (C.c_int, [C.c_int, C.c_double, _DP]),
     "md_maxwell_boltzmann": (C.c_int, [C.c_size_t, _DP, C.c_double, C.c_uint32, _DP]),
+    "md_adjoint_run": (C.c_int, [_P, _P, C.POINTER(AdjointConfig), C.POINTER(AdjointResult), _DP, _DP]),
 }
+
+
+def adjoint_run(state, nl, lj, dt: float, steps: int, checkpoint_every: int, pressure_target: float = 1.0):
+    """md_adjoint_run: (result, dL/dr0 (3, n), dL/dv0 (3, n)); `state` is left unchanged."""
+    cfg = AdjointConfig(dt, steps, checkpoint_every, lj, pressure_target)
+    res = AdjointResult()
+    gp = np.empty((3, state.n))
+    gv = np.empty((3, state.n))
+    check(lib().md_adjoint_run(state.h, nl.h, C.byref(cfg), C.byref(res), gp.ctypes.data_as(_DP),
+                               gv.ctypes.data_as(_DP)))
+    return res, gp, gv
 
 
 def fcc_lattice(cells: int, lattice_constant: float) -> np.ndarray:
