@@ -27,27 +27,40 @@ const char* get_error() { return g_error.c_str(); }
 // ----------------------------------------------------------------------------------------------------------
 // Reductions
 
-// One block sums rows of `width` partials: thread t takes rows t, t + 256, ... (fixed order), then the block
-// combines the 256 thread sums in a fixed tree.
-__global__ void __launch_bounds__(256) k_reduce_partials(const double* __restrict__ partials, int blocks,
-                                                         int width, double* __restrict__ out) {
-    __shared__ double lds[256];
-    for (int k = 0; k < width; ++k) {
-        double s = 0.0;
-        for (int b = threadIdx.x; b < blocks; b += 256) s += partials[static_cast<size_t>(b) * width + k];
-        lds[threadIdx.x] = s;
-        __syncthreads();
-        for (int half = 128; half > 0; half >>= 1) {
-            if (threadIdx.x < half) lds[threadIdx.x] += lds[threadIdx.x + half];
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) out[k] = lds[0];
-        __syncthreads();
+// One block sums rows of W partials: thread t takes rows t, t + 1024, ... (fixed order), then each wavefront
+// reduces by butterfly shuffles and thread 0 adds the 16 wavefront sums in order.
+template <int W>
+__global__ void __launch_bounds__(1024) k_reduce_partials(const double* __restrict__ partials, int blocks,
+                                                          double* __restrict__ out) {
+    __shared__ double lds[1024 / WAVE][W];
+    double s[W];
+#pragma unroll
+    for (int k = 0; k < W; ++k) s[k] = 0.0;
+    for (int b = threadIdx.x; b < blocks; b += 1024) {
+#pragma unroll
+        for (int k = 0; k < W; ++k) s[k] += partials[static_cast<size_t>(b) * W + k];
     }
+#pragma unroll
+    for (int k = 0; k < W; ++k) s[k] = wave_sum(s[k]);
+    if (threadIdx.x % WAVE == 0)
+#pragma unroll
+        for (int k = 0; k < W; ++k) lds[threadIdx.x / WAVE][k] = s[k];
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int k = 0; k < W; ++k) {
+            double t = 0.0;
+            for (int w = 0; w < 1024 / WAVE; ++w) t += lds[w][k];
+            out[k] = t;
+        }
 }
 
 void reduce_partials_enqueue(hipStream_t st, const double* partials, int blocks, int width, double* out_dev) {
-    k_reduce_partials<<<1, 256, 0, st>>>(partials, blocks, width, out_dev);
+    switch (width) {
+        case 1: k_reduce_partials<1><<<1, 1024, 0, st>>>(partials, blocks, out_dev); break;
+        case 3: k_reduce_partials<3><<<1, 1024, 0, st>>>(partials, blocks, out_dev); break;
+        case 6: k_reduce_partials<6><<<1, 1024, 0, st>>>(partials, blocks, out_dev); break;
+        default: fail(MD_ERR_ARG, "unsupported reduction width");
+    }
     MD_HIP(hipGetLastError());
 }
 

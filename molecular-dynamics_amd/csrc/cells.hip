@@ -3,7 +3,7 @@
 //
 // Rebuild pipeline, every kernel optionally gated by a device flag so a captured step can carry it:
 //   k_cell_count   cell id per particle (CellList.cpp:26-38) + atomic histogram
-//   k_scan_cells   exclusive prefix sum of the histogram -> cell start offsets (CellList.cpp:41-44)
+//   k_scan_*       exclusive prefix sum of the histogram -> cell start offsets (CellList.cpp:41-44)
 //   k_cell_scatter particle index into its cell's range, unordered (atomics)
 //   k_cell_sort    one wavefront per cell ranks its members by index, restoring the counting sort's stable order
 //                  (CellList.cpp:47-51); when re-sorting, it also gathers every particle array into the new order
@@ -66,40 +66,98 @@ __global__ void __launch_bounds__(256) k_cell_count(const double4* __restrict__ 
     atomicAdd(&counts[c], 1u);
 }
 
-// Exclusive scan of `ncells` counts by one workgroup: each thread owns a contiguous chunk, the chunk totals are
-// scanned across the block, then every chunk is rewritten with its running offset. start[ncells] = total.
-__global__ void __launch_bounds__(1024) k_scan_cells(const uint32_t* __restrict__ counts, size_t ncells,
-                                                      uint32_t* __restrict__ start, const int* gate) {
+// Three-phase exclusive scan of the cell histogram, SCAN_ITEMS cells per block: block totals, a scan of the
+// totals, then each block rescans its cells from its offset. start[ncells] receives the total.
+constexpr int SCAN_THREADS = 256;
+constexpr int SCAN_PER_THREAD = 8;
+constexpr int SCAN_ITEMS = SCAN_THREADS * SCAN_PER_THREAD;
+
+__global__ void __launch_bounds__(SCAN_THREADS) k_scan_totals(const uint32_t* __restrict__ counts, size_t ncells,
+                                                               uint32_t* __restrict__ totals, const int* gate) {
     if (gated_off(gate)) return;
-    __shared__ uint32_t wave_total[1024 / WAVE];
-    const size_t per = (ncells + 1023) / 1024;
-    const size_t b = threadIdx.x * per;
-    const size_t e = b + per < ncells ? b + per : ncells;
-    uint32_t sum = 0;
-    for (size_t c = b; c < e; ++c) sum += counts[c];
+    __shared__ uint32_t wsum[SCAN_THREADS / WAVE];
+    const size_t base = static_cast<size_t>(blockIdx.x) * SCAN_ITEMS + threadIdx.x * SCAN_PER_THREAD;
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_PER_THREAD; ++k)
+        if (base + k < ncells) s += counts[base + k];
+#pragma unroll
+    for (int off = WAVE / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, WAVE);
+    if (threadIdx.x % WAVE == 0) wsum[threadIdx.x / WAVE] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (int w = 0; w < SCAN_THREADS / WAVE; ++w) t += wsum[w];
+        totals[blockIdx.x] = t;
+    }
+}
+
+__global__ void __launch_bounds__(1024) k_scan_top(uint32_t* __restrict__ totals, int blocks,
+                                                    uint32_t* __restrict__ grand, const int* gate) {
+    if (gated_off(gate)) return;
+    __shared__ uint32_t wsum[1024 / WAVE];
+    __shared__ uint32_t carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
     const int lane = threadIdx.x % WAVE, wave = threadIdx.x / WAVE;
-    uint32_t incl = sum;
+    for (int b0 = 0; b0 < blocks; b0 += 1024) {
+        const int b = b0 + threadIdx.x;
+        const uint32_t v = b < blocks ? totals[b] : 0u;
+        uint32_t incl = v;
+#pragma unroll
+        for (int off = 1; off < WAVE; off <<= 1) {
+            const uint32_t o = __shfl_up(incl, off, WAVE);
+            if (lane >= off) incl += o;
+        }
+        if (lane == WAVE - 1) wsum[wave] = incl;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t acc = 0;
+            for (int w = 0; w < 1024 / WAVE; ++w) {
+                const uint32_t t = wsum[w];
+                wsum[w] = acc;
+                acc += t;
+            }
+        }
+        __syncthreads();
+        const uint32_t excl = carry + wsum[wave] + incl - v;
+        if (b < blocks) totals[b] = excl;
+        __syncthreads();
+        if (threadIdx.x == 1023) carry = excl + v;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *grand = carry;
+}
+
+__global__ void __launch_bounds__(SCAN_THREADS) k_scan_apply(const uint32_t* __restrict__ counts, size_t ncells,
+                                                              const uint32_t* __restrict__ totals,
+                                                              uint32_t* __restrict__ start, const int* gate) {
+    if (gated_off(gate)) return;
+    __shared__ uint32_t wsum[SCAN_THREADS / WAVE];
+    const size_t base = static_cast<size_t>(blockIdx.x) * SCAN_ITEMS + threadIdx.x * SCAN_PER_THREAD;
+    uint32_t v[SCAN_PER_THREAD];
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_PER_THREAD; ++k) {
+        v[k] = base + k < ncells ? counts[base + k] : 0u;
+        s += v[k];
+    }
+    const int lane = threadIdx.x % WAVE, wave = threadIdx.x / WAVE;
+    uint32_t incl = s;
 #pragma unroll
     for (int off = 1; off < WAVE; off <<= 1) {
         const uint32_t o = __shfl_up(incl, off, WAVE);
         if (lane >= off) incl += o;
     }
-    if (lane == WAVE - 1) wave_total[wave] = incl;
+    if (lane == WAVE - 1) wsum[wave] = incl;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t acc = 0;
-        for (int w = 0; w < 1024 / WAVE; ++w) {
-            const uint32_t t = wave_total[w];
-            wave_total[w] = acc;
-            acc += t;
-        }
-        start[ncells] = acc;
-    }
-    __syncthreads();
-    uint32_t run = wave_total[wave] + incl - sum;
-    for (size_t c = b; c < e; ++c) {
-        start[c] = run;
-        run += counts[c];
+    uint32_t before = totals[blockIdx.x];
+    for (int w = 0; w < wave; ++w) before += wsum[w];
+    uint32_t run = before + incl - s;
+#pragma unroll
+    for (int k = 0; k < SCAN_PER_THREAD; ++k) {
+        if (base + k < ncells) start[base + k] = run;
+        run += v[k];
     }
 }
 
@@ -318,9 +376,11 @@ void list_prepare(md_list* l, md_state* s) {
         dev_free(l->cell_count);
         dev_free(l->cell_start);
         dev_free(l->scan_tmp);
+        dev_free(l->scan_totals);
         l->cell_count = dev_alloc<uint32_t>(ncells + 1);
         l->cell_start = dev_alloc<uint32_t>(ncells + 1);
         l->scan_tmp = dev_alloc<uint32_t>(ncells + 1);  // per-cell fill counters of the scatter
+        l->scan_totals = dev_alloc<uint32_t>(ncells / 2048 + 2);
         l->ncells_cap = ncells + 1;
         MD_HIP(hipMemsetAsync(l->cell_count, 0, sizeof(uint32_t) * (ncells + 1), st));
         MD_HIP(hipMemsetAsync(l->scan_tmp, 0, sizeof(uint32_t) * (ncells + 1), st));
@@ -351,6 +411,30 @@ void list_prepare(md_list* l, md_state* s) {
         l->rows = dev_alloc<int32_t>(need);
         l->rows_cap = need;
     }
+    if (need > l->rows16_cap || !l->rows16) {
+        dev_free(l->rows16);
+        l->rows16 = dev_alloc<uint16_t>(need);
+        l->rows16_cap = need;
+    }
+    const size_t tiles = static_cast<size_t>(tile_count(s->n ? s->n : 1));
+    if (tiles > l->tiles_cap || !l->tile_nr) {
+        dev_free(l->tile_ranges);
+        dev_free(l->tile_nr);
+        l->tile_ranges = dev_alloc<int2>(tiles * TILE_RANGES);
+        l->tile_nr = dev_alloc<int>(tiles);
+        l->tiles_cap = tiles;
+    }
+    // LDS room for a tile's halo: 1.5x the expected union of the 27-cell stencils of the cells a tile spans
+    // (x | y | z doubles, at most 2560 particles = 60 KB), or the whole system when that is smaller
+    {
+        const double per_cell = static_cast<double>(s->n) / static_cast<double>(ncells);
+        const double est = 1.5 * 9.0 * (TILE_PARTICLES / (per_cell > 0.5 ? per_cell : 0.5) + 3.0) * per_cell;
+        long h = static_cast<long>(std::ceil(est / 8.0)) * 8;
+        h = std::max<long>(256, std::min<long>(2560, h));
+        const long all = static_cast<long>((s->n + 7) / 8 * 8);
+        l->hmax = static_cast<int>(std::min<long>(h, all > 0 ? all : 8));
+        if (static_cast<long>(s->n) <= 2560) l->hmax = static_cast<int>(all > 0 ? all : 8);
+    }
     l->n = s->n;
 }
 
@@ -362,7 +446,12 @@ static void cells_enqueue(md_list* l, md_state* s, double per_length, bool permu
     if (n == 0) return;
     k_cell_count<<<blocks_for(n, 256), 256, 0, st>>>(s->pos, n, per_length, l->side, l->cell_of, l->cell_count,
                                                      gate);
-    k_scan_cells<<<1, 1024, 0, st>>>(l->cell_count, l->ncells, l->cell_start, gate);
+    {
+        const int nb = static_cast<int>((l->ncells + SCAN_ITEMS - 1) / SCAN_ITEMS);
+        k_scan_totals<<<nb, SCAN_THREADS, 0, st>>>(l->cell_count, l->ncells, l->scan_totals, gate);
+        k_scan_top<<<1, 1024, 0, st>>>(l->scan_totals, nb, l->cell_start + l->ncells, gate);
+        k_scan_apply<<<nb, SCAN_THREADS, 0, st>>>(l->cell_count, l->ncells, l->scan_totals, l->cell_start, gate);
+    }
     k_cell_scatter<<<blocks_for(n, 256), 256, 0, st>>>(l->cell_of, n, l->cell_start, l->scan_tmp, l->order_raw,
                                                        gate);
     PermuteArgs pa{};
@@ -405,11 +494,17 @@ static void list_kernel_enqueue(md_list* l, md_state* s, bool from_alt, const in
         cb.cell_of = l->cell_of;
     }
     k_clear_build_flags<<<1, 1, 0, st>>>(l->flags, pair_total_of(l), gate);
+    if (tiling_enabled()) {
+        tiled_list_enqueue(l, s, from_alt, gate);
+        l->tiled = true;
+        return;
+    }
     k_list_build<<<blocks_for(n ? n : 1, 256), 256, 0, st>>>(
         from_alt ? s->pos_alt : s->pos, n, from_alt ? l->cell_of_alt : l->cell_of, l->cell_start, l->side, s->box,
         s->inv_box, l->radius2, l->side >= 3 ? 1 : 0, l->padding, l->rows, l->count, l->half, l->flags,
         pair_total_of(l), cb, gate);
     MD_HIP(hipGetLastError());
+    l->tiled = false;
 }
 
 void cells_build(md_list* l, md_state* s, double min_cell, int side) {
@@ -536,6 +631,7 @@ static void list_fixed_enqueue(md_list* l, md_state* s, const int* gate) {
     const size_t n = s->n;
     cells_enqueue(l, s, l->side / s->box, false, gate);
     k_clear_build_flags<<<1, 1, 0, st>>>(l->flags, pair_total_of(l), gate);
+    l->tiled = false;
     k_list_build_fixed<<<blocks_for(n ? n : 1, 256), 256, 0, st>>>(
         s->pos, n, l->cell_of, l->cell_start, l->order, l->side, s->box, s->inv_box, l->radius2,
         l->side >= 3 ? 1 : 0, l->padding, l->rows, l->count, l->half, l->flags, pair_total_of(l), l->ref, gate);
@@ -582,6 +678,11 @@ static void release_list_buffers(md_list* l) {
     dev_free(l->half);
     dev_free(l->rows);
     dev_free(l->flags);
+    dev_free(l->scan_totals);
+    dev_free(l->rows16);
+    dev_free(l->tile_ranges);
+    dev_free(l->tile_nr);
+    l->rows16_cap = l->tiles_cap = 0;
 }
 
 void set_error(const std::string& m);

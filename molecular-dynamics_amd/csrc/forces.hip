@@ -189,8 +189,13 @@ void forces_enqueue(md_state* s, md_list* l, const md_lj& p, int precision, doub
     hipStream_t st = s->ctx->stream;
     const size_t n = s->n;
     const unsigned blocks = blocks_for(n ? n : 1, FORCE_THREADS);
-    state_reserve_partials(s, static_cast<size_t>(blocks) * 3);
     const LJConst lj = make_lj(p);
+    if (l->tiled) {
+        tiled_forces_enqueue(s, l, lj.sig2, lj.c24, lj.c4, lj.shift, lj.rc2, precision, out_dev, clear_flag,
+                             ev_start, ev_stop);
+        return;
+    }
+    state_reserve_partials(s, static_cast<size_t>(blocks) * 3);
     if (ev_start) MD_HIP(hipEventRecord(ev_start, st));
     if (precision == MD_MIXED)
         k_forces_mixed<<<blocks, FORCE_THREADS, 0, st>>>(s->pos, n, l->rows, l->count, l->half, s->box, s->inv_box,

@@ -127,7 +127,8 @@ struct md_list {
     // cell grid
     uint32_t* cell_count = nullptr;  // ncells
     uint32_t* cell_start = nullptr;  // ncells + 1
-    uint32_t* scan_tmp = nullptr;    // block sums of the scan
+    uint32_t* scan_tmp = nullptr;    // per-cell fill counters of the scatter
+    uint32_t* scan_totals = nullptr; // per-block totals of the cell scan
     size_t scan_cap = 0;
     int* cell_of = nullptr;          // n (cell of particle i in the current order)
     int* cell_of_alt = nullptr;
@@ -139,6 +140,14 @@ struct md_list {
     int32_t* count = nullptr;        // n
     int32_t* half = nullptr;         // n
     int* flags = nullptr;            // FLAG_COUNT ints on the device
+    // tiled view of the same rows (tiled.hip): LDS slots of each tile's staged halo
+    uint16_t* rows16 = nullptr;      // padding * n, column-major
+    size_t rows16_cap = 0;
+    int2* tile_ranges = nullptr;     // tiles * RMAX (first particle, count) of the halo ranges
+    int* tile_nr = nullptr;          // tiles; number of ranges, -1 if the tile is not staged
+    size_t tiles_cap = 0;
+    int hmax = 0;                    // LDS capacity of a tile's halo, in particles
+    bool tiled = false;              // rows16 / tile_* describe the current rows
 };
 
 namespace mdgpu {
@@ -167,6 +176,16 @@ void check_cutoff(const md_state* s, const md_lj& lj);
 void validate_list(const md_state* s, const md_list* l);
 void read_force_result(md_state* s, md_force_result* out);
 void set_error(const std::string& m);
+
+// tiled.hip. The tiled list and force path is the default; MDGPU_UNTILED=1 selects the per-particle global
+// gather kernels instead (same results: the rows are identical).
+bool tiling_enabled();
+int tile_count(size_t n);
+constexpr int TILE_PARTICLES = 128;
+constexpr int TILE_RANGES = 64;
+void tiled_list_enqueue(md_list* l, md_state* s, bool from_alt, const int* gate);
+void tiled_forces_enqueue(md_state* s, md_list* l, double sig2, double c24, double c4, double shift, double rc2,
+                          int precision, double* out_dev, int* clear_flag, hipEvent_t ev_start, hipEvent_t ev_stop);
 
 // integrate.hip
 void kick_drift_enqueue(md_state* s, double dt, md_list* check_list);
