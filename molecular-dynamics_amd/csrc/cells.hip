@@ -411,10 +411,11 @@ void list_prepare(md_list* l, md_state* s) {
         l->rows = dev_alloc<int32_t>(need);
         l->rows_cap = need;
     }
-    if (need > l->rows16_cap || !l->rows16) {
+    const size_t need16 = tiled_rows16_entries(s->n, l->padding);
+    if (need16 > l->rows16_cap || !l->rows16) {
         dev_free(l->rows16);
-        l->rows16 = dev_alloc<uint16_t>(need);
-        l->rows16_cap = need;
+        l->rows16 = dev_alloc<uint16_t>(need16);
+        l->rows16_cap = need16;
     }
     const size_t tiles = static_cast<size_t>(tile_count(s->n ? s->n : 1));
     if (tiles > l->tiles_cap || !l->tile_nr) {

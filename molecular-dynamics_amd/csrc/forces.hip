@@ -190,8 +190,10 @@ void forces_enqueue(md_state* s, md_list* l, const md_lj& p, int precision, doub
     const size_t n = s->n;
     const unsigned blocks = blocks_for(n ? n : 1, FORCE_THREADS);
     const LJConst lj = make_lj(p);
-    if (l->tiled) {
-        tiled_forces_enqueue(s, l, lj.sig2, lj.c24, lj.c4, lj.shift, lj.rc2, precision, out_dev, clear_flag,
+    // mixed precision runs on the tiled rows; FP64 keeps the per-particle gather kernel, whose single lane per
+    // particle sums in the reference's order
+    if (l->tiled && precision == MD_MIXED) {
+        tiled_forces_enqueue(s, l, lj.sig2, lj.c24, lj.c4, lj.shift, lj.rc2, p.cutoff, out_dev, clear_flag,
                              ev_start, ev_stop);
         return;
     }

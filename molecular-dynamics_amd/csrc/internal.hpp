@@ -183,9 +183,11 @@ bool tiling_enabled();
 int tile_count(size_t n);
 constexpr int TILE_PARTICLES = 128;
 constexpr int TILE_RANGES = 64;
+size_t tiled_rows16_entries(size_t n, int padding);
 void tiled_list_enqueue(md_list* l, md_state* s, bool from_alt, const int* gate);
+// mixed-precision forces over the tiled rows
 void tiled_forces_enqueue(md_state* s, md_list* l, double sig2, double c24, double c4, double shift, double rc2,
-                          int precision, double* out_dev, int* clear_flag, hipEvent_t ev_start, hipEvent_t ev_stop);
+                          double cutoff, double* out_dev, int* clear_flag, hipEvent_t ev_start, hipEvent_t ev_stop);
 
 // integrate.hip
 void kick_drift_enqueue(md_state* s, double dt, md_list* check_list);
