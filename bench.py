@@ -142,8 +142,10 @@ def main():
     drift1000, max_drift = drift_stats(all_en, e0, n)
     k_full = 2.0 * info.half_pairs / n
     # force kernel: per atom it reads its position (32 B), its two row lengths (8 B) and its row (4 B per
-    # full-list neighbour), and writes its force (24 B); neighbour positions are re-reads of the same array
-    bytes_per_atom = 32.0 + 8.0 + 4.0 * k_full + 24.0
+    # full-list neighbour as int32, 2 B as the tiled kernel's 16-bit LDS slots), and writes its force (24 B);
+    # neighbour positions are re-reads of the same array
+    tiled = bool(info.tiled) and prec == mdgpu.MD_MIXED
+    bytes_per_atom = 32.0 + 8.0 + (2.0 if tiled else 4.0) * k_full + 24.0
     roofline = None
     if timing:
         avg_ms = timing[0] / args.steps
@@ -156,7 +158,8 @@ def main():
                 traffic = tr.get("hbm_bytes_per_launch")
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                    "kernel": "k_forces_mixed" if prec == mdgpu.MD_MIXED else "k_forces_fp64",
+                    "kernel": ("k_forces_tiled_mixed" if tiled else "k_forces_mixed") if prec == mdgpu.MD_MIXED
+                    else "k_forces_fp64",
                     "avg_launch_ms": round(avg_ms, 5), "algorithmic_bytes_per_launch": round(bytes_per_atom * n)}
     cpu = None
     if not args.no_cpu:

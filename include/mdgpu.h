@@ -73,6 +73,7 @@ typedef struct md_list_info {
     int max_neighbours;    /* longest full-list row at the last build */
     int padding;           /* allocated row length of the padded full list */
     size_t particles;      /* particle count of the last build (rows of the list) */
+    int tiled;             /* 1 when the rows also carry the 16-bit LDS-slot view read by the tiled kernels */
 } md_list_info;
 
 /* ---- host-side input generation (Initialisation.hpp:9-12) --------------------------------------------- */

@@ -421,7 +421,7 @@ void list_prepare(md_list* l, md_state* s) {
     if (tiles > l->tiles_cap || !l->tile_nr) {
         dev_free(l->tile_ranges);
         dev_free(l->tile_nr);
-        l->tile_ranges = dev_alloc<int2>(tiles * TILE_RANGES);
+        l->tile_ranges = dev_alloc<int4>(tiles * TILE_RANGES);
         l->tile_nr = dev_alloc<int>(tiles);
         l->tiles_cap = tiles;
     }
@@ -431,10 +431,10 @@ void list_prepare(md_list* l, md_state* s) {
         const double per_cell = static_cast<double>(s->n) / static_cast<double>(ncells);
         const double est = 1.5 * 9.0 * (TILE_PARTICLES / (per_cell > 0.5 ? per_cell : 0.5) + 3.0) * per_cell;
         long h = static_cast<long>(std::ceil(est / 8.0)) * 8;
-        h = std::max<long>(256, std::min<long>(2560, h));
+        h = std::max<long>(256, std::min<long>(TILE_HALO_MAX, h));
         const long all = static_cast<long>((s->n + 7) / 8 * 8);
         l->hmax = static_cast<int>(std::min<long>(h, all > 0 ? all : 8));
-        if (static_cast<long>(s->n) <= 2560) l->hmax = static_cast<int>(all > 0 ? all : 8);
+        if (static_cast<long>(s->n) <= TILE_HALO_MAX) l->hmax = static_cast<int>(all > 0 ? all : 8);
     }
     l->n = s->n;
 }
@@ -798,6 +798,7 @@ int md_list_get_info(const md_list* l, md_list_info* info) {
     info->max_neighbours = l->max_count;
     info->padding = l->padding;
     info->particles = l->built ? l->n : 0;
+    info->tiled = l->built && l->tiled ? 1 : 0;
     MD_API_END
 }
 

@@ -41,6 +41,17 @@ __device__ __forceinline__ int wrap_index(int c, int side) {
     return c < 0 ? c + side : (c >= side ? c - side : c);
 }
 
+// gfx950 deals the workgroups of a launch to its 8 XCDs round-robin (block b runs on XCD b % 8), and each XCD has
+// its own L2. Mapping block b to item (b % 8) * (items / 8) + b / 8 gives every XCD one contiguous eighth of the
+// items, so neighbouring tiles (which share most of their halo) meet in the same L2. Items past the largest
+// multiple of 8 map to themselves.
+constexpr int XCDS = 8;
+__device__ __forceinline__ int xcd_item(int b, int items) {
+    const int full = items & ~(XCDS - 1);
+    if (b >= full) return b;
+    return (b % XCDS) * (full / XCDS) + b / XCDS;
+}
+
 // Wavefront sum by butterfly shuffles; every lane returns the same value, combined in a fixed order.
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll

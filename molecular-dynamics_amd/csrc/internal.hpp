@@ -143,7 +143,7 @@ struct md_list {
     // tiled view of the same rows (tiled.hip): LDS slots of each tile's staged halo
     uint16_t* rows16 = nullptr;      // padding * n, column-major
     size_t rows16_cap = 0;
-    int2* tile_ranges = nullptr;     // tiles * RMAX (first particle, count) of the halo ranges
+    int4* tile_ranges = nullptr;     // tiles * TILE_RANGES (first cell, first particle, first slot, count)
     int* tile_nr = nullptr;          // tiles; number of ranges, -1 if the tile is not staged
     size_t tiles_cap = 0;
     int hmax = 0;                    // LDS capacity of a tile's halo, in particles
@@ -183,6 +183,7 @@ bool tiling_enabled();
 int tile_count(size_t n);
 constexpr int TILE_PARTICLES = 128;
 constexpr int TILE_RANGES = 64;
+constexpr int TILE_HALO_MAX = 2560;  // LDS capacity bound of a tile's halo, in particles
 size_t tiled_rows16_entries(size_t n, int padding);
 void tiled_list_enqueue(md_list* l, md_state* s, bool from_alt, const int* gate);
 // mixed-precision forces over the tiled rows

@@ -4,39 +4,49 @@
 // A tile is TILE = 128 consecutive particles of the cell-sorted state, worked by 512 threads: LANES = 4 lanes per
 // particle. Its halo is the union of the 27-cell stencils of the cells it spans, kept as at most RMAX ranges of
 // whole cells (each a contiguous run of particle indices, since particles are sorted by cell). At rho* = 0.8442
-// and rc + skin = 2.8 a halo holds ~1700 particles (~2400 when the tile crosses a cell column); the capacity
-// `hmax` (<= 2560, 24 B each = 60 KB) is set from the density, and a tile that does not fit is marked
-// tile_nr = -1 and handled through global gathers.
+// and rc + skin = 2.8 a halo holds ~1500 particles (~2400 when the tile crosses a cell column); the capacity
+// `hmax` (<= TILE_HALO_MAX = 2560, 24 B each = 60 KB) is set from the density, and a tile that does not fit is
+// marked tile_nr = -1 and handled through global gathers.
 //
-// The build writes each row twice: as global indices (int32, column-major: what the reference's half-list view,
-// the RDF, the adjoint and the FP64 force kernel read) and as LDS slots of the tile's halo (uint16, in chunks of
-// LANES entries: entry k of particle i at ((k / 4) * n + i) * 4 + k % 4, so a wavefront of 16 particles x 4 lanes
-// reads 128 contiguous bytes per chunk). Rows keep the reference's order (j > i in 27-cell stencil order, then
-// j < i increasing): the 4 lanes test 4 consecutive candidates and a wavefront ballot compacts the accepted ones
-// in order.
+// A rebuild runs two kernels: k_tile_setup (one wavefront per tile) finds the halo ranges and their LDS slots;
+// k_list_build_tiled stages the halo, tabulates the 27 stencil cells of every cell the tile spans, and tests the
+// candidates. It writes each row twice: as global indices (int32, column-major: what the reference's half-list
+// view, the RDF, the adjoint and the FP64 force kernel read) and as LDS slots of the tile's halo (uint16). The
+// slot rows are stored in chunks of 32 entries per particle, each lane's 8 entries (k = lane, lane + 4, ...)
+// contiguous, so that the force kernel fetches 8 slots with one 16-byte load per lane. Rows keep the reference's
+// order (j > i in 27-cell stencil order, then j < i increasing): the 4 lanes test 4 consecutive candidates and a
+// wavefront ballot compacts the accepted ones in order.
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <type_traits>
 
 #include "device.hpp"
 #include "internal.hpp"
 
 namespace mdgpu {
 
-constexpr int TILE = 128;
+constexpr int TILE = TILE_PARTICLES;
 constexpr int LANES = 4;
 constexpr int TILE_THREADS = TILE * LANES;
-constexpr int RMAX = 64;
+constexpr int RMAX = TILE_RANGES;
 constexpr int CR_CAP = 192;  // cell ranges before merging (9 neighbour columns x 2 wrap pieces x segments)
+constexpr int CMAX = 24;     // own cells of a tile covered by the stencil table
+constexpr int STAGE_ROUNDS = (TILE_HALO_MAX + TILE_THREADS - 1) / TILE_THREADS;
+constexpr int CHUNK = 32;    // slot-row entries per particle and chunk
+constexpr int SETUP_TILES = 4;
+static_assert(RMAX == WAVE, "one lane per halo range in k_tile_setup");
 
-__device__ __forceinline__ size_t slot_index(size_t n, size_t i, int k) {
-    return ((static_cast<size_t>(k) >> 2) * n + i) * LANES + (k & 3);
-}
-
-struct TileArgs {
-    int side;
-    int use_cells;
-    int hmax;  // LDS capacity in particles
+// Three cell coordinates in increasing order, indexed without a private array (which would live in scratch).
+struct Sorted3 {
+    int v0, v1, v2;
+    __device__ Sorted3(int a, int b, int c) {
+        const int lo = min(a, min(b, c)), hi = max(a, max(b, c));
+        v0 = lo;
+        v1 = a + b + c - lo - hi;
+        v2 = hi;
+    }
+    __device__ int operator[](int k) const { return k == 0 ? v0 : (k == 1 ? v1 : v2); }
 };
 
 __device__ __forceinline__ void push_range(int2* cr, int& ncr, int a, int b) {
@@ -93,19 +103,124 @@ __device__ int tile_cell_ranges(int c_first, int c_last, int s, int2* cr) {
     return out <= RMAX ? out : -1;
 }
 
-// LDS slot of the first particle of cell `nc`, by binary search over the merged cell ranges.
-__device__ __forceinline__ int slot_base_of_cell(int nc, const int* ra, const int* rofs, const int* rgs,
-                                                 const uint32_t* __restrict__ start, int nr) {
+// Halo ranges of every tile: one wavefront per tile. Lane 0 merges the cell ranges (a short serial pass; the
+// launch keeps 32 tiles per CU in flight to hide it), then lane r reads the particle span of range r and a
+// wavefront scan assigns the LDS slots. tile_nr = -1 marks a tile whose halo does not fit in `hmax` slots.
+__global__ void __launch_bounds__(SETUP_TILES * WAVE)
+    k_tile_setup(size_t n, const int* __restrict__ cell_of, const uint32_t* __restrict__ start, int side,
+                 int use_cells, int hmax, int tiles, int4* __restrict__ tile_ranges, int* __restrict__ tile_nr,
+                 const int* gate) {
+    if (gate && *gate == 0) return;
+    __shared__ int2 cr_all[SETUP_TILES][CR_CAP];
+    __shared__ int nr_all[SETUP_TILES];
+    const int w = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    const int t = blockIdx.x * SETUP_TILES + w;
+    const bool live = t < tiles;
+    int2* cr = cr_all[w];
+    if (live && lane == 0) {
+        const size_t i0 = static_cast<size_t>(t) * TILE;
+        const size_t i1 = (i0 + TILE) < n ? (i0 + TILE) : n;
+        int nr;
+        if (use_cells) {
+            nr = tile_cell_ranges(cell_of[i0], cell_of[i1 - 1], side, cr);
+        } else {
+            cr[0] = make_int2(0, side * side * side - 1);  // every particle
+            nr = 1;
+        }
+        nr_all[w] = nr;
+    }
+    __syncthreads();
+    if (!live) return;
+    const int nr = nr_all[w];
+    int first_cell = 0, first = 0, cnt = 0;
+    if (lane < nr) {
+        first_cell = cr[lane].x;
+        first = static_cast<int>(start[first_cell]);
+        cnt = static_cast<int>(start[cr[lane].y + 1]) - first;
+    }
+    int incl = cnt;  // inclusive scan of the range lengths
+#pragma unroll
+    for (int d = 1; d < WAVE; d <<= 1) {
+        const int o = __shfl_up(incl, d, WAVE);
+        if (lane >= d) incl += o;
+    }
+    const int total = __shfl(incl, WAVE - 1, WAVE);
+    if (lane < nr)
+        tile_ranges[static_cast<size_t>(t) * RMAX + lane] = make_int4(first_cell, first, incl - cnt, cnt);
+    if (lane == 0) tile_nr[t] = (nr > 0 && total <= hmax) ? nr : -1;
+}
+
+// Tile header shared by the build and the force kernel: the tile's halo ranges in LDS.
+struct HaloRanges {
+    int* ra;    // first cell of each range
+    int* rgs;   // first particle of each range
+    int* rofs;  // first LDS slot of each range; rofs[nr] = halo size
+};
+
+__device__ __forceinline__ void load_ranges(const int4* __restrict__ tile_ranges, int tile, int nr, HaloRanges h) {
+    if (static_cast<int>(threadIdx.x) < nr) {
+        const int4 r = tile_ranges[static_cast<size_t>(tile) * RMAX + threadIdx.x];
+        h.ra[threadIdx.x] = r.x;
+        h.rgs[threadIdx.x] = r.y;
+        h.rofs[threadIdx.x] = r.z;
+        if (static_cast<int>(threadIdx.x) == nr - 1) h.rofs[nr] = r.z + r.w;
+    }
+}
+
+// Global particle index held by LDS slot q of the halo.
+__device__ __forceinline__ int halo_particle(int q, HaloRanges h, int nr) {
     int lo = 0, hi = nr - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
-        if (ra[mid] <= nc)
+        if (h.rofs[mid] <= q)
             lo = mid;
         else
             hi = mid - 1;
     }
-    return rofs[lo] + static_cast<int>(start[nc]) - rgs[lo];
+    return h.rgs[lo] + (q - h.rofs[lo]);
 }
+
+// LDS slot of the first particle of cell `nc` (a cell inside the halo).
+__device__ __forceinline__ int slot_base_of_cell(int nc, HaloRanges h, const uint32_t* __restrict__ start,
+                                                 int nr) {
+    int lo = 0, hi = nr - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (h.ra[mid] <= nc)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return h.rofs[lo] + static_cast<int>(start[nc]) - h.rgs[lo];
+}
+
+// Stages the halo: thread q loads slots q, q + 512, ... (all loads issued before any is used), and hands each
+// loaded position with its slot to `store`.
+template <class Store>
+__device__ __forceinline__ void stage_halo(const double4* __restrict__ pos, HaloRanges h, int nr, int total,
+                                           Store store) {
+    int src[STAGE_ROUNDS];
+#pragma unroll
+    for (int u = 0; u < STAGE_ROUNDS; ++u) {
+        const int q = static_cast<int>(threadIdx.x) + u * TILE_THREADS;
+        src[u] = q < total ? halo_particle(q, h, nr) : -1;
+    }
+    double4 p[STAGE_ROUNDS];
+#pragma unroll
+    for (int u = 0; u < STAGE_ROUNDS; ++u)
+        if (src[u] >= 0) p[u] = pos[src[u]];
+#pragma unroll
+    for (int u = 0; u < STAGE_ROUNDS; ++u)
+        if (src[u] >= 0) store(static_cast<int>(threadIdx.x) + u * TILE_THREADS, p[u]);
+}
+
+struct TileArgs {
+    int side;
+    int use_cells;
+    int hmax;   // LDS capacity in particles
+    int tiles;
+    int interior_ok;  // side >= 7: an interior cell's 27 neighbours are nearer than L/2 along every axis
+};
 
 struct TiledCopyBack {
     int enabled;
@@ -122,61 +237,67 @@ struct TiledCopyBack {
     int* cell_of;
 };
 
+// Stencil table entry of one (own cell, neighbour cell) pair: particles [b, e) of the neighbour cell, LDS slot of
+// its first particle, neighbour cell id.
+using StencilEntry = int4;
+
 __global__ void __launch_bounds__(TILE_THREADS)
     k_list_build_tiled(const double4* __restrict__ pos, size_t n, const int* __restrict__ cell_of,
                        const uint32_t* __restrict__ start, TileArgs ta, double box, double inv_box, double radius2,
                        int padding, int32_t* __restrict__ rows, uint16_t* __restrict__ rows16,
                        int32_t* __restrict__ count, int32_t* __restrict__ half, int* __restrict__ flags,
-                       unsigned long long* __restrict__ pair_total, int2* __restrict__ tile_ranges,
-                       int* __restrict__ tile_nr, TiledCopyBack cb, const int* gate) {
+                       unsigned long long* __restrict__ pair_total, const int4* __restrict__ tile_ranges,
+                       const int* __restrict__ tile_nr, TiledCopyBack cb, const int* gate) {
     if (gate && *gate == 0) return;
     extern __shared__ double lds[];
     double* xs = lds;
     double* ys = lds + ta.hmax;
     double* zs = lds + 2 * ta.hmax;
-    __shared__ int2 cr[CR_CAP];
-    __shared__ int ra[RMAX], rofs[RMAX + 1], rgs[RMAX];
-    __shared__ int s_nr;
-    const size_t i0 = static_cast<size_t>(blockIdx.x) * TILE;
+    __shared__ int ra[RMAX], rgs[RMAX], rofs[RMAX + 1];
+    __shared__ StencilEntry table[CMAX * 27];
+    __shared__ unsigned char sorted_stencil[CMAX * 27];
+    __shared__ int s_outside;
+    const HaloRanges hr{ra, rgs, rofs};
+    const int tile = xcd_item(static_cast<int>(blockIdx.x), ta.tiles);
+    const size_t i0 = static_cast<size_t>(tile) * TILE;
     const size_t i1 = (i0 + TILE) < n ? (i0 + TILE) : n;
     const int s = ta.side;
-    if (threadIdx.x == 0) {
-        int nr;
-        if (ta.use_cells) {
-            nr = tile_cell_ranges(cell_of[i0], cell_of[i1 - 1], s, cr);
-        } else {
-            cr[0] = make_int2(0, s * s * s - 1);  // every particle
-            nr = 1;
-        }
-        int total = 0;
-        if (nr > 0) {
-            for (int r = 0; r < nr; ++r) {
-                ra[r] = cr[r].x;
-                rgs[r] = static_cast<int>(start[cr[r].x]);
-                rofs[r] = total;
-                total += static_cast<int>(start[cr[r].y + 1]) - rgs[r];
-            }
-            rofs[nr] = total;
-            if (total > ta.hmax) nr = -1;
-        }
-        s_nr = nr;
-        tile_nr[blockIdx.x] = nr;
-        for (int r = 0; r < (nr > 0 ? nr : 0); ++r)
-            tile_ranges[static_cast<size_t>(blockIdx.x) * RMAX + r] = make_int2(rgs[r], rofs[r + 1] - rofs[r]);
+    const int nr = tile_nr[tile];
+    const bool staged = nr > 0;
+    const int c_first = ta.use_cells ? cell_of[i0] : 0;
+    const int ncl = ta.use_cells ? cell_of[i1 - 1] - c_first + 1 : 0;
+    const bool use_table = ta.use_cells && ncl <= CMAX;
+    if (threadIdx.x == 0) s_outside = 0;
+    if (staged) load_ranges(tile_ranges, tile, nr, hr);
+    __syncthreads();
+    if (staged) {
+        const int total = rofs[nr];
+        int outside = 0;
+        stage_halo(pos, hr, nr, total, [&](int q, const double4& p) {
+            xs[q] = p.x;
+            ys[q] = p.y;
+            zs[q] = p.z;
+            outside |= !(p.x >= 0.0 && p.x < box && p.y >= 0.0 && p.y < box && p.z >= 0.0 && p.z < box);
+        });
+        if (outside) s_outside = 1;
+    }
+    for (int t = threadIdx.x; use_table && t < ncl * 27; t += TILE_THREADS) {
+        const int cl = t / 27, st = t % 27;
+        const int c = c_first + cl;
+        const int cx = c / (s * s), cy = (c / s) % s, cz = c % s;
+        const int nc = (wrap_index(cx + st / 9 - 1, s) * s + wrap_index(cy + (st / 3) % 3 - 1, s)) * s +
+                       wrap_index(cz + st % 3 - 1, s);
+        const int b = static_cast<int>(start[nc]), e = static_cast<int>(start[nc + 1]);
+        table[t] = make_int4(b, e, staged ? slot_base_of_cell(nc, hr, start, nr) : 0, nc);
     }
     __syncthreads();
-    const int nr = s_nr;
-    const bool staged = nr > 0;
-    if (staged) {
-        for (int r = 0; r < nr; ++r) {
-            const int cnt = rofs[r + 1] - rofs[r];
-            for (int q = threadIdx.x; q < cnt; q += TILE_THREADS) {
-                const double4 p = pos[rgs[r] + q];
-                xs[rofs[r] + q] = p.x;
-                ys[rofs[r] + q] = p.y;
-                zs[rofs[r] + q] = p.z;
-            }
-        }
+    for (int t = threadIdx.x; use_table && t < ncl * 27; t += TILE_THREADS) {
+        // the same 27 cells in increasing id (all distinct for side >= 3)
+        const int cl = t / 27, st = t % 27;
+        const int nc = table[t].w;
+        int rank = 0;
+        for (int q = 0; q < 27; ++q) rank += table[cl * 27 + q].w < nc ? 1 : 0;
+        sorted_stencil[cl * 27 + rank] = static_cast<unsigned char>(st);
     }
     __syncthreads();
     const int lane = threadIdx.x & (LANES - 1);
@@ -197,16 +318,26 @@ __global__ void __launch_bounds__(TILE_THREADS)
             cb.cell_of[i] = cb.cell_of2[i];
         }
         const uint32_t self = static_cast<uint32_t>(i);
-        // candidates j in [b, e) of one cell, 4 at a time; `base` is the LDS slot of j = b
-        auto scan = [&](uint32_t b, uint32_t e, int base) {
+        const uint32_t n32 = static_cast<uint32_t>(n);
+        int32_t* const row = rows + i;
+        uint16_t* const row16 = rows16 + i * CHUNK;
+        const uint64_t chunk_stride = static_cast<uint64_t>(n32) * CHUNK;
+        // Candidates j in [b, e) of one cell, 4 at a time; `base` is the LDS slot of j = b. Staged tiles read the
+        // halo from LDS, the others gather from global memory (a compile-time choice: a run-time select between
+        // the two would turn every read into a generic flat access). With INTERIOR the separation needs no
+        // minimum image: every candidate lies within 3 cells = 3L/side < L/2 along each axis, so round(d / L) = 0
+        // and the reference's d - L * round(d / L) is d itself.
+        auto scan = [&](uint32_t b, uint32_t e, int base, auto staged_tag, auto interior_tag) {
+            constexpr bool STAGED = decltype(staged_tag)::value;
+            constexpr bool INTERIOR = decltype(interior_tag)::value;
             for (uint32_t jb = b; jb < e; jb += LANES) {
                 const uint32_t j = jb + lane;
                 bool accept = false;
                 int slot = 0;
                 if (j < e) {
                     double xj, yj, zj;
-                    slot = base + static_cast<int>(j - b);
-                    if (staged) {
+                    if constexpr (STAGED) {
+                        slot = base + static_cast<int>(j - b);
                         xj = xs[slot];
                         yj = ys[slot];
                         zj = zs[slot];
@@ -216,66 +347,104 @@ __global__ void __launch_bounds__(TILE_THREADS)
                         yj = rj.y;
                         zj = rj.z;
                     }
-                    const double dx = min_image(ri.x - xj, box, inv_box);
-                    const double dy = min_image(ri.y - yj, box, inv_box);
-                    const double dz = min_image(ri.z - zj, box, inv_box);
+                    double dx = ri.x - xj, dy = ri.y - yj, dz = ri.z - zj;
+                    if constexpr (!INTERIOR) {
+                        dx = min_image(dx, box, inv_box);
+                        dy = min_image(dy, box, inv_box);
+                        dz = min_image(dz, box, inv_box);
+                    }
                     accept = dx * dx + dy * dy + dz * dz < radius2;
                 }
                 const unsigned group = static_cast<unsigned>((__ballot(accept) >> group_shift) & 0xFull);
                 if (accept) {
-                    const int at = cnt + __popc(group & lower_mask);
-                    if (at < padding) {
-                        rows[static_cast<size_t>(at) * n + i] = static_cast<int32_t>(j);
-                        rows16[slot_index(n, i, at)] = static_cast<uint16_t>(staged ? slot : 0);
+                    const unsigned at = static_cast<unsigned>(cnt) + __popc(group & lower_mask);
+                    if (at < static_cast<unsigned>(padding)) {
+                        row[static_cast<uint64_t>(at) * n32] = static_cast<int32_t>(j);
+                        row16[static_cast<uint64_t>(at >> 5) * chunk_stride + ((at & 3u) << 3) + ((at >> 2) & 7u)] =
+                            static_cast<uint16_t>(slot);
                     }
                 }
                 cnt += __popc(group);
             }
         };
-        if (ta.use_cells) {
+        auto particle_rows = [&](auto staged_tag) {
+            constexpr bool STAGED = decltype(staged_tag)::value;
             const int c = cell_of[i];
-            const int cx = c / (s * s), cy = (c / s) % s, cz = c % s;
-            for (int ox = -1; ox <= 1; ++ox) {
-                const int x = wrap_index(cx + ox, s);
-                for (int oy = -1; oy <= 1; ++oy) {
-                    const int y = wrap_index(cy + oy, s);
-                    for (int oz = -1; oz <= 1; ++oz) {
-                        const int nc = (x * s + y) * s + wrap_index(cz + oz, s);
-                        if (nc < c) continue;
-                        const int base = staged ? slot_base_of_cell(nc, ra, rofs, rgs, start, nr) : 0;
-                        const uint32_t b = start[nc], e = start[nc + 1];
-                        if (nc == c)
-                            scan(self + 1, e, base + static_cast<int>(self + 1 - b));
+            if (use_table) {
+                const int cl = c - c_first;
+                auto rows_of = [&](auto interior_tag) {
+#pragma unroll 1
+                    for (int st = 0; st < 27; ++st) {  // j > i: the reference's stencil order
+                        const StencilEntry en = table[cl * 27 + st];
+                        if (en.w < c) continue;
+                        if (en.w == c)
+                            scan(self + 1, static_cast<uint32_t>(en.y), en.z + static_cast<int>(self + 1) - en.x,
+                                 staged_tag, interior_tag);
                         else
-                            scan(b, e, base);
+                            scan(static_cast<uint32_t>(en.x), static_cast<uint32_t>(en.y), en.z, staged_tag,
+                                 interior_tag);
                     }
+                    hcnt = cnt;
+#pragma unroll 1
+                    for (int r = 0; r < 27; ++r) {  // j < i: cells in increasing id
+                        const StencilEntry en = table[cl * 27 + sorted_stencil[cl * 27 + r]];
+                        if (en.w > c) break;
+                        scan(static_cast<uint32_t>(en.x), en.w == c ? self : static_cast<uint32_t>(en.y), en.z,
+                             staged_tag, interior_tag);
+                    }
+                };
+                if constexpr (STAGED) {
+                    const int cx = c / (s * s), cy = (c / s) % s, cz = c % s;
+                    const bool interior = ta.interior_ok && !s_outside && cx >= 1 && cx <= s - 2 && cy >= 1 &&
+                                          cy <= s - 2 && cz >= 1 && cz <= s - 2;
+                    if (interior)
+                        rows_of(std::true_type{});
+                    else
+                        rows_of(std::false_type{});
+                } else {
+                    rows_of(std::false_type{});
                 }
+            } else if (ta.use_cells) {
+                const int cx = c / (s * s), cy = (c / s) % s, cz = c % s;
+                auto base_of = [&](int nc) {
+                    if constexpr (STAGED)
+                        return slot_base_of_cell(nc, hr, start, nr);
+                    else
+                        return 0;
+                };
+#pragma unroll 1
+                for (int st = 0; st < 27; ++st) {
+                    const int nc = (wrap_index(cx + st / 9 - 1, s) * s + wrap_index(cy + (st / 3) % 3 - 1, s)) * s +
+                                   wrap_index(cz + st % 3 - 1, s);
+                    if (nc < c) continue;
+                    const int base = base_of(nc);
+                    const uint32_t b = start[nc], e = start[nc + 1];
+                    if (nc == c)
+                        scan(self + 1, e, base + static_cast<int>(self + 1 - b), staged_tag, std::false_type{});
+                    else
+                        scan(b, e, base, staged_tag, std::false_type{});
+                }
+                hcnt = cnt;
+                const Sorted3 xo(wrap_index(cx - 1, s), cx, wrap_index(cx + 1, s));
+                const Sorted3 yo(wrap_index(cy - 1, s), cy, wrap_index(cy + 1, s));
+                const Sorted3 zo(wrap_index(cz - 1, s), cz, wrap_index(cz + 1, s));
+#pragma unroll 1
+                for (int st = 0; st < 27; ++st) {
+                    const int nc = (xo[st / 9] * s + yo[(st / 3) % 3]) * s + zo[st % 3];
+                    if (nc > c) break;
+                    scan(start[nc], nc == c ? self : start[nc + 1], base_of(nc), staged_tag, std::false_type{});
+                }
+            } else {
+                // every particle is in the single staged range, at slot = its index
+                scan(self + 1, n32, static_cast<int>(self + 1), staged_tag, std::false_type{});
+                hcnt = cnt;
+                scan(0, self, 0, staged_tag, std::false_type{});
             }
-            hcnt = cnt;
-            int xo[3] = {wrap_index(cx - 1, s), cx, wrap_index(cx + 1, s)};
-            int yo[3] = {wrap_index(cy - 1, s), cy, wrap_index(cy + 1, s)};
-            int zo[3] = {wrap_index(cz - 1, s), cz, wrap_index(cz + 1, s)};
-            for (int a = 0; a < 2; ++a)
-                for (int k = 0; k < 2 - a; ++k) {
-                    if (xo[k] > xo[k + 1]) { const int t = xo[k]; xo[k] = xo[k + 1]; xo[k + 1] = t; }
-                    if (yo[k] > yo[k + 1]) { const int t = yo[k]; yo[k] = yo[k + 1]; yo[k + 1] = t; }
-                    if (zo[k] > zo[k + 1]) { const int t = zo[k]; zo[k] = zo[k + 1]; zo[k + 1] = t; }
-                }
-            for (int a = 0; a < 3; ++a)
-                for (int b2 = 0; b2 < 3; ++b2)
-                    for (int d = 0; d < 3; ++d) {
-                        const int nc = (xo[a] * s + yo[b2]) * s + zo[d];
-                        if (nc > c) continue;
-                        const int base = staged ? slot_base_of_cell(nc, ra, rofs, rgs, start, nr) : 0;
-                        const uint32_t b = start[nc];
-                        scan(b, nc == c ? self : start[nc + 1], base);
-                    }
-        } else {
-            // every particle is in the single staged range, at slot = its index
-            scan(self + 1, static_cast<uint32_t>(n), static_cast<int>(self + 1));
-            hcnt = cnt;
-            scan(0, self, 0);
-        }
+        };
+        if (staged)
+            particle_rows(std::true_type{});
+        else
+            particle_rows(std::false_type{});
         if (lane == 0) {
             count[i] = cnt < padding ? cnt : padding;
             half[i] = hcnt < padding ? hcnt : padding;
@@ -302,6 +471,7 @@ struct MixedLJ {
     float box_hi, box_lo, inv_box;
     double box, inv_box_d, reach;  // reach: rc + list radius + skin, for the tile compactness test
     double frame;                  // 1.5 * 2^ceil(log2 L): the centre of the tile frame's binade
+    int tiles;
 };
 
 // Position as a float pair in the tile frame: the image of x nearest to `ref`, moved so that `ref` sits at `frame`:
@@ -349,39 +519,38 @@ __device__ __forceinline__ void mixed_pair(float dxh, float dyh, float dzh, floa
     }
 }
 
-// Mixed-precision forces over the tiled list: lane `lane` of particle i takes row entries k = lane, lane + 4, ...
-// Staged tiles read float hi/lo positions from LDS by slot; unstaged tiles gather doubles through the int32 rows.
+// Mixed-precision forces over the tiled list. Lane `lane` of particle i takes row entries k = lane, lane + 4, ...:
+// in staged tiles it reads 8 slots per 16-byte load (the next chunk's load is issued before the current chunk is
+// evaluated) and the float hi/lo positions from LDS; unstaged tiles gather doubles through the int32 rows.
 __global__ void __launch_bounds__(TILE_THREADS)
     k_forces_tiled_mixed(const double4* __restrict__ pos, size_t n, const uint16_t* __restrict__ rows16,
                          const int32_t* __restrict__ rows, const int32_t* __restrict__ count,
-                         const int32_t* __restrict__ half, const int2* __restrict__ tile_ranges,
+                         const int32_t* __restrict__ half, const int4* __restrict__ tile_ranges,
                          const int* __restrict__ tile_nr, int hmax, MixedLJ lj, double* __restrict__ frc,
                          double* __restrict__ partials, int* __restrict__ clear_flag) {
     extern __shared__ float4 ldsf[];
     float4* ha = ldsf;                                      // xh, yh, zh, xl
     float2* hb = reinterpret_cast<float2*>(ldsf + hmax);  // yl, zl
+    __shared__ int ra[RMAX], rgs[RMAX], rofs[RMAX + 1];
     __shared__ float extent[3];
-    const int nr = tile_nr[blockIdx.x];
+    const HaloRanges hr{ra, rgs, rofs};
+    const int tile = xcd_item(static_cast<int>(blockIdx.x), lj.tiles);
+    const int nr = tile_nr[tile];
     const bool staged = nr > 0;
-    const size_t i0 = static_cast<size_t>(blockIdx.x) * TILE;
+    const size_t i0 = static_cast<size_t>(tile) * TILE;
     const double4 ref = pos[i0];
     if (threadIdx.x < 3) extent[threadIdx.x] = 0.0f;
+    if (staged) load_ranges(tile_ranges, tile, nr, hr);
     __syncthreads();
     if (staged) {
-        int off = 0;
-        for (int r = 0; r < nr; ++r) {
-            const int2 rg = tile_ranges[static_cast<size_t>(blockIdx.x) * RMAX + r];
-            for (int q = threadIdx.x; q < rg.y; q += TILE_THREADS) {
-                const double4 p = pos[rg.x + q];
-                float xh, yh, zh, xl, yl, zl;
-                split_near(p.x, ref.x, lj, xh, xl);
-                split_near(p.y, ref.y, lj, yh, yl);
-                split_near(p.z, ref.z, lj, zh, zl);
-                ha[off + q] = make_float4(xh, yh, zh, xl);
-                hb[off + q] = make_float2(yl, zl);
-            }
-            off += rg.y;
-        }
+        stage_halo(pos, hr, nr, rofs[nr], [&](int q, const double4& p) {
+            float xh, yh, zh, xl, yl, zl;
+            split_near(p.x, ref.x, lj, xh, xl);
+            split_near(p.y, ref.y, lj, yh, yl);
+            split_near(p.z, ref.z, lj, zh, zl);
+            ha[q] = make_float4(xh, yh, zh, xl);
+            hb[q] = make_float2(yl, zl);
+        });
     }
     const int lane = threadIdx.x & (LANES - 1);
     const size_t i = i0 + (threadIdx.x >> 2);
@@ -408,29 +577,45 @@ __global__ void __launch_bounds__(TILE_THREADS)
     if (active) {
         const int m = count[i], h = half[i];
         if (staged) {
-            for (int k = lane; k < m; k += LANES) {
-                const int sl = rows16[slot_index(n, i, k)];
-                const float4 a = ha[sl];
-                const float2 b = hb[sl];
-                if (compact)
-                    mixed_pair<false>(xih - a.x, yih - a.y, zih - a.z, xil - a.w, yil - b.x, zil - b.y, lj, k < h, fx,
-                                      fy, fz, acc[0], acc[1], acc[2]);
-                else
-                    mixed_pair<true>(xih - a.x, yih - a.y, zih - a.z, xil - a.w, yil - b.x, zil - b.y, lj, k < h, fx,
-                                     fy, fz, acc[0], acc[1], acc[2]);
-            }
+            auto run = [&](auto wrap) {
+                constexpr bool W = decltype(wrap)::value;
+                const int chunks = (m + CHUNK - 1) / CHUNK;
+                const uint4* rp = reinterpret_cast<const uint4*>(rows16 + i * CHUNK + lane * 8);
+                const size_t stride = n * (CHUNK / 8);  // uint4 words between chunks
+                uint4 cur = chunks > 0 ? rp[0] : make_uint4(0u, 0u, 0u, 0u);
+                for (int c = 0; c < chunks; ++c) {
+                    const uint4 nxt = c + 1 < chunks ? rp[(c + 1) * stride] : make_uint4(0u, 0u, 0u, 0u);
+                    const unsigned words[4] = {cur.x, cur.y, cur.z, cur.w};
+#pragma unroll
+                    for (int t = 0; t < 8; ++t) {
+                        const int k = c * CHUNK + t * LANES + lane;
+                        if (k < m) {
+                            const int sl = static_cast<int>((words[t >> 1] >> ((t & 1) * 16)) & 0xFFFFu);
+                            const float4 a = ha[sl];
+                            const float2 b = hb[sl];
+                            mixed_pair<W>(xih - a.x, yih - a.y, zih - a.z, xil - a.w, yil - b.x, zil - b.y, lj,
+                                          k < h, fx, fy, fz, acc[0], acc[1], acc[2]);
+                        }
+                    }
+                    cur = nxt;
+                }
+            };
+            if (compact)
+                run(std::false_type{});
+            else
+                run(std::true_type{});
         } else {
             const double4 ri = pos[i];
+            float xh, yh, zh, xl, yl, zl;
+            split_near(ri.x, ri.x, lj, xh, xl);
+            split_near(ri.y, ri.y, lj, yh, yl);
+            split_near(ri.z, ri.z, lj, zh, zl);
             for (int k = lane; k < m; k += LANES) {
                 const double4 rj = pos[rows[static_cast<size_t>(k) * n + i]];
                 float xjh, yjh, zjh, xjl, yjl, zjl;
                 split_near(rj.x, ri.x, lj, xjh, xjl);
                 split_near(rj.y, ri.y, lj, yjh, yjl);
                 split_near(rj.z, ri.z, lj, zjh, zjl);
-                float xh, yh, zh, xl, yl, zl;
-                split_near(ri.x, ri.x, lj, xh, xl);
-                split_near(ri.y, ri.y, lj, yh, yl);
-                split_near(ri.z, ri.z, lj, zh, zl);
                 mixed_pair<false>(xh - xjh, yh - yjh, zh - zjh, xl - xjl, yl - yjl, zl - zjl, lj, k < h, fx, fy, fz,
                                   acc[0], acc[1], acc[2]);
             }
@@ -449,7 +634,7 @@ __global__ void __launch_bounds__(TILE_THREADS)
         frc[2 * n + i] = fz;
     }
     if (clear_flag && blockIdx.x == 0 && threadIdx.x == 0) *clear_flag = 0;
-    block_sum<3, TILE_THREADS>(acc, partials + static_cast<size_t>(blockIdx.x) * 3);
+    block_sum<3, TILE_THREADS>(acc, partials + static_cast<size_t>(tile) * 3);
 }
 
 bool tiling_enabled() {
@@ -463,13 +648,14 @@ bool tiling_enabled() {
 int tile_count(size_t n) { return static_cast<int>((n + TILE - 1) / TILE); }
 
 size_t tiled_rows16_entries(size_t n, int padding) {
-    return static_cast<size_t>((padding + LANES - 1) / LANES) * LANES * (n ? n : 1);
+    return static_cast<size_t>((padding + CHUNK - 1) / CHUNK) * CHUNK * (n ? n : 1);
 }
 
 void tiled_list_enqueue(md_list* l, md_state* s, bool from_alt, const int* gate) {
     hipStream_t st = l->ctx->stream;
     const size_t n = s->n;
     if (n == 0) return;
+    if (l->hmax > TILE_HALO_MAX) fail(MD_ERR_STATE, "tile halo capacity exceeds the LDS bound");
     TiledCopyBack cb{};
     cb.enabled = from_alt ? 1 : 0;
     if (from_alt) {
@@ -485,13 +671,17 @@ void tiled_list_enqueue(md_list* l, md_state* s, bool from_alt, const int* gate)
         cb.id = s->id;
         cb.cell_of = l->cell_of;
     }
-    TileArgs ta{l->side, l->side >= 3 ? 1 : 0, l->hmax};
-    hipLaunchKernelGGL(k_list_build_tiled, dim3(tile_count(n)), dim3(TILE_THREADS),
-                       static_cast<size_t>(l->hmax) * 3 * sizeof(double), st, from_alt ? s->pos_alt : s->pos, n,
-                       from_alt ? l->cell_of_alt : l->cell_of, l->cell_start, ta, s->box, s->inv_box, l->radius2,
-                       l->padding, l->rows, l->rows16, l->count, l->half, l->flags,
-                       reinterpret_cast<unsigned long long*>(l->flags + FLAG_COUNT), l->tile_ranges, l->tile_nr, cb,
+    const int tiles = tile_count(n);
+    TileArgs ta{l->side, l->side >= 3 ? 1 : 0, l->hmax, tiles, l->side >= 7 ? 1 : 0};
+    const int* cells = from_alt ? l->cell_of_alt : l->cell_of;
+    hipLaunchKernelGGL(k_tile_setup, dim3((tiles + SETUP_TILES - 1) / SETUP_TILES), dim3(SETUP_TILES * WAVE), 0, st,
+                       n, cells, l->cell_start, ta.side, ta.use_cells, ta.hmax, tiles, l->tile_ranges, l->tile_nr,
                        gate);
+    hipLaunchKernelGGL(k_list_build_tiled, dim3(tiles), dim3(TILE_THREADS),
+                       static_cast<size_t>(l->hmax) * 3 * sizeof(double), st, from_alt ? s->pos_alt : s->pos, n,
+                       cells, l->cell_start, ta, s->box, s->inv_box, l->radius2, l->padding, l->rows, l->rows16,
+                       l->count, l->half, l->flags, reinterpret_cast<unsigned long long*>(l->flags + FLAG_COUNT),
+                       l->tile_ranges, l->tile_nr, cb, gate);
     MD_HIP(hipGetLastError());
 }
 
@@ -514,6 +704,7 @@ void tiled_forces_enqueue(md_state* s, md_list* l, double sig2, double c24, doub
     lj.inv_box = static_cast<float>(s->inv_box);
     lj.reach = cutoff + l->radius + l->skin;
     lj.frame = 1.5 * std::exp2(std::ceil(std::log2(s->box)));
+    lj.tiles = tiles;
     if (ev_start) MD_HIP(hipEventRecord(ev_start, st));
     if (n > 0) {
         const size_t lds = static_cast<size_t>(l->hmax) * (sizeof(float4) + sizeof(float2));

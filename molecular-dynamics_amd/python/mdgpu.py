@@ -36,7 +36,8 @@ class ForceResult(C.Structure):
 
 class ListInfo(C.Structure):
     _fields_ = [("half_pairs", C.c_size_t), ("rebuilds", C.c_int), ("cells_per_side", C.c_int),
-                ("max_neighbours", C.c_int), ("padding", C.c_int), ("particles", C.c_size_t)]
+                ("max_neighbours", C.c_int), ("padding", C.c_int), ("particles", C.c_size_t),
+                ("tiled", C.c_int)]
 
 
 class RunConfig(C.Structure):
