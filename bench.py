@@ -94,12 +94,154 @@ def calibrate_copy(nbytes):
     print(json.dumps({"copy_bytes": nbytes, "launches": 5}))
 
 
+def batched_replicas(count, base_seed=1):
+    """The batched workload: replica r has FCC n_r ~ U{5..20} cells (N = 4 n^3), rho ~ U[0.70, 0.90],
+    T ~ U[0.7, 1.5] and seed base + r, drawn from numpy's PCG64 seeded with the base seed."""
+    rng = np.random.default_rng(base_seed)
+    cells = rng.integers(5, 21, size=count)
+    rho = rng.uniform(0.70, 0.90, size=count)
+    temp = rng.uniform(0.7, 1.5, size=count)
+    return [(int(cells[r]), float(rho[r]), float(temp[r]), base_seed + r) for r in range(count)]
+
+
+def run_batched(args):
+    """Batched replicas (Bussi tau 0.1, mixed, skin 0.3) in one md_batch against the same replicas advanced one
+    after another by md_run (each captured as a hipGraph, the fastest plain path)."""
+    ctx = mdgpu.Context(0)
+    lj = mdgpu.LJ(1.0, 1.0, 2.5)
+    specs = batched_replicas(args.replicas)
+    steps, warm = args.steps, args.warmup
+    t_setup = time.perf_counter()
+    states = [mdgpu.fcc_system(ctx, c, rho, t, seed) for c, rho, t, seed in specs]
+    n_list = [s.n for s in states]
+    n_total = sum(n_list)
+    temps = np.array([t for _, _, t, _ in specs])
+    rngs = [mdgpu.BussiRng(seed) for *_, seed in specs]
+    draws_w = np.stack([rngs[r].draws(n_list[r], max(warm, 1)) for r in range(len(specs))])
+    draws = np.stack([rngs[r].draws(n_list[r], steps) for r in range(len(specs))])
+    batch = mdgpu.Batch(ctx, states, 2.5, 0.3)
+    log(f"batched setup: {len(specs)} replicas, N_total={n_total} in {time.perf_counter() - t_setup:.1f} s")
+    best, tune_ms = batch.tune(lj, 0.005, max(warm, 1), min(20, max(warm, 1)), precision=mdgpu.MD_MIXED,
+                               temperatures=temps, draws=draws_w)
+    log(f"autotuner: lanes {best}, ms/step for 1/2/4 lanes: {np.round(tune_ms, 3).tolist()}")
+    if warm:
+        batch.run(lj, 0.005, warm, precision=mdgpu.MD_MIXED, temperatures=temps, draws=draws_w, use_graph=True,
+                  record=False)
+    ctx.synchronize()
+    reb0 = batch.info().rebuilds
+    t0 = time.perf_counter()
+    en = batch.run(lj, 0.005, steps, precision=mdgpu.MD_MIXED, temperatures=temps, draws=draws, use_graph=True)
+    ctx.synchronize()
+    elapsed = time.perf_counter() - t0
+    info = batch.info()
+    value = n_total * steps / elapsed
+    log(f"batched: {steps} steps in {elapsed:.3f} s: {value:.4g} atom-steps/s, {info.rebuilds - reb0} rebuilds")
+    # force-kernel roofline from an eager stretch of the same run (event-timed)
+    k_steps = min(50, steps)
+    draws_k = np.stack([rngs[r].draws(n_list[r], k_steps) for r in range(len(specs))])
+    timing = []
+    batch.run(lj, 0.005, k_steps, precision=mdgpu.MD_MIXED, temperatures=temps, draws=draws_k, timing=timing,
+              record=False)
+    k_full = 2.0 * info.half_pairs / n_total
+    bytes_per_atom = 32.0 + 8.0 + 4.0 * k_full + 24.0
+    avg_ms = timing[0] / k_steps
+    achieved = bytes_per_atom * n_total / (avg_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_b_forces",
+                "avg_launch_ms": round(avg_ms, 5), "algorithmic_bytes_per_launch": round(bytes_per_atom * n_total)}
+    # plain: the first `plain_replicas` replicas one after another, each its own md_run
+    sample = specs[:args.plain_replicas]
+    p_steps = min(steps, args.plain_steps)
+    plain_states = [mdgpu.fcc_system(ctx, c, rho, t, seed) for c, rho, t, seed in sample]
+    lists = []
+    for st in plain_states:
+        nl = mdgpu.NeighbourList(ctx, 2.5, 0.3)
+        nl.build(st)
+        nl.forces(st, lj, mdgpu.MD_MIXED)
+        lists.append(nl)
+    p_draws = [mdgpu.BussiRng(seed).draws(st.n, p_steps) for st, (*_, seed) in zip(plain_states, sample)]
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for st, nl, d, (_, _, t, _) in zip(plain_states, lists, p_draws, sample):
+        nl.run(st, lj, 0.005, p_steps, precision=mdgpu.MD_MIXED, draws=d, temperature=t, tau=0.1, use_graph=True,
+               record=False)
+    ctx.synchronize()
+    p_el = time.perf_counter() - t0
+    p_atoms = sum(st.n for st in plain_states)
+    plain = p_atoms * p_steps / p_el
+    log(f"plain replica-by-replica: {len(sample)} replicas x {p_steps} steps in {p_el:.3f} s: {plain:.4g}")
+    out = {
+        "metric": "batched atom-steps/s (512 Bussi replicas, 1 GPU)" if args.replicas == 512
+        else "batched atom-steps/s (1 GPU)",
+        "value": value, "unit": "atom-steps/s", "n_gpus": 1, "steps": steps, "warmup": warm,
+        "ms_per_step": 1000.0 * elapsed / steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 pair maths, f64 positions/accumulation",
+        "data": "synthetic: per-replica FCC lattice + Maxwell-Boltzmann velocities generated by the library",
+        "config": {"workload": f"{len(specs)} LJ replicas, FCC n~U{{5..20}}, rho~U[0.70,0.90], T~U[0.7,1.5], "
+                               f"Bussi tau=0.1, seed 1+r, rc=2.5 skin=0.3 dt=0.005 mixed",
+                   "replicas": len(specs), "N_total": n_total, "lanes": best, "graph": True},
+        "roofline": roofline, "cpu_baseline": None,
+        "plain_replica_by_replica": {"value": plain, "unit": "atom-steps/s", "replicas": len(sample),
+                                     "steps": p_steps, "path": "md_run per replica, hipGraph"},
+        "speedup_vs_plain": value / plain,
+        "autotuner_ms_per_step": {"lanes1": tune_ms[0], "lanes2": tune_ms[1], "lanes4": tune_ms[2]},
+        "rebuilds": int(info.rebuilds - reb0), "k_full_mean": k_full,
+        "mean_final_temperature_over_target": float(np.mean(
+            2.0 * en[-1, :, 1] / (3.0 * np.array(n_list) - 3.0) / temps)),
+    }
+    print(json.dumps(out), flush=True)
+
+
+def run_adjoint(args):
+    """Adjoint of the 32k-atom FP64 NVE run (FCC 20^3 x 4, rho 0.8442, T 0.9, seed 1, skin 0.3, dt 0.005),
+    checkpoints every 50 steps: reverse-sweep time over forward time, and over a plain md_run of the same steps."""
+    ctx = mdgpu.Context(0)
+    lj = mdgpu.LJ(1.0, 1.0, 2.5)
+    steps = args.steps
+    state = mdgpu.fcc_system(ctx, 20, 0.8442, 0.9, 1)
+    nl = mdgpu.NeighbourList(ctx, 2.5, 0.3)
+    nl.build(state)
+    nl.forces(state, lj, mdgpu.MD_FP64)
+    # warm-up of both paths on a short horizon
+    mdgpu.adjoint_run(state, nl, lj, 0.005, 50, 50)
+    t0 = time.perf_counter()
+    res, gp, gv = mdgpu.adjoint_run(state, nl, lj, 0.005, steps, 50)
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    twin = mdgpu.fcc_system(ctx, 20, 0.8442, 0.9, 1)
+    nl2 = mdgpu.NeighbourList(ctx, 2.5, 0.3)
+    nl2.build(twin)
+    nl2.forces(twin, lj, mdgpu.MD_FP64)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    nl2.run(twin, lj, 0.005, steps, precision=mdgpu.MD_FP64, use_graph=True, record=False)
+    ctx.synchronize()
+    plain_ms = 1000.0 * (time.perf_counter() - t0)
+    log(f"adjoint: forward {res.forward_ms:.1f} ms, reverse {res.reverse_ms:.1f} ms, plain md_run {plain_ms:.1f} ms")
+    out = {
+        "metric": "adjoint time / forward time (N=32k, 2000 steps, checkpoints every 50)",
+        "value": res.reverse_ms / res.forward_ms, "unit": "ratio", "n_gpus": 1, "steps": steps, "warmup": 50,
+        "ms_per_step": (res.forward_ms + res.reverse_ms) / steps, "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic: FCC lattice + Maxwell-Boltzmann velocities",
+        "config": {"workload": "LJ NVE N=32,000 (FCC 20^3x4) rho*=0.8442 T*=0.9 seed 1 rc=2.5 skin=0.3 dt=0.005 "
+                               "fp64, loss = mean U + (mean P - 1)^2, checkpoints every 50 steps", "N": state.n},
+        "roofline": None, "cpu_baseline": None,
+        "forward_ms": res.forward_ms, "reverse_ms": res.reverse_ms, "wall_ms": 1000.0 * wall,
+        "plain_md_run_ms": plain_ms, "reverse_over_plain_forward": res.reverse_ms / plain_ms,
+        "loss": res.loss, "d_epsilon": res.d_epsilon, "d_sigma": res.d_sigma,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=100)
-    ap.add_argument("--workload", default="lj4m", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="lj4m", choices=sorted(WORKLOADS) + ["batched", "adjoint"])
+    ap.add_argument("--replicas", type=int, default=512, help="batched workload: replica count")
+    ap.add_argument("--plain-replicas", type=int, default=32, help="batched workload: replicas in the plain sample")
+    ap.add_argument("--plain-steps", type=int, default=200, help="batched workload: steps of the plain sample")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--graph", action="store_true", help="replay a captured step instead of eager launches")
@@ -111,6 +253,12 @@ def main():
         sys.exit(2)
     if args.calibrate_copy:
         calibrate_copy(args.calibrate_copy)
+        return
+    if args.workload == "batched":
+        run_batched(args)
+        return
+    if args.workload == "adjoint":
+        run_adjoint(args)
         return
 
     cells, rho, temp, seed, skin, prec, desc = WORKLOADS[args.workload]

@@ -213,6 +213,58 @@ typedef struct md_adjoint_result {
 int md_adjoint_run(md_state* s, md_list* l, const md_adjoint_config* cfg, md_adjoint_result* out,
                    double* grad_positions, double* grad_velocities);
 
+/* ---- Batched replicas (no counterpart in the reference) -------------------------------------------------- */
+/* Many independent Systems, each with its own box, cell grid and neighbour list, advanced together: every
+   kernel launch covers all replicas, one step is one fixed sequence of launches (optionally one hipGraph
+   replay), and each replica re-sorts and rebuilds on its own half-skin trigger. Per replica the step is the
+   one md_run performs on a single state (same arithmetic in FP64: the trajectories are bit-equal). It stands
+   for the per-replica loop of the reference's validation driver (bench/validate.py:96-120), which runs the
+   single-System API once per replica. */
+typedef struct md_batch md_batch;
+
+typedef struct md_batch_config {
+    double dt;
+    int precision;               /* md_precision */
+    int steps;
+    md_lj lj;
+    int thermostat;              /* 0 = NVE, 1 = Bussi on every replica */
+    double tau;
+    const double* temperatures;  /* [count] Bussi targets (thermostat only) */
+    const double* draws;         /* [count][steps][2] Bussi draws (r1, sum_sq), replica-major (thermostat only) */
+    int use_graph;               /* capture one batched step as a hipGraph and replay it */
+    int lanes;                   /* lanes per particle of the force kernel: 1, 2 or 4 (MIXED only; FP64 uses 1
+                                    to keep the reference's summation order); 0 = the batch's tuned value */
+    double* force_kernel_ms;     /* optional out (eager runs only): summed duration of the force kernel launches */
+} md_batch_config;
+
+typedef struct md_batch_info {
+    int count;                   /* replicas */
+    size_t particles;            /* sum of the replicas' particle counts */
+    size_t slots;                /* particle slots allocated (each replica starts on a 256-slot boundary) */
+    int padding;                 /* padded row length shared by every replica's list */
+    int lanes;                   /* tuned lanes per particle of the mixed force kernel */
+    long long rebuilds;          /* list rebuilds summed over replicas, including the initial ones */
+    size_t half_pairs;           /* half-list pairs summed over replicas, as of each replica's last build */
+} md_batch_info;
+
+/* Copies the current state of each states[r] (which stays owned by the caller) into one batch, builds every
+   replica's cell grid and list with the given cutoff and skin (re-sorting each replica, as md_list_build does),
+   and fails with the reference's message if some box is shorter than twice cutoff + skin. */
+int md_batch_create(md_ctx* ctx, md_state* const* states, int count, double cutoff, double skin, md_batch** out);
+int md_batch_destroy(md_batch* b);
+/* Advances every replica cfg->steps steps. The first run of a batch first evaluates the forces of the built
+   state. energies (host, may be NULL): [steps][count][4] doubles (potential, kinetic, virial, inside pairs), the
+   kinetic energy as md_run reports it. */
+int md_batch_run(md_batch* b, const md_batch_config* cfg, double* energies);
+int md_batch_get_info(const md_batch* b, md_batch_info* info);
+/* Copies replica r's current state (its particle order after the re-sorts) into dst, which must hold as many
+   particles; rebuilds and half_pairs (may be NULL) receive that replica's counters. */
+int md_batch_replica(md_batch* b, int r, md_state* dst, int* rebuilds, size_t* half_pairs);
+/* Autotuner: times `trial_steps` steps of cfg for 1, 2 and 4 lanes per particle on a copy of the batch (the
+   batch itself does not move), keeps the fastest as the batch's lanes, and writes the three timings (ms per
+   step) to ms_per_step[3] (may be NULL). MIXED only. */
+int md_batch_tune(md_batch* b, const md_batch_config* cfg, int trial_steps, double* ms_per_step, int* best_lanes);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,1096 @@
+// Batched replicas: many independent Systems advanced together, every kernel launch covering all of them.
+//
+// Layout. The replicas' particles are concatenated: replica r owns slots [first_r, first_r + n_r), first_r a
+// multiple of BT = 256, so every 256-slot block belongs to exactly one replica (block_rep[]). Every per-particle
+// array has `slots` entries; the component-major velocity and force arrays have stride `slots`. Cells are
+// concatenated the same way (replica r owns cell words [cell_first_r, cell_first_r + ncells_r], the last one
+// its end offset; cell_rep[] maps a word back to its replica) and the cell start offsets hold batch slots, so
+// list rows hold batch slots too. One padded full list serves all replicas: padding shared, column-major with
+// stride `slots`.
+//
+// A step is one fixed sequence of launches (and so one hipGraph): k_b_kick_drift raises a replica's stale flag
+// (NeighbourList::update's half-skin test) → the rebuild kernels, each exiting at once for a replica that is not
+// stale → k_b_forces → kick and kinetic energy (+ Bussi) → k_b_record, which also clears the stale flags. Each
+// kernel repeats the arithmetic of the single-system kernel for the same phase (cells.hip, forces.hip,
+// integrate.hip, api.hip), so an FP64 replica follows exactly the trajectory md_run gives its System.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "device.hpp"
+#include "internal.hpp"
+#include "pair.hpp"
+
+namespace mdgpu {
+
+constexpr int BT = 256;          // particle slots per block; replicas start on block boundaries
+constexpr int KE_TERMS = 1024;   // the reference's deterministicSum block (Parallel.hpp:32-57)
+constexpr int SCALARS = 8;       // per replica: U, W, inside, KE, -, alpha, KE', -
+
+struct Rep {
+    uint32_t first, n;            // particle slots [first, first + n)
+    uint32_t cell_first, ncells;  // cell words [cell_first, cell_first + ncells]
+    int side, use_cells;
+    double box, inv_box, per_length;
+    uint32_t ke_first, ke_blocks; // reference-order kinetic energy blocks of 1024 particles
+};
+
+struct BatchPermute {
+    const double4* pos;
+    const double* vel;
+    const double* frc;
+    const double* mass;
+    const uint64_t* id;
+    double4* pos2;
+    double* vel2;
+    double* frc2;
+    double* mass2;
+    uint64_t* id2;
+    double4* ref;
+    int* cell_of2;
+};
+
+struct BatchCopyBack {
+    const double* vel2;
+    const double* frc2;
+    const double* mass2;
+    const uint64_t* id2;
+    double4* pos;
+    double* vel;
+    double* frc;
+    double* mass;
+    uint64_t* id;
+};
+
+// ---- integration -----------------------------------------------------------------------------------------
+
+// k_kick_drift per replica box; the staleness test raises stale[r].
+__global__ void __launch_bounds__(BT)
+    k_b_kick_drift(const Rep* __restrict__ reps, const uint16_t* __restrict__ block_rep, double4* __restrict__ pos,
+                   double* __restrict__ vel, const double* __restrict__ frc, const double* __restrict__ mass,
+                   size_t slots, double half_dt, double dt, const double4* __restrict__ ref, double max_disp2,
+                   int* __restrict__ stale) {
+    const int r = block_rep[blockIdx.x];
+    const Rep rp = reps[r];
+    const size_t i = static_cast<size_t>(blockIdx.x) * BT + threadIdx.x;
+    if (i >= static_cast<size_t>(rp.first) + rp.n) return;
+    const double m = mass[i];
+    double4 p = pos[i];
+    double v[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const double kick = half_dt * frc[c * slots + i];
+        v[c] = vel[c * slots + i] + kick / m;
+        vel[c * slots + i] = v[c];
+    }
+    p.x = wrap_coord(p.x + dt * v[0], rp.box, rp.inv_box);
+    p.y = wrap_coord(p.y + dt * v[1], rp.box, rp.inv_box);
+    p.z = wrap_coord(p.z + dt * v[2], rp.box, rp.inv_box);
+    pos[i] = p;
+    const double4 q = ref[i];
+    const double dx = min_image(p.x - q.x, rp.box, rp.inv_box);
+    const double dy = min_image(p.y - q.y, rp.box, rp.inv_box);
+    const double dz = min_image(p.z - q.z, rp.box, rp.inv_box);
+    if (dx * dx + dy * dy + dz * dz > max_disp2) stale[r] = 1;
+}
+
+// k_kick; with `partials`, per-block sums of (0.5 m) v^2 (blocks never straddle replicas).
+__global__ void __launch_bounds__(BT)
+    k_b_kick(const Rep* __restrict__ reps, const uint16_t* __restrict__ block_rep, double* __restrict__ vel,
+             const double* __restrict__ frc, const double* __restrict__ mass, size_t slots, double half_dt,
+             double* __restrict__ partials) {
+    const Rep rp = reps[block_rep[blockIdx.x]];
+    const size_t i = static_cast<size_t>(blockIdx.x) * BT + threadIdx.x;
+    double ke[1] = {0.0};
+    if (i < static_cast<size_t>(rp.first) + rp.n) {
+        const double m = mass[i];
+        double v[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const double kick = half_dt * frc[c * slots + i];
+            v[c] = vel[c * slots + i] + kick / m;
+            vel[c * slots + i] = v[c];
+        }
+        const double half_m = 0.5 * m;
+        ke[0] = half_m * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    }
+    if (partials) block_sum<1, BT>(ke, partials + blockIdx.x);
+}
+
+// Per replica, in block order: the kick's kinetic-energy partials -> scalars[3].
+__global__ void k_b_reduce_ke(const Rep* __restrict__ reps, int count, const double* __restrict__ partials,
+                              double* __restrict__ scalars) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= count) return;
+    const Rep rp = reps[r];
+    const uint32_t b0 = rp.first / BT, b1 = (rp.first + rp.n + BT - 1) / BT;
+    double s = 0.0;
+    for (uint32_t b = b0; b < b1; ++b) s += partials[b];
+    scalars[r * SCALARS + 3] = s;
+}
+
+// Reference-order kinetic energy (k_ke_blocks / k_ke_final per replica): 1024-particle blocks summed
+// sequentially, then the block sums sequentially.
+__global__ void __launch_bounds__(BT)
+    k_b_ke_blocks(const Rep* __restrict__ reps, const uint16_t* __restrict__ block_rep,
+                  const uint32_t* __restrict__ ke_start, const double* __restrict__ vel,
+                  const double* __restrict__ mass, size_t slots, double* __restrict__ ke_part) {
+    __shared__ double terms[KE_TERMS];
+    const uint32_t base = ke_start[blockIdx.x];
+    const Rep rp = reps[block_rep[base / BT]];
+    const uint32_t end = rp.first + rp.n;
+    for (int t = threadIdx.x; t < KE_TERMS; t += BT) {
+        const size_t i = static_cast<size_t>(base) + t;
+        double term = 0.0;
+        if (i < end) {
+            const double vx = vel[i], vy = vel[slots + i], vz = vel[2 * slots + i];
+            const double half_m = 0.5 * mass[i];
+            term = half_m * (vx * vx + vy * vy + vz * vz);
+        }
+        terms[t] = term;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t cnt = (end - base) < static_cast<uint32_t>(KE_TERMS) ? (end - base) : KE_TERMS;
+        double s = 0.0;
+        for (uint32_t t = 0; t < cnt; ++t) s += terms[t];
+        ke_part[blockIdx.x] = s;
+    }
+}
+
+__global__ void k_b_ke_final(const Rep* __restrict__ reps, int count, const double* __restrict__ ke_part,
+                             double* __restrict__ scalars) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= count) return;
+    const Rep rp = reps[r];
+    double total = 0.0;
+    for (uint32_t b = 0; b < rp.ke_blocks; ++b) total += ke_part[rp.ke_first + b];
+    scalars[r * SCALARS + 3] = total;
+}
+
+// k_bussi per replica: eq. A7 from the replica's KE and its own draw stream (consumed only when KE > 0).
+__global__ void k_b_bussi(int count, double* __restrict__ scalars, const double* __restrict__ draws, int steps,
+                          int* __restrict__ draw_index, const double2* __restrict__ dof_kbar, double c) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= count) return;
+    double* sc = scalars + r * SCALARS;
+    const double ke = sc[3];
+    if (!(ke > 0.0)) {
+        sc[5] = 1.0;
+        sc[6] = ke;
+        return;
+    }
+    const double dof = dof_kbar[r].x, kbar = dof_kbar[r].y;
+    const int k = draw_index[r];
+    draw_index[r] = k + 1;
+    const double* d = draws + (static_cast<size_t>(r) * steps + k) * 2;
+    const double r1 = d[0], sum_sq = d[1];
+    const double relax = (1.0 - c) * (kbar * (sum_sq + r1 * r1) / dof - ke);
+    const double noise = 2.0 * r1 * sqrt(kbar / dof * ke * (1.0 - c) * c);
+    const double ke_new = ke + relax + noise;
+    sc[5] = sqrt(ke_new / ke);
+    sc[6] = ke_new;
+}
+
+__global__ void __launch_bounds__(BT)
+    k_b_scale(const Rep* __restrict__ reps, const uint16_t* __restrict__ block_rep, double* __restrict__ vel,
+              size_t slots, const double* __restrict__ scalars) {
+    const int r = block_rep[blockIdx.x];
+    const Rep rp = reps[r];
+    const size_t i = static_cast<size_t>(blockIdx.x) * BT + threadIdx.x;
+    if (i >= static_cast<size_t>(rp.first) + rp.n) return;
+    const double s = scalars[r * SCALARS + 5];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) vel[c * slots + i] = vel[c * slots + i] * s;
+}
+
+// Per-step record of every replica, the rebuild counters, and the stale flags cleared for the next step.
+// One block; flags[2] is the step counter.
+__global__ void __launch_bounds__(1024)
+    k_b_record(int count, const double* __restrict__ scalars, int ke_slot, double* __restrict__ energies,
+               int* __restrict__ stale, int* __restrict__ rebuilds, int* __restrict__ flags) {
+    const int k = flags[2];
+    for (int r = threadIdx.x; r < count; r += blockDim.x) {
+        const double* sc = scalars + r * SCALARS;
+        double* e = energies + (static_cast<size_t>(k) * count + r) * 4;
+        e[0] = sc[0];
+        e[1] = sc[ke_slot];
+        e[2] = sc[1];
+        e[3] = sc[2];
+        rebuilds[r] += stale[r];
+        stale[r] = 0;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) flags[2] = k + 1;
+}
+
+// ---- cells and list (each kernel exits at once for a replica that is not stale) --------------------------
+
+__global__ void __launch_bounds__(BT)
+    k_b_cell_count(const Rep* __restrict__ reps, const uint16_t* __restrict__ block_rep,
+                   const double4* __restrict__ pos, int* __restrict__ cell_of, uint32_t* __restrict__ counts,
+                   const int* __restrict__ stale) {
+    const int r = block_rep[blockIdx.x];
+    if (!stale[r]) return;
+    const Rep rp = reps[r];
+    const size_t i = static_cast<size_t>(blockIdx.x) * BT + threadIdx.x;
+    if (i >= static_cast<size_t>(rp.first) + rp.n) return;
+    const double4 p = pos[i];
+    const int c = cell_of_position(p.x, p.y, p.z, rp.per_length, rp.side);
+    cell_of[i] = c;
+    atomicAdd(&counts[rp.cell_first + c], 1u);
+}
+
+// One block per replica: exclusive scan of its histogram into batch slots; also clears its pair total.
+__global__ void __launch_bounds__(BT)
+    k_b_scan(const Rep* __restrict__ reps, const uint32_t* __restrict__ counts, uint32_t* __restrict__ start,
+             const int* __restrict__ stale, unsigned long long* __restrict__ pairs) {
+    const int r = blockIdx.x;
+    if (!stale[r]) return;
+    const Rep rp = reps[r];
+    __shared__ uint32_t wsum[BT / WAVE];
+    __shared__ uint32_t carry;
+    if (threadIdx.x == 0) {
+        carry = 0;
+        pairs[r] = 0ull;
+    }
+    __syncthreads();
+    const int lane = threadIdx.x % WAVE, wave = threadIdx.x / WAVE;
+    for (uint32_t b0 = 0; b0 < rp.ncells; b0 += BT) {
+        const uint32_t c = b0 + threadIdx.x;
+        const uint32_t v = c < rp.ncells ? counts[rp.cell_first + c] : 0u;
+        uint32_t incl = v;
+#pragma unroll
+        for (int off = 1; off < WAVE; off <<= 1) {
+            const uint32_t o = __shfl_up(incl, off, WAVE);
+            if (lane >= off) incl += o;
+        }
+        if (lane == WAVE - 1) wsum[wave] = incl;
+        __syncthreads();
+        uint32_t before = carry;
+        for (int w = 0; w < wave; ++w) before += wsum[w];
+        if (c < rp.ncells) start[rp.cell_first + c] = rp.first + before + incl - v;
+        __syncthreads();
+        if (threadIdx.x == BT - 1) carry = before + incl;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) start[rp.cell_first + rp.ncells] = rp.first + rp.n;
+}
+
+__global__ void __launch_bounds__(BT)
+    k_b_scatter(const Rep* __restrict__ reps, const uint16_t* __restrict__ block_rep,
+                const int* __restrict__ cell_of, const uint32_t* __restrict__ start, uint32_t* __restrict__ fill,
+                uint32_t* __restrict__ order_raw, const int* __restrict__ stale) {
+    const int r = block_rep[blockIdx.x];
+    if (!stale[r]) return;
+    const Rep rp = reps[r];
+    const size_t i = static_cast<size_t>(blockIdx.x) * BT + threadIdx.x;
+    if (i >= static_cast<size_t>(rp.first) + rp.n) return;
+    const uint32_t cw = rp.cell_first + static_cast<uint32_t>(cell_of[i]);
+    const uint32_t slot = atomicAdd(&fill[cw], 1u);
+    order_raw[start[cw] + slot] = static_cast<uint32_t>(i);
+}
+
+// k_cell_sort over every replica's cells: one wavefront per cell word.
+__global__ void __launch_bounds__(BT)
+    k_b_sort(const Rep* __restrict__ reps, const uint16_t* __restrict__ cell_rep, size_t cells_total, size_t slots,
+             const uint32_t* __restrict__ start, const uint32_t* __restrict__ order_raw, uint32_t* __restrict__ fill,
+             uint32_t* __restrict__ counts, BatchPermute pa, const int* __restrict__ stale) {
+    const size_t g = (static_cast<size_t>(blockIdx.x) * BT + threadIdx.x) / WAVE;
+    const int lane = threadIdx.x % WAVE;
+    if (g >= cells_total) return;
+    const int r = cell_rep[g];
+    if (!stale[r]) return;
+    const Rep rp = reps[r];
+    const uint32_t cell = static_cast<uint32_t>(g) - rp.cell_first;
+    if (cell >= rp.ncells) return;  // the replica's end word
+    const uint32_t b = start[g];
+    const uint32_t m = start[g + 1] - b;
+    if (lane == 0) {
+        fill[g] = 0;
+        counts[g] = 0;
+    }
+    for (uint32_t base = 0; base < m; base += WAVE) {
+        const bool valid = base + lane < m;
+        const uint32_t v = valid ? order_raw[b + base + lane] : 0xffffffffu;
+        uint32_t rank = 0;
+        if (m <= WAVE) {
+            for (uint32_t s = 0; s < m; ++s) rank += (static_cast<uint32_t>(__shfl(static_cast<int>(v), s, WAVE)) < v);
+        } else {
+            for (uint32_t s = 0; s < m; ++s) rank += (order_raw[b + s] < v);
+        }
+        if (valid) {
+            const uint32_t k = b + rank;
+            const double4 p = pa.pos[v];
+            pa.pos2[k] = p;
+            pa.ref[k] = p;
+            for (int c = 0; c < 3; ++c) {
+                pa.vel2[c * slots + k] = pa.vel[c * slots + v];
+                pa.frc2[c * slots + k] = pa.frc[c * slots + v];
+            }
+            pa.mass2[k] = pa.mass[v];
+            pa.id2[k] = pa.id[v];
+            pa.cell_of2[k] = static_cast<int>(cell);
+        }
+    }
+}
+
+// k_list_build per replica: rows in the reference's order (j > i in stencil order, then j < i increasing),
+// from the re-sorted arrays, which it also moves back into place.
+__global__ void __launch_bounds__(BT)
+    k_b_list_build(const Rep* __restrict__ reps, const uint16_t* __restrict__ block_rep,
+                   const double4* __restrict__ pos, size_t slots, const int* __restrict__ cell_of,
+                   const uint32_t* __restrict__ start, double radius2, int padding, int32_t* __restrict__ rows,
+                   int32_t* __restrict__ count, int32_t* __restrict__ half, int* __restrict__ flags,
+                   unsigned long long* __restrict__ pairs, BatchCopyBack cb, const int* __restrict__ stale) {
+    const int r = block_rep[blockIdx.x];
+    if (!stale[r]) return;
+    const Rep rp = reps[r];
+    const size_t i = static_cast<size_t>(blockIdx.x) * BT + threadIdx.x;
+    const uint32_t end = rp.first + rp.n;
+    int cnt = 0, hcnt = 0;
+    if (i < end) {
+        const double4 ri = pos[i];
+        cb.pos[i] = ri;
+        for (int c = 0; c < 3; ++c) {
+            cb.vel[c * slots + i] = cb.vel2[c * slots + i];
+            cb.frc[c * slots + i] = cb.frc2[c * slots + i];
+        }
+        cb.mass[i] = cb.mass2[i];
+        cb.id[i] = cb.id2[i];
+        const double box = rp.box, inv_box = rp.inv_box;
+        auto consider = [&](uint32_t j) {
+            const double4 rj = pos[j];
+            const double dx = min_image(ri.x - rj.x, box, inv_box);
+            const double dy = min_image(ri.y - rj.y, box, inv_box);
+            const double dz = min_image(ri.z - rj.z, box, inv_box);
+            const double r2 = dx * dx + dy * dy + dz * dz;
+            if (r2 < radius2) {
+                if (cnt < padding) rows[static_cast<size_t>(cnt) * slots + i] = static_cast<int32_t>(j);
+                ++cnt;
+            }
+        };
+        const uint32_t self = static_cast<uint32_t>(i);
+        const int side = rp.side;
+        const uint32_t* st = start + rp.cell_first;
+        if (rp.use_cells) {
+            const int c = cell_of[i];
+            const int cx = c / (side * side), cy = (c / side) % side, cz = c % side;
+            for (int ox = -1; ox <= 1; ++ox) {
+                const int x = wrap_index(cx + ox, side);
+                for (int oy = -1; oy <= 1; ++oy) {
+                    const int y = wrap_index(cy + oy, side);
+                    for (int oz = -1; oz <= 1; ++oz) {
+                        const int nc = (x * side + y) * side + wrap_index(cz + oz, side);
+                        if (nc < c) continue;
+                        const uint32_t e = st[nc + 1];
+                        for (uint32_t j = (nc == c ? self + 1 : st[nc]); j < e; ++j) consider(j);
+                    }
+                }
+            }
+            hcnt = cnt;
+            int xs[3] = {wrap_index(cx - 1, side), cx, wrap_index(cx + 1, side)};
+            int ys[3] = {wrap_index(cy - 1, side), cy, wrap_index(cy + 1, side)};
+            int zs[3] = {wrap_index(cz - 1, side), cz, wrap_index(cz + 1, side)};
+            for (int a = 0; a < 2; ++a)
+                for (int k = 0; k < 2 - a; ++k) {
+                    if (xs[k] > xs[k + 1]) { const int t = xs[k]; xs[k] = xs[k + 1]; xs[k + 1] = t; }
+                    if (ys[k] > ys[k + 1]) { const int t = ys[k]; ys[k] = ys[k + 1]; ys[k + 1] = t; }
+                    if (zs[k] > zs[k + 1]) { const int t = zs[k]; zs[k] = zs[k + 1]; zs[k + 1] = t; }
+                }
+            for (int a = 0; a < 3; ++a)
+                for (int b2 = 0; b2 < 3; ++b2)
+                    for (int d = 0; d < 3; ++d) {
+                        const int nc = (xs[a] * side + ys[b2]) * side + zs[d];
+                        if (nc > c) continue;
+                        const uint32_t e = (nc == c) ? self : st[nc + 1];
+                        for (uint32_t j = st[nc]; j < e; ++j) consider(j);
+                    }
+        } else {
+            for (uint32_t j = self + 1; j < end; ++j) consider(j);
+            hcnt = cnt;
+            for (uint32_t j = rp.first; j < self; ++j) consider(j);
+        }
+        count[i] = cnt < padding ? cnt : padding;
+        half[i] = hcnt < padding ? hcnt : padding;
+    }
+    int mx = cnt;
+    unsigned long long hs = static_cast<unsigned long long>(hcnt);
+#pragma unroll
+    for (int off = WAVE / 2; off > 0; off >>= 1) {
+        const int o = __shfl_xor(mx, off, WAVE);
+        mx = o > mx ? o : mx;
+        hs += __shfl_xor(hs, off, WAVE);
+    }
+    if (threadIdx.x % WAVE == 0) {
+        atomicMax(&flags[1], mx);
+        if (mx > padding) atomicOr(&flags[0], 1);
+        atomicAdd(&pairs[r], hs);
+    }
+}
+
+// ---- forces --------------------------------------------------------------------------------------------
+
+// LANES lanes per particle take row entries k = lane, lane + LANES, ...; LANES = 1 is the single-system kernel.
+template <bool MIXED, int LANES>
+__global__ void __launch_bounds__(BT)
+    k_b_forces(const Rep* __restrict__ reps, const uint16_t* __restrict__ block_rep, const double4* __restrict__ pos,
+               size_t slots, const int32_t* __restrict__ rows, const int32_t* __restrict__ count,
+               const int32_t* __restrict__ half, LJConst lj, double* __restrict__ frc,
+               double* __restrict__ partials) {
+    constexpr int PB = BT / LANES;
+    const size_t p0 = static_cast<size_t>(blockIdx.x) * PB;
+    const Rep rp = reps[block_rep[p0 / BT]];
+    const int lane = threadIdx.x % LANES;
+    const size_t i = p0 + threadIdx.x / LANES;
+    const bool active = i < static_cast<size_t>(rp.first) + rp.n;
+    PairSums a;
+    if (active) {
+        const double4 ri = pos[i];
+        const int m = count[i], h = half[i];
+        const int32_t* row = rows + i;
+        for (int k = lane; k < m; k += LANES) {
+            const double4 rj = pos[row[static_cast<size_t>(k) * slots]];
+            if constexpr (MIXED)
+                mixed_pair_gather(ri, rj, rp.box, rp.inv_box, lj, k < h, a);
+            else
+                fp64_pair(ri, rj, rp.box, rp.inv_box, lj, k < h, a);
+        }
+    }
+#pragma unroll
+    for (int off = 1; off < LANES; off <<= 1) {
+        a.fx += __shfl_xor(a.fx, off, WAVE);
+        a.fy += __shfl_xor(a.fy, off, WAVE);
+        a.fz += __shfl_xor(a.fz, off, WAVE);
+    }
+    if (active && lane == 0) {
+        frc[i] = a.fx;
+        frc[slots + i] = a.fy;
+        frc[2 * slots + i] = a.fz;
+    }
+    double acc[3] = {a.u, a.w, a.c};
+    block_sum<3, BT>(acc, partials + static_cast<size_t>(blockIdx.x) * 3);
+}
+
+// Per replica, in block order: potential, virial, inside pairs -> scalars[0..2].
+__global__ void k_b_reduce_forces(const Rep* __restrict__ reps, int count, int lanes,
+                                  const double* __restrict__ partials, double* __restrict__ scalars) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= count) return;
+    const Rep rp = reps[r];
+    const uint32_t pb = BT / lanes;
+    const uint32_t b0 = rp.first / pb, b1 = (rp.first + rp.n + pb - 1) / pb;
+    double s[3] = {0.0, 0.0, 0.0};
+    for (uint32_t b = b0; b < b1; ++b)
+        for (int k = 0; k < 3; ++k) s[k] += partials[static_cast<size_t>(b) * 3 + k];
+    for (int k = 0; k < 3; ++k) scalars[r * SCALARS + k] = s[k];
+}
+
+// After a host-driven build: count it and clear the stale flags.
+__global__ void k_b_finish_build(int count, int* __restrict__ stale, int* __restrict__ rebuilds) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= count) return;
+    rebuilds[r] += stale[r];
+    stale[r] = 0;
+}
+
+__global__ void k_b_fill(int* __restrict__ p, int count, int value) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < count) p[k] = value;
+}
+
+int grid_side_of(double box, double min_cell) {
+    const int s = static_cast<int>(box / min_cell);
+    return s > 1 ? s : 1;
+}
+
+int padding_for(size_t n, double box, double radius) {
+    const double density = static_cast<double>(n) / (box * box * box);
+    const double expected = 4.0 / 3.0 * M_PI * radius * radius * radius * density;
+    long p = static_cast<long>(std::ceil(expected * 1.35 + 24.0));
+    p = (p + 7) / 8 * 8;
+    const long cap = n > 1 ? static_cast<long>(n - 1) : 1;
+    return static_cast<int>(std::min<long>(p, std::max<long>(cap, 1)));
+}
+
+}  // namespace mdgpu
+
+struct md_batch {
+    md_ctx* ctx = nullptr;
+    int count = 0;
+    size_t slots = 0, particles = 0, cells_total = 0;
+    int blocks = 0, ke_total = 0;
+    double cutoff = 0.0, skin = 0.0, radius2 = 0.0, max_disp2 = 0.0;
+    int padding = 0;
+    int lanes = 1;
+    bool forces_valid = false;
+    std::vector<mdgpu::Rep> reps;
+    mdgpu::Rep* d_reps = nullptr;
+    uint16_t* block_rep = nullptr;
+    uint16_t* cell_rep = nullptr;
+    uint32_t* ke_start = nullptr;
+    double4 *pos = nullptr, *pos_alt = nullptr, *ref = nullptr;
+    double *vel = nullptr, *vel_alt = nullptr, *frc = nullptr, *frc_alt = nullptr;
+    double *mass = nullptr, *mass_alt = nullptr;
+    uint64_t *id = nullptr, *id_alt = nullptr;
+    int *cell_of = nullptr, *cell_of_alt = nullptr;
+    uint32_t *counts = nullptr, *start = nullptr, *fill = nullptr, *order_raw = nullptr;
+    int32_t *rows = nullptr, *rcount = nullptr, *half = nullptr;
+    int* stale = nullptr;
+    int* rebuilds = nullptr;
+    unsigned long long* pairs = nullptr;
+    int* flags = nullptr;       // [0] overflow, [1] longest row, [2] step counter, [3] spare
+    double* partials = nullptr; // force partials (up to blocks * 4 lanes * 3), then the kick's (blocks)
+    double* ke_part = nullptr;
+    double* scalars = nullptr;
+};
+
+namespace mdgpu {
+namespace {
+
+void batch_free(md_batch* b) {
+    dev_free(b->d_reps);
+    dev_free(b->block_rep);
+    dev_free(b->cell_rep);
+    dev_free(b->ke_start);
+    dev_free(b->pos);
+    dev_free(b->pos_alt);
+    dev_free(b->ref);
+    dev_free(b->vel);
+    dev_free(b->vel_alt);
+    dev_free(b->frc);
+    dev_free(b->frc_alt);
+    dev_free(b->mass);
+    dev_free(b->mass_alt);
+    dev_free(b->id);
+    dev_free(b->id_alt);
+    dev_free(b->cell_of);
+    dev_free(b->cell_of_alt);
+    dev_free(b->counts);
+    dev_free(b->start);
+    dev_free(b->fill);
+    dev_free(b->order_raw);
+    dev_free(b->rows);
+    dev_free(b->rcount);
+    dev_free(b->half);
+    dev_free(b->stale);
+    dev_free(b->rebuilds);
+    dev_free(b->pairs);
+    dev_free(b->flags);
+    dev_free(b->partials);
+    dev_free(b->ke_part);
+    dev_free(b->scalars);
+}
+
+// Allocates every device buffer for the layout in b (sizes, reps) and uploads the tables.
+void batch_alloc(md_batch* b) {
+    hipStream_t st = b->ctx->stream;
+    const size_t S = b->slots;
+    b->d_reps = dev_alloc<Rep>(b->count);
+    b->block_rep = dev_alloc<uint16_t>(b->blocks);
+    b->cell_rep = dev_alloc<uint16_t>(b->cells_total);
+    b->ke_start = dev_alloc<uint32_t>(b->ke_total);
+    b->pos = dev_alloc<double4>(S);
+    b->pos_alt = dev_alloc<double4>(S);
+    b->ref = dev_alloc<double4>(S);
+    b->vel = dev_alloc<double>(3 * S);
+    b->vel_alt = dev_alloc<double>(3 * S);
+    b->frc = dev_alloc<double>(3 * S);
+    b->frc_alt = dev_alloc<double>(3 * S);
+    b->mass = dev_alloc<double>(S);
+    b->mass_alt = dev_alloc<double>(S);
+    b->id = dev_alloc<uint64_t>(S);
+    b->id_alt = dev_alloc<uint64_t>(S);
+    b->cell_of = dev_alloc<int>(S);
+    b->cell_of_alt = dev_alloc<int>(S);
+    b->counts = dev_alloc<uint32_t>(b->cells_total);
+    b->start = dev_alloc<uint32_t>(b->cells_total);
+    b->fill = dev_alloc<uint32_t>(b->cells_total);
+    b->order_raw = dev_alloc<uint32_t>(S);
+    b->rows = dev_alloc<int32_t>(static_cast<size_t>(b->padding) * S);
+    b->rcount = dev_alloc<int32_t>(S);
+    b->half = dev_alloc<int32_t>(S);
+    b->stale = dev_alloc<int>(b->count);
+    b->rebuilds = dev_alloc<int>(b->count);
+    b->pairs = dev_alloc<unsigned long long>(b->count);
+    b->flags = dev_alloc<int>(4);
+    b->partials = dev_alloc<double>(static_cast<size_t>(b->blocks) * (4 * 3 + 1));
+    b->ke_part = dev_alloc<double>(b->ke_total);
+    b->scalars = dev_alloc<double>(static_cast<size_t>(b->count) * SCALARS);
+    MD_HIP(hipMemsetAsync(b->counts, 0, sizeof(uint32_t) * b->cells_total, st));
+    MD_HIP(hipMemsetAsync(b->fill, 0, sizeof(uint32_t) * b->cells_total, st));
+    MD_HIP(hipMemsetAsync(b->rebuilds, 0, sizeof(int) * b->count, st));
+    MD_HIP(hipMemsetAsync(b->flags, 0, sizeof(int) * 4, st));
+    MD_HIP(hipMemsetAsync(b->scalars, 0, sizeof(double) * b->count * SCALARS, st));
+    // padding slots: zero mass/velocity so that no kernel reads garbage there
+    MD_HIP(hipMemsetAsync(b->pos, 0, sizeof(double4) * S, st));
+    MD_HIP(hipMemsetAsync(b->vel, 0, sizeof(double) * 3 * S, st));
+    MD_HIP(hipMemsetAsync(b->frc, 0, sizeof(double) * 3 * S, st));
+    MD_HIP(hipMemsetAsync(b->mass, 0, sizeof(double) * S, st));
+    MD_HIP(hipMemsetAsync(b->id, 0, sizeof(uint64_t) * S, st));
+
+    std::vector<uint16_t> brep(b->blocks), crep(b->cells_total);
+    std::vector<uint32_t> kstart(b->ke_total);
+    for (int r = 0; r < b->count; ++r) {
+        const Rep& rp = b->reps[r];
+        const uint32_t nb = (rp.n + BT - 1) / BT;
+        for (uint32_t k = 0; k < (nb ? nb : 1); ++k) brep[rp.first / BT + k] = static_cast<uint16_t>(r);
+        for (uint32_t c = 0; c <= rp.ncells; ++c) crep[rp.cell_first + c] = static_cast<uint16_t>(r);
+        for (uint32_t k = 0; k < rp.ke_blocks; ++k) kstart[rp.ke_first + k] = rp.first + k * KE_TERMS;
+    }
+    MD_HIP(hipMemcpyAsync(b->d_reps, b->reps.data(), sizeof(Rep) * b->count, hipMemcpyHostToDevice, st));
+    MD_HIP(hipMemcpyAsync(b->block_rep, brep.data(), sizeof(uint16_t) * brep.size(), hipMemcpyHostToDevice, st));
+    MD_HIP(hipMemcpyAsync(b->cell_rep, crep.data(), sizeof(uint16_t) * crep.size(), hipMemcpyHostToDevice, st));
+    if (b->ke_total)
+        MD_HIP(hipMemcpyAsync(b->ke_start, kstart.data(), sizeof(uint32_t) * kstart.size(), hipMemcpyHostToDevice,
+                              st));
+    MD_HIP(hipStreamSynchronize(st));  // the host vectors above go out of scope
+}
+
+void enqueue_rebuild(md_batch* b) {
+    hipStream_t st = b->ctx->stream;
+    const size_t S = b->slots;
+    k_b_cell_count<<<b->blocks, BT, 0, st>>>(b->d_reps, b->block_rep, b->pos, b->cell_of, b->counts, b->stale);
+    k_b_scan<<<b->count, BT, 0, st>>>(b->d_reps, b->counts, b->start, b->stale, b->pairs);
+    k_b_scatter<<<b->blocks, BT, 0, st>>>(b->d_reps, b->block_rep, b->cell_of, b->start, b->fill, b->order_raw,
+                                          b->stale);
+    BatchPermute pa{b->pos, b->vel, b->frc, b->mass, b->id, b->pos_alt, b->vel_alt,
+                    b->frc_alt, b->mass_alt, b->id_alt, b->ref, b->cell_of_alt};
+    k_b_sort<<<blocks_for(b->cells_total * WAVE, BT), BT, 0, st>>>(b->d_reps, b->cell_rep, b->cells_total, S,
+                                                                   b->start, b->order_raw, b->fill, b->counts, pa,
+                                                                   b->stale);
+    BatchCopyBack cb{b->vel_alt, b->frc_alt, b->mass_alt, b->id_alt, b->pos, b->vel, b->frc, b->mass, b->id};
+    k_b_list_build<<<b->blocks, BT, 0, st>>>(b->d_reps, b->block_rep, b->pos_alt, S, b->cell_of_alt, b->start,
+                                             b->radius2, b->padding, b->rows, b->rcount, b->half, b->flags,
+                                             b->pairs, cb, b->stale);
+    MD_HIP(hipGetLastError());
+}
+
+void enqueue_forces(md_batch* b, const LJConst& lj, int precision, int lanes, hipEvent_t e0, hipEvent_t e1) {
+    hipStream_t st = b->ctx->stream;
+    const size_t S = b->slots;
+    if (precision != MD_MIXED) lanes = 1;
+    const unsigned grid = static_cast<unsigned>(b->blocks) * lanes;
+    if (e0) MD_HIP(hipEventRecord(e0, st));
+#define MD_B_FORCES(M, L)                                                                                  \
+    k_b_forces<M, L><<<grid, BT, 0, st>>>(b->d_reps, b->block_rep, b->pos, S, b->rows, b->rcount, b->half, lj, \
+                                          b->frc, b->partials)
+    if (precision == MD_MIXED) {
+        if (lanes == 4)
+            MD_B_FORCES(true, 4);
+        else if (lanes == 2)
+            MD_B_FORCES(true, 2);
+        else
+            MD_B_FORCES(true, 1);
+    } else {
+        MD_B_FORCES(false, 1);
+    }
+#undef MD_B_FORCES
+    MD_HIP(hipGetLastError());
+    if (e1) MD_HIP(hipEventRecord(e1, st));
+    k_b_reduce_forces<<<blocks_for(b->count, 256), 256, 0, st>>>(b->d_reps, b->count, lanes, b->partials,
+                                                                 b->scalars);
+    MD_HIP(hipGetLastError());
+}
+
+void check_boxes(const md_batch* b, double cutoff) {
+    for (const Rep& rp : b->reps)
+        if (2.0 * cutoff > rp.box) fail(MD_ERR_RUNTIME, "Cutoff must be less than half the box length");
+}
+
+// Host-driven build of the stale replicas with overflow regrowth (list_rebuild for a batch).
+void batch_build(md_batch* b) {
+    hipStream_t st = b->ctx->stream;
+    for (int attempt = 0;; ++attempt) {
+        MD_HIP(hipMemsetAsync(b->flags, 0, 2 * sizeof(int), st));
+        enqueue_rebuild(b);
+        int hf[2];
+        MD_HIP(hipMemcpyAsync(b->ctx->host_flags, b->flags, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
+        MD_HIP(hipStreamSynchronize(st));
+        std::memcpy(hf, b->ctx->host_flags, sizeof(hf));
+        if (!hf[0]) break;
+        if (attempt > 8) fail(MD_ERR_OVERFLOW, "neighbour-list rows keep overflowing");
+        // the replicas are sorted now (a re-sort is the identity); rebuild all with longer rows
+        b->padding = (hf[1] + hf[1] / 4 + 16 + 7) / 8 * 8;
+        dev_free(b->rows);
+        b->rows = dev_alloc<int32_t>(static_cast<size_t>(b->padding) * b->slots);
+        k_b_fill<<<blocks_for(b->count, 256), 256, 0, st>>>(b->stale, b->count, 1);
+    }
+    k_b_finish_build<<<blocks_for(b->count, 256), 256, 0, st>>>(b->count, b->stale, b->rebuilds);
+    MD_HIP(hipGetLastError());
+}
+
+}  // namespace
+}  // namespace mdgpu
+
+using namespace mdgpu;
+
+extern "C" {
+
+int md_batch_create(md_ctx* ctx, md_state* const* states, int count, double cutoff, double skin, md_batch** out) {
+    MD_API_BEGIN
+    if (!ctx || !states || !out) fail(MD_ERR_ARG, "ctx/states/out is NULL");
+    if (count <= 0 || count > 65535) fail(MD_ERR_ARG, "replica count must be in 1..65535");
+    if (!(cutoff > 0.0) || !(skin >= 0.0)) fail(MD_ERR_ARG, "cutoff must be > 0 and skin >= 0");
+    *out = nullptr;
+    const double radius = cutoff + skin;
+    auto b = new md_batch();
+    struct Guard {
+        md_batch* b;
+        ~Guard() {
+            if (b) {
+                batch_free(b);
+                delete b;
+            }
+        }
+    } guard{b};
+    b->ctx = ctx;
+    b->count = count;
+    b->cutoff = cutoff;
+    b->skin = skin;
+    b->radius2 = radius * radius;
+    b->max_disp2 = 0.25 * skin * skin;
+    b->reps.resize(count);
+    size_t slot = 0, cell = 0, ke = 0;
+    int padding = 8;
+    for (int r = 0; r < count; ++r) {
+        const md_state* s = states[r];
+        if (!s) fail(MD_ERR_ARG, "a replica state is NULL");
+        if (s->ctx != ctx) fail(MD_ERR_ARG, "every replica state must belong to the batch's context");
+        if (s->n == 0) fail(MD_ERR_ARG, "a replica has no particles");
+        if (2.0 * radius > s->box) fail(MD_ERR_RUNTIME, "Cutoff + skin must be less than half the box length");
+        Rep& rp = b->reps[r];
+        rp.first = static_cast<uint32_t>(slot);
+        rp.n = static_cast<uint32_t>(s->n);
+        rp.side = grid_side_of(s->box, radius);
+        rp.use_cells = rp.side >= 3 ? 1 : 0;
+        rp.ncells = static_cast<uint32_t>(rp.side) * rp.side * rp.side;
+        rp.cell_first = static_cast<uint32_t>(cell);
+        rp.box = s->box;
+        rp.inv_box = s->inv_box;
+        rp.per_length = rp.side / s->box;
+        rp.ke_first = static_cast<uint32_t>(ke);
+        rp.ke_blocks = static_cast<uint32_t>((s->n + KE_TERMS - 1) / KE_TERMS);
+        slot += (s->n + BT - 1) / BT * BT;
+        cell += rp.ncells + 1;
+        ke += rp.ke_blocks;
+        padding = std::max(padding, padding_for(s->n, s->box, radius));
+        b->particles += s->n;
+        if (slot > 0x7fffffffu || cell > 0xffffffffu) fail(MD_ERR_ARG, "batch too large for 32-bit slot indices");
+    }
+    b->slots = slot;
+    b->cells_total = cell;
+    b->blocks = static_cast<int>(slot / BT);
+    b->ke_total = static_cast<int>(ke);
+    b->padding = padding;
+    batch_alloc(b);
+    hipStream_t st = ctx->stream;
+    for (int r = 0; r < count; ++r) {
+        const md_state* s = states[r];
+        const Rep& rp = b->reps[r];
+        const size_t n = s->n;
+        MD_HIP(hipMemcpyAsync(b->pos + rp.first, s->pos, sizeof(double4) * n, hipMemcpyDeviceToDevice, st));
+        for (int c = 0; c < 3; ++c) {
+            MD_HIP(hipMemcpyAsync(b->vel + c * b->slots + rp.first, s->vel + c * n, sizeof(double) * n,
+                                  hipMemcpyDeviceToDevice, st));
+            MD_HIP(hipMemcpyAsync(b->frc + c * b->slots + rp.first, s->frc + c * n, sizeof(double) * n,
+                                  hipMemcpyDeviceToDevice, st));
+        }
+        MD_HIP(hipMemcpyAsync(b->mass + rp.first, s->mass, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+        MD_HIP(hipMemcpyAsync(b->id + rp.first, s->id, sizeof(uint64_t) * n, hipMemcpyDeviceToDevice, st));
+    }
+    k_b_fill<<<blocks_for(count, 256), 256, 0, st>>>(b->stale, count, 1);
+    MD_HIP(hipGetLastError());
+    batch_build(b);
+    MD_HIP(hipStreamSynchronize(st));
+    ++ctx->live;
+    guard.b = nullptr;
+    *out = b;
+    MD_API_END
+}
+
+int md_batch_destroy(md_batch* b) {
+    MD_API_BEGIN
+    if (!b) return MD_OK;
+    (void)hipStreamSynchronize(b->ctx->stream);
+    --b->ctx->live;
+    batch_free(b);
+    delete b;
+    MD_API_END
+}
+
+int md_batch_get_info(const md_batch* b, md_batch_info* info) {
+    MD_API_BEGIN
+    if (!b || !info) fail(MD_ERR_ARG, "batch/info is NULL");
+    std::vector<int> reb(b->count);
+    std::vector<unsigned long long> pairs(b->count);
+    hipStream_t st = b->ctx->stream;
+    MD_HIP(hipMemcpyAsync(reb.data(), b->rebuilds, sizeof(int) * b->count, hipMemcpyDeviceToHost, st));
+    MD_HIP(hipMemcpyAsync(pairs.data(), b->pairs, sizeof(unsigned long long) * b->count, hipMemcpyDeviceToHost, st));
+    MD_HIP(hipStreamSynchronize(st));
+    info->count = b->count;
+    info->particles = b->particles;
+    info->slots = b->slots;
+    info->padding = b->padding;
+    info->lanes = b->lanes;
+    info->rebuilds = 0;
+    info->half_pairs = 0;
+    for (int r = 0; r < b->count; ++r) {
+        info->rebuilds += reb[r];
+        info->half_pairs += pairs[r];
+    }
+    MD_API_END
+}
+
+int md_batch_replica(md_batch* b, int r, md_state* dst, int* rebuilds, size_t* half_pairs) {
+    MD_API_BEGIN
+    if (!b) fail(MD_ERR_ARG, "batch is NULL");
+    if (r < 0 || r >= b->count) fail(MD_ERR_ARG, "replica index out of range");
+    const Rep& rp = b->reps[r];
+    hipStream_t st = b->ctx->stream;
+    if (dst) {
+        if (dst->n != rp.n) fail(MD_ERR_ARG, "destination state holds a different particle count");
+        const size_t n = rp.n;
+        MD_HIP(hipMemcpyAsync(dst->pos, b->pos + rp.first, sizeof(double4) * n, hipMemcpyDeviceToDevice, st));
+        for (int c = 0; c < 3; ++c) {
+            MD_HIP(hipMemcpyAsync(dst->vel + c * n, b->vel + c * b->slots + rp.first, sizeof(double) * n,
+                                  hipMemcpyDeviceToDevice, st));
+            MD_HIP(hipMemcpyAsync(dst->frc + c * n, b->frc + c * b->slots + rp.first, sizeof(double) * n,
+                                  hipMemcpyDeviceToDevice, st));
+        }
+        MD_HIP(hipMemcpyAsync(dst->mass, b->mass + rp.first, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+        MD_HIP(hipMemcpyAsync(dst->id, b->id + rp.first, sizeof(uint64_t) * n, hipMemcpyDeviceToDevice, st));
+        dst->box = rp.box;
+        dst->inv_box = rp.inv_box;
+    }
+    int reb = 0;
+    unsigned long long pairs = 0;
+    MD_HIP(hipMemcpyAsync(&reb, b->rebuilds + r, sizeof(int), hipMemcpyDeviceToHost, st));
+    MD_HIP(hipMemcpyAsync(&pairs, b->pairs + r, sizeof(pairs), hipMemcpyDeviceToHost, st));
+    MD_HIP(hipStreamSynchronize(st));
+    if (rebuilds) *rebuilds = reb;
+    if (half_pairs) *half_pairs = static_cast<size_t>(pairs);
+    MD_API_END
+}
+
+int md_batch_run(md_batch* b, const md_batch_config* cfg, double* energies) {
+    MD_API_BEGIN
+    if (!b || !cfg) fail(MD_ERR_ARG, "batch/config is NULL");
+    if (cfg->steps < 0) fail(MD_ERR_ARG, "steps must be >= 0");
+    if (cfg->precision != MD_FP64 && cfg->precision != MD_MIXED) fail(MD_ERR_ARG, "unknown precision");
+    if (cfg->thermostat && (!cfg->draws || !cfg->temperatures))
+        fail(MD_ERR_ARG, "Bussi thermostat needs per-replica temperatures and draws");
+    const int lanes = cfg->lanes ? cfg->lanes : b->lanes;
+    if (lanes != 1 && lanes != 2 && lanes != 4) fail(MD_ERR_ARG, "lanes must be 1, 2 or 4");
+    check_boxes(b, cfg->lj.cutoff);
+    hipStream_t st = b->ctx->stream;
+    const LJConst lj = make_lj(cfg->lj);
+    const size_t S = b->slots;
+    const int count = b->count;
+    if (!b->forces_valid) {
+        enqueue_forces(b, lj, cfg->precision, lanes, nullptr, nullptr);
+        b->forces_valid = true;
+    }
+    const int steps = cfg->steps;
+    if (steps == 0) {
+        MD_HIP(hipStreamSynchronize(st));
+        return MD_OK;
+    }
+    struct RunBuffers {
+        double* energies = nullptr;
+        double* draws = nullptr;
+        int* draw_index = nullptr;
+        double2* dof_kbar = nullptr;
+        ~RunBuffers() {
+            dev_free(energies);
+            dev_free(draws);
+            dev_free(draw_index);
+            dev_free(dof_kbar);
+        }
+    } rb;
+    rb.energies = dev_alloc<double>(static_cast<size_t>(steps) * count * 4);
+    const double c = cfg->thermostat ? std::exp(-cfg->dt / cfg->tau) : 0.0;
+    if (cfg->thermostat) {
+        rb.draws = dev_alloc<double>(static_cast<size_t>(count) * steps * 2);
+        MD_HIP(hipMemcpyAsync(rb.draws, cfg->draws, sizeof(double) * count * steps * 2, hipMemcpyHostToDevice, st));
+        rb.draw_index = dev_alloc<int>(count);
+        MD_HIP(hipMemsetAsync(rb.draw_index, 0, sizeof(int) * count, st));
+        std::vector<double2> dk(count);
+        for (int r = 0; r < count; ++r) {
+            const double dof = 3.0 * static_cast<double>(b->reps[r].n) - 3.0;
+            dk[r] = make_double2(dof, 0.5 * dof * cfg->temperatures[r]);
+        }
+        rb.dof_kbar = dev_alloc<double2>(count);
+        MD_HIP(hipMemcpyAsync(rb.dof_kbar, dk.data(), sizeof(double2) * count, hipMemcpyHostToDevice, st));
+        MD_HIP(hipStreamSynchronize(st));  // dk goes out of scope
+    }
+    MD_HIP(hipMemsetAsync(b->flags, 0, 4 * sizeof(int), st));
+    MD_HIP(hipMemsetAsync(b->stale, 0, sizeof(int) * count, st));
+
+    const bool timed = cfg->force_kernel_ms != nullptr && !cfg->use_graph;
+    std::vector<hipEvent_t> events;
+    struct EventGuard {
+        std::vector<hipEvent_t>& ev;
+        ~EventGuard() {
+            for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+        }
+    } eguard{events};
+    if (timed) {
+        events.resize(2 * static_cast<size_t>(steps), nullptr);
+        for (auto& e : events) MD_HIP(hipEventCreate(&e));
+    }
+    int step_index = 0;
+    const double half_dt = 0.5 * cfg->dt;
+    auto enqueue_step = [&]() {
+        k_b_kick_drift<<<b->blocks, BT, 0, st>>>(b->d_reps, b->block_rep, b->pos, b->vel, b->frc, b->mass, S, half_dt,
+                                                 cfg->dt, b->ref, b->max_disp2, b->stale);
+        enqueue_rebuild(b);
+        hipEvent_t e0 = timed ? events[2 * step_index] : nullptr;
+        hipEvent_t e1 = timed ? events[2 * step_index + 1] : nullptr;
+        ++step_index;
+        enqueue_forces(b, lj, cfg->precision, lanes, e0, e1);
+        if (cfg->thermostat) {
+            k_b_kick<<<b->blocks, BT, 0, st>>>(b->d_reps, b->block_rep, b->vel, b->frc, b->mass, S, half_dt, nullptr);
+            k_b_ke_blocks<<<b->ke_total, BT, 0, st>>>(b->d_reps, b->block_rep, b->ke_start, b->vel, b->mass, S,
+                                                      b->ke_part);
+            k_b_ke_final<<<blocks_for(count, 256), 256, 0, st>>>(b->d_reps, count, b->ke_part, b->scalars);
+            k_b_bussi<<<blocks_for(count, 256), 256, 0, st>>>(count, b->scalars, rb.draws, steps, rb.draw_index,
+                                                              rb.dof_kbar, c);
+            k_b_scale<<<b->blocks, BT, 0, st>>>(b->d_reps, b->block_rep, b->vel, S, b->scalars);
+            k_b_record<<<1, 1024, 0, st>>>(count, b->scalars, 6, rb.energies, b->stale, b->rebuilds, b->flags);
+        } else {
+            double* kick_partials = b->partials + static_cast<size_t>(b->blocks) * 4 * 3;
+            k_b_kick<<<b->blocks, BT, 0, st>>>(b->d_reps, b->block_rep, b->vel, b->frc, b->mass, S, half_dt,
+                                               kick_partials);
+            k_b_reduce_ke<<<blocks_for(count, 256), 256, 0, st>>>(b->d_reps, count, kick_partials, b->scalars);
+            k_b_record<<<1, 1024, 0, st>>>(count, b->scalars, 3, rb.energies, b->stale, b->rebuilds, b->flags);
+        }
+        MD_HIP(hipGetLastError());
+    };
+    if (cfg->use_graph) {
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        MD_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        try {
+            enqueue_step();
+        } catch (...) {
+            (void)hipStreamEndCapture(st, &graph);
+            if (graph) (void)hipGraphDestroy(graph);
+            throw;
+        }
+        MD_HIP(hipStreamEndCapture(st, &graph));
+        hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            (void)hipGraphDestroy(graph);
+            hip_check(e, "hipGraphInstantiate");
+        }
+        for (int k = 0; k < steps; ++k) {
+            e = hipGraphLaunch(exec, st);
+            if (e != hipSuccess) break;
+        }
+        (void)hipStreamSynchronize(st);
+        (void)hipGraphExecDestroy(exec);
+        (void)hipGraphDestroy(graph);
+        hip_check(e, "hipGraphLaunch");
+    } else {
+        for (int k = 0; k < steps; ++k) enqueue_step();
+    }
+    int* hf = b->ctx->host_flags;
+    MD_HIP(hipMemcpyAsync(hf, b->flags, sizeof(int) * 4, hipMemcpyDeviceToHost, st));
+    if (energies)
+        MD_HIP(hipMemcpyAsync(energies, rb.energies, sizeof(double) * steps * count * 4, hipMemcpyDeviceToHost, st));
+    MD_HIP(hipStreamSynchronize(st));
+    if (timed) {
+        double total_ms = 0.0;
+        for (int k = 0; k < steps; ++k) {
+            float ms = 0.0f;
+            MD_HIP(hipEventElapsedTime(&ms, events[2 * k], events[2 * k + 1]));
+            total_ms += ms;
+        }
+        *cfg->force_kernel_ms = total_ms;
+    }
+    if (hf[0]) {
+        const int longest = hf[1];
+        b->padding = (longest + longest / 4 + 16 + 7) / 8 * 8;
+        dev_free(b->rows);
+        b->rows = dev_alloc<int32_t>(static_cast<size_t>(b->padding) * b->slots);
+        b->forces_valid = false;
+        k_b_fill<<<blocks_for(count, 256), 256, 0, st>>>(b->stale, count, 1);
+        MD_HIP(hipStreamSynchronize(st));
+        fail(MD_ERR_OVERFLOW, "a neighbour-list row outgrew its padding during md_batch_run; state is invalid");
+    }
+    MD_API_END
+}
+
+int md_batch_tune(md_batch* b, const md_batch_config* cfg, int trial_steps, double* ms_per_step, int* best_lanes) {
+    MD_API_BEGIN
+    if (!b || !cfg) fail(MD_ERR_ARG, "batch/config is NULL");
+    if (cfg->precision != MD_MIXED) fail(MD_ERR_ARG, "the autotuner picks lanes for the mixed-precision kernel");
+    if (trial_steps <= 0 || trial_steps > cfg->steps) fail(MD_ERR_ARG, "trial_steps must be in 1..cfg->steps");
+    hipStream_t st = b->ctx->stream;
+    // a scratch copy of the batch, so that the trials leave the batch itself where it is
+    md_batch tmp;
+    tmp.ctx = b->ctx;
+    tmp.count = b->count;
+    tmp.slots = b->slots;
+    tmp.particles = b->particles;
+    tmp.cells_total = b->cells_total;
+    tmp.blocks = b->blocks;
+    tmp.ke_total = b->ke_total;
+    tmp.cutoff = b->cutoff;
+    tmp.skin = b->skin;
+    tmp.radius2 = b->radius2;
+    tmp.max_disp2 = b->max_disp2;
+    tmp.padding = b->padding;
+    tmp.lanes = b->lanes;
+    tmp.forces_valid = b->forces_valid;
+    tmp.reps = b->reps;
+    struct Guard {
+        md_batch* t;
+        ~Guard() { batch_free(t); }
+    } guard{&tmp};
+    batch_alloc(&tmp);
+    const size_t S = b->slots;
+    auto copy = [&](void* dst, const void* src, size_t bytes) {
+        MD_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, st));
+    };
+    copy(tmp.pos, b->pos, sizeof(double4) * S);
+    copy(tmp.ref, b->ref, sizeof(double4) * S);
+    copy(tmp.vel, b->vel, sizeof(double) * 3 * S);
+    copy(tmp.frc, b->frc, sizeof(double) * 3 * S);
+    copy(tmp.mass, b->mass, sizeof(double) * S);
+    copy(tmp.id, b->id, sizeof(uint64_t) * S);
+    copy(tmp.cell_of, b->cell_of, sizeof(int) * S);
+    copy(tmp.start, b->start, sizeof(uint32_t) * b->cells_total);
+    copy(tmp.rows, b->rows, sizeof(int32_t) * b->padding * S);
+    copy(tmp.rcount, b->rcount, sizeof(int32_t) * S);
+    copy(tmp.half, b->half, sizeof(int32_t) * S);
+    copy(tmp.pairs, b->pairs, sizeof(unsigned long long) * b->count);
+    MD_HIP(hipStreamSynchronize(st));
+    const int options[3] = {1, 2, 4};
+    double best = 0.0;
+    int best_l = b->lanes;
+    for (int k = 0; k < 3; ++k) {
+        md_batch_config c = *cfg;
+        c.steps = trial_steps;
+        c.lanes = options[k];
+        c.force_kernel_ms = nullptr;
+        const auto t0 = std::chrono::steady_clock::now();
+        const int rc = md_batch_run(&tmp, &c, nullptr);
+        if (rc != MD_OK) fail(rc, std::string("autotuner trial failed: ") + md_last_error());
+        const double ms =
+            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / trial_steps;
+        if (ms_per_step) ms_per_step[k] = ms;
+        if (k == 0 || ms < best) {
+            best = ms;
+            best_l = options[k];
+        }
+    }
+    b->lanes = best_l;
+    if (best_lanes) *best_lanes = best_l;
+    MD_API_END
+}
+
+}  // extern "C"

@@ -9,21 +9,16 @@
 // FP64: each pair term is F/r * d rounded once and added in row order: the j > i part reproduces the reference's
 // deterministic-mode "own pairs" sum and the j < i part (increasing j) its "pairs as second" subtraction, since
 // d_ij = -d_ji exactly. Forces are therefore bit-equal to Forcefield.cpp:214-326 and to the O(N^2) path.
-// MIXED: double minimum image narrowed to float, float pair maths (Forcefield.cpp:352-446), double accumulation.
+// MIXED: double minimum image and r^2, float LJ factor (Forcefield.cpp:352-446), double accumulation.
 #include <cmath>
 
 #include "device.hpp"
 #include "internal.hpp"
+#include "pair.hpp"
 
 namespace mdgpu {
 
-struct LJConst {
-    double sig2, c24, c4, shift, rc2;
-    float sig2f, c24f, c4f, shiftf, rc2f;
-};
-
-// Constructor constants (Forcefield.cpp:8-15): shift = V(rc) with std::pow, as the reference computes it.
-static LJConst make_lj(const md_lj& p) {
+LJConst make_lj(const md_lj& p) {
     LJConst c;
     c.sig2 = p.sigma * p.sigma;
     c.c24 = 24.0 * p.epsilon;
@@ -47,37 +42,17 @@ __global__ void __launch_bounds__(FORCE_THREADS)
                   const int32_t* __restrict__ count, const int32_t* __restrict__ half, double box, double inv_box,
                   LJConst lj, double* __restrict__ frc, double* __restrict__ partials, int* __restrict__ clear_flag) {
     const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    double fx = 0.0, fy = 0.0, fz = 0.0;
-    double acc[3] = {0.0, 0.0, 0.0};  // potential, virial, inside pairs
+    PairSums a;
     if (i < n) {
         const double4 ri = pos[i];
         const int m = count[i], h = half[i];
         const int32_t* row = rows + i;
-        for (int k = 0; k < m; ++k) {
-            const double4 rj = pos[row[static_cast<size_t>(k) * n]];
-            const double dx = min_image(ri.x - rj.x, box, inv_box);
-            const double dy = min_image(ri.y - rj.y, box, inv_box);
-            const double dz = min_image(ri.z - rj.z, box, inv_box);
-            const double r2 = dx * dx + dy * dy + dz * dz;
-            const double inv_r2 = 1.0 / r2;
-            const double s2 = lj.sig2 * inv_r2;
-            const double s6 = s2 * s2 * s2;
-            const double s12 = s6 * s6;
-            const bool inside = r2 < lj.rc2;
-            const double f = inside ? lj.c24 * (2.0 * s12 - s6) * inv_r2 : 0.0;
-            fx += f * dx;
-            fy += f * dy;
-            fz += f * dz;
-            if (k < h) {
-                acc[0] += inside ? lj.c4 * (s12 - s6) - lj.shift : 0.0;
-                acc[1] += f * r2;
-                acc[2] += inside ? 1.0 : 0.0;
-            }
-        }
-        frc[i] = fx;
-        frc[n + i] = fy;
-        frc[2 * n + i] = fz;
+        for (int k = 0; k < m; ++k) fp64_pair(ri, pos[row[static_cast<size_t>(k) * n]], box, inv_box, lj, k < h, a);
+        frc[i] = a.fx;
+        frc[n + i] = a.fy;
+        frc[2 * n + i] = a.fz;
     }
+    double acc[3] = {a.u, a.w, a.c};  // potential, virial, inside pairs
     if (clear_flag && i == 0) *clear_flag = 0;
     block_sum<3, FORCE_THREADS>(acc, partials + static_cast<size_t>(blockIdx.x) * 3);
 }
@@ -87,45 +62,18 @@ __global__ void __launch_bounds__(FORCE_THREADS)
                    const int32_t* __restrict__ count, const int32_t* __restrict__ half, double box, double inv_box,
                    LJConst lj, double* __restrict__ frc, double* __restrict__ partials, int* __restrict__ clear_flag) {
     const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    double fx = 0.0, fy = 0.0, fz = 0.0;
-    double acc[3] = {0.0, 0.0, 0.0};
+    PairSums a;
     if (i < n) {
         const double4 ri = pos[i];
         const int m = count[i], h = half[i];
         const int32_t* row = rows + i;
-        double u = 0.0, w = 0.0, c = 0.0;
-        for (int k = 0; k < m; ++k) {
-            const double4 rj = pos[row[static_cast<size_t>(k) * n]];
-            double ex = ri.x - rj.x, ey = ri.y - rj.y, ez = ri.z - rj.z;
-            ex = fma(-box, rint(ex * inv_box), ex);
-            ey = fma(-box, rint(ey * inv_box), ey);
-            ez = fma(-box, rint(ez * inv_box), ez);
-            const float dx = static_cast<float>(ex), dy = static_cast<float>(ey), dz = static_cast<float>(ez);
-            const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-            // hardware reciprocal (1 ulp) plus one Newton step
-            const float q = __builtin_amdgcn_rcpf(r2);
-            const float inv_r2 = fmaf(q, fmaf(-r2, q, 1.0f), q);
-            const float s2 = lj.sig2f * inv_r2;
-            const float s6 = s2 * s2 * s2;
-            const float s12 = s6 * s6;
-            const bool inside = r2 < lj.rc2f;
-            const float f = inside ? lj.c24f * fmaf(2.0f, s12, -s6) * inv_r2 : 0.0f;
-            fx += static_cast<double>(f * dx);
-            fy += static_cast<double>(f * dy);
-            fz += static_cast<double>(f * dz);
-            if (k < h) {
-                u += inside ? static_cast<double>(fmaf(lj.c4f, s12 - s6, -lj.shiftf)) : 0.0;
-                w += static_cast<double>(f * r2);
-                c += inside ? 1.0 : 0.0;
-            }
-        }
-        acc[0] = u;
-        acc[1] = w;
-        acc[2] = c;
-        frc[i] = fx;
-        frc[n + i] = fy;
-        frc[2 * n + i] = fz;
+        for (int k = 0; k < m; ++k)
+            mixed_pair_gather(ri, pos[row[static_cast<size_t>(k) * n]], box, inv_box, lj, k < h, a);
+        frc[i] = a.fx;
+        frc[n + i] = a.fy;
+        frc[2 * n + i] = a.fz;
     }
+    double acc[3] = {a.u, a.w, a.c};
     if (clear_flag && i == 0) *clear_flag = 0;
     block_sum<3, FORCE_THREADS>(acc, partials + static_cast<size_t>(blockIdx.x) * 3);
 }
@@ -193,8 +141,7 @@ void forces_enqueue(md_state* s, md_list* l, const md_lj& p, int precision, doub
     // mixed precision runs on the tiled rows; FP64 keeps the per-particle gather kernel, whose single lane per
     // particle sums in the reference's order
     if (l->tiled && precision == MD_MIXED) {
-        tiled_forces_enqueue(s, l, lj.sig2, lj.c24, lj.c4, lj.shift, lj.rc2, p.cutoff, out_dev, clear_flag,
-                             ev_start, ev_stop);
+        tiled_forces_enqueue(s, l, lj, p.cutoff, out_dev, clear_flag, ev_start, ev_stop);
         return;
     }
     state_reserve_partials(s, static_cast<size_t>(blocks) * 3);

@@ -187,8 +187,9 @@ constexpr int TILE_HALO_MAX = 2560;  // LDS capacity bound of a tile's halo, in 
 size_t tiled_rows16_entries(size_t n, int padding);
 void tiled_list_enqueue(md_list* l, md_state* s, bool from_alt, const int* gate);
 // mixed-precision forces over the tiled rows
-void tiled_forces_enqueue(md_state* s, md_list* l, double sig2, double c24, double c4, double shift, double rc2,
-                          double cutoff, double* out_dev, int* clear_flag, hipEvent_t ev_start, hipEvent_t ev_stop);
+struct LJConst;
+void tiled_forces_enqueue(md_state* s, md_list* l, const LJConst& lj, double cutoff, double* out_dev,
+                          int* clear_flag, hipEvent_t ev_start, hipEvent_t ev_stop);
 
 // integrate.hip
 void kick_drift_enqueue(md_state* s, double dt, md_list* check_list);

@@ -23,6 +23,7 @@
 
 #include "device.hpp"
 #include "internal.hpp"
+#include "pair.hpp"
 
 namespace mdgpu {
 
@@ -466,114 +467,70 @@ __global__ void __launch_bounds__(TILE_THREADS)
     if (gate && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&flags[FLAG_REBUILDS], 1);
 }
 
-struct MixedLJ {
-    float sig2, c24, c4, shift, rc2;
-    float box_hi, box_lo, inv_box;
-    double box, inv_box_d, reach;  // reach: rc + list radius + skin, for the tile compactness test
-    double frame;                  // 1.5 * 2^ceil(log2 L): the centre of the tile frame's binade
+// Frame of the mixed force kernel: positions are staged relative to the tile's first particle (its nearest image),
+// in double, so a separation is one double subtraction.
+struct TileFrame {
+    double box, inv_box;
+    double reach;  // rc + list radius + skin, for the compactness test
     int tiles;
 };
 
-// Position as a float pair in the tile frame: the image of x nearest to `ref`, moved so that `ref` sits at `frame`:
-// x' = x - L round((x - ref) / L) - ref + frame (double), hi = float(x'), lo = float(x' - hi). Every x' lies in
-// [frame - L/2, frame + L/2], inside one binade [2^m, 2^(m+1)), so all hi values share one spacing and any
-// difference of two of them (also L-sized ones across the box) is exact in float: (hi_i - hi_j) + (lo_i - lo_j)
-// is then the separation rounded once to float, as accurate as narrowing the double separation.
-__device__ __forceinline__ void split_near(double x, double ref, const MixedLJ& lj, float& hi, float& lo) {
-    const double xs = x - lj.box * rint((x - ref) * lj.inv_box_d) - ref + lj.frame;
-    hi = static_cast<float>(xs);
-    lo = static_cast<float>(xs - static_cast<double>(hi));
-}
-
-// One pair in float: separation, LJ terms (Forcefield.cpp:352-446 arithmetic, with FMA), accumulation.
-template <bool WRAP>
-__device__ __forceinline__ void mixed_pair(float dxh, float dyh, float dzh, float dxl, float dyl, float dzl,
-                                           const MixedLJ& lj, bool count_energy, double& fx, double& fy, double& fz,
-                                           double& u, double& w, double& c) {
-    if (WRAP) {  // minimum image on the exact hi differences (tiles not compact against the box)
-        const float kx = rintf(dxh * lj.inv_box), ky = rintf(dyh * lj.inv_box), kz = rintf(dzh * lj.inv_box);
-        dxh = fmaf(-kx, lj.box_hi, dxh);
-        dyh = fmaf(-ky, lj.box_hi, dyh);
-        dzh = fmaf(-kz, lj.box_hi, dzh);
-        dxl = fmaf(-kx, lj.box_lo, dxl);
-        dyl = fmaf(-ky, lj.box_lo, dyl);
-        dzl = fmaf(-kz, lj.box_lo, dzl);
-    }
-    const float dx = dxh + dxl, dy = dyh + dyl, dz = dzh + dzl;
-    const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-    const float q = __builtin_amdgcn_rcpf(r2);
-    const float inv_r2 = fmaf(q, fmaf(-r2, q, 1.0f), q);
-    const float s2 = lj.sig2 * inv_r2;
-    const float s6 = s2 * s2 * s2;
-    const float s12 = s6 * s6;
-    const bool inside = r2 < lj.rc2;
-    const float f = inside ? lj.c24 * fmaf(2.0f, s12, -s6) * inv_r2 : 0.0f;
-    // pair forces reach a few hundred near contact and cancel in the sum: accumulate in double
-    fx += static_cast<double>(f * dx);
-    fy += static_cast<double>(f * dy);
-    fz += static_cast<double>(f * dz);
-    if (count_energy) {
-        u += inside ? static_cast<double>(fmaf(lj.c4, s12 - s6, -lj.shift)) : 0.0;
-        w += static_cast<double>(f * r2);
-        c += inside ? 1.0 : 0.0;
-    }
+__device__ __forceinline__ double frame_coord(double x, double ref, const TileFrame& fr) {
+    const double t = x - ref;
+    return fma(-fr.box, rint(t * fr.inv_box), t);
 }
 
 // Mixed-precision forces over the tiled list. Lane `lane` of particle i takes row entries k = lane, lane + 4, ...:
 // in staged tiles it reads 8 slots per 16-byte load (the next chunk's load is issued before the current chunk is
-// evaluated) and the float hi/lo positions from LDS; unstaged tiles gather doubles through the int32 rows.
+// evaluated) and the frame positions from LDS; unstaged tiles gather positions through the int32 rows. In a
+// compact tile (its own particles within e of the reference particle, with 2e + reach < L - 1) every listed
+// separation is already the minimum image; otherwise it is taken per pair.
 __global__ void __launch_bounds__(TILE_THREADS)
     k_forces_tiled_mixed(const double4* __restrict__ pos, size_t n, const uint16_t* __restrict__ rows16,
                          const int32_t* __restrict__ rows, const int32_t* __restrict__ count,
                          const int32_t* __restrict__ half, const int4* __restrict__ tile_ranges,
-                         const int* __restrict__ tile_nr, int hmax, MixedLJ lj, double* __restrict__ frc,
-                         double* __restrict__ partials, int* __restrict__ clear_flag) {
-    extern __shared__ float4 ldsf[];
-    float4* ha = ldsf;                                      // xh, yh, zh, xl
-    float2* hb = reinterpret_cast<float2*>(ldsf + hmax);  // yl, zl
+                         const int* __restrict__ tile_nr, int hmax, LJConst lj, TileFrame fr,
+                         double* __restrict__ frc, double* __restrict__ partials, int* __restrict__ clear_flag) {
+    extern __shared__ double ldsd[];
+    double* xs = ldsd;
+    double* ys = ldsd + hmax;
+    double* zs = ldsd + 2 * hmax;
     __shared__ int ra[RMAX], rgs[RMAX], rofs[RMAX + 1];
-    __shared__ float extent[3];
+    __shared__ int extent_bits;
     const HaloRanges hr{ra, rgs, rofs};
-    const int tile = xcd_item(static_cast<int>(blockIdx.x), lj.tiles);
+    const int tile = xcd_item(static_cast<int>(blockIdx.x), fr.tiles);
     const int nr = tile_nr[tile];
     const bool staged = nr > 0;
     const size_t i0 = static_cast<size_t>(tile) * TILE;
     const double4 ref = pos[i0];
-    if (threadIdx.x < 3) extent[threadIdx.x] = 0.0f;
+    if (threadIdx.x == 0) extent_bits = 0;
     if (staged) load_ranges(tile_ranges, tile, nr, hr);
     __syncthreads();
     if (staged) {
         stage_halo(pos, hr, nr, rofs[nr], [&](int q, const double4& p) {
-            float xh, yh, zh, xl, yl, zl;
-            split_near(p.x, ref.x, lj, xh, xl);
-            split_near(p.y, ref.y, lj, yh, yl);
-            split_near(p.z, ref.z, lj, zh, zl);
-            ha[q] = make_float4(xh, yh, zh, xl);
-            hb[q] = make_float2(yl, zl);
+            xs[q] = frame_coord(p.x, ref.x, fr);
+            ys[q] = frame_coord(p.y, ref.y, fr);
+            zs[q] = frame_coord(p.z, ref.z, fr);
         });
     }
     const int lane = threadIdx.x & (LANES - 1);
     const size_t i = i0 + (threadIdx.x >> 2);
     const bool active = i < n;
-    float xih = 0.f, yih = 0.f, zih = 0.f, xil = 0.f, yil = 0.f, zil = 0.f;
+    double xi = 0.0, yi = 0.0, zi = 0.0;
     if (active) {
         const double4 ri = pos[i];
-        split_near(ri.x, ref.x, lj, xih, xil);
-        split_near(ri.y, ref.y, lj, yih, yil);
-        split_near(ri.z, ref.z, lj, zih, zil);
-        if (lane == 0) {  // how far the tile's own particles lie from the reference particle, per axis
-            const float c = static_cast<float>(lj.frame);
-            atomicMax(reinterpret_cast<int*>(&extent[0]), __float_as_int(fabsf(xih - c)));
-            atomicMax(reinterpret_cast<int*>(&extent[1]), __float_as_int(fabsf(yih - c)));
-            atomicMax(reinterpret_cast<int*>(&extent[2]), __float_as_int(fabsf(zih - c)));
+        xi = frame_coord(ri.x, ref.x, fr);
+        yi = frame_coord(ri.y, ref.y, fr);
+        zi = frame_coord(ri.z, ref.z, fr);
+        if (lane == 0) {  // how far the tile's own particles lie from the reference particle
+            const float e = static_cast<float>(fmax(fabs(xi), fmax(fabs(yi), fabs(zi))));
+            atomicMax(&extent_bits, __float_as_int(e));
         }
     }
     __syncthreads();
-    // compact tile: every listed pair within the cutoff is separated by less than L/2 in the shifted frame
-    const float e = fmaxf(extent[0], fmaxf(extent[1], extent[2]));
-    const bool compact = staged && (2.0f * e + static_cast<float>(lj.reach) < static_cast<float>(lj.box) - 1.0f);
-    double fx = 0.0, fy = 0.0, fz = 0.0;
-    double acc[3] = {0.0, 0.0, 0.0};
+    const float e = __int_as_float(extent_bits);
+    const bool compact = staged && (2.0f * e + static_cast<float>(fr.reach) < static_cast<float>(fr.box) - 1.0f);
+    PairSums a;
     if (active) {
         const int m = count[i], h = half[i];
         if (staged) {
@@ -591,10 +548,13 @@ __global__ void __launch_bounds__(TILE_THREADS)
                         const int k = c * CHUNK + t * LANES + lane;
                         if (k < m) {
                             const int sl = static_cast<int>((words[t >> 1] >> ((t & 1) * 16)) & 0xFFFFu);
-                            const float4 a = ha[sl];
-                            const float2 b = hb[sl];
-                            mixed_pair<W>(xih - a.x, yih - a.y, zih - a.z, xil - a.w, yil - b.x, zil - b.y, lj,
-                                          k < h, fx, fy, fz, acc[0], acc[1], acc[2]);
+                            double dx = xi - xs[sl], dy = yi - ys[sl], dz = zi - zs[sl];
+                            if constexpr (W) {
+                                dx = fma(-fr.box, rint(dx * fr.inv_box), dx);
+                                dy = fma(-fr.box, rint(dy * fr.inv_box), dy);
+                                dz = fma(-fr.box, rint(dz * fr.inv_box), dz);
+                            }
+                            mixed_pair_term(dx, dy, dz, lj, k < h, a);
                         }
                     }
                     cur = nxt;
@@ -606,34 +566,24 @@ __global__ void __launch_bounds__(TILE_THREADS)
                 run(std::true_type{});
         } else {
             const double4 ri = pos[i];
-            float xh, yh, zh, xl, yl, zl;
-            split_near(ri.x, ri.x, lj, xh, xl);
-            split_near(ri.y, ri.y, lj, yh, yl);
-            split_near(ri.z, ri.z, lj, zh, zl);
-            for (int k = lane; k < m; k += LANES) {
-                const double4 rj = pos[rows[static_cast<size_t>(k) * n + i]];
-                float xjh, yjh, zjh, xjl, yjl, zjl;
-                split_near(rj.x, ri.x, lj, xjh, xjl);
-                split_near(rj.y, ri.y, lj, yjh, yjl);
-                split_near(rj.z, ri.z, lj, zjh, zjl);
-                mixed_pair<false>(xh - xjh, yh - yjh, zh - zjh, xl - xjl, yl - yjl, zl - zjl, lj, k < h, fx, fy, fz,
-                                  acc[0], acc[1], acc[2]);
-            }
+            for (int k = lane; k < m; k += LANES)
+                mixed_pair_gather(ri, pos[rows[static_cast<size_t>(k) * n + i]], fr.box, fr.inv_box, lj, k < h, a);
         }
     }
     // the particle's four lanes are adjacent in the wavefront
-    fx += __shfl_xor(fx, 1, WAVE);
-    fy += __shfl_xor(fy, 1, WAVE);
-    fz += __shfl_xor(fz, 1, WAVE);
-    fx += __shfl_xor(fx, 2, WAVE);
-    fy += __shfl_xor(fy, 2, WAVE);
-    fz += __shfl_xor(fz, 2, WAVE);
+    a.fx += __shfl_xor(a.fx, 1, WAVE);
+    a.fy += __shfl_xor(a.fy, 1, WAVE);
+    a.fz += __shfl_xor(a.fz, 1, WAVE);
+    a.fx += __shfl_xor(a.fx, 2, WAVE);
+    a.fy += __shfl_xor(a.fy, 2, WAVE);
+    a.fz += __shfl_xor(a.fz, 2, WAVE);
     if (active && lane == 0) {
-        frc[i] = fx;
-        frc[n + i] = fy;
-        frc[2 * n + i] = fz;
+        frc[i] = a.fx;
+        frc[n + i] = a.fy;
+        frc[2 * n + i] = a.fz;
     }
     if (clear_flag && blockIdx.x == 0 && threadIdx.x == 0) *clear_flag = 0;
+    double acc[3] = {a.u, a.w, a.c};
     block_sum<3, TILE_THREADS>(acc, partials + static_cast<size_t>(tile) * 3);
 }
 
@@ -685,32 +635,23 @@ void tiled_list_enqueue(md_list* l, md_state* s, bool from_alt, const int* gate)
     MD_HIP(hipGetLastError());
 }
 
-void tiled_forces_enqueue(md_state* s, md_list* l, double sig2, double c24, double c4, double shift, double rc2,
-                          double cutoff, double* out_dev, int* clear_flag, hipEvent_t ev_start, hipEvent_t ev_stop) {
+void tiled_forces_enqueue(md_state* s, md_list* l, const LJConst& lj, double cutoff, double* out_dev,
+                          int* clear_flag, hipEvent_t ev_start, hipEvent_t ev_stop) {
     hipStream_t st = s->ctx->stream;
     const size_t n = s->n;
     const int tiles = tile_count(n ? n : 1);
     state_reserve_partials(s, static_cast<size_t>(tiles) * 3);
-    MixedLJ lj;
-    lj.sig2 = static_cast<float>(sig2);
-    lj.c24 = static_cast<float>(c24);
-    lj.c4 = static_cast<float>(c4);
-    lj.shift = static_cast<float>(shift);
-    lj.rc2 = static_cast<float>(rc2);
-    lj.box = s->box;
-    lj.inv_box_d = s->inv_box;
-    lj.box_hi = static_cast<float>(s->box);
-    lj.box_lo = static_cast<float>(s->box - static_cast<double>(lj.box_hi));
-    lj.inv_box = static_cast<float>(s->inv_box);
-    lj.reach = cutoff + l->radius + l->skin;
-    lj.frame = 1.5 * std::exp2(std::ceil(std::log2(s->box)));
-    lj.tiles = tiles;
+    TileFrame fr;
+    fr.box = s->box;
+    fr.inv_box = s->inv_box;
+    fr.reach = cutoff + l->radius + l->skin;
+    fr.tiles = tiles;
     if (ev_start) MD_HIP(hipEventRecord(ev_start, st));
     if (n > 0) {
-        const size_t lds = static_cast<size_t>(l->hmax) * (sizeof(float4) + sizeof(float2));
+        const size_t lds = static_cast<size_t>(l->hmax) * 3 * sizeof(double);
         hipLaunchKernelGGL(k_forces_tiled_mixed, dim3(tiles), dim3(TILE_THREADS), lds, st, s->pos, n, l->rows16,
-                           l->rows, l->count, l->half, l->tile_ranges, l->tile_nr, l->hmax, lj, s->frc, s->partials,
-                           clear_flag);
+                           l->rows, l->count, l->half, l->tile_ranges, l->tile_nr, l->hmax, lj, fr, s->frc,
+                           s->partials, clear_flag);
         MD_HIP(hipGetLastError());
     }
     if (ev_stop) MD_HIP(hipEventRecord(ev_stop, st));

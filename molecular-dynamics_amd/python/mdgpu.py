@@ -47,6 +47,18 @@ class RunConfig(C.Structure):
                 ("force_kernel_ms", C.POINTER(C.c_double))]
 
 
+class BatchConfig(C.Structure):
+    _fields_ = [("dt", C.c_double), ("precision", C.c_int), ("steps", C.c_int), ("lj", LJ),
+                ("thermostat", C.c_int), ("tau", C.c_double), ("temperatures", C.POINTER(C.c_double)),
+                ("draws", C.POINTER(C.c_double)), ("use_graph", C.c_int), ("lanes", C.c_int),
+                ("force_kernel_ms", C.POINTER(C.c_double))]
+
+
+class BatchInfo(C.Structure):
+    _fields_ = [("count", C.c_int), ("particles", C.c_size_t), ("slots", C.c_size_t), ("padding", C.c_int),
+                ("lanes", C.c_int), ("rebuilds", C.c_longlong), ("half_pairs", C.c_size_t)]
+
+
 class AdjointConfig(C.Structure):
     _fields_ = [("dt", C.c_double), ("steps", C.c_int), ("checkpoint_every", C.c_int), ("lj", LJ),
                 ("pressure_target", C.c_double)]
@@ -104,6 +116,12 @@ _SIGS = {
     "md_fcc_lattice": (C.c_int, [C.c_int, C.c_double, _DP]),
     "md_maxwell_boltzmann": (C.c_int, [C.c_size_t, _DP, C.c_double, C.c_uint32, _DP]),
     "md_adjoint_run": (C.c_int, [_P, _P, C.POINTER(AdjointConfig), C.POINTER(AdjointResult), _DP, _DP]),
+    "md_batch_create": (C.c_int, [_P, _PP, C.c_int, C.c_double, C.c_double, _PP]),
+    "md_batch_destroy": (C.c_int, [_P]),
+    "md_batch_run": (C.c_int, [_P, C.POINTER(BatchConfig), _DP]),
+    "md_batch_get_info": (C.c_int, [_P, C.POINTER(BatchInfo)]),
+    "md_batch_replica": (C.c_int, [_P, C.c_int, _P, _IP, C.POINTER(C.c_size_t)]),
+    "md_batch_tune": (C.c_int, [_P, C.POINTER(BatchConfig), C.c_int, _DP, _IP]),
 }
 
 
@@ -442,6 +460,86 @@ class BussiRng:
         if getattr(self, "h", None):
             lib().md_bussi_rng_destroy(self.h)
             self.h = None
+
+
+class Batch:
+    """Independent replicas advanced together (md_batch_*). The states are copied in; they stay the caller's."""
+
+    def __init__(self, ctx: Context, states, cutoff: float, skin: float):
+        self.ctx = ctx
+        self.count = len(states)
+        self.n = [s.n for s in states]
+        self.boxes = [s.box for s in states]
+        handles = (C.c_void_p * self.count)(*[s.h.value for s in states])
+        self.h = C.c_void_p()
+        check(lib().md_batch_create(ctx.h, C.cast(handles, _PP), self.count, cutoff, skin, C.byref(self.h)))
+
+    def _config(self, lj, dt, steps, precision, temperatures, draws, tau, use_graph, lanes):
+        cfg = BatchConfig()
+        cfg.dt = dt
+        cfg.precision = precision
+        cfg.steps = steps
+        cfg.lj = lj
+        cfg.tau = tau
+        cfg.use_graph = 1 if use_graph else 0
+        cfg.lanes = lanes
+        keep = []
+        if draws is not None:
+            t = np.ascontiguousarray(temperatures, dtype=np.float64).reshape(self.count)
+            d = np.ascontiguousarray(draws, dtype=np.float64).reshape(self.count, steps, 2)
+            keep = [t, d]
+            cfg.thermostat = 1
+            cfg.temperatures = t.ctypes.data_as(_DP)
+            cfg.draws = d.ctypes.data_as(_DP)
+        return cfg, keep
+
+    def run(self, lj: LJ, dt: float, steps: int, precision=MD_MIXED, temperatures=None, draws=None, tau=0.1,
+            use_graph=True, lanes=0, record=True, timing=None):
+        """md_batch_run; energies (steps, count, 4). draws: (count, steps, 2) per-replica Bussi draws."""
+        cfg, keep = self._config(lj, dt, steps, precision, temperatures, draws, tau,
+                                 use_graph and timing is None, lanes)
+        ms = C.c_double(0.0)
+        if timing is not None:
+            cfg.force_kernel_ms = C.pointer(ms)
+        energies = np.empty((steps, self.count, 4)) if (record and steps > 0) else None
+        check(lib().md_batch_run(self.h, C.byref(cfg), None if energies is None else energies.ctypes.data_as(_DP)))
+        if timing is not None:
+            timing.append(ms.value)
+        del keep
+        return energies
+
+    def tune(self, lj: LJ, dt: float, steps: int, trial_steps: int, precision=MD_MIXED, temperatures=None,
+             draws=None, tau=0.1):
+        """md_batch_tune: (best lanes, ms per step for 1, 2, 4 lanes)."""
+        cfg, keep = self._config(lj, dt, steps, precision, temperatures, draws, tau, False, 0)
+        ms = np.zeros(3)
+        best = C.c_int()
+        check(lib().md_batch_tune(self.h, C.byref(cfg), trial_steps, ms.ctypes.data_as(_DP), C.byref(best)))
+        del keep
+        return best.value, ms
+
+    def info(self) -> BatchInfo:
+        out = BatchInfo()
+        check(lib().md_batch_get_info(self.h, C.byref(out)))
+        return out
+
+    def replica(self, r: int, dst: "State"):
+        """Copies replica r into dst; returns (rebuilds, half_pairs)."""
+        reb = C.c_int()
+        hp = C.c_size_t()
+        check(lib().md_batch_replica(self.h, r, dst.h, C.byref(reb), C.byref(hp)))
+        return reb.value, hp.value
+
+    def close(self):
+        if self.h:
+            lib().md_batch_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def device_count() -> int:
