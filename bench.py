@@ -30,6 +30,8 @@ WORKLOADS = {
              "LJ NVE N=4,000,000 (FCC 100^3x4) rho*=0.8442 T*=0.722 seed 1 rc=2.5 skin=0.3 dt=0.005 mixed"),
     "lj256k": (40, 0.8442, 1.0, 1, 0.3, mdgpu.MD_MIXED,
                "LJ NVE N=256,000 (FCC 40^3x4) rho*=0.8442 T*=1.0 seed 1 rc=2.5 skin=0.3 dt=0.005 mixed"),
+    "lj4m_fp64": (100, 0.8442, 0.722, 1, 0.3, mdgpu.MD_FP64,
+                  "LJ NVE N=4,000,000 (FCC 100^3x4) rho*=0.8442 T*=0.722 seed 1 rc=2.5 skin=0.3 dt=0.005 fp64"),
     "lj2048": (8, 0.8442, 0.722, 1, 0.3, mdgpu.MD_FP64,
                "LJ NVE N=2048 (FCC 8^3x4) rho*=0.8442 T*=0.722 seed 1 rc=2.5 skin=0.3 dt=0.005 fp64"),
 }
@@ -150,7 +152,7 @@ def run_batched(args):
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_b_forces",
                 "avg_launch_ms": round(avg_ms, 5), "algorithmic_bytes_per_launch": round(bytes_per_atom * n_total)}
     # plain: the first `plain_replicas` replicas one after another, each its own md_run
-    sample = specs[:args.plain_replicas]
+    sample = specs[:args.plain_replicas] if not args.no_plain else []
     p_steps = min(steps, args.plain_steps)
     plain_states = [mdgpu.fcc_system(ctx, c, rho, t, seed) for c, rho, t, seed in sample]
     lists = []
@@ -168,8 +170,9 @@ def run_batched(args):
     ctx.synchronize()
     p_el = time.perf_counter() - t0
     p_atoms = sum(st.n for st in plain_states)
-    plain = p_atoms * p_steps / p_el
-    log(f"plain replica-by-replica: {len(sample)} replicas x {p_steps} steps in {p_el:.3f} s: {plain:.4g}")
+    plain = p_atoms * p_steps / p_el if sample else None
+    if sample:
+        log(f"plain replica-by-replica: {len(sample)} replicas x {p_steps} steps in {p_el:.3f} s: {plain:.4g}")
     out = {
         "metric": "batched atom-steps/s (512 Bussi replicas, 1 GPU)" if args.replicas == 512
         else "batched atom-steps/s (1 GPU)",
@@ -183,7 +186,7 @@ def run_batched(args):
         "roofline": roofline, "cpu_baseline": None,
         "plain_replica_by_replica": {"value": plain, "unit": "atom-steps/s", "replicas": len(sample),
                                      "steps": p_steps, "path": "md_run per replica, hipGraph"},
-        "speedup_vs_plain": value / plain,
+        "speedup_vs_plain": value / plain if plain else None,
         "autotuner_ms_per_step": {"lanes1": tune_ms[0], "lanes2": tune_ms[1], "lanes4": tune_ms[2]},
         "rebuilds": int(info.rebuilds - reb0), "k_full_mean": k_full,
         "mean_final_temperature_over_target": float(np.mean(
@@ -208,16 +211,18 @@ def run_adjoint(args):
     res, gp, gv = mdgpu.adjoint_run(state, nl, lj, 0.005, steps, 50)
     ctx.synchronize()
     wall = time.perf_counter() - t0
-    twin = mdgpu.fcc_system(ctx, 20, 0.8442, 0.9, 1)
-    nl2 = mdgpu.NeighbourList(ctx, 2.5, 0.3)
-    nl2.build(twin)
-    nl2.forces(twin, lj, mdgpu.MD_FP64)
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    nl2.run(twin, lj, 0.005, steps, precision=mdgpu.MD_FP64, use_graph=True, record=False)
-    ctx.synchronize()
-    plain_ms = 1000.0 * (time.perf_counter() - t0)
-    log(f"adjoint: forward {res.forward_ms:.1f} ms, reverse {res.reverse_ms:.1f} ms, plain md_run {plain_ms:.1f} ms")
+    plain_ms = None
+    if not args.no_plain:
+        twin = mdgpu.fcc_system(ctx, 20, 0.8442, 0.9, 1)
+        nl2 = mdgpu.NeighbourList(ctx, 2.5, 0.3)
+        nl2.build(twin)
+        nl2.forces(twin, lj, mdgpu.MD_FP64)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        nl2.run(twin, lj, 0.005, steps, precision=mdgpu.MD_FP64, use_graph=True, record=False)
+        ctx.synchronize()
+        plain_ms = 1000.0 * (time.perf_counter() - t0)
+    log(f"adjoint: forward {res.forward_ms:.1f} ms, reverse {res.reverse_ms:.1f} ms, plain md_run {plain_ms} ms")
     out = {
         "metric": "adjoint time / forward time (N=32k, 2000 steps, checkpoints every 50)",
         "value": res.reverse_ms / res.forward_ms, "unit": "ratio", "n_gpus": 1, "steps": steps, "warmup": 50,
@@ -227,7 +232,7 @@ def run_adjoint(args):
                                "fp64, loss = mean U + (mean P - 1)^2, checkpoints every 50 steps", "N": state.n},
         "roofline": None, "cpu_baseline": None,
         "forward_ms": res.forward_ms, "reverse_ms": res.reverse_ms, "wall_ms": 1000.0 * wall,
-        "plain_md_run_ms": plain_ms, "reverse_over_plain_forward": res.reverse_ms / plain_ms,
+        "plain_md_run_ms": plain_ms, "reverse_over_plain_forward": res.reverse_ms / plain_ms if plain_ms else None,
         "loss": res.loss, "d_epsilon": res.d_epsilon, "d_sigma": res.d_sigma,
     }
     print(json.dumps(out), flush=True)
@@ -242,6 +247,8 @@ def main():
     ap.add_argument("--replicas", type=int, default=512, help="batched workload: replica count")
     ap.add_argument("--plain-replicas", type=int, default=32, help="batched workload: replicas in the plain sample")
     ap.add_argument("--plain-steps", type=int, default=200, help="batched workload: steps of the plain sample")
+    ap.add_argument("--no-plain", action="store_true",
+                    help="batched / adjoint workloads: skip the plain md_run comparison")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--graph", action="store_true", help="replay a captured step instead of eager launches")

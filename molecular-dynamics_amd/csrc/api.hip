@@ -106,7 +106,15 @@ __global__ void __launch_bounds__(256) k_ke_final(const double* __restrict__ par
     if (threadIdx.x == 0) *out = total;
 }
 
-void state_reserve_partials(md_state* s, size_t count) { dev_reserve(s->partials, s->partials_cap, count); }
+void state_reserve_partials(md_state* s, size_t count) {
+    if (count > s->partials_cap || !s->partials) {
+        // growing frees and allocates: never inside a stream capture (callers reserve before they capture)
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        MD_HIP(hipStreamIsCapturing(s->ctx->stream, &cs));
+        if (cs != hipStreamCaptureStatusNone) fail(MD_ERR_STATE, "reduction buffer would grow inside a graph capture");
+    }
+    dev_reserve(s->partials, s->partials_cap, count);
+}
 
 void kinetic_exact_enqueue(md_state* s, double* out_dev) {
     hipStream_t st = s->ctx->stream;

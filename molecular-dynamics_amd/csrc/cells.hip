@@ -495,7 +495,7 @@ static void list_kernel_enqueue(md_list* l, md_state* s, bool from_alt, const in
         cb.cell_of = l->cell_of;
     }
     k_clear_build_flags<<<1, 1, 0, st>>>(l->flags, pair_total_of(l), gate);
-    if (tiling_enabled()) {
+    if (tiling_enabled(n)) {
         tiled_list_enqueue(l, s, from_alt, gate);
         l->tiled = true;
         return;

@@ -1,5 +1,6 @@
 // Velocity Verlet (reference: Integrator.cpp:7-71), the Bussi thermostat update (Thermostat.cpp:36-58) and the
 // multi-step driver md_run (md_sim's production loop, main.cpp:272-304) that keeps the whole step on the GPU.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -216,8 +217,9 @@ int md_run(md_state* s, md_list* l, const md_run_config* cfg, double* energies) 
     const double kbar = 0.5 * dof * cfg->temperature;
     const double c = cfg->thermostat ? std::exp(-cfg->dt / cfg->tau) : 0.0;
     const size_t nvel = 3 * s->n;
-    // reserve the partials once, before any capture
-    state_reserve_partials(s, static_cast<size_t>(blocks_for(s->n, 256)) * 3 + 8);
+    // reserve the partials once, before any capture (no allocation may happen inside it): per-block rows of the
+    // 256-thread kernels, or per-tile rows of the tiled force kernel, whichever is more
+    state_reserve_partials(s, static_cast<size_t>(std::max<size_t>(blocks_for(s->n, 256), tile_count(s->n))) * 3 + 8);
 
     // optional force-kernel timing: one event pair per step, eager launches only
     const bool timed = cfg->force_kernel_ms != nullptr && !cfg->use_graph;

@@ -141,7 +141,7 @@ struct md_list {
     int32_t* half = nullptr;         // n
     int* flags = nullptr;            // FLAG_COUNT ints on the device
     // tiled view of the same rows (tiled.hip): LDS slots of each tile's staged halo
-    uint16_t* rows16 = nullptr;      // padding * n, column-major
+    uint16_t* rows16 = nullptr;      // ceil(padding / 32) chunks of 32 entries per particle, chunk-major
     size_t rows16_cap = 0;
     int4* tile_ranges = nullptr;     // tiles * TILE_RANGES (first cell, first particle, first slot, count)
     int* tile_nr = nullptr;          // tiles; number of ranges, -1 if the tile is not staged
@@ -179,7 +179,7 @@ void set_error(const std::string& m);
 
 // tiled.hip. The tiled list and force path is the default; MDGPU_UNTILED=1 selects the per-particle global
 // gather kernels instead (same results: the rows are identical).
-bool tiling_enabled();
+bool tiling_enabled(size_t n);
 int tile_count(size_t n);
 constexpr int TILE_PARTICLES = 128;
 constexpr int TILE_RANGES = 64;

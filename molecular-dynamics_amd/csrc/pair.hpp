@@ -61,12 +61,14 @@ __device__ __forceinline__ void mixed_pair_term(double dx, double dy, double dz,
     const float s6 = s2 * s2 * s2;
     const float s12 = s6 * s6;
     const bool inside = r2 < lj.rc2f;
-    const double f = inside ? static_cast<double>(lj.c24f * fmaf(2.0f, s12, -s6) * inv_r2) : 0.0;
+    const float ff = inside ? lj.c24f * fmaf(2.0f, s12, -s6) * inv_r2 : 0.0f;
+    const double f = static_cast<double>(ff);
     a.fx = fma(f, dx, a.fx);
     a.fy = fma(f, dy, a.fy);
     a.fz = fma(f, dz, a.fz);
     if (count_energy) {
-        a.u += inside ? static_cast<double>(fmaf(lj.c4f, s12 - s6, -lj.shiftf)) : 0.0;
+        const float up = inside ? fmaf(lj.c4f, s12 - s6, -lj.shiftf) : 0.0f;
+        a.u += static_cast<double>(up);
         a.w = fma(f, r2d, a.w);
         a.c += inside ? 1.0 : 0.0;
     }

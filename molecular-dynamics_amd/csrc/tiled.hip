@@ -11,9 +11,10 @@
 // A rebuild runs two kernels: k_tile_setup (one wavefront per tile) finds the halo ranges and their LDS slots;
 // k_list_build_tiled stages the halo, tabulates the 27 stencil cells of every cell the tile spans, and tests the
 // candidates. It writes each row twice: as global indices (int32, column-major: what the reference's half-list
-// view, the RDF, the adjoint and the FP64 force kernel read) and as LDS slots of the tile's halo (uint16). The
-// slot rows are stored in chunks of 32 entries per particle, each lane's 8 entries (k = lane, lane + 4, ...)
-// contiguous, so that the force kernel fetches 8 slots with one 16-byte load per lane. Rows keep the reference's
+// view, the RDF, the adjoint and the FP64 force kernel read) and as LDS slots of the tile's halo (uint16, in
+// chunks of 32 entries: chunk c of particle i holds entries 32c..32c+31 at ((c n + i) 32 + 8 (k % 4) + (k / 4) % 8),
+// so each of the force kernel's 4 lanes fetches its 8 entries k = lane, lane + 4, ... of a chunk with one 16-byte
+// load and a wavefront's 16 particles read 1 KB contiguously). Rows keep the reference's
 // order (j > i in 27-cell stencil order, then j < i increasing): the 4 lanes test 4 consecutive candidates and a
 // wavefront ballot compacts the accepted ones in order.
 #include <cmath>
@@ -251,9 +252,9 @@ __global__ void __launch_bounds__(TILE_THREADS)
                        const int* __restrict__ tile_nr, TiledCopyBack cb, const int* gate) {
     if (gate && *gate == 0) return;
     extern __shared__ double lds[];
-    double* xs = lds;
-    double* ys = lds + ta.hmax;
-    double* zs = lds + 2 * ta.hmax;
+    double* xs = lds;  // fixed strides: the three reads of a slot share one address plus immediate offsets
+    double* ys = lds + TILE_HALO_MAX;
+    double* zs = lds + 2 * TILE_HALO_MAX;
     __shared__ int ra[RMAX], rgs[RMAX], rofs[RMAX + 1];
     __shared__ StencilEntry table[CMAX * 27];
     __shared__ unsigned char sorted_stencil[CMAX * 27];
@@ -322,7 +323,7 @@ __global__ void __launch_bounds__(TILE_THREADS)
         const uint32_t n32 = static_cast<uint32_t>(n);
         int32_t* const row = rows + i;
         uint16_t* const row16 = rows16 + i * CHUNK;
-        const uint64_t chunk_stride = static_cast<uint64_t>(n32) * CHUNK;
+        const uint32_t chunk_stride = n32 * CHUNK;  // entries between chunks (n < 2^27 on this path)
         // Candidates j in [b, e) of one cell, 4 at a time; `base` is the LDS slot of j = b. Staged tiles read the
         // halo from LDS, the others gather from global memory (a compile-time choice: a run-time select between
         // the two would turn every read into a generic flat access). With INTERIOR the separation needs no
@@ -332,37 +333,38 @@ __global__ void __launch_bounds__(TILE_THREADS)
             constexpr bool STAGED = decltype(staged_tag)::value;
             constexpr bool INTERIOR = decltype(interior_tag)::value;
             for (uint32_t jb = b; jb < e; jb += LANES) {
+                // past the cell's end a lane re-reads its first candidate and rejects it
                 const uint32_t j = jb + lane;
-                bool accept = false;
+                const bool valid = j < e;
+                const uint32_t jj = valid ? j : b;
                 int slot = 0;
-                if (j < e) {
-                    double xj, yj, zj;
-                    if constexpr (STAGED) {
-                        slot = base + static_cast<int>(j - b);
-                        xj = xs[slot];
-                        yj = ys[slot];
-                        zj = zs[slot];
-                    } else {
-                        const double4 rj = pos[j];
-                        xj = rj.x;
-                        yj = rj.y;
-                        zj = rj.z;
-                    }
-                    double dx = ri.x - xj, dy = ri.y - yj, dz = ri.z - zj;
-                    if constexpr (!INTERIOR) {
-                        dx = min_image(dx, box, inv_box);
-                        dy = min_image(dy, box, inv_box);
-                        dz = min_image(dz, box, inv_box);
-                    }
-                    accept = dx * dx + dy * dy + dz * dz < radius2;
+                double xj, yj, zj;
+                if constexpr (STAGED) {
+                    slot = base + static_cast<int>(jj - b);
+                    xj = xs[slot];
+                    yj = ys[slot];
+                    zj = zs[slot];
+                } else {
+                    const double4 rj = pos[jj];
+                    xj = rj.x;
+                    yj = rj.y;
+                    zj = rj.z;
                 }
+                double dx = ri.x - xj, dy = ri.y - yj, dz = ri.z - zj;
+                if constexpr (!INTERIOR) {
+                    dx = min_image(dx, box, inv_box);
+                    dy = min_image(dy, box, inv_box);
+                    dz = min_image(dz, box, inv_box);
+                }
+                const bool near = dx * dx + dy * dy + dz * dz < radius2;
+                const bool accept = valid & near;  // both evaluated: no branch around the distance test
                 const unsigned group = static_cast<unsigned>((__ballot(accept) >> group_shift) & 0xFull);
                 if (accept) {
                     const unsigned at = static_cast<unsigned>(cnt) + __popc(group & lower_mask);
                     if (at < static_cast<unsigned>(padding)) {
                         row[static_cast<uint64_t>(at) * n32] = static_cast<int32_t>(j);
-                        row16[static_cast<uint64_t>(at >> 5) * chunk_stride + ((at & 3u) << 3) + ((at >> 2) & 7u)] =
-                            static_cast<uint16_t>(slot);
+                        const uint32_t within = ((at & 3u) << 3) | ((at >> 2) & 7u);
+                        row16[static_cast<uint64_t>(at >> 5) * chunk_stride + within] = static_cast<uint16_t>(slot);
                     }
                 }
                 cnt += __popc(group);
@@ -489,12 +491,12 @@ __global__ void __launch_bounds__(TILE_THREADS)
     k_forces_tiled_mixed(const double4* __restrict__ pos, size_t n, const uint16_t* __restrict__ rows16,
                          const int32_t* __restrict__ rows, const int32_t* __restrict__ count,
                          const int32_t* __restrict__ half, const int4* __restrict__ tile_ranges,
-                         const int* __restrict__ tile_nr, int hmax, LJConst lj, TileFrame fr,
+                         const int* __restrict__ tile_nr, LJConst lj, TileFrame fr,
                          double* __restrict__ frc, double* __restrict__ partials, int* __restrict__ clear_flag) {
     extern __shared__ double ldsd[];
     double* xs = ldsd;
-    double* ys = ldsd + hmax;
-    double* zs = ldsd + 2 * hmax;
+    double* ys = ldsd + TILE_HALO_MAX;
+    double* zs = ldsd + 2 * TILE_HALO_MAX;
     __shared__ int ra[RMAX], rgs[RMAX], rofs[RMAX + 1];
     __shared__ int extent_bits;
     const HaloRanges hr{ra, rgs, rofs};
@@ -587,7 +589,9 @@ __global__ void __launch_bounds__(TILE_THREADS)
     block_sum<3, TILE_THREADS>(acc, partials + static_cast<size_t>(tile) * 3);
 }
 
-bool tiling_enabled() {
+bool tiling_enabled(size_t n) {
+    // the slot rows index chunks with 32-bit strides: 32 n < 2^32
+    if (n >= (size_t{1} << 27)) return false;
     static const bool enabled = [] {
         const char* v = std::getenv("MDGPU_UNTILED");
         return !(v && v[0] == '1');
@@ -628,8 +632,8 @@ void tiled_list_enqueue(md_list* l, md_state* s, bool from_alt, const int* gate)
                        n, cells, l->cell_start, ta.side, ta.use_cells, ta.hmax, tiles, l->tile_ranges, l->tile_nr,
                        gate);
     hipLaunchKernelGGL(k_list_build_tiled, dim3(tiles), dim3(TILE_THREADS),
-                       static_cast<size_t>(l->hmax) * 3 * sizeof(double), st, from_alt ? s->pos_alt : s->pos, n,
-                       cells, l->cell_start, ta, s->box, s->inv_box, l->radius2, l->padding, l->rows, l->rows16,
+                       static_cast<size_t>(TILE_HALO_MAX) * 3 * sizeof(double), st, from_alt ? s->pos_alt : s->pos,
+                       n, cells, l->cell_start, ta, s->box, s->inv_box, l->radius2, l->padding, l->rows, l->rows16,
                        l->count, l->half, l->flags, reinterpret_cast<unsigned long long*>(l->flags + FLAG_COUNT),
                        l->tile_ranges, l->tile_nr, cb, gate);
     MD_HIP(hipGetLastError());
@@ -648,10 +652,10 @@ void tiled_forces_enqueue(md_state* s, md_list* l, const LJConst& lj, double cut
     fr.tiles = tiles;
     if (ev_start) MD_HIP(hipEventRecord(ev_start, st));
     if (n > 0) {
-        const size_t lds = static_cast<size_t>(l->hmax) * 3 * sizeof(double);
+        const size_t lds = static_cast<size_t>(TILE_HALO_MAX) * 3 * sizeof(double);
         hipLaunchKernelGGL(k_forces_tiled_mixed, dim3(tiles), dim3(TILE_THREADS), lds, st, s->pos, n, l->rows16,
-                           l->rows, l->count, l->half, l->tile_ranges, l->tile_nr, l->hmax, lj, fr, s->frc,
-                           s->partials, clear_flag);
+                           l->rows, l->count, l->half, l->tile_ranges, l->tile_nr, lj, fr, s->frc, s->partials,
+                           clear_flag);
         MD_HIP(hipGetLastError());
     }
     if (ev_stop) MD_HIP(hipEventRecord(ev_stop, st));
