@@ -431,10 +431,10 @@ void list_prepare(md_list* l, md_state* s) {
         const double per_cell = static_cast<double>(s->n) / static_cast<double>(ncells);
         const double est = 1.5 * 9.0 * (TILE_PARTICLES / (per_cell > 0.5 ? per_cell : 0.5) + 3.0) * per_cell;
         long h = static_cast<long>(std::ceil(est / 8.0)) * 8;
-        h = std::max<long>(256, std::min<long>(TILE_HALO_MAX, h));
+        h = std::max<long>(256, std::min<long>(TILE_HALO_MAX - 8, h));  // the last slot is the build's ghost
         const long all = static_cast<long>((s->n + 7) / 8 * 8);
         l->hmax = static_cast<int>(std::min<long>(h, all > 0 ? all : 8));
-        if (static_cast<long>(s->n) <= TILE_HALO_MAX) l->hmax = static_cast<int>(all > 0 ? all : 8);
+        if (static_cast<long>(s->n) <= TILE_HALO_MAX - 8) l->hmax = static_cast<int>(all > 0 ? all : 8);
     }
     l->n = s->n;
 }
@@ -496,8 +496,9 @@ static void list_kernel_enqueue(md_list* l, md_state* s, bool from_alt, const in
     }
     k_clear_build_flags<<<1, 1, 0, st>>>(l->flags, pair_total_of(l), gate);
     if (tiling_enabled(n)) {
-        tiled_list_enqueue(l, s, from_alt, gate);
+        tiled_list_enqueue(l, s, from_alt, gate, !l->slots_only);
         l->tiled = true;
+        if (!l->slots_only) l->rows32_valid = true;
         return;
     }
     k_list_build<<<blocks_for(n ? n : 1, 256), 256, 0, st>>>(
@@ -506,6 +507,14 @@ static void list_kernel_enqueue(md_list* l, md_state* s, bool from_alt, const in
         pair_total_of(l), cb, gate);
     MD_HIP(hipGetLastError());
     l->tiled = false;
+    l->rows32_valid = true;
+}
+
+void ensure_rows32(md_list* l) {
+    if (l->rows32_valid || !l->built) return;
+    if (!l->tiled) fail(MD_ERR_STATE, "internal: int32 rows missing without slot rows");
+    tiled_rows32_enqueue(l);
+    l->rows32_valid = true;
 }
 
 void cells_build(md_list* l, md_state* s, double min_cell, int side) {
@@ -633,6 +642,7 @@ static void list_fixed_enqueue(md_list* l, md_state* s, const int* gate) {
     cells_enqueue(l, s, l->side / s->box, false, gate);
     k_clear_build_flags<<<1, 1, 0, st>>>(l->flags, pair_total_of(l), gate);
     l->tiled = false;
+    l->rows32_valid = true;
     k_list_build_fixed<<<blocks_for(n ? n : 1, 256), 256, 0, st>>>(
         s->pos, n, l->cell_of, l->cell_start, l->order, l->side, s->box, s->inv_box, l->radius2,
         l->side >= 3 ? 1 : 0, l->padding, l->rows, l->count, l->half, l->flags, pair_total_of(l), l->ref, gate);
@@ -811,6 +821,7 @@ int md_list_half(md_list* l, uint64_t* offsets, uint64_t* neighbours) {
     std::vector<int32_t> half(n), rows(neighbours ? static_cast<size_t>(l->padding) * n : 0);
     if (n) MD_HIP(hipMemcpyAsync(half.data(), l->half, sizeof(int32_t) * n, hipMemcpyDeviceToHost, st));
     if (neighbours && n)
+        ensure_rows32(l);
         MD_HIP(hipMemcpyAsync(rows.data(), l->rows, sizeof(int32_t) * rows.size(), hipMemcpyDeviceToHost, st));
     MD_HIP(hipStreamSynchronize(st));
     uint64_t at = 0;
@@ -834,6 +845,7 @@ int md_list_full(md_list* l, int32_t* counts, int32_t* half_counts, int32_t* row
     if (half_counts && n) MD_HIP(hipMemcpyAsync(half_counts, l->half, sizeof(int32_t) * n, hipMemcpyDeviceToHost, st));
     std::vector<int32_t> rows(rows_out ? static_cast<size_t>(l->padding) * n : 0);
     if (rows_out && n)
+        ensure_rows32(l);
         MD_HIP(hipMemcpyAsync(rows.data(), l->rows, sizeof(int32_t) * rows.size(), hipMemcpyDeviceToHost, st));
     MD_HIP(hipStreamSynchronize(st));
     if (rows_out)

@@ -144,6 +144,7 @@ void forces_enqueue(md_state* s, md_list* l, const md_lj& p, int precision, doub
         tiled_forces_enqueue(s, l, lj, p.cutoff, out_dev, clear_flag, ev_start, ev_stop);
         return;
     }
+    ensure_rows32(l);
     state_reserve_partials(s, static_cast<size_t>(blocks) * 3);
     if (ev_start) MD_HIP(hipEventRecord(ev_start, st));
     if (precision == MD_MIXED)

@@ -197,8 +197,22 @@ int md_run(md_state* s, md_list* l, const md_run_config* cfg, double* energies) 
     if (cfg->precision != MD_FP64 && cfg->precision != MD_MIXED) fail(MD_ERR_ARG, "unknown precision");
     if (cfg->thermostat && !cfg->draws) fail(MD_ERR_ARG, "Bussi thermostat needs draws");
     check_cutoff(s, cfg->lj);
-    if (!l->built || l->n != s->n) list_rebuild(l, s);
+    // (a list last built without the tiled slot rows, e.g. by the adjoint's fixed-order build, is rebuilt first:
+    // the captured step picks the tiled kernels)
+    if (!l->built || l->n != s->n || (tiling_enabled(s->n) && !l->tiled)) list_rebuild(l, s);
     if (cfg->steps == 0 || s->n == 0) return MD_OK;
+    // mixed runs on the tiled path need only the 16-bit slot rows: the device-gated rebuilds skip the int32 rows,
+    // which are materialised again when a caller needs them
+    if (cfg->precision == MD_FP64) ensure_rows32(l);
+    struct SlotsOnly {
+        md_list* l;
+        bool rebuilt = false;
+        ~SlotsOnly() {
+            if (l->slots_only && rebuilt) l->rows32_valid = false;
+            l->slots_only = false;
+        }
+    } slots_guard{l};
+    l->slots_only = cfg->precision == MD_MIXED && tiling_enabled(s->n);
     hipStream_t st = s->ctx->stream;
     const int steps = cfg->steps;
     RunBuffers rb;
@@ -299,6 +313,7 @@ int md_run(md_state* s, md_list* l, const md_run_config* cfg, double* energies) 
         *cfg->force_kernel_ms = total_ms;
     }
     l->rebuilds += hf[FLAG_REBUILDS];
+    slots_guard.rebuilt = hf[FLAG_REBUILDS] > 0;
     if (hf[FLAG_REBUILDS] > 0) {
         // the words after the flags hold the half-pair total of the last build
         unsigned long long total;

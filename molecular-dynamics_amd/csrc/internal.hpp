@@ -148,6 +148,8 @@ struct md_list {
     size_t tiles_cap = 0;
     int hmax = 0;                    // LDS capacity of a tile's halo, in particles
     bool tiled = false;              // rows16 / tile_* describe the current rows
+    bool rows32_valid = false;       // `rows` (int32) matches the current list
+    bool slots_only = false;         // device-gated builds may write only rows16 (mixed md_run)
 };
 
 namespace mdgpu {
@@ -185,7 +187,9 @@ constexpr int TILE_PARTICLES = 128;
 constexpr int TILE_RANGES = 64;
 constexpr int TILE_HALO_MAX = 2560;  // LDS capacity bound of a tile's halo, in particles
 size_t tiled_rows16_entries(size_t n, int padding);
-void tiled_list_enqueue(md_list* l, md_state* s, bool from_alt, const int* gate);
+void tiled_list_enqueue(md_list* l, md_state* s, bool from_alt, const int* gate, bool write32);
+void tiled_rows32_enqueue(md_list* l);                          // int32 rows from the slot rows
+void ensure_rows32(md_list* l);                                 // (cells.hip) materialise if stale
 // mixed-precision forces over the tiled rows
 struct LJConst;
 void tiled_forces_enqueue(md_state* s, md_list* l, const LJConst& lj, double cutoff, double* out_dev,

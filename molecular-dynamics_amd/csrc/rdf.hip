@@ -57,6 +57,7 @@ extern "C" int md_rdf_sample(md_state* s, md_list* l, double rmax, int bins, uin
         if (!s || !l || !counts) fail(MD_ERR_ARG, "state/list/counts is NULL");
         if (bins <= 0) fail(MD_ERR_ARG, "bins must be positive");
         validate_list(s, l);
+        ensure_rows32(l);
         hipStream_t st = s->ctx->stream;
         unsigned long long* hist = dev_alloc<unsigned long long>(static_cast<size_t>(bins));
         std::vector<unsigned long long> host(static_cast<size_t>(bins));

@@ -11,10 +11,11 @@
 // A rebuild runs two kernels: k_tile_setup (one wavefront per tile) finds the halo ranges and their LDS slots;
 // k_list_build_tiled stages the halo, tabulates the 27 stencil cells of every cell the tile spans, and tests the
 // candidates. It writes each row twice: as global indices (int32, column-major: what the reference's half-list
-// view, the RDF, the adjoint and the FP64 force kernel read) and as LDS slots of the tile's halo (uint16, in
-// chunks of 32 entries: chunk c of particle i holds entries 32c..32c+31 at ((c n + i) 32 + 8 (k % 4) + (k / 4) % 8),
-// so each of the force kernel's 4 lanes fetches its 8 entries k = lane, lane + 4, ... of a chunk with one 16-byte
-// load and a wavefront's 16 particles read 1 KB contiguously). Rows keep the reference's
+// view, the RDF, the adjoint and the FP64 force kernel read) and as LDS slots of the tile's halo (uint16, tile-major:
+// each tile owns row_len / 32 chunks of 128 particles x 32 entries; entry k of the tile's particle p sits at chunk
+// k / 32, offset 32 p + 8 (k % 4) + (k / 4) % 8, so each of the force kernel's 4 lanes fetches its 8 entries
+// k = lane, lane + 4, ... of a chunk with one 16-byte load, a wavefront's 16 particles read 1 KB contiguously, and
+// chunks are a constant 4096 entries apart). Rows keep the reference's
 // order (j > i in 27-cell stencil order, then j < i increasing): the 4 lanes test 4 consecutive candidates and a
 // wavefront ballot compacts the accepted ones in order.
 #include <cmath>
@@ -36,8 +37,19 @@ constexpr int CR_CAP = 192;  // cell ranges before merging (9 neighbour columns 
 constexpr int CMAX = 24;     // own cells of a tile covered by the stencil table
 constexpr int STAGE_ROUNDS = (TILE_HALO_MAX + TILE_THREADS - 1) / TILE_THREADS;
 constexpr int CHUNK = 32;    // slot-row entries per particle and chunk
+constexpr int CHUNK_STRIDE = TILE * CHUNK;  // entries between a tile's chunks
+constexpr int GHOST_SLOT = TILE_HALO_MAX - 1;  // LDS slot holding NaN coordinates: never a neighbour
+
+// Position of a particle's first slot-row entry, and of entry k relative to it.
+__device__ __forceinline__ size_t slot_row_base(size_t i, int row_len) {
+    return (i / TILE) * static_cast<size_t>(TILE) * row_len + (i % TILE) * CHUNK;
+}
+__device__ __forceinline__ uint32_t slot_entry(uint32_t k) {
+    return ((k & ~31u) << 7) | ((k & 3u) << 3) | ((k >> 2) & 7u);  // (k / 32) * 4096 + 8 (k % 4) + (k / 4) % 8
+}
 constexpr int SETUP_TILES = 4;
 static_assert(RMAX == WAVE, "one lane per halo range in k_tile_setup");
+static_assert(CHUNK_STRIDE == 4096, "slot_entry shifts the chunk index by 7 + 5 bits");
 
 // Three cell coordinates in increasing order, indexed without a private array (which would live in scratch).
 struct Sorted3 {
@@ -202,10 +214,29 @@ template <class Store>
 __device__ __forceinline__ void stage_halo(const double4* __restrict__ pos, HaloRanges h, int nr, int total,
                                            Store store) {
     int src[STAGE_ROUNDS];
+    // the thread's slots grow by TILE_THREADS per round: one binary search, then a short forward walk per round
+    int r = 0;
+    {
+        int lo = 0, hi = nr - 1;
+        const int q = static_cast<int>(threadIdx.x);
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (h.rofs[mid] <= q)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        r = lo;
+    }
 #pragma unroll
     for (int u = 0; u < STAGE_ROUNDS; ++u) {
         const int q = static_cast<int>(threadIdx.x) + u * TILE_THREADS;
-        src[u] = q < total ? halo_particle(q, h, nr) : -1;
+        if (q < total) {
+            while (h.rofs[r + 1] <= q) ++r;  // rofs[nr] = total > q ends the walk
+            src[u] = h.rgs[r] + (q - h.rofs[r]);
+        } else {
+            src[u] = -1;
+        }
     }
     double4 p[STAGE_ROUNDS];
 #pragma unroll
@@ -239,14 +270,87 @@ struct TiledCopyBack {
     int* cell_of;
 };
 
-// Stencil table entry of one (own cell, neighbour cell) pair: particles [b, e) of the neighbour cell, LDS slot of
-// its first particle, neighbour cell id.
-using StencilEntry = int4;
+// One stencil segment of an own cell: neighbour cell id, LDS slot of its first particle, its particle count.
+struct Segment {
+    int nc;
+    uint16_t base, len;
+};
 
+constexpr int LIST_CAP = 6144;  // flat candidate lists of a tile's own cells, in LDS slots
+constexpr int OWN_STENCIL = 13; // stencil index of (0, 0, 0)
+
+// Candidates j of particle i, walked in the order of its row. `STAGED` reads the halo from LDS, else from global
+// memory; `INTERIOR` drops the minimum image where it is the identity (every candidate within 3 cells = 3L/side
+// < L/2 per axis, so round(d / L) = 0 and the reference's d - L * round(d / L) is d itself). Runs on the 4 lanes
+// of a particle: they test 4 consecutive candidates and a wavefront ballot compacts the accepted ones in order.
+struct LaneScan {
+    const double* xs;
+    const double* ys;
+    const double* zs;
+    const double4* pos;
+    double4 ri;
+    double box, inv_box, radius2;
+    size_t n;
+    size_t i;
+    int lane, group_shift;
+    unsigned lower_mask;
+    int padding;
+    int32_t* row;
+    uint16_t* row16;  // the particle's slot_row_base
+    int cnt;
+
+    template <bool STAGED, bool INTERIOR>
+    __device__ __forceinline__ void scan(uint32_t b, uint32_t e, int base) {
+        for (uint32_t jb = b; jb < e; jb += LANES) {
+            // past the cell's end a lane re-reads its first candidate and rejects it
+            const uint32_t j = jb + lane;
+            const bool valid = j < e;
+            const uint32_t jj = valid ? j : b;
+            int slot = 0;
+            double xj, yj, zj;
+            if constexpr (STAGED) {
+                slot = base + static_cast<int>(jj - b);
+                xj = xs[slot];
+                yj = ys[slot];
+                zj = zs[slot];
+            } else {
+                const double4 rj = pos[jj];
+                xj = rj.x;
+                yj = rj.y;
+                zj = rj.z;
+            }
+            double dx = ri.x - xj, dy = ri.y - yj, dz = ri.z - zj;
+            if constexpr (!INTERIOR) {
+                dx = min_image(dx, box, inv_box);
+                dy = min_image(dy, box, inv_box);
+                dz = min_image(dz, box, inv_box);
+            }
+            const bool near = dx * dx + dy * dy + dz * dz < radius2;
+            const bool accept = valid & near;  // both evaluated: no branch around the distance test
+            const unsigned group = static_cast<unsigned>((__ballot(accept) >> group_shift) & 0xFull);
+            if (accept) {
+                const unsigned at = static_cast<unsigned>(cnt) + __popc(group & lower_mask);
+                if (at < static_cast<unsigned>(padding)) {
+                    row[static_cast<uint64_t>(at) * n] = static_cast<int32_t>(j);
+                    row16[slot_entry(at)] = static_cast<uint16_t>(slot);
+                }
+            }
+            cnt += __popc(group);
+        }
+    }
+};
+
+// List build of one tile. Fast path (staged halo, own cells' candidate lists fit in LDS): one wavefront per
+// particle walks a flat list of LDS slots — for each own cell c, U_c = the slots of the stencil cells with id >= c
+// in the reference's stencil order, L_c = those with id <= c in increasing id — 64 candidates at a time; the
+// accepted ones (j > i from U_c, j < i from L_c, slot order being index order) are ranked by mbcnt over the
+// ballot, so the row comes out in the reference's order with no divergence between lanes. Other tiles use the
+// 4-lanes-per-particle scan with a binary search per stencil cell (LaneScan).
 __global__ void __launch_bounds__(TILE_THREADS)
     k_list_build_tiled(const double4* __restrict__ pos, size_t n, const int* __restrict__ cell_of,
                        const uint32_t* __restrict__ start, TileArgs ta, double box, double inv_box, double radius2,
-                       int padding, int32_t* __restrict__ rows, uint16_t* __restrict__ rows16,
+                       int padding, int write32, int row_len, int32_t* __restrict__ rows,
+                       uint16_t* __restrict__ rows16,
                        int32_t* __restrict__ count, int32_t* __restrict__ half, int* __restrict__ flags,
                        unsigned long long* __restrict__ pair_total, const int4* __restrict__ tile_ranges,
                        const int* __restrict__ tile_nr, TiledCopyBack cb, const int* gate) {
@@ -256,9 +360,12 @@ __global__ void __launch_bounds__(TILE_THREADS)
     double* ys = lds + TILE_HALO_MAX;
     double* zs = lds + 2 * TILE_HALO_MAX;
     __shared__ int ra[RMAX], rgs[RMAX], rofs[RMAX + 1];
-    __shared__ StencilEntry table[CMAX * 27];
-    __shared__ unsigned char sorted_stencil[CMAX * 27];
-    __shared__ int s_outside;
+    __shared__ Segment seg[CMAX * 27];
+    __shared__ uint16_t lists[LIST_CAP];
+    __shared__ int lst[CMAX][4];  // U offset, U length, L offset, L length (lengths padded to 64 with the ghost)
+    __shared__ int cl_first[CMAX];
+    __shared__ unsigned char cl_interior[CMAX];
+    __shared__ int s_outside, s_lists_ok;
     const HaloRanges hr{ra, rgs, rofs};
     const int tile = xcd_item(static_cast<int>(blockIdx.x), ta.tiles);
     const size_t i0 = static_cast<size_t>(tile) * TILE;
@@ -268,193 +375,247 @@ __global__ void __launch_bounds__(TILE_THREADS)
     const bool staged = nr > 0;
     const int c_first = ta.use_cells ? cell_of[i0] : 0;
     const int ncl = ta.use_cells ? cell_of[i1 - 1] - c_first + 1 : 0;
-    const bool use_table = ta.use_cells && ncl <= CMAX;
-    if (threadIdx.x == 0) s_outside = 0;
+    const bool use_lists = staged && ta.use_cells && ncl <= CMAX;
+    const uint32_t n32 = static_cast<uint32_t>(n);
+    if (threadIdx.x == 0) {
+        s_outside = 0;
+        s_lists_ok = 0;
+    }
+    if (cb.enabled && static_cast<size_t>(threadIdx.x) < i1 - i0) {  // move the re-sorted arrays into place
+        const size_t i = i0 + threadIdx.x;
+        cb.pos[i] = pos[i];
+        for (int c = 0; c < 3; ++c) {
+            cb.vel[c * n + i] = cb.vel2[c * n + i];
+            cb.frc[c * n + i] = cb.frc2[c * n + i];
+        }
+        cb.mass[i] = cb.mass2[i];
+        cb.id[i] = cb.id2[i];
+        cb.cell_of[i] = cb.cell_of2[i];
+    }
     if (staged) load_ranges(tile_ranges, tile, nr, hr);
     __syncthreads();
     if (staged) {
-        const int total = rofs[nr];
         int outside = 0;
-        stage_halo(pos, hr, nr, total, [&](int q, const double4& p) {
+        stage_halo(pos, hr, nr, rofs[nr], [&](int q, const double4& p) {
             xs[q] = p.x;
             ys[q] = p.y;
             zs[q] = p.z;
             outside |= !(p.x >= 0.0 && p.x < box && p.y >= 0.0 && p.y < box && p.z >= 0.0 && p.z < box);
         });
         if (outside) s_outside = 1;
+        if (threadIdx.x == 0) {
+            const double ghost = __builtin_nan("");
+            xs[GHOST_SLOT] = ghost;
+            ys[GHOST_SLOT] = ghost;
+            zs[GHOST_SLOT] = ghost;
+        }
     }
-    for (int t = threadIdx.x; use_table && t < ncl * 27; t += TILE_THREADS) {
+    // stencil segments of every own cell
+    for (int t = threadIdx.x; use_lists && t < ncl * 27; t += TILE_THREADS) {
         const int cl = t / 27, st = t % 27;
         const int c = c_first + cl;
         const int cx = c / (s * s), cy = (c / s) % s, cz = c % s;
         const int nc = (wrap_index(cx + st / 9 - 1, s) * s + wrap_index(cy + (st / 3) % 3 - 1, s)) * s +
                        wrap_index(cz + st % 3 - 1, s);
-        const int b = static_cast<int>(start[nc]), e = static_cast<int>(start[nc + 1]);
-        table[t] = make_int4(b, e, staged ? slot_base_of_cell(nc, hr, start, nr) : 0, nc);
+        Segment g;
+        g.nc = nc;
+        g.base = static_cast<uint16_t>(slot_base_of_cell(nc, hr, start, nr));
+        g.len = static_cast<uint16_t>(start[nc + 1] - start[nc]);
+        seg[t] = g;
     }
     __syncthreads();
-    for (int t = threadIdx.x; use_table && t < ncl * 27; t += TILE_THREADS) {
-        // the same 27 cells in increasing id (all distinct for side >= 3)
-        const int cl = t / 27, st = t % 27;
-        const int nc = table[t].w;
-        int rank = 0;
-        for (int q = 0; q < 27; ++q) rank += table[cl * 27 + q].w < nc ? 1 : 0;
-        sorted_stencil[cl * 27 + rank] = static_cast<unsigned char>(st);
-    }
-    __syncthreads();
-    const int lane = threadIdx.x & (LANES - 1);
-    const int group_shift = (threadIdx.x & (WAVE - 1)) & ~(LANES - 1);
-    const unsigned lower_mask = (1u << lane) - 1u;
-    const size_t i = i0 + (threadIdx.x >> 2);
-    int cnt = 0, hcnt = 0;
-    if (i < i1) {
-        const double4 ri = pos[i];
-        if (cb.enabled && lane == 0) {
-            cb.pos[i] = ri;
-            for (int c = 0; c < 3; ++c) {
-                cb.vel[c * n + i] = cb.vel2[c * n + i];
-                cb.frc[c * n + i] = cb.frc2[c * n + i];
-            }
-            cb.mass[i] = cb.mass2[i];
-            cb.id[i] = cb.id2[i];
-            cb.cell_of[i] = cb.cell_of2[i];
+    if (use_lists && static_cast<int>(threadIdx.x) < ncl) {
+        const int cl = threadIdx.x, c = c_first + cl;
+        int ul = 0, ll = 0;
+        for (int st = 0; st < 27; ++st) {
+            const Segment g = seg[cl * 27 + st];
+            if (g.nc >= c) ul += g.len;
+            if (g.nc <= c) ll += g.len;
         }
-        const uint32_t self = static_cast<uint32_t>(i);
-        const uint32_t n32 = static_cast<uint32_t>(n);
-        int32_t* const row = rows + i;
-        uint16_t* const row16 = rows16 + i * CHUNK;
-        const uint32_t chunk_stride = n32 * CHUNK;  // entries between chunks (n < 2^27 on this path)
-        // Candidates j in [b, e) of one cell, 4 at a time; `base` is the LDS slot of j = b. Staged tiles read the
-        // halo from LDS, the others gather from global memory (a compile-time choice: a run-time select between
-        // the two would turn every read into a generic flat access). With INTERIOR the separation needs no
-        // minimum image: every candidate lies within 3 cells = 3L/side < L/2 along each axis, so round(d / L) = 0
-        // and the reference's d - L * round(d / L) is d itself.
-        auto scan = [&](uint32_t b, uint32_t e, int base, auto staged_tag, auto interior_tag) {
-            constexpr bool STAGED = decltype(staged_tag)::value;
-            constexpr bool INTERIOR = decltype(interior_tag)::value;
-            for (uint32_t jb = b; jb < e; jb += LANES) {
-                // past the cell's end a lane re-reads its first candidate and rejects it
-                const uint32_t j = jb + lane;
-                const bool valid = j < e;
-                const uint32_t jj = valid ? j : b;
-                int slot = 0;
-                double xj, yj, zj;
-                if constexpr (STAGED) {
-                    slot = base + static_cast<int>(jj - b);
-                    xj = xs[slot];
-                    yj = ys[slot];
-                    zj = zs[slot];
-                } else {
-                    const double4 rj = pos[jj];
-                    xj = rj.x;
-                    yj = rj.y;
-                    zj = rj.z;
-                }
-                double dx = ri.x - xj, dy = ri.y - yj, dz = ri.z - zj;
-                if constexpr (!INTERIOR) {
+        lst[cl][1] = (ul + WAVE - 1) / WAVE * WAVE;
+        lst[cl][3] = (ll + WAVE - 1) / WAVE * WAVE;
+        cl_first[cl] = static_cast<int>(start[c]);
+        const int cx = c / (s * s), cy = (c / s) % s, cz = c % s;
+        cl_interior[cl] = ta.interior_ok && cx >= 1 && cx <= s - 2 && cy >= 1 && cy <= s - 2 && cz >= 1 &&
+                          cz <= s - 2;
+    }
+    __syncthreads();
+    if (use_lists && threadIdx.x == 0) {
+        int off = 0;
+        for (int cl = 0; cl < ncl; ++cl) {
+            lst[cl][0] = off;
+            off += lst[cl][1];
+            lst[cl][2] = off;
+            off += lst[cl][3];
+        }
+        s_lists_ok = off <= LIST_CAP ? 1 : 0;
+    }
+    __syncthreads();
+    const bool lists_ok = use_lists && s_lists_ok;
+    // fill: each (own cell, stencil cell, U or L) task copies its segment's slots to its place in the list
+    for (int t = threadIdx.x; lists_ok && t < 2 * ncl * 27; t += TILE_THREADS) {
+        const bool lower = t >= ncl * 27;
+        const int k = lower ? t - ncl * 27 : t;
+        const int cl = k / 27, st = k % 27;
+        const int c = c_first + cl;
+        const Segment g = seg[k];
+        if (lower ? g.nc > c : g.nc < c) continue;
+        int off = lst[cl][lower ? 2 : 0];
+        for (int q = 0; q < 27; ++q) {  // earlier segments of the same list
+            const Segment h = seg[cl * 27 + q];
+            if (lower ? (h.nc < g.nc) : (q < st && h.nc >= c)) off += h.len;
+        }
+        for (int m = 0; m < g.len; ++m) lists[off + m] = static_cast<uint16_t>(g.base + m);
+    }
+    for (int t = threadIdx.x; lists_ok && t < 2 * ncl * WAVE; t += TILE_THREADS) {  // ghost tails
+        const int cl = t / (2 * WAVE), lower = (t / WAVE) & 1, q = t % WAVE;
+        const int c = c_first + cl;
+        int used = 0;  // unpadded length of this list
+        for (int st = 0; st < 27; ++st) {
+            const Segment g = seg[cl * 27 + st];
+            if (lower ? g.nc <= c : g.nc >= c) used += g.len;
+        }
+        const int padded = lst[cl][lower ? 3 : 1];
+        if (used + q < padded) lists[lst[cl][lower ? 2 : 0] + used + q] = static_cast<uint16_t>(GHOST_SLOT);
+    }
+    __syncthreads();
+    int ep_max = 0;                  // longest row seen by this lane's wavefront / lane group
+    unsigned long long ep_half = 0;  // half-row lengths summed once per particle
+    if (lists_ok) {
+        const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+        for (size_t p = wave; p < i1 - i0; p += TILE_THREADS / WAVE) {
+            const size_t i = i0 + p;
+            const int cl = cell_of[i] - c_first;
+            const int slot_i = seg[cl * 27 + OWN_STENCIL].base + (static_cast<int>(i) - cl_first[cl]);
+            const double xi = xs[slot_i], yi = ys[slot_i], zi = zs[slot_i];
+            const bool interior = cl_interior[cl] && !s_outside;
+            uint16_t* const row16 = rows16 + slot_row_base(i, row_len);
+            int32_t* const row = rows + i;
+            int cnt = 0;
+            // one batch of 64 candidates: test, rank the accepted ones over the ballot, store, advance the count
+            auto batch = [&](int q, bool upper, auto interior_tag) {
+                const int sl = lists[q];
+                double dx = xi - xs[sl], dy = yi - ys[sl], dz = zi - zs[sl];
+                if constexpr (!decltype(interior_tag)::value) {
                     dx = min_image(dx, box, inv_box);
                     dy = min_image(dy, box, inv_box);
                     dz = min_image(dz, box, inv_box);
                 }
-                const bool near = dx * dx + dy * dy + dz * dz < radius2;
-                const bool accept = valid & near;  // both evaluated: no branch around the distance test
-                const unsigned group = static_cast<unsigned>((__ballot(accept) >> group_shift) & 0xFull);
+                const bool near = dx * dx + dy * dy + dz * dz < radius2;  // false for the ghost (NaN)
+                const bool order = upper ? sl > slot_i : sl < slot_i;
+                const bool accept = order & near;
+                const unsigned long long mask = __ballot(accept);
                 if (accept) {
-                    const unsigned at = static_cast<unsigned>(cnt) + __popc(group & lower_mask);
+                    const unsigned rank = __builtin_amdgcn_mbcnt_hi(
+                        static_cast<unsigned>(mask >> 32), __builtin_amdgcn_mbcnt_lo(static_cast<unsigned>(mask), 0u));
+                    const unsigned at = static_cast<unsigned>(cnt) + rank;
                     if (at < static_cast<unsigned>(padding)) {
-                        row[static_cast<uint64_t>(at) * n32] = static_cast<int32_t>(j);
-                        const uint32_t within = ((at & 3u) << 3) | ((at >> 2) & 7u);
-                        row16[static_cast<uint64_t>(at >> 5) * chunk_stride + within] = static_cast<uint16_t>(slot);
+                        row16[slot_entry(at)] = static_cast<uint16_t>(sl);
+                        if (write32) row[static_cast<uint64_t>(at) * n] = halo_particle(sl, hr, nr);
                     }
                 }
-                cnt += __popc(group);
+                cnt += __popcll(mask);
+            };
+            auto walk = [&](int off, int len, bool upper, auto interior_tag) {
+                int q0 = 0;
+                for (; q0 + 2 * WAVE <= len; q0 += 2 * WAVE) {  // two batches in flight
+                    batch(off + q0 + lane, upper, interior_tag);
+                    batch(off + q0 + WAVE + lane, upper, interior_tag);
+                }
+                if (q0 < len) batch(off + q0 + lane, upper, interior_tag);
+            };
+            auto both = [&](auto interior_tag) {
+                walk(lst[cl][0], lst[cl][1], true, interior_tag);
+                const int h = cnt;
+                walk(lst[cl][2], lst[cl][3], false, interior_tag);
+                return h;
+            };
+            const int hcnt = interior ? both(std::true_type{}) : both(std::false_type{});
+            if (lane == 0) {
+                count[i] = cnt < padding ? cnt : padding;
+                half[i] = hcnt < padding ? hcnt : padding;
             }
-        };
-        auto particle_rows = [&](auto staged_tag) {
-            constexpr bool STAGED = decltype(staged_tag)::value;
-            const int c = cell_of[i];
-            if (use_table) {
-                const int cl = c - c_first;
-                auto rows_of = [&](auto interior_tag) {
-#pragma unroll 1
-                    for (int st = 0; st < 27; ++st) {  // j > i: the reference's stencil order
-                        const StencilEntry en = table[cl * 27 + st];
-                        if (en.w < c) continue;
-                        if (en.w == c)
-                            scan(self + 1, static_cast<uint32_t>(en.y), en.z + static_cast<int>(self + 1) - en.x,
-                                 staged_tag, interior_tag);
-                        else
-                            scan(static_cast<uint32_t>(en.x), static_cast<uint32_t>(en.y), en.z, staged_tag,
-                                 interior_tag);
-                    }
-                    hcnt = cnt;
-#pragma unroll 1
-                    for (int r = 0; r < 27; ++r) {  // j < i: cells in increasing id
-                        const StencilEntry en = table[cl * 27 + sorted_stencil[cl * 27 + r]];
-                        if (en.w > c) break;
-                        scan(static_cast<uint32_t>(en.x), en.w == c ? self : static_cast<uint32_t>(en.y), en.z,
-                             staged_tag, interior_tag);
-                    }
-                };
-                if constexpr (STAGED) {
-                    const int cx = c / (s * s), cy = (c / s) % s, cz = c % s;
-                    const bool interior = ta.interior_ok && !s_outside && cx >= 1 && cx <= s - 2 && cy >= 1 &&
-                                          cy <= s - 2 && cz >= 1 && cz <= s - 2;
-                    if (interior)
-                        rows_of(std::true_type{});
-                    else
-                        rows_of(std::false_type{});
-                } else {
-                    rows_of(std::false_type{});
-                }
-            } else if (ta.use_cells) {
-                const int cx = c / (s * s), cy = (c / s) % s, cz = c % s;
+            ep_max = cnt > ep_max ? cnt : ep_max;
+            if (lane == 0) ep_half += static_cast<unsigned long long>(hcnt);
+        }
+    } else {
+        const int lane = threadIdx.x & (LANES - 1);
+        const size_t i = i0 + (threadIdx.x >> 2);
+        if (i < i1) {
+            LaneScan sc;
+            sc.xs = xs;
+            sc.ys = ys;
+            sc.zs = zs;
+            sc.pos = pos;
+            sc.ri = pos[i];
+            sc.box = box;
+            sc.inv_box = inv_box;
+            sc.radius2 = radius2;
+            sc.n = n;
+            sc.i = i;
+            sc.lane = lane;
+            sc.group_shift = (threadIdx.x & (WAVE - 1)) & ~(LANES - 1);
+            sc.lower_mask = (1u << lane) - 1u;
+            sc.padding = padding;
+            sc.row = rows + i;
+            sc.row16 = rows16 + slot_row_base(i, row_len);
+            sc.cnt = 0;
+            const uint32_t self = static_cast<uint32_t>(i);
+            int hcnt = 0;
+            auto run = [&](auto staged_tag) {
+                constexpr bool STAGED = decltype(staged_tag)::value;
                 auto base_of = [&](int nc) {
                     if constexpr (STAGED)
                         return slot_base_of_cell(nc, hr, start, nr);
                     else
                         return 0;
                 };
+                if (ta.use_cells) {
+                    const int c = cell_of[i];
+                    const int cx = c / (s * s), cy = (c / s) % s, cz = c % s;
 #pragma unroll 1
-                for (int st = 0; st < 27; ++st) {
-                    const int nc = (wrap_index(cx + st / 9 - 1, s) * s + wrap_index(cy + (st / 3) % 3 - 1, s)) * s +
-                                   wrap_index(cz + st % 3 - 1, s);
-                    if (nc < c) continue;
-                    const int base = base_of(nc);
-                    const uint32_t b = start[nc], e = start[nc + 1];
-                    if (nc == c)
-                        scan(self + 1, e, base + static_cast<int>(self + 1 - b), staged_tag, std::false_type{});
-                    else
-                        scan(b, e, base, staged_tag, std::false_type{});
-                }
-                hcnt = cnt;
-                const Sorted3 xo(wrap_index(cx - 1, s), cx, wrap_index(cx + 1, s));
-                const Sorted3 yo(wrap_index(cy - 1, s), cy, wrap_index(cy + 1, s));
-                const Sorted3 zo(wrap_index(cz - 1, s), cz, wrap_index(cz + 1, s));
+                    for (int st = 0; st < 27; ++st) {  // j > i: the reference's stencil order
+                        const int nc = (wrap_index(cx + st / 9 - 1, s) * s + wrap_index(cy + (st / 3) % 3 - 1, s)) * s +
+                                       wrap_index(cz + st % 3 - 1, s);
+                        if (nc < c) continue;
+                        const int base = base_of(nc);
+                        const uint32_t b = start[nc], e = start[nc + 1];
+                        if (nc == c)
+                            sc.scan<STAGED, false>(self + 1, e, base + static_cast<int>(self + 1 - b));
+                        else
+                            sc.scan<STAGED, false>(b, e, base);
+                    }
+                    hcnt = sc.cnt;
+                    const Sorted3 xo(wrap_index(cx - 1, s), cx, wrap_index(cx + 1, s));
+                    const Sorted3 yo(wrap_index(cy - 1, s), cy, wrap_index(cy + 1, s));
+                    const Sorted3 zo(wrap_index(cz - 1, s), cz, wrap_index(cz + 1, s));
 #pragma unroll 1
-                for (int st = 0; st < 27; ++st) {
-                    const int nc = (xo[st / 9] * s + yo[(st / 3) % 3]) * s + zo[st % 3];
-                    if (nc > c) break;
-                    scan(start[nc], nc == c ? self : start[nc + 1], base_of(nc), staged_tag, std::false_type{});
+                    for (int st = 0; st < 27; ++st) {  // j < i: cells in increasing id
+                        const int nc = (xo[st / 9] * s + yo[(st / 3) % 3]) * s + zo[st % 3];
+                        if (nc > c) break;
+                        sc.scan<STAGED, false>(start[nc], nc == c ? self : start[nc + 1], base_of(nc));
+                    }
+                } else {
+                    // every particle is in the single staged range, at slot = its index
+                    sc.scan<STAGED, false>(self + 1, n32, static_cast<int>(self + 1));
+                    hcnt = sc.cnt;
+                    sc.scan<STAGED, false>(0, self, 0);
                 }
-            } else {
-                // every particle is in the single staged range, at slot = its index
-                scan(self + 1, n32, static_cast<int>(self + 1), staged_tag, std::false_type{});
-                hcnt = cnt;
-                scan(0, self, 0, staged_tag, std::false_type{});
+            };
+            if (staged)
+                run(std::true_type{});
+            else
+                run(std::false_type{});
+            if (lane == 0) {
+                count[i] = sc.cnt < padding ? sc.cnt : padding;
+                half[i] = hcnt < padding ? hcnt : padding;
+                ep_half = static_cast<unsigned long long>(hcnt);
             }
-        };
-        if (staged)
-            particle_rows(std::true_type{});
-        else
-            particle_rows(std::false_type{});
-        if (lane == 0) {
-            count[i] = cnt < padding ? cnt : padding;
-            half[i] = hcnt < padding ? hcnt : padding;
+            ep_max = sc.cnt;
         }
     }
-    int mx = cnt;
-    unsigned long long hs = lane == 0 ? static_cast<unsigned long long>(hcnt) : 0ull;
+    int mx = ep_max;
+    unsigned long long hs = ep_half;
 #pragma unroll
     for (int off = WAVE / 2; off > 0; off >>= 1) {
         const int o = __shfl_xor(mx, off, WAVE);
@@ -467,6 +628,28 @@ __global__ void __launch_bounds__(TILE_THREADS)
         atomicAdd(pair_total, hs);
     }
     if (gate && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&flags[FLAG_REBUILDS], 1);
+}
+
+// int32 rows from the 16-bit slot rows (builds that wrote only the slots): slot -> particle through the tile's
+// halo ranges. Unstaged tiles already hold int32 rows.
+__global__ void __launch_bounds__(TILE_THREADS)
+    k_rows_from_slots(size_t n, const uint16_t* __restrict__ rows16, int row_len, const int32_t* __restrict__ count,
+                      const int4* __restrict__ tile_ranges, const int* __restrict__ tile_nr,
+                      int32_t* __restrict__ rows) {
+    __shared__ int ra[RMAX], rgs[RMAX], rofs[RMAX + 1];
+    const HaloRanges hr{ra, rgs, rofs};
+    const int tile = blockIdx.x;
+    const int nr = tile_nr[tile];
+    if (nr <= 0) return;
+    load_ranges(tile_ranges, tile, nr, hr);
+    __syncthreads();
+    const size_t i = static_cast<size_t>(tile) * TILE + (threadIdx.x >> 2);
+    const int lane = threadIdx.x & (LANES - 1);
+    if (i >= n) return;
+    const int m = count[i];
+    const uint16_t* row16 = rows16 + slot_row_base(i, row_len);
+    for (int k = lane; k < m; k += LANES)
+        rows[static_cast<size_t>(k) * n + i] = halo_particle(row16[slot_entry(static_cast<uint32_t>(k))], hr, nr);
 }
 
 // Frame of the mixed force kernel: positions are staged relative to the tile's first particle (its nearest image),
@@ -491,7 +674,7 @@ __global__ void __launch_bounds__(TILE_THREADS)
     k_forces_tiled_mixed(const double4* __restrict__ pos, size_t n, const uint16_t* __restrict__ rows16,
                          const int32_t* __restrict__ rows, const int32_t* __restrict__ count,
                          const int32_t* __restrict__ half, const int4* __restrict__ tile_ranges,
-                         const int* __restrict__ tile_nr, LJConst lj, TileFrame fr,
+                         const int* __restrict__ tile_nr, int row_len, LJConst lj, TileFrame fr,
                          double* __restrict__ frc, double* __restrict__ partials, int* __restrict__ clear_flag) {
     extern __shared__ double ldsd[];
     double* xs = ldsd;
@@ -539,8 +722,8 @@ __global__ void __launch_bounds__(TILE_THREADS)
             auto run = [&](auto wrap) {
                 constexpr bool W = decltype(wrap)::value;
                 const int chunks = (m + CHUNK - 1) / CHUNK;
-                const uint4* rp = reinterpret_cast<const uint4*>(rows16 + i * CHUNK + lane * 8);
-                const size_t stride = n * (CHUNK / 8);  // uint4 words between chunks
+                const uint4* rp = reinterpret_cast<const uint4*>(rows16 + slot_row_base(i, row_len) + lane * 8);
+                constexpr int stride = CHUNK_STRIDE / 8;  // uint4 words between chunks
                 uint4 cur = chunks > 0 ? rp[0] : make_uint4(0u, 0u, 0u, 0u);
                 for (int c = 0; c < chunks; ++c) {
                     const uint4 nxt = c + 1 < chunks ? rp[(c + 1) * stride] : make_uint4(0u, 0u, 0u, 0u);
@@ -601,11 +784,22 @@ bool tiling_enabled(size_t n) {
 
 int tile_count(size_t n) { return static_cast<int>((n + TILE - 1) / TILE); }
 
-size_t tiled_rows16_entries(size_t n, int padding) {
-    return static_cast<size_t>((padding + CHUNK - 1) / CHUNK) * CHUNK * (n ? n : 1);
+static int slot_row_len(int padding) { return (padding + CHUNK - 1) / CHUNK * CHUNK; }
+
+void tiled_rows32_enqueue(md_list* l) {
+    const size_t n = l->n;
+    if (n == 0) return;
+    const int tiles = tile_count(n);
+    hipLaunchKernelGGL(k_rows_from_slots, dim3(tiles), dim3(TILE_THREADS), 0, l->ctx->stream, n, l->rows16,
+                       slot_row_len(l->padding), l->count, l->tile_ranges, l->tile_nr, l->rows);
+    MD_HIP(hipGetLastError());
 }
 
-void tiled_list_enqueue(md_list* l, md_state* s, bool from_alt, const int* gate) {
+size_t tiled_rows16_entries(size_t n, int padding) {
+    return static_cast<size_t>(tile_count(n ? n : 1)) * TILE * slot_row_len(padding);
+}
+
+void tiled_list_enqueue(md_list* l, md_state* s, bool from_alt, const int* gate, bool write32) {
     hipStream_t st = l->ctx->stream;
     const size_t n = s->n;
     if (n == 0) return;
@@ -633,8 +827,8 @@ void tiled_list_enqueue(md_list* l, md_state* s, bool from_alt, const int* gate)
                        gate);
     hipLaunchKernelGGL(k_list_build_tiled, dim3(tiles), dim3(TILE_THREADS),
                        static_cast<size_t>(TILE_HALO_MAX) * 3 * sizeof(double), st, from_alt ? s->pos_alt : s->pos,
-                       n, cells, l->cell_start, ta, s->box, s->inv_box, l->radius2, l->padding, l->rows, l->rows16,
-                       l->count, l->half, l->flags, reinterpret_cast<unsigned long long*>(l->flags + FLAG_COUNT),
+                       n, cells, l->cell_start, ta, s->box, s->inv_box, l->radius2, l->padding, write32 ? 1 : 0,
+                       slot_row_len(l->padding), l->rows, l->rows16, l->count, l->half, l->flags, reinterpret_cast<unsigned long long*>(l->flags + FLAG_COUNT),
                        l->tile_ranges, l->tile_nr, cb, gate);
     MD_HIP(hipGetLastError());
 }
@@ -654,8 +848,8 @@ void tiled_forces_enqueue(md_state* s, md_list* l, const LJConst& lj, double cut
     if (n > 0) {
         const size_t lds = static_cast<size_t>(TILE_HALO_MAX) * 3 * sizeof(double);
         hipLaunchKernelGGL(k_forces_tiled_mixed, dim3(tiles), dim3(TILE_THREADS), lds, st, s->pos, n, l->rows16,
-                           l->rows, l->count, l->half, l->tile_ranges, l->tile_nr, lj, fr, s->frc, s->partials,
-                           clear_flag);
+                           l->rows, l->count, l->half, l->tile_ranges, l->tile_nr, slot_row_len(l->padding), lj,
+                           fr, s->frc, s->partials, clear_flag);
         MD_HIP(hipGetLastError());
     }
     if (ev_stop) MD_HIP(hipEventRecord(ev_stop, st));

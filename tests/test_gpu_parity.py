@@ -219,6 +219,25 @@ def test_mixed_graph_run_matches_eager_run(ctx):
     assert ga.state_hash() == gb.state_hash()
 
 
+def test_int32_rows_rebuilt_after_mixed_run(ctx):
+    """A mixed md_run rebuilds only the 16-bit slot rows on the device; the int32 rows are materialised from them
+    when the FP64 kernel needs them, and hold every pair within the cutoff."""
+    s = O.make_liquid(10, 40, skin=0.3)
+    g = to_gpu(ctx, s)
+    nl = mdgpu.NeighbourList(ctx, 2.5, 0.3)
+    nl.build(g)
+    nl.forces(g, LJ, mdgpu.MD_MIXED)
+    nl.run(g, LJ, 0.005, 150, precision=mdgpu.MD_MIXED, use_graph=True)
+    assert nl.info().rebuilds > 2
+    r = nl.forces(g, LJ, mdgpu.MD_FP64)
+    f = g.download(mdgpu.FIELD_FORCE)
+    ra = g.forces_allpairs(LJ)
+    fa = g.download(mdgpu.FIELD_FORCE)
+    assert r.inside_pairs == ra.inside_pairs
+    assert np.abs(f - fa).max() <= 1e-12 * np.abs(fa).max()
+    assert r.potential == pytest.approx(ra.potential, rel=1e-12, abs=0)
+
+
 def test_bussi_run_matches_golden(ctx, golden):
     """FCC 6, T 1.0, Bussi tau 0.1 seed 7, FP64, 300 steps: the device-side thermostat reproduces the reference."""
     s = O.Sim.fcc(6, 0.8442, 1.0, 1)
