@@ -27,19 +27,29 @@ const char* get_error() { return g_error.c_str(); }
 // ----------------------------------------------------------------------------------------------------------
 // Reductions
 
-// One block sums rows of W partials: thread t takes rows t, t + 1024, ... (fixed order), then each wavefront
-// reduces by butterfly shuffles and thread 0 adds the 16 wavefront sums in order.
+// One block sums rows of W partials: thread t takes rows t, t + 1024, ... into four running sums (rows
+// t + 4096 q + 1024 r go to sum r, so four loads are in flight), adds them in order, then each wavefront reduces by
+// butterfly shuffles and thread 0 adds the 16 wavefront sums in order. The order is fixed for a given row count.
 template <int W>
 __global__ void __launch_bounds__(1024) k_reduce_partials(const double* __restrict__ partials, int blocks,
                                                           double* __restrict__ out) {
     __shared__ double lds[1024 / WAVE][W];
-    double s[W];
+    double s[W], t4[4][W];
 #pragma unroll
-    for (int k = 0; k < W; ++k) s[k] = 0.0;
-    for (int b = threadIdx.x; b < blocks; b += 1024) {
+    for (int r = 0; r < 4; ++r)
 #pragma unroll
-        for (int k = 0; k < W; ++k) s[k] += partials[static_cast<size_t>(b) * W + k];
+        for (int k = 0; k < W; ++k) t4[r][k] = 0.0;
+    for (int b = threadIdx.x; b < blocks; b += 4 * 1024) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int row = b + r * 1024;
+            if (row < blocks)
+#pragma unroll
+                for (int k = 0; k < W; ++k) t4[r][k] += partials[static_cast<size_t>(row) * W + k];
+        }
     }
+#pragma unroll
+    for (int k = 0; k < W; ++k) s[k] = (t4[0][k] + t4[1][k]) + (t4[2][k] + t4[3][k]);
 #pragma unroll
     for (int k = 0; k < W; ++k) s[k] = wave_sum(s[k]);
     if (threadIdx.x % WAVE == 0)

@@ -63,6 +63,47 @@ __global__ void __launch_bounds__(VV_THREADS)
     if (partials) block_sum<1, VV_THREADS>(ke, partials + blockIdx.x);
 }
 
+// The closing half kick of one step fused with the opening kick-drift of the next (NVE md_run): the same two
+// updates v += (dt/2 f) / m as k_kick then k_kick_drift, in the same order, so results are bit-equal; v, f and m
+// are read once instead of twice. The block partials are the closing kick's kinetic energy, as k_kick's.
+__global__ void __launch_bounds__(VV_THREADS)
+    k_kick_kick_drift(double4* __restrict__ pos, double* __restrict__ vel, const double* __restrict__ frc,
+                      const double* __restrict__ mass, size_t n, double half_dt, double dt, double box,
+                      double inv_box, const double4* __restrict__ ref, double max_disp2, int* __restrict__ stale,
+                      double* __restrict__ partials) {
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    double ke[1] = {0.0};
+    if (i < n) {
+        const double m = mass[i];
+        double v[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const double kick = half_dt * frc[c * n + i];
+            const double closed = vel[c * n + i] + kick / m;
+            v[c] = closed;
+        }
+        const double half_m = 0.5 * m;
+        ke[0] = half_m * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const double kick = half_dt * frc[c * n + i];
+            v[c] = v[c] + kick / m;
+            vel[c * n + i] = v[c];
+        }
+        double4 r = pos[i];
+        r.x = wrap_coord(r.x + dt * v[0], box, inv_box);
+        r.y = wrap_coord(r.y + dt * v[1], box, inv_box);
+        r.z = wrap_coord(r.z + dt * v[2], box, inv_box);
+        pos[i] = r;
+        const double4 q = ref[i];
+        const double dx = min_image(r.x - q.x, box, inv_box);
+        const double dy = min_image(r.y - q.y, box, inv_box);
+        const double dz = min_image(r.z - q.z, box, inv_box);
+        if (dx * dx + dy * dy + dz * dz > max_disp2) *stale = 1;
+    }
+    block_sum<1, VV_THREADS>(ke, partials + blockIdx.x);
+}
+
 // Bussi eq. A7 on the device from the reference-order kinetic energy in scalars[3] and the caller's draws
 // (Thermostat.cpp:41-57). Writes the velocity scale to scalars[5] and the new kinetic energy to scalars[6].
 // Draws are consumed only when KE > 0, as the reference draws only then.
@@ -250,13 +291,29 @@ int md_run(md_state* s, md_list* l, const md_run_config* cfg, double* energies) 
     }
     int step_index = 0;
 
-    auto enqueue_step = [&]() {
-        kick_drift_enqueue(s, cfg->dt, l);
+    auto enqueue_forces = [&]() {
         list_enqueue_conditional_rebuild(l, s);
         hipEvent_t e0 = timed ? events[2 * step_index] : nullptr;
         hipEvent_t e1 = timed ? events[2 * step_index + 1] : nullptr;
         ++step_index;
         forces_enqueue(s, l, cfg->lj, cfg->precision, s->scalars, l->flags + FLAG_STALE, e0, e1);
+    };
+    // NVE: steps k and k+1 meet in one fused kernel (closing kick of k, its kinetic energy, opening kick-drift of
+    // k+1); the run is head (kick-drift, forces of step 1), steps - 1 bodies (fused kernel, record of step k,
+    // forces of step k+1) and a tail (closing kick and record of the last step)
+    const unsigned vv_blocks = blocks_for(s->n, VV_THREADS);
+    auto enqueue_nve_body = [&]() {
+        k_kick_kick_drift<<<vv_blocks, VV_THREADS, 0, st>>>(s->pos, s->vel, s->frc, s->mass, s->n, 0.5 * cfg->dt,
+                                                           cfg->dt, s->box, s->inv_box, l->ref, l->max_disp2,
+                                                           l->flags + FLAG_STALE, s->partials);
+        reduce_partials_enqueue(st, s->partials, static_cast<int>(vv_blocks), 1, s->scalars + 3);
+        k_record<<<1, 1, 0, st>>>(s->scalars, 3, rb.energies, rb.counters);
+        enqueue_forces();
+        MD_HIP(hipGetLastError());
+    };
+    auto enqueue_step = [&]() {
+        kick_drift_enqueue(s, cfg->dt, l);
+        enqueue_forces();
         if (cfg->thermostat) {
             kick_enqueue(s, cfg->dt, nullptr);
             kinetic_exact_enqueue(s, s->scalars + 3);
@@ -270,12 +327,21 @@ int md_run(md_state* s, md_list* l, const md_run_config* cfg, double* energies) 
         MD_HIP(hipGetLastError());
     };
 
-    if (cfg->use_graph) {
+    const bool fused = !cfg->thermostat;
+    if (fused) {  // head: the first step's kick-drift and forces
+        kick_drift_enqueue(s, cfg->dt, l);
+        enqueue_forces();
+    }
+    const int replays = fused ? steps - 1 : steps;
+    if (cfg->use_graph && replays > 0) {
         hipGraph_t graph = nullptr;
         hipGraphExec_t exec = nullptr;
         MD_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
         try {
-            enqueue_step();
+            if (fused)
+                enqueue_nve_body();
+            else
+                enqueue_step();
         } catch (...) {
             (void)hipStreamEndCapture(st, &graph);
             if (graph) (void)hipGraphDestroy(graph);
@@ -287,7 +353,7 @@ int md_run(md_state* s, md_list* l, const md_run_config* cfg, double* energies) 
             (void)hipGraphDestroy(graph);
             hip_check(e, "hipGraphInstantiate");
         }
-        for (int k = 0; k < steps; ++k) {
+        for (int k = 0; k < replays; ++k) {
             e = hipGraphLaunch(exec, st);
             if (e != hipSuccess) break;
         }
@@ -296,7 +362,17 @@ int md_run(md_state* s, md_list* l, const md_run_config* cfg, double* energies) 
         (void)hipGraphDestroy(graph);
         hip_check(e, "hipGraphLaunch");
     } else {
-        for (int k = 0; k < steps; ++k) enqueue_step();
+        for (int k = 0; k < replays; ++k) {
+            if (fused)
+                enqueue_nve_body();
+            else
+                enqueue_step();
+        }
+    }
+    if (fused) {  // tail: the last step's closing kick and record
+        kick_enqueue(s, cfg->dt, s->scalars + 3);
+        k_record<<<1, 1, 0, st>>>(s->scalars, 3, rb.energies, rb.counters);
+        MD_HIP(hipGetLastError());
     }
     int* hf = s->ctx->host_flags;
     MD_HIP(hipMemcpyAsync(hf, l->flags, sizeof(int) * (FLAG_COUNT + 2), hipMemcpyDeviceToHost, st));
