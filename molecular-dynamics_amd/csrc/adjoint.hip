@@ -310,7 +310,7 @@ extern "C" int md_adjoint_run(md_state* src, md_list* l, const md_adjoint_config
         list_enqueue_conditional_rebuild_fixed(l, s);
         k_adj_pair<<<bn, ADJ_THREADS, 0, st>>>(s->pos, n, l->rows, l->count, l->half, s->box, s->inv_box, alj, w, hw,
                                                gu, gw, s->partials);
-        reduce_partials_enqueue(st, s->partials, static_cast<int>(bn), ADJ_PARTS, point);
+        reduce_partials_enqueue(s, static_cast<int>(bn), ADJ_PARTS, point);
         k_adj_theta<<<1, 1, 0, st>>>(point, theta, coef_u, coef_w, dt);
         // the stale flag is left set only by a check that rebuilt; clear it for the next point
         MD_HIP(hipMemsetAsync(l->flags + FLAG_STALE, 0, sizeof(int), st));

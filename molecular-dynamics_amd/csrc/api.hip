@@ -27,48 +27,75 @@ const char* get_error() { return g_error.c_str(); }
 // ----------------------------------------------------------------------------------------------------------
 // Reductions
 
-// One block sums rows of W partials: thread t takes rows t, t + 1024, ... into four running sums (rows
-// t + 4096 q + 1024 r go to sum r, so four loads are in flight), adds them in order, then each wavefront reduces by
-// butterfly shuffles and thread 0 adds the 16 wavefront sums in order. The order is fixed for a given row count.
+// Rows of W partials summed by RED_BLOCKS blocks: block g takes the contiguous rows [g R, (g + 1) R), R = ceil(rows /
+// RED_BLOCKS); thread t sums rows g R + t, g R + t + 256, ... in order, the block adds its threads' sums (wavefront
+// butterflies, then the 4 wavefronts in order) into block_sums[g], and the last block to finish (it draws the
+// last ticket, and resets the counter for the next launch) adds the RED_BLOCKS block sums in order. The order is
+// fixed for a given row count, so results repeat bit for bit; no memory is cleared by the host, so the launch can
+// sit in a captured graph.
+constexpr int RED_BLOCKS = 64;
+constexpr int RED_THREADS = 256;
+constexpr int RED_WIDTH_MAX = 8;
+
 template <int W>
-__global__ void __launch_bounds__(1024) k_reduce_partials(const double* __restrict__ partials, int blocks,
-                                                          double* __restrict__ out) {
-    __shared__ double lds[1024 / WAVE][W];
-    double s[W], t4[4][W];
+__global__ void __launch_bounds__(RED_THREADS) k_reduce_partials(const double* __restrict__ partials, int rows,
+                                                                 double* __restrict__ block_sums,
+                                                                 unsigned* __restrict__ ticket,
+                                                                 double* __restrict__ out) {
+    __shared__ double lds[RED_THREADS / WAVE][W];
+    __shared__ double fin[RED_BLOCKS][W];
+    __shared__ int s_last;
+    const int per = (rows + RED_BLOCKS - 1) / RED_BLOCKS;
+    const int r0 = blockIdx.x * per;
+    const int r1 = min(rows, r0 + per);
+    double s[W];
 #pragma unroll
-    for (int r = 0; r < 4; ++r)
+    for (int k = 0; k < W; ++k) s[k] = 0.0;
+    for (int r = r0 + static_cast<int>(threadIdx.x); r < r1; r += RED_THREADS)
 #pragma unroll
-        for (int k = 0; k < W; ++k) t4[r][k] = 0.0;
-    for (int b = threadIdx.x; b < blocks; b += 4 * 1024) {
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int row = b + r * 1024;
-            if (row < blocks)
-#pragma unroll
-                for (int k = 0; k < W; ++k) t4[r][k] += partials[static_cast<size_t>(row) * W + k];
-        }
-    }
-#pragma unroll
-    for (int k = 0; k < W; ++k) s[k] = (t4[0][k] + t4[1][k]) + (t4[2][k] + t4[3][k]);
+        for (int k = 0; k < W; ++k) s[k] += partials[static_cast<size_t>(r) * W + k];
 #pragma unroll
     for (int k = 0; k < W; ++k) s[k] = wave_sum(s[k]);
     if (threadIdx.x % WAVE == 0)
 #pragma unroll
         for (int k = 0; k < W; ++k) lds[threadIdx.x / WAVE][k] = s[k];
     __syncthreads();
-    if (threadIdx.x == 0)
+    if (threadIdx.x == 0) {
         for (int k = 0; k < W; ++k) {
             double t = 0.0;
-            for (int w = 0; w < 1024 / WAVE; ++w) t += lds[w][k];
+            for (int w = 0; w < RED_THREADS / WAVE; ++w) t += lds[w][k];
+            block_sums[blockIdx.x * RED_WIDTH_MAX + k] = t;
+        }
+        __threadfence();  // the block's sums are visible before its ticket is drawn
+        s_last = atomicAdd(ticket, 1u) == RED_BLOCKS - 1;
+    }
+    __syncthreads();
+    if (!s_last) return;
+    if (threadIdx.x < RED_BLOCKS) {
+        __threadfence();
+#pragma unroll
+        for (int k = 0; k < W; ++k) fin[threadIdx.x][k] = block_sums[threadIdx.x * RED_WIDTH_MAX + k];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 0; k < W; ++k) {
+            double t = 0.0;
+            for (int g = 0; g < RED_BLOCKS; ++g) t += fin[g][k];
             out[k] = t;
         }
+        *ticket = 0u;
+    }
 }
 
-void reduce_partials_enqueue(hipStream_t st, const double* partials, int blocks, int width, double* out_dev) {
+void reduce_partials_enqueue(md_state* s, int blocks, int width, double* out_dev) {
+    hipStream_t st = s->ctx->stream;
+    const double* partials = s->partials;
+    double* bs = s->red_blocks;
+    unsigned* tk = s->red_ticket;
     switch (width) {
-        case 1: k_reduce_partials<1><<<1, 1024, 0, st>>>(partials, blocks, out_dev); break;
-        case 3: k_reduce_partials<3><<<1, 1024, 0, st>>>(partials, blocks, out_dev); break;
-        case 6: k_reduce_partials<6><<<1, 1024, 0, st>>>(partials, blocks, out_dev); break;
+        case 1: k_reduce_partials<1><<<RED_BLOCKS, RED_THREADS, 0, st>>>(partials, blocks, bs, tk, out_dev); break;
+        case 3: k_reduce_partials<3><<<RED_BLOCKS, RED_THREADS, 0, st>>>(partials, blocks, bs, tk, out_dev); break;
+        case 6: k_reduce_partials<6><<<RED_BLOCKS, RED_THREADS, 0, st>>>(partials, blocks, bs, tk, out_dev); break;
         default: fail(MD_ERR_ARG, "unsupported reduction width");
     }
     MD_HIP(hipGetLastError());
@@ -257,7 +284,10 @@ int md_state_create(md_ctx* ctx, size_t n, double box_length, md_state** out) {
         s->mass_alt = dev_alloc<double>(n);
         s->id_alt = dev_alloc<uint64_t>(n);
         s->scalars = dev_alloc<double>(16);
+        s->red_blocks = dev_alloc<double>(RED_BLOCKS * RED_WIDTH_MAX);
+        s->red_ticket = dev_alloc<unsigned>(1);
         hipStream_t st = ctx->stream;
+        MD_HIP(hipMemsetAsync(s->red_ticket, 0, sizeof(unsigned), st));
         MD_HIP(hipMemsetAsync(s->pos, 0, sizeof(double4) * (n ? n : 1), st));
         MD_HIP(hipMemsetAsync(s->vel, 0, sizeof(double) * 3 * (n ? n : 1), st));
         MD_HIP(hipMemsetAsync(s->frc, 0, sizeof(double) * 3 * (n ? n : 1), st));
@@ -293,6 +323,8 @@ int md_state_destroy(md_state* s) {
     dev_free(s->id_alt);
     dev_free(s->partials);
     dev_free(s->scalars);
+    dev_free(s->red_blocks);
+    dev_free(s->red_ticket);
     delete s;
     MD_API_END
 }

@@ -236,11 +236,15 @@ __global__ void __launch_bounds__(BT)
     if (!stale[r]) return;
     const Rep rp = reps[r];
     const size_t i = static_cast<size_t>(blockIdx.x) * BT + threadIdx.x;
-    if (i >= static_cast<size_t>(rp.first) + rp.n) return;
-    const double4 p = pos[i];
-    const int c = cell_of_position(p.x, p.y, p.z, rp.per_length, rp.side);
-    cell_of[i] = c;
-    atomicAdd(&counts[rp.cell_first + c], 1u);
+    const bool valid = i < static_cast<size_t>(rp.first) + rp.n;
+    int c = -1;
+    if (valid) {
+        const double4 p = pos[i];
+        c = cell_of_position(p.x, p.y, p.z, rp.per_length, rp.side);
+        cell_of[i] = c;
+    }
+    const CellRun run = cell_run(c, valid);
+    if (run.head) atomicAdd(&counts[rp.cell_first + c], static_cast<uint32_t>(run.len));
 }
 
 // One block per replica: exclusive scan of its histogram into batch slots; also clears its pair total.
@@ -287,10 +291,14 @@ __global__ void __launch_bounds__(BT)
     if (!stale[r]) return;
     const Rep rp = reps[r];
     const size_t i = static_cast<size_t>(blockIdx.x) * BT + threadIdx.x;
-    if (i >= static_cast<size_t>(rp.first) + rp.n) return;
-    const uint32_t cw = rp.cell_first + static_cast<uint32_t>(cell_of[i]);
-    const uint32_t slot = atomicAdd(&fill[cw], 1u);
-    order_raw[start[cw] + slot] = static_cast<uint32_t>(i);
+    const bool valid = i < static_cast<size_t>(rp.first) + rp.n;
+    const int c = valid ? cell_of[i] : -1;
+    const uint32_t cw = rp.cell_first + static_cast<uint32_t>(c);
+    const CellRun run = cell_run(c, valid);
+    uint32_t base = 0;
+    if (run.head) base = atomicAdd(&fill[cw], static_cast<uint32_t>(run.len));
+    base = static_cast<uint32_t>(__shfl(static_cast<int>(base), run.first, WAVE));
+    if (valid) order_raw[start[cw] + base + static_cast<uint32_t>((threadIdx.x % WAVE) - run.first)] = static_cast<uint32_t>(i);
 }
 
 // k_cell_sort over every replica's cells: one wavefront per cell word.
@@ -343,8 +351,8 @@ __global__ void __launch_bounds__(BT)
     k_b_list_build(const Rep* __restrict__ reps, const uint16_t* __restrict__ block_rep,
                    const double4* __restrict__ pos, size_t slots, const int* __restrict__ cell_of,
                    const uint32_t* __restrict__ start, double radius2, int padding, int32_t* __restrict__ rows,
-                   int32_t* __restrict__ count, int32_t* __restrict__ half, int* __restrict__ flags,
-                   unsigned long long* __restrict__ pairs, BatchCopyBack cb, const int* __restrict__ stale) {
+                   int32_t* __restrict__ count, int32_t* __restrict__ half, int2* __restrict__ bstats,
+                   BatchCopyBack cb, const int* __restrict__ stale) {
     const int r = block_rep[blockIdx.x];
     if (!stale[r]) return;
     const Rep rp = reps[r];
@@ -416,19 +424,28 @@ __global__ void __launch_bounds__(BT)
         count[i] = cnt < padding ? cnt : padding;
         half[i] = hcnt < padding ? hcnt : padding;
     }
-    int mx = cnt;
-    unsigned long long hs = static_cast<unsigned long long>(hcnt);
-#pragma unroll
-    for (int off = WAVE / 2; off > 0; off >>= 1) {
-        const int o = __shfl_xor(mx, off, WAVE);
-        mx = o > mx ? o : mx;
-        hs += __shfl_xor(hs, off, WAVE);
+    block_build_stat<BT>(cnt, static_cast<unsigned>(hcnt), bstats + blockIdx.x);
+}
+
+// Per-replica totals of a build from its blocks' records (a replica's slots are whole blocks): half-pair count,
+// and the longest row and overflow over the stale replicas into flags[1] and flags[0].
+__global__ void k_b_build_totals(const Rep* __restrict__ reps, int count, const int2* __restrict__ bstats,
+                                 int padding, int* __restrict__ flags, unsigned long long* __restrict__ pairs,
+                                 const int* __restrict__ stale) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= count || !stale[r]) return;
+    const Rep rp = reps[r];
+    const uint32_t b0 = rp.first / BT, b1 = (rp.first + rp.n + BT - 1) / BT;
+    int mx = 0;
+    unsigned long long hs = 0;
+    for (uint32_t b = b0; b < b1; ++b) {
+        const int2 v = bstats[b];
+        mx = v.x > mx ? v.x : mx;
+        hs += static_cast<unsigned>(v.y);
     }
-    if (threadIdx.x % WAVE == 0) {
-        atomicMax(&flags[1], mx);
-        if (mx > padding) atomicOr(&flags[0], 1);
-        atomicAdd(&pairs[r], hs);
-    }
+    pairs[r] = hs;
+    atomicMax(&flags[1], mx);
+    if (mx > padding) atomicOr(&flags[0], 1);
 }
 
 // ---- forces --------------------------------------------------------------------------------------------
@@ -541,6 +558,7 @@ struct md_batch {
     int* stale = nullptr;
     int* rebuilds = nullptr;
     unsigned long long* pairs = nullptr;
+    int2* bstats = nullptr;     // per build block: longest row, half-row sum
     int* flags = nullptr;       // [0] overflow, [1] longest row, [2] step counter, [3] spare
     double* partials = nullptr; // force partials (up to blocks * 4 lanes * 3), then the kick's (blocks)
     double* ke_part = nullptr;
@@ -578,6 +596,7 @@ void batch_free(md_batch* b) {
     dev_free(b->stale);
     dev_free(b->rebuilds);
     dev_free(b->pairs);
+    dev_free(b->bstats);
     dev_free(b->flags);
     dev_free(b->partials);
     dev_free(b->ke_part);
@@ -615,6 +634,7 @@ void batch_alloc(md_batch* b) {
     b->stale = dev_alloc<int>(b->count);
     b->rebuilds = dev_alloc<int>(b->count);
     b->pairs = dev_alloc<unsigned long long>(b->count);
+    b->bstats = dev_alloc<int2>(b->blocks);
     b->flags = dev_alloc<int>(4);
     b->partials = dev_alloc<double>(static_cast<size_t>(b->blocks) * (4 * 3 + 1));
     b->ke_part = dev_alloc<double>(b->ke_total);
@@ -663,8 +683,10 @@ void enqueue_rebuild(md_batch* b) {
                                                                    b->stale);
     BatchCopyBack cb{b->vel_alt, b->frc_alt, b->mass_alt, b->id_alt, b->pos, b->vel, b->frc, b->mass, b->id};
     k_b_list_build<<<b->blocks, BT, 0, st>>>(b->d_reps, b->block_rep, b->pos_alt, S, b->cell_of_alt, b->start,
-                                             b->radius2, b->padding, b->rows, b->rcount, b->half, b->flags,
-                                             b->pairs, cb, b->stale);
+                                             b->radius2, b->padding, b->rows, b->rcount, b->half, b->bstats, cb,
+                                             b->stale);
+    k_b_build_totals<<<blocks_for(b->count, 256), 256, 0, st>>>(b->d_reps, b->count, b->bstats, b->padding,
+                                                                 b->flags, b->pairs, b->stale);
     MD_HIP(hipGetLastError());
 }
 

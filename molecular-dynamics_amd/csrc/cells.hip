@@ -54,16 +54,21 @@ struct CopyBackArgs {
 
 __device__ __forceinline__ bool gated_off(const int* gate) { return gate && *gate == 0; }
 
+// Cell of every particle and the cell histogram, one atomic per run of equal cells.
 __global__ void __launch_bounds__(256) k_cell_count(const double4* __restrict__ pos, size_t n, double per_length,
                                                      int side, int* __restrict__ cell_of,
                                                      uint32_t* __restrict__ counts, const int* gate) {
     if (gated_off(gate)) return;
     const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const double4 r = pos[i];
-    const int c = cell_of_position(r.x, r.y, r.z, per_length, side);
-    cell_of[i] = c;
-    atomicAdd(&counts[c], 1u);
+    const bool valid = i < n;
+    int c = -1;
+    if (valid) {
+        const double4 r = pos[i];
+        c = cell_of_position(r.x, r.y, r.z, per_length, side);
+        cell_of[i] = c;
+    }
+    const CellRun run = cell_run(c, valid);
+    if (run.head) atomicAdd(&counts[c], static_cast<uint32_t>(run.len));
 }
 
 // Three-phase exclusive scan of the cell histogram, SCAN_ITEMS cells per block: block totals, a scan of the
@@ -167,10 +172,14 @@ __global__ void __launch_bounds__(256) k_cell_scatter(const int* __restrict__ ce
                                                        const int* gate) {
     if (gated_off(gate)) return;
     const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int c = cell_of[i];
-    const uint32_t slot = atomicAdd(&fill[c], 1u);
-    order_raw[start[c] + slot] = static_cast<uint32_t>(i);
+    const bool valid = i < n;
+    const int c = valid ? cell_of[i] : -1;
+    // one atomic per run of equal cells reserves the run's slots; its lanes take them in lane order
+    const CellRun run = cell_run(c, valid);
+    uint32_t base = 0;
+    if (run.head) base = atomicAdd(&fill[c], static_cast<uint32_t>(run.len));
+    base = static_cast<uint32_t>(__shfl(static_cast<int>(base), run.first, WAVE));
+    if (valid) order_raw[start[c] + base + static_cast<uint32_t>((threadIdx.x % WAVE) - run.first)] = static_cast<uint32_t>(i);
 }
 
 // One wavefront per cell: rank = number of members with a smaller index, so the members land in increasing
@@ -230,8 +239,7 @@ __global__ void __launch_bounds__(256) k_list_build(const double4* __restrict__ 
                                                      const uint32_t* __restrict__ start, int side, double box,
                                                      double inv_box, double radius2, int use_cells, int padding,
                                                      int32_t* __restrict__ rows, int32_t* __restrict__ count,
-                                                     int32_t* __restrict__ half, int* __restrict__ flags,
-                                                     unsigned long long* __restrict__ pair_total,
+                                                     int32_t* __restrict__ half, int2* __restrict__ stats,
                                                      CopyBackArgs cb, const int* gate) {
     if (gated_off(gate)) return;
     const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -304,21 +312,7 @@ __global__ void __launch_bounds__(256) k_list_build(const double4* __restrict__ 
         count[i] = cnt < padding ? cnt : padding;
         half[i] = hcnt < padding ? hcnt : padding;
     }
-    // wavefront max / sum, then one atomic per wavefront
-    int mx = cnt;
-    unsigned long long hs = static_cast<unsigned long long>(hcnt);
-#pragma unroll
-    for (int off = WAVE / 2; off > 0; off >>= 1) {
-        const int o = __shfl_xor(mx, off, WAVE);
-        mx = o > mx ? o : mx;
-        hs += __shfl_xor(hs, off, WAVE);
-    }
-    if (threadIdx.x % WAVE == 0) {
-        atomicMax(&flags[FLAG_MAXCOUNT], mx);
-        if (mx > padding) atomicOr(&flags[FLAG_OVERFLOW], 1);
-        atomicAdd(pair_total, hs);
-    }
-    if (gate && i == 0) atomicAdd(&flags[FLAG_REBUILDS], 1);
+    block_build_stat<256>(cnt, static_cast<unsigned>(hcnt), stats + blockIdx.x);
 }
 
 __global__ void __launch_bounds__(256) k_displacement(const double4* __restrict__ pos,
@@ -333,11 +327,44 @@ __global__ void __launch_bounds__(256) k_displacement(const double4* __restrict_
     if (dx * dx + dy * dy + dz * dz > max_disp2) flags[FLAG_STALE] = 1;
 }
 
-__global__ void k_clear_build_flags(int* flags, unsigned long long* pair_total, const int* gate) {
+// Totals of a build from its per-block records: longest row, overflow, half-pair total, and in device-gated
+// builds the rebuild counter. One block; each thread folds a strided share of the records.
+constexpr int FINISH_THREADS = 1024;
+__global__ void __launch_bounds__(FINISH_THREADS) k_finish_build(const int2* __restrict__ stats, int blocks,
+                                                                 int padding, int* __restrict__ flags,
+                                                                 unsigned long long* __restrict__ pair_total,
+                                                                 const int* gate) {
     if (gated_off(gate)) return;
-    flags[FLAG_OVERFLOW] = 0;
-    flags[FLAG_MAXCOUNT] = 0;
-    *pair_total = 0ull;
+    __shared__ int s_mx[FINISH_THREADS / WAVE];
+    __shared__ unsigned long long s_hs[FINISH_THREADS / WAVE];
+    int mx = 0;
+    unsigned long long hs = 0;
+    for (int b = threadIdx.x; b < blocks; b += FINISH_THREADS) {
+        const int2 v = stats[b];
+        mx = v.x > mx ? v.x : mx;
+        hs += static_cast<unsigned>(v.y);
+    }
+#pragma unroll
+    for (int off = WAVE / 2; off > 0; off >>= 1) {
+        const int o = __shfl_xor(mx, off, WAVE);
+        mx = o > mx ? o : mx;
+        hs += __shfl_xor(hs, off, WAVE);
+    }
+    if (threadIdx.x % WAVE == 0) {
+        s_mx[threadIdx.x / WAVE] = mx;
+        s_hs[threadIdx.x / WAVE] = hs;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < FINISH_THREADS / WAVE; ++w) {
+            mx = s_mx[w] > mx ? s_mx[w] : mx;
+            hs += s_hs[w];
+        }
+        flags[FLAG_MAXCOUNT] = mx;
+        flags[FLAG_OVERFLOW] = mx > padding ? 1 : 0;
+        *pair_total = hs;
+        if (gate) flags[FLAG_REBUILDS] += 1;
+    }
 }
 
 namespace {
@@ -362,6 +389,12 @@ int auto_padding(size_t n, double box, double radius) {
 // Scratch words kept per list next to the flags: the running half-pair total.
 static unsigned long long* pair_total_of(md_list* l) {
     return reinterpret_cast<unsigned long long*>(l->flags + FLAG_COUNT);
+}
+
+void finish_build_enqueue(md_list* l, int blocks, const int* gate) {
+    k_finish_build<<<1, FINISH_THREADS, 0, l->ctx->stream>>>(l->build_stats, blocks, l->padding, l->flags,
+                                                             pair_total_of(l), gate);
+    MD_HIP(hipGetLastError());
 }
 
 void list_prepare(md_list* l, md_state* s) {
@@ -418,6 +451,12 @@ void list_prepare(md_list* l, md_state* s) {
         l->rows16_cap = need16;
     }
     const size_t tiles = static_cast<size_t>(tile_count(s->n ? s->n : 1));
+    const size_t stat_blocks = std::max<size_t>(tiles, blocks_for(s->n ? s->n : 1, 256));
+    if (stat_blocks > l->stats_cap || !l->build_stats) {
+        dev_free(l->build_stats);
+        l->build_stats = dev_alloc<int2>(stat_blocks);
+        l->stats_cap = stat_blocks;
+    }
     if (tiles > l->tiles_cap || !l->tile_nr) {
         dev_free(l->tile_ranges);
         dev_free(l->tile_nr);
@@ -494,7 +533,6 @@ static void list_kernel_enqueue(md_list* l, md_state* s, bool from_alt, const in
         cb.id = s->id;
         cb.cell_of = l->cell_of;
     }
-    k_clear_build_flags<<<1, 1, 0, st>>>(l->flags, pair_total_of(l), gate);
     if (tiling_enabled(n)) {
         tiled_list_enqueue(l, s, from_alt, gate, !l->slots_only);
         l->tiled = true;
@@ -503,9 +541,10 @@ static void list_kernel_enqueue(md_list* l, md_state* s, bool from_alt, const in
     }
     k_list_build<<<blocks_for(n ? n : 1, 256), 256, 0, st>>>(
         from_alt ? s->pos_alt : s->pos, n, from_alt ? l->cell_of_alt : l->cell_of, l->cell_start, l->side, s->box,
-        s->inv_box, l->radius2, l->side >= 3 ? 1 : 0, l->padding, l->rows, l->count, l->half, l->flags,
-        pair_total_of(l), cb, gate);
+        s->inv_box, l->radius2, l->side >= 3 ? 1 : 0, l->padding, l->rows, l->count, l->half, l->build_stats, cb,
+        gate);
     MD_HIP(hipGetLastError());
+    finish_build_enqueue(l, static_cast<int>(blocks_for(n ? n : 1, 256)), gate);
     l->tiled = false;
     l->rows32_valid = true;
 }
@@ -577,8 +616,7 @@ __global__ void __launch_bounds__(256) k_list_build_fixed(const double4* __restr
                                                            const uint32_t* __restrict__ order, int side, double box,
                                                            double inv_box, double radius2, int use_cells, int padding,
                                                            int32_t* __restrict__ rows, int32_t* __restrict__ count,
-                                                           int32_t* __restrict__ half, int* __restrict__ flags,
-                                                           unsigned long long* __restrict__ pair_total,
+                                                           int32_t* __restrict__ half, int2* __restrict__ stats,
                                                            double4* __restrict__ ref, const int* gate) {
     if (gated_off(gate)) return;
     const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -620,33 +658,20 @@ __global__ void __launch_bounds__(256) k_list_build_fixed(const double4* __restr
         count[i] = cnt < padding ? cnt : padding;
         half[i] = hcnt < padding ? hcnt : padding;
     }
-    int mx = cnt;
-    unsigned long long hs = static_cast<unsigned long long>(hcnt);
-#pragma unroll
-    for (int off = WAVE / 2; off > 0; off >>= 1) {
-        const int o = __shfl_xor(mx, off, WAVE);
-        mx = o > mx ? o : mx;
-        hs += __shfl_xor(hs, off, WAVE);
-    }
-    if (threadIdx.x % WAVE == 0) {
-        atomicMax(&flags[FLAG_MAXCOUNT], mx);
-        if (mx > padding) atomicOr(&flags[FLAG_OVERFLOW], 1);
-        atomicAdd(pair_total, hs);
-    }
-    if (gate && i == 0) atomicAdd(&flags[FLAG_REBUILDS], 1);
+    block_build_stat<256>(cnt, static_cast<unsigned>(hcnt), stats + blockIdx.x);
 }
 
 static void list_fixed_enqueue(md_list* l, md_state* s, const int* gate) {
     hipStream_t st = l->ctx->stream;
     const size_t n = s->n;
     cells_enqueue(l, s, l->side / s->box, false, gate);
-    k_clear_build_flags<<<1, 1, 0, st>>>(l->flags, pair_total_of(l), gate);
     l->tiled = false;
     l->rows32_valid = true;
     k_list_build_fixed<<<blocks_for(n ? n : 1, 256), 256, 0, st>>>(
         s->pos, n, l->cell_of, l->cell_start, l->order, l->side, s->box, s->inv_box, l->radius2,
-        l->side >= 3 ? 1 : 0, l->padding, l->rows, l->count, l->half, l->flags, pair_total_of(l), l->ref, gate);
+        l->side >= 3 ? 1 : 0, l->padding, l->rows, l->count, l->half, l->build_stats, l->ref, gate);
     MD_HIP(hipGetLastError());
+    finish_build_enqueue(l, static_cast<int>(blocks_for(n ? n : 1, 256)), gate);
 }
 
 void list_rebuild_fixed(md_list* l, md_state* s) {
@@ -693,7 +718,8 @@ static void release_list_buffers(md_list* l) {
     dev_free(l->rows16);
     dev_free(l->tile_ranges);
     dev_free(l->tile_nr);
-    l->rows16_cap = l->tiles_cap = 0;
+    dev_free(l->build_stats);
+    l->rows16_cap = l->tiles_cap = l->stats_cap = 0;
 }
 
 void set_error(const std::string& m);

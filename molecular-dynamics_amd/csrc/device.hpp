@@ -87,4 +87,52 @@ __device__ __forceinline__ void block_sum(double (&v)[W], double* out) {
     }
 }
 
+// Runs of equal cell ids among a wavefront's consecutive particles (the state is still nearly sorted by cell from
+// the last build): lane `head` marks the first lane of a run, `len` (valid in head lanes) its length, `first` the
+// run's first lane for every lane. Lanes past the end (valid = false) belong to no run.
+struct CellRun {
+    bool head;
+    int len, first;
+};
+
+__device__ __forceinline__ CellRun cell_run(int c, bool valid) {
+    const int lane = threadIdx.x % WAVE;
+    const int prev = __shfl_up(c, 1, WAVE);
+    const bool head = valid && (lane == 0 || prev != c);
+    const unsigned long long heads = __ballot(head);
+    const int live = __popcll(__ballot(valid));  // valid lanes are a prefix of the wavefront
+    CellRun r;
+    r.head = head;
+    const unsigned long long later = lane == WAVE - 1 ? 0ull : heads & (~0ull << (lane + 1));
+    r.len = (later ? __ffsll(static_cast<long long>(later)) - 1 : live) - lane;
+    const unsigned long long upto = lane == WAVE - 1 ? heads : heads & ((2ull << lane) - 1ull);
+    r.first = upto ? WAVE - 1 - __clzll(static_cast<long long>(upto)) : 0;
+    return r;
+}
+
+// Per-block record of a list build: (longest row seen, sum of the block's half-row lengths). A block covers at
+// most 512 rows of at most `padding` entries each, and padding x n int32 rows fit in device memory, so the sum
+// fits in 32 bits.
+template <int THREADS>
+__device__ __forceinline__ void block_build_stat(int mx, unsigned hs, int2* out) {
+    __shared__ int2 lds[THREADS / WAVE];
+#pragma unroll
+    for (int off = WAVE / 2; off > 0; off >>= 1) {
+        const int o = __shfl_xor(mx, off, WAVE);
+        mx = o > mx ? o : mx;
+        hs += __shfl_xor(hs, off, WAVE);
+    }
+    if (threadIdx.x % WAVE == 0) lds[threadIdx.x / WAVE] = make_int2(mx, static_cast<int>(hs));
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int m = 0;
+        unsigned h = 0;
+        for (int w = 0; w < THREADS / WAVE; ++w) {
+            m = lds[w].x > m ? lds[w].x : m;
+            h += static_cast<unsigned>(lds[w].y);
+        }
+        *out = make_int2(m, static_cast<int>(h));
+    }
+}
+
 }  // namespace mdgpu

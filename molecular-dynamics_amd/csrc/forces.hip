@@ -155,7 +155,7 @@ void forces_enqueue(md_state* s, md_list* l, const md_lj& p, int precision, doub
                                                          lj, s->frc, s->partials, clear_flag);
     MD_HIP(hipGetLastError());
     if (ev_stop) MD_HIP(hipEventRecord(ev_stop, st));
-    reduce_partials_enqueue(st, s->partials, static_cast<int>(blocks), 3, out_dev);
+    reduce_partials_enqueue(s, static_cast<int>(blocks), 3, out_dev);
 }
 
 void forces_enqueue(md_state* s, md_list* l, const md_lj& p, int precision, double* out_dev) {
@@ -170,7 +170,7 @@ void forces_allpairs_enqueue(md_state* s, const md_lj& p, double* out_dev) {
     k_forces_allpairs<<<blocks, FORCE_THREADS, 0, st>>>(s->pos, n, s->box, s->inv_box, make_lj(p), s->frc,
                                                          s->partials);
     MD_HIP(hipGetLastError());
-    reduce_partials_enqueue(st, s->partials, static_cast<int>(blocks), 3, out_dev);
+    reduce_partials_enqueue(s, static_cast<int>(blocks), 3, out_dev);
 }
 
 void set_error(const std::string& m);

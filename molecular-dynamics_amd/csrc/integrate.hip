@@ -157,7 +157,7 @@ void kick_enqueue(md_state* s, double dt, double* ke_dev) {
     k_kick<<<blocks, VV_THREADS, 0, s->ctx->stream>>>(s->vel, s->frc, s->mass, s->n, 0.5 * dt,
                                                       ke_dev ? s->partials : nullptr);
     MD_HIP(hipGetLastError());
-    if (ke_dev) reduce_partials_enqueue(s->ctx->stream, s->partials, static_cast<int>(blocks), 1, ke_dev);
+    if (ke_dev) reduce_partials_enqueue(s, static_cast<int>(blocks), 1, ke_dev);
 }
 
 // The list update of the eager step: staleness test, read-back, host-driven rebuild (NeighbourList.cpp:11-36).
@@ -306,7 +306,7 @@ int md_run(md_state* s, md_list* l, const md_run_config* cfg, double* energies) 
         k_kick_kick_drift<<<vv_blocks, VV_THREADS, 0, st>>>(s->pos, s->vel, s->frc, s->mass, s->n, 0.5 * cfg->dt,
                                                            cfg->dt, s->box, s->inv_box, l->ref, l->max_disp2,
                                                            l->flags + FLAG_STALE, s->partials);
-        reduce_partials_enqueue(st, s->partials, static_cast<int>(vv_blocks), 1, s->scalars + 3);
+        reduce_partials_enqueue(s, static_cast<int>(vv_blocks), 1, s->scalars + 3);
         k_record<<<1, 1, 0, st>>>(s->scalars, 3, rb.energies, rb.counters);
         enqueue_forces();
         MD_HIP(hipGetLastError());

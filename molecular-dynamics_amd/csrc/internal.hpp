@@ -110,6 +110,8 @@ struct md_state {
     double* partials = nullptr;
     size_t partials_cap = 0;
     double* scalars = nullptr;  // [0..2] U, W, inside; [3] KE; [4] KE fast; [5..7] scratch
+    double* red_blocks = nullptr;   // per-block sums of the partials reduction
+    unsigned* red_ticket = nullptr; // its completion counter (zero between launches)
 };
 
 struct md_list {
@@ -140,6 +142,8 @@ struct md_list {
     int32_t* count = nullptr;        // n
     int32_t* half = nullptr;         // n
     int* flags = nullptr;            // FLAG_COUNT ints on the device
+    int2* build_stats = nullptr;     // per build block: longest row, half-row sum (reduced by k_finish_build)
+    size_t stats_cap = 0;
     // tiled view of the same rows (tiled.hip): LDS slots of each tile's staged halo
     uint16_t* rows16 = nullptr;      // ceil(padding / 32) chunks of 32 entries per particle, chunk-major
     size_t rows16_cap = 0;
@@ -162,6 +166,8 @@ void list_prepare(md_list* l, md_state* s);                            // grid/b
 void displacement_check(md_list* l, md_state* s);                      // sets FLAG_STALE
 void list_rebuild_fixed(md_list* l, md_state* s);                      // list without re-sorting (host-driven)
 void list_enqueue_conditional_rebuild_fixed(md_list* l, md_state* s); // same, gated by FLAG_STALE
+// build totals (FLAG_MAXCOUNT, FLAG_OVERFLOW, half-pair total, FLAG_REBUILDS when gated) from `blocks` records
+void finish_build_enqueue(md_list* l, int blocks, const int* gate);
 
 // api.hip
 void state_reserve_partials(md_state* s, size_t count);
@@ -199,7 +205,7 @@ void tiled_forces_enqueue(md_state* s, md_list* l, const LJConst& lj, double cut
 void kick_drift_enqueue(md_state* s, double dt, md_list* check_list);
 void kick_enqueue(md_state* s, double dt, double* ke_dev);
 
-// Final fixed-order reduction of `blocks` rows of `width` partials into out_dev[0..width).
-void reduce_partials_enqueue(hipStream_t st, const double* partials, int blocks, int width, double* out_dev);
+// Final fixed-order reduction of `blocks` rows of `width` partials (s->partials) into out_dev[0..width).
+void reduce_partials_enqueue(md_state* s, int blocks, int width, double* out_dev);
 
 }  // namespace mdgpu

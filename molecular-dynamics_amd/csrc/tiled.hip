@@ -39,6 +39,7 @@ constexpr int STAGE_ROUNDS = (TILE_HALO_MAX + TILE_THREADS - 1) / TILE_THREADS;
 constexpr int CHUNK = 32;    // slot-row entries per particle and chunk
 constexpr int CHUNK_STRIDE = TILE * CHUNK;  // entries between a tile's chunks
 constexpr int GHOST_SLOT = TILE_HALO_MAX - 1;  // LDS slot holding NaN coordinates: never a neighbour
+constexpr int PRE_CHUNKS = 4;  // slot chunks the force kernel loads per lane ahead of its pair loop
 
 // Position of a particle's first slot-row entry, and of entry k relative to it.
 __device__ __forceinline__ size_t slot_row_base(size_t i, int row_len) {
@@ -48,6 +49,14 @@ __device__ __forceinline__ uint32_t slot_entry(uint32_t k) {
     return ((k & ~31u) << 7) | ((k & 3u) << 3) | ((k >> 2) & 7u);  // (k / 32) * 4096 + 8 (k % 4) + (k / 4) % 8
 }
 constexpr int SETUP_TILES = 4;
+// tile_nr[t] holds the tile's number of halo ranges (-1: not staged), plus TILE_FLAT when no periodic image can
+// enter the tile's frame: its cells lie in one cell column at least two cells from every face of the grid, so its
+// halo cells are at least one cell from every face (a particle moves less than a cell between builds, so none
+// has wrapped), and the tile and its halo span less than half the box. Then t - L rint(t / L) = t for every
+// staged coordinate t = x - ref, and staging skips the image term.
+constexpr int TILE_FLAT = 1 << 16;
+__device__ __forceinline__ int tile_range_count(int raw) { return raw < 0 ? -1 : (raw & (TILE_FLAT - 1)); }
+__device__ __forceinline__ bool tile_is_flat(int raw) { return raw > 0 && (raw & TILE_FLAT) != 0; }
 static_assert(RMAX == WAVE, "one lane per halo range in k_tile_setup");
 static_assert(CHUNK_STRIDE == 4096, "slot_entry shifts the chunk index by 7 + 5 bits");
 
@@ -122,11 +131,12 @@ __device__ int tile_cell_ranges(int c_first, int c_last, int s, int2* cr) {
 // wavefront scan assigns the LDS slots. tile_nr = -1 marks a tile whose halo does not fit in `hmax` slots.
 __global__ void __launch_bounds__(SETUP_TILES * WAVE)
     k_tile_setup(size_t n, const int* __restrict__ cell_of, const uint32_t* __restrict__ start, int side,
-                 int use_cells, int hmax, int tiles, int4* __restrict__ tile_ranges, int* __restrict__ tile_nr,
-                 const int* gate) {
+                 int use_cells, int hmax, int tiles, double flat_cells, int4* __restrict__ tile_ranges,
+                 int* __restrict__ tile_nr, const int* gate) {
     if (gate && *gate == 0) return;
     __shared__ int2 cr_all[SETUP_TILES][CR_CAP];
     __shared__ int nr_all[SETUP_TILES];
+    __shared__ int flat_all[SETUP_TILES];
     const int w = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
     const int t = blockIdx.x * SETUP_TILES + w;
     const bool live = t < tiles;
@@ -134,14 +144,19 @@ __global__ void __launch_bounds__(SETUP_TILES * WAVE)
     if (live && lane == 0) {
         const size_t i0 = static_cast<size_t>(t) * TILE;
         const size_t i1 = (i0 + TILE) < n ? (i0 + TILE) : n;
-        int nr;
+        int nr, flat = 0;
         if (use_cells) {
-            nr = tile_cell_ranges(cell_of[i0], cell_of[i1 - 1], side, cr);
+            const int c0 = cell_of[i0], c1 = cell_of[i1 - 1];
+            nr = tile_cell_ranges(c0, c1, side, cr);
+            const int col = c0 / side, x = col / side, y = col % side, z0 = c0 % side, z1 = c1 % side;
+            flat = col == c1 / side && x >= 2 && x <= side - 3 && y >= 2 && y <= side - 3 && z0 >= 2 &&
+                   z1 <= side - 3 && static_cast<double>(c1 - c0 + 3) < flat_cells;
         } else {
             cr[0] = make_int2(0, side * side * side - 1);  // every particle
             nr = 1;
         }
         nr_all[w] = nr;
+        flat_all[w] = flat;
     }
     __syncthreads();
     if (!live) return;
@@ -161,7 +176,7 @@ __global__ void __launch_bounds__(SETUP_TILES * WAVE)
     const int total = __shfl(incl, WAVE - 1, WAVE);
     if (lane < nr)
         tile_ranges[static_cast<size_t>(t) * RMAX + lane] = make_int4(first_cell, first, incl - cnt, cnt);
-    if (lane == 0) tile_nr[t] = (nr > 0 && total <= hmax) ? nr : -1;
+    if (lane == 0) tile_nr[t] = (nr > 0 && total <= hmax) ? nr + (flat_all[w] ? TILE_FLAT : 0) : -1;
 }
 
 // Tile header shared by the build and the force kernel: the tile's halo ranges in LDS.
@@ -277,6 +292,7 @@ struct Segment {
 };
 
 constexpr int LIST_CAP = 6144;  // flat candidate lists of a tile's own cells, in LDS slots
+constexpr int ROWBUF_CAP = 320; // row entries a wavefront of the list walk buffers in LDS (longer rows go direct)
 constexpr int OWN_STENCIL = 13; // stencil index of (0, 0, 0)
 
 // Candidates j of particle i, walked in the order of its row. `STAGED` reads the halo from LDS, else from global
@@ -351,8 +367,8 @@ __global__ void __launch_bounds__(TILE_THREADS)
                        const uint32_t* __restrict__ start, TileArgs ta, double box, double inv_box, double radius2,
                        int padding, int write32, int row_len, int32_t* __restrict__ rows,
                        uint16_t* __restrict__ rows16,
-                       int32_t* __restrict__ count, int32_t* __restrict__ half, int* __restrict__ flags,
-                       unsigned long long* __restrict__ pair_total, const int4* __restrict__ tile_ranges,
+                       int32_t* __restrict__ count, int32_t* __restrict__ half, int2* __restrict__ stats,
+                       const int4* __restrict__ tile_ranges,
                        const int* __restrict__ tile_nr, TiledCopyBack cb, const int* gate) {
     if (gate && *gate == 0) return;
     extern __shared__ double lds[];
@@ -360,10 +376,17 @@ __global__ void __launch_bounds__(TILE_THREADS)
     double* ys = lds + TILE_HALO_MAX;
     double* zs = lds + 2 * TILE_HALO_MAX;
     __shared__ int ra[RMAX], rgs[RMAX], rofs[RMAX + 1];
-    __shared__ Segment seg[CMAX * 27];
+    // the stencil segments are dead once the candidate lists are filled; the walk then keeps each wavefront's
+    // current row in the same memory
+    __shared__ union {
+        Segment seg[CMAX * 27];
+        uint16_t rowbuf[TILE_THREADS / WAVE][ROWBUF_CAP];
+    } sm;
+    Segment* const seg = sm.seg;
     __shared__ uint16_t lists[LIST_CAP];
     __shared__ int lst[CMAX][4];  // U offset, U length, L offset, L length (lengths padded to 64 with the ghost)
     __shared__ int cl_first[CMAX];
+    __shared__ int cl_slot[CMAX];  // LDS slot of the own cell's first particle
     __shared__ unsigned char cl_interior[CMAX];
     __shared__ int s_outside, s_lists_ok;
     const HaloRanges hr{ra, rgs, rofs};
@@ -371,7 +394,7 @@ __global__ void __launch_bounds__(TILE_THREADS)
     const size_t i0 = static_cast<size_t>(tile) * TILE;
     const size_t i1 = (i0 + TILE) < n ? (i0 + TILE) : n;
     const int s = ta.side;
-    const int nr = tile_nr[tile];
+    const int nr = tile_range_count(tile_nr[tile]);
     const bool staged = nr > 0;
     const int c_first = ta.use_cells ? cell_of[i0] : 0;
     const int ncl = ta.use_cells ? cell_of[i1 - 1] - c_first + 1 : 0;
@@ -424,73 +447,97 @@ __global__ void __launch_bounds__(TILE_THREADS)
         seg[t] = g;
     }
     __syncthreads();
+    // Candidate lists: list L = 2 cl + lower (U_c, then L_c). Half-wavefront h takes lists h, h + 16, h + 32; its
+    // lane j holds the list's j-th segment (U_c: stencil index j, kept if nc >= c; L_c: the j-th stencil cell in
+    // increasing id, kept if nc <= c), and a 32-lane scan of the kept lengths gives every segment's offset and the
+    // list's length. The scan results stay in registers until the fill below.
+    constexpr int HALF = WAVE / 2;
+    constexpr int LIST_ROUNDS = (2 * CMAX + TILE_THREADS / HALF - 1) / (TILE_THREADS / HALF);
+    const int hw = threadIdx.x / HALF, hl = threadIdx.x % HALF;
+    int seg_off[LIST_ROUNDS], seg_base[LIST_ROUNDS], seg_len[LIST_ROUNDS], list_used[LIST_ROUNDS];
+#pragma unroll
+    for (int u = 0; u < LIST_ROUNDS; ++u) {
+        const int L = hw + u * (TILE_THREADS / HALF);
+        seg_off[u] = seg_base[u] = seg_len[u] = list_used[u] = 0;
+        if (!use_lists || L >= 2 * ncl) continue;  // uniform over the half-wavefront
+        const int cl = L >> 1, lower = L & 1;
+        const int c = c_first + cl;
+        int len = 0, base = 0;
+        if (hl < 27) {
+            int st = hl;
+            if (lower) {
+                // stencil offset index (0, 1, 2 for -1, 0, +1) of the a-th smallest wrapped coordinate
+                auto axis = [&](int v, int a) { return v == 0 ? (a + 1) % 3 : (v == s - 1 ? (a + 2) % 3 : a); };
+                const int cx = c / (s * s), cy = (c / s) % s, cz = c % s;
+                st = axis(cx, hl / 9) * 9 + axis(cy, (hl / 3) % 3) * 3 + axis(cz, hl % 3);
+            }
+            const Segment g = seg[cl * 27 + st];
+            if (lower ? g.nc <= c : g.nc >= c) {
+                len = g.len;
+                base = g.base;
+            }
+        }
+        int incl = len;
+#pragma unroll
+        for (int d = 1; d < HALF; d <<= 1) {
+            const int o = __shfl_up(incl, d, HALF);
+            if (hl >= d) incl += o;
+        }
+        seg_off[u] = incl - len;
+        seg_base[u] = base;
+        seg_len[u] = len;
+        list_used[u] = __shfl(incl, HALF - 1, HALF);
+        if (hl == 0) lst[cl][lower ? 3 : 1] = (list_used[u] + WAVE - 1) / WAVE * WAVE;
+    }
     if (use_lists && static_cast<int>(threadIdx.x) < ncl) {
         const int cl = threadIdx.x, c = c_first + cl;
-        int ul = 0, ll = 0;
-        for (int st = 0; st < 27; ++st) {
-            const Segment g = seg[cl * 27 + st];
-            if (g.nc >= c) ul += g.len;
-            if (g.nc <= c) ll += g.len;
-        }
-        lst[cl][1] = (ul + WAVE - 1) / WAVE * WAVE;
-        lst[cl][3] = (ll + WAVE - 1) / WAVE * WAVE;
         cl_first[cl] = static_cast<int>(start[c]);
+        cl_slot[cl] = seg[cl * 27 + OWN_STENCIL].base;
         const int cx = c / (s * s), cy = (c / s) % s, cz = c % s;
         cl_interior[cl] = ta.interior_ok && cx >= 1 && cx <= s - 2 && cy >= 1 && cy <= s - 2 && cz >= 1 &&
                           cz <= s - 2;
     }
     __syncthreads();
-    if (use_lists && threadIdx.x == 0) {
-        int off = 0;
-        for (int cl = 0; cl < ncl; ++cl) {
-            lst[cl][0] = off;
-            off += lst[cl][1];
-            lst[cl][2] = off;
-            off += lst[cl][3];
+    if (use_lists && threadIdx.x < WAVE) {  // list offsets in order U_0, L_0, U_1, ...: one wavefront scan
+        const int L = threadIdx.x;
+        const int len = L < 2 * ncl ? lst[L >> 1][(L & 1) ? 3 : 1] : 0;
+        int incl = len;
+#pragma unroll
+        for (int d = 1; d < WAVE; d <<= 1) {
+            const int o = __shfl_up(incl, d, WAVE);
+            if (L >= d) incl += o;
         }
-        s_lists_ok = off <= LIST_CAP ? 1 : 0;
+        if (L < 2 * ncl) lst[L >> 1][(L & 1) ? 2 : 0] = incl - len;
+        if (L == WAVE - 1) s_lists_ok = incl <= LIST_CAP ? 1 : 0;
     }
     __syncthreads();
     const bool lists_ok = use_lists && s_lists_ok;
-    // fill: each (own cell, stencil cell, U or L) task copies its segment's slots to its place in the list
-    for (int t = threadIdx.x; lists_ok && t < 2 * ncl * 27; t += TILE_THREADS) {
-        const bool lower = t >= ncl * 27;
-        const int k = lower ? t - ncl * 27 : t;
-        const int cl = k / 27, st = k % 27;
-        const int c = c_first + cl;
-        const Segment g = seg[k];
-        if (lower ? g.nc > c : g.nc < c) continue;
-        int off = lst[cl][lower ? 2 : 0];
-        for (int q = 0; q < 27; ++q) {  // earlier segments of the same list
-            const Segment h = seg[cl * 27 + q];
-            if (lower ? (h.nc < g.nc) : (q < st && h.nc >= c)) off += h.len;
-        }
-        for (int m = 0; m < g.len; ++m) lists[off + m] = static_cast<uint16_t>(g.base + m);
-    }
-    for (int t = threadIdx.x; lists_ok && t < 2 * ncl * WAVE; t += TILE_THREADS) {  // ghost tails
-        const int cl = t / (2 * WAVE), lower = (t / WAVE) & 1, q = t % WAVE;
-        const int c = c_first + cl;
-        int used = 0;  // unpadded length of this list
-        for (int st = 0; st < 27; ++st) {
-            const Segment g = seg[cl * 27 + st];
-            if (lower ? g.nc <= c : g.nc >= c) used += g.len;
-        }
+    // fill: each lane copies its segment's slots; the list's lanes then pad it to a multiple of 64 with the ghost
+#pragma unroll
+    for (int u = 0; u < LIST_ROUNDS; ++u) {
+        const int L = hw + u * (TILE_THREADS / HALF);
+        if (!lists_ok || L >= 2 * ncl) continue;
+        const int cl = L >> 1, lower = L & 1;
+        const int off = lst[cl][lower ? 2 : 0];
+        for (int m = 0; m < seg_len[u]; ++m) lists[off + seg_off[u] + m] = static_cast<uint16_t>(seg_base[u] + m);
         const int padded = lst[cl][lower ? 3 : 1];
-        if (used + q < padded) lists[lst[cl][lower ? 2 : 0] + used + q] = static_cast<uint16_t>(GHOST_SLOT);
+        for (int q = list_used[u] + hl; q < padded; q += HALF) lists[off + q] = static_cast<uint16_t>(GHOST_SLOT);
     }
     __syncthreads();
-    int ep_max = 0;                  // longest row seen by this lane's wavefront / lane group
-    unsigned long long ep_half = 0;  // half-row lengths summed once per particle
+    int ep_max = 0;      // longest row seen by this lane's wavefront / lane group
+    unsigned ep_half = 0;  // half-row lengths summed once per particle
     if (lists_ok) {
         const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
         for (size_t p = wave; p < i1 - i0; p += TILE_THREADS / WAVE) {
             const size_t i = i0 + p;
             const int cl = cell_of[i] - c_first;
-            const int slot_i = seg[cl * 27 + OWN_STENCIL].base + (static_cast<int>(i) - cl_first[cl]);
+            const int slot_i = cl_slot[cl] + (static_cast<int>(i) - cl_first[cl]);
             const double xi = xs[slot_i], yi = ys[slot_i], zi = zs[slot_i];
             const bool interior = cl_interior[cl] && !s_outside;
             uint16_t* const row16 = rows16 + slot_row_base(i, row_len);
             int32_t* const row = rows + i;
+            uint16_t* const rb = sm.rowbuf[wave];
+            const bool buffered = padding <= ROWBUF_CAP;
             int cnt = 0;
             // one batch of 64 candidates: test, rank the accepted ones over the ballot, store, advance the count
             auto batch = [&](int q, bool upper, auto interior_tag) {
@@ -510,7 +557,10 @@ __global__ void __launch_bounds__(TILE_THREADS)
                         static_cast<unsigned>(mask >> 32), __builtin_amdgcn_mbcnt_lo(static_cast<unsigned>(mask), 0u));
                     const unsigned at = static_cast<unsigned>(cnt) + rank;
                     if (at < static_cast<unsigned>(padding)) {
-                        row16[slot_entry(at)] = static_cast<uint16_t>(sl);
+                        if (buffered)
+                            rb[at] = static_cast<uint16_t>(sl);
+                        else
+                            row16[slot_entry(at)] = static_cast<uint16_t>(sl);
                         if (write32) row[static_cast<uint64_t>(at) * n] = halo_particle(sl, hr, nr);
                     }
                 }
@@ -531,12 +581,29 @@ __global__ void __launch_bounds__(TILE_THREADS)
                 return h;
             };
             const int hcnt = interior ? both(std::true_type{}) : both(std::false_type{});
+            if (buffered) {
+                // the row leaves LDS as 16-byte chunk words: lane 4 c + w packs entries 32 c + w + 4 t, t = 0..7
+                // (the wavefront's LDS accesses complete in issue order, so every lane's entries are in place)
+                __builtin_amdgcn_wave_barrier();
+                const int stored = cnt < padding ? cnt : padding;
+                if (lane < (stored + CHUNK - 1) / CHUNK * LANES) {
+                    const int c = lane >> 2, w = lane & 3;
+                    const uint16_t* e = rb + c * CHUNK + w;
+                    uint4 v;
+                    v.x = static_cast<unsigned>(e[0]) | (static_cast<unsigned>(e[4]) << 16);
+                    v.y = static_cast<unsigned>(e[8]) | (static_cast<unsigned>(e[12]) << 16);
+                    v.z = static_cast<unsigned>(e[16]) | (static_cast<unsigned>(e[20]) << 16);
+                    v.w = static_cast<unsigned>(e[24]) | (static_cast<unsigned>(e[28]) << 16);
+                    *reinterpret_cast<uint4*>(row16 + c * CHUNK_STRIDE + 8 * w) = v;
+                }
+                __builtin_amdgcn_wave_barrier();
+            }
             if (lane == 0) {
                 count[i] = cnt < padding ? cnt : padding;
                 half[i] = hcnt < padding ? hcnt : padding;
             }
             ep_max = cnt > ep_max ? cnt : ep_max;
-            if (lane == 0) ep_half += static_cast<unsigned long long>(hcnt);
+            if (lane == 0) ep_half += static_cast<unsigned>(hcnt);
         }
     } else {
         const int lane = threadIdx.x & (LANES - 1);
@@ -609,25 +676,12 @@ __global__ void __launch_bounds__(TILE_THREADS)
             if (lane == 0) {
                 count[i] = sc.cnt < padding ? sc.cnt : padding;
                 half[i] = hcnt < padding ? hcnt : padding;
-                ep_half = static_cast<unsigned long long>(hcnt);
+                ep_half = static_cast<unsigned>(hcnt);
             }
             ep_max = sc.cnt;
         }
     }
-    int mx = ep_max;
-    unsigned long long hs = ep_half;
-#pragma unroll
-    for (int off = WAVE / 2; off > 0; off >>= 1) {
-        const int o = __shfl_xor(mx, off, WAVE);
-        mx = o > mx ? o : mx;
-        hs += __shfl_xor(hs, off, WAVE);
-    }
-    if ((threadIdx.x & (WAVE - 1)) == 0) {
-        atomicMax(&flags[FLAG_MAXCOUNT], mx);
-        if (mx > padding) atomicOr(&flags[FLAG_OVERFLOW], 1);
-        atomicAdd(pair_total, hs);
-    }
-    if (gate && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&flags[FLAG_REBUILDS], 1);
+    block_build_stat<TILE_THREADS>(ep_max, ep_half, stats + tile);
 }
 
 // int32 rows from the 16-bit slot rows (builds that wrote only the slots): slot -> particle through the tile's
@@ -639,7 +693,7 @@ __global__ void __launch_bounds__(TILE_THREADS)
     __shared__ int ra[RMAX], rgs[RMAX], rofs[RMAX + 1];
     const HaloRanges hr{ra, rgs, rofs};
     const int tile = blockIdx.x;
-    const int nr = tile_nr[tile];
+    const int nr = tile_range_count(tile_nr[tile]);
     if (nr <= 0) return;
     load_ranges(tile_ranges, tile, nr, hr);
     __syncthreads();
@@ -684,29 +738,57 @@ __global__ void __launch_bounds__(TILE_THREADS)
     __shared__ int extent_bits;
     const HaloRanges hr{ra, rgs, rofs};
     const int tile = xcd_item(static_cast<int>(blockIdx.x), fr.tiles);
-    const int nr = tile_nr[tile];
+    const int raw_nr = tile_nr[tile];
+    const int nr = tile_range_count(raw_nr);
     const bool staged = nr > 0;
+    const bool flat = tile_is_flat(raw_nr);
     const size_t i0 = static_cast<size_t>(tile) * TILE;
+    const int lane = threadIdx.x & (LANES - 1);
+    const size_t i = i0 + (threadIdx.x >> 2);
+    const bool active = i < n;
+    int m = 0, h = 0;  // row lengths, loaded first: the slot loads below depend on them
+    if (active) {
+        m = count[i];
+        h = half[i];
+    }
     const double4 ref = pos[i0];
     if (threadIdx.x == 0) extent_bits = 0;
     if (staged) load_ranges(tile_ranges, tile, nr, hr);
     __syncthreads();
-    if (staged) {
+    if (staged && flat) {
+        stage_halo(pos, hr, nr, rofs[nr], [&](int q, const double4& p) {
+            xs[q] = p.x - ref.x;
+            ys[q] = p.y - ref.y;
+            zs[q] = p.z - ref.z;
+        });
+    } else if (staged) {
         stage_halo(pos, hr, nr, rofs[nr], [&](int q, const double4& p) {
             xs[q] = frame_coord(p.x, ref.x, fr);
             ys[q] = frame_coord(p.y, ref.y, fr);
             zs[q] = frame_coord(p.z, ref.z, fr);
         });
     }
-    const int lane = threadIdx.x & (LANES - 1);
-    const size_t i = i0 + (threadIdx.x >> 2);
-    const bool active = i < n;
+    // the lane's slot words of the row's first PRE_CHUNKS chunks, all issued before the barrier, so the pair loop
+    // below runs from registers (longer rows load their remaining chunks in the loop)
+    const int chunks = (m + CHUNK - 1) / CHUNK;
+    const uint4* rp = reinterpret_cast<const uint4*>(rows16 + slot_row_base(i, row_len) + lane * 8);
+    constexpr int stride = CHUNK_STRIDE / 8;  // uint4 words between chunks
+    uint4 pre[PRE_CHUNKS];
+#pragma unroll
+    for (int c = 0; c < PRE_CHUNKS; ++c)
+        pre[c] = staged && c < chunks ? rp[c * stride] : make_uint4(0u, 0u, 0u, 0u);
     double xi = 0.0, yi = 0.0, zi = 0.0;
     if (active) {
         const double4 ri = pos[i];
-        xi = frame_coord(ri.x, ref.x, fr);
-        yi = frame_coord(ri.y, ref.y, fr);
-        zi = frame_coord(ri.z, ref.z, fr);
+        if (flat) {
+            xi = ri.x - ref.x;
+            yi = ri.y - ref.y;
+            zi = ri.z - ref.z;
+        } else {
+            xi = frame_coord(ri.x, ref.x, fr);
+            yi = frame_coord(ri.y, ref.y, fr);
+            zi = frame_coord(ri.z, ref.z, fr);
+        }
         if (lane == 0) {  // how far the tile's own particles lie from the reference particle
             const float e = static_cast<float>(fmax(fabs(xi), fmax(fabs(yi), fabs(zi))));
             atomicMax(&extent_bits, __float_as_int(e));
@@ -717,17 +799,11 @@ __global__ void __launch_bounds__(TILE_THREADS)
     const bool compact = staged && (2.0f * e + static_cast<float>(fr.reach) < static_cast<float>(fr.box) - 1.0f);
     PairSums a;
     if (active) {
-        const int m = count[i], h = half[i];
         if (staged) {
             auto run = [&](auto wrap) {
                 constexpr bool W = decltype(wrap)::value;
-                const int chunks = (m + CHUNK - 1) / CHUNK;
-                const uint4* rp = reinterpret_cast<const uint4*>(rows16 + slot_row_base(i, row_len) + lane * 8);
-                constexpr int stride = CHUNK_STRIDE / 8;  // uint4 words between chunks
-                uint4 cur = chunks > 0 ? rp[0] : make_uint4(0u, 0u, 0u, 0u);
-                for (int c = 0; c < chunks; ++c) {
-                    const uint4 nxt = c + 1 < chunks ? rp[(c + 1) * stride] : make_uint4(0u, 0u, 0u, 0u);
-                    const unsigned words[4] = {cur.x, cur.y, cur.z, cur.w};
+                auto chunk_pairs = [&](const uint4& cw, int c) {
+                    const unsigned words[4] = {cw.x, cw.y, cw.z, cw.w};
 #pragma unroll
                     for (int t = 0; t < 8; ++t) {
                         const int k = c * CHUNK + t * LANES + lane;
@@ -742,8 +818,10 @@ __global__ void __launch_bounds__(TILE_THREADS)
                             mixed_pair_term(dx, dy, dz, lj, k < h, a);
                         }
                     }
-                    cur = nxt;
-                }
+                };
+#pragma unroll
+                for (int c = 0; c < PRE_CHUNKS; ++c) chunk_pairs(pre[c], c);
+                for (int c = PRE_CHUNKS; c < chunks; ++c) chunk_pairs(rp[c * stride], c);
             };
             if (compact)
                 run(std::false_type{});
@@ -802,7 +880,10 @@ size_t tiled_rows16_entries(size_t n, int padding) {
 void tiled_list_enqueue(md_list* l, md_state* s, bool from_alt, const int* gate, bool write32) {
     hipStream_t st = l->ctx->stream;
     const size_t n = s->n;
-    if (n == 0) return;
+    if (n == 0) {
+        finish_build_enqueue(l, 0, gate);
+        return;
+    }
     if (l->hmax > TILE_HALO_MAX) fail(MD_ERR_STATE, "tile halo capacity exceeds the LDS bound");
     TiledCopyBack cb{};
     cb.enabled = from_alt ? 1 : 0;
@@ -822,14 +903,18 @@ void tiled_list_enqueue(md_list* l, md_state* s, bool from_alt, const int* gate,
     const int tiles = tile_count(n);
     TileArgs ta{l->side, l->side >= 3 ? 1 : 0, l->hmax, tiles, l->side >= 7 ? 1 : 0};
     const int* cells = from_alt ? l->cell_of_alt : l->cell_of;
+    // cells that a tile and its halo may span with every coordinate inside half a box (skin: the largest drift)
+    const double flat_cells = (0.5 * s->box - l->skin) / (s->box / l->side);
     hipLaunchKernelGGL(k_tile_setup, dim3((tiles + SETUP_TILES - 1) / SETUP_TILES), dim3(SETUP_TILES * WAVE), 0, st,
-                       n, cells, l->cell_start, ta.side, ta.use_cells, ta.hmax, tiles, l->tile_ranges, l->tile_nr,
-                       gate);
+                       n, cells, l->cell_start, ta.side, ta.use_cells, ta.hmax, tiles, flat_cells, l->tile_ranges,
+                       l->tile_nr, gate);
     hipLaunchKernelGGL(k_list_build_tiled, dim3(tiles), dim3(TILE_THREADS),
                        static_cast<size_t>(TILE_HALO_MAX) * 3 * sizeof(double), st, from_alt ? s->pos_alt : s->pos,
                        n, cells, l->cell_start, ta, s->box, s->inv_box, l->radius2, l->padding, write32 ? 1 : 0,
-                       slot_row_len(l->padding), l->rows, l->rows16, l->count, l->half, l->flags, reinterpret_cast<unsigned long long*>(l->flags + FLAG_COUNT),
+                       slot_row_len(l->padding), l->rows, l->rows16, l->count, l->half, l->build_stats,
                        l->tile_ranges, l->tile_nr, cb, gate);
+    MD_HIP(hipGetLastError());
+    finish_build_enqueue(l, tiles, gate);
     MD_HIP(hipGetLastError());
 }
 
@@ -853,7 +938,7 @@ void tiled_forces_enqueue(md_state* s, md_list* l, const LJConst& lj, double cut
         MD_HIP(hipGetLastError());
     }
     if (ev_stop) MD_HIP(hipEventRecord(ev_stop, st));
-    reduce_partials_enqueue(st, s->partials, n > 0 ? tiles : 0, 3, out_dev);
+    reduce_partials_enqueue(s, n > 0 ? tiles : 0, 3, out_dev);
 }
 
 }  // namespace mdgpu

@@ -377,3 +377,40 @@ def test_config2_size_invariants(ctx):
     assert np.array_equal(s.state()["force"], f)
     assert np.all(np.abs(f.sum(axis=1)) < 1e-9)
     assert r.inside_pairs == s.inside_pairs
+
+
+def test_mixed_forces_large_box_within_tolerance(ctx):
+    """N = 256,000 with every particle displaced (sigma 0.06) from FCC: the box is wide enough (24 cells) that
+    most tiles stage without the periodic image term, and the rest (near the faces, or spanning a cell column
+    end) with it. Mixed forces, energy and virial against the FP64 list path (bit-exact to the reference elsewhere),
+    then a short mixed md_run against the same run in FP64."""
+    s = O.Sim.fcc(40, 0.8442, 0.722, 1)
+    st = s.state()
+    pos = st["pos"].copy()
+    pos[:, :3] = np.mod(pos[:, :3] + np.random.default_rng(5).normal(scale=0.06, size=(len(pos), 3)), s.box)
+    def state():
+        g = mdgpu.State.from_arrays(ctx, s.box, pos, vel=st["vel"], mass=st["mass"], force=st["force"])
+        g.upload(mdgpu.FIELD_ID, st["ids"])
+        return g
+    g = state()
+    nl = mdgpu.NeighbourList(ctx, 2.5, 0.3)
+    nl.build(g)
+    r64 = nl.forces(g, LJ, mdgpu.MD_FP64)
+    f64 = g.download(mdgpu.FIELD_FORCE)
+    rm = nl.forces(g, LJ, mdgpu.MD_MIXED)
+    fm = g.download(mdgpu.FIELD_FORCE)
+    assert np.abs(fm - f64).max() <= 1e-6 * np.abs(f64).max()
+    assert rm.potential == pytest.approx(r64.potential, rel=1e-6)
+    assert rm.virial == pytest.approx(r64.virial, rel=1e-6)
+    assert abs(rm.inside_pairs - r64.inside_pairs) <= 2
+    # 60 steps (a few rebuilds): mixed trajectory stays within 1e-6 of FP64 in energy
+    ga, gb = state(), state()
+    la, lb = mdgpu.NeighbourList(ctx, 2.5, 0.3), mdgpu.NeighbourList(ctx, 2.5, 0.3)
+    la.build(ga)
+    lb.build(gb)
+    la.forces(ga, LJ, mdgpu.MD_FP64)
+    lb.forces(gb, LJ, mdgpu.MD_MIXED)
+    ea = la.run(ga, LJ, 0.005, 60, precision=mdgpu.MD_FP64)
+    eb = lb.run(gb, LJ, 0.005, 60, precision=mdgpu.MD_MIXED)
+    tot_a, tot_b = ea[:, 0] + ea[:, 1], eb[:, 0] + eb[:, 1]
+    assert np.abs(tot_b - tot_a).max() <= 1e-6 * np.abs(tot_a).max()
