@@ -185,14 +185,10 @@ __global__ void __launch_bounds__(256) k_cell_scatter(const int* __restrict__ ce
 // One wavefront per cell: rank = number of members with a smaller index, so the members land in increasing
 // index order — exactly the order of the reference's serial counting sort. Also clears the cell's histogram
 // and fill counters for the next build.
-__global__ void __launch_bounds__(256) k_cell_sort(const uint32_t* __restrict__ start, size_t ncells, size_t n,
-                                                    const uint32_t* __restrict__ order_raw,
-                                                    uint32_t* __restrict__ order, uint32_t* __restrict__ fill,
-                                                    uint32_t* __restrict__ counts, PermuteArgs pa, const int* gate) {
-    if (gated_off(gate)) return;
-    const size_t cell = (static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x) / WAVE;
-    const int lane = threadIdx.x % WAVE;
-    if (cell >= ncells) return;
+__device__ __forceinline__ void sort_cell(size_t cell, int lane, const uint32_t* __restrict__ start, size_t n,
+                                          const uint32_t* __restrict__ order_raw, uint32_t* __restrict__ order,
+                                          uint32_t* __restrict__ fill, uint32_t* __restrict__ counts,
+                                          const PermuteArgs& pa) {
     const uint32_t b = start[cell];
     const uint32_t m = start[cell + 1] - b;
     if (lane == 0) {
@@ -215,16 +211,29 @@ __global__ void __launch_bounds__(256) k_cell_sort(const uint32_t* __restrict__ 
                 const double4 r = pa.pos[v];
                 pa.pos2[k] = r;
                 pa.ref[k] = r;
-                for (int c = 0; c < 3; ++c) {
-                    pa.vel2[c * n + k] = pa.vel[c * n + v];
-                    pa.frc2[c * n + k] = pa.frc[c * n + v];
-                }
+                for (int c = 0; c < 3; ++c) pa.vel2[c * n + k] = pa.vel[c * n + v];
+                if (pa.frc)
+                    for (int c = 0; c < 3; ++c) pa.frc2[c * n + k] = pa.frc[c * n + v];
                 pa.mass2[k] = pa.mass[v];
                 pa.id2[k] = pa.id[v];
                 pa.cell_of2[k] = static_cast<int>(cell);
             }
         }
     }
+}
+
+// Wavefronts stride over the cells (a small grid, so a launch gated off costs little).
+constexpr unsigned SORT_BLOCKS = 2048;
+__global__ void __launch_bounds__(256) k_cell_sort(const uint32_t* __restrict__ start, size_t ncells, size_t n,
+                                                    const uint32_t* __restrict__ order_raw,
+                                                    uint32_t* __restrict__ order, uint32_t* __restrict__ fill,
+                                                    uint32_t* __restrict__ counts, PermuteArgs pa, const int* gate) {
+    if (gated_off(gate)) return;
+    const int lane = threadIdx.x % WAVE;
+    const size_t waves = static_cast<size_t>(gridDim.x) * (blockDim.x / WAVE);
+    for (size_t cell = (static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x) / WAVE; cell < ncells;
+         cell += waves)
+        sort_cell(cell, lane, start, n, order_raw, order, fill, counts, pa);
 }
 
 __device__ __forceinline__ void sort3(int& a, int& b, int& c) {
@@ -248,10 +257,9 @@ __global__ void __launch_bounds__(256) k_list_build(const double4* __restrict__ 
         const double4 ri = pos[i];
         if (cb.enabled) {
             cb.pos[i] = ri;
-            for (int c = 0; c < 3; ++c) {
-                cb.vel[c * n + i] = cb.vel2[c * n + i];
-                cb.frc[c * n + i] = cb.frc2[c * n + i];
-            }
+            for (int c = 0; c < 3; ++c) cb.vel[c * n + i] = cb.vel2[c * n + i];
+            if (cb.frc)
+                for (int c = 0; c < 3; ++c) cb.frc[c * n + i] = cb.frc2[c * n + i];
             cb.mass[i] = cb.mass2[i];
             cb.id[i] = cb.id2[i];
             cb.cell_of[i] = cb.cell_of2[i];
@@ -360,8 +368,10 @@ __global__ void __launch_bounds__(FINISH_THREADS) k_finish_build(const int2* __r
             mx = s_mx[w] > mx ? s_mx[w] : mx;
             hs += s_hs[w];
         }
-        flags[FLAG_MAXCOUNT] = mx;
-        flags[FLAG_OVERFLOW] = mx > padding ? 1 : 0;
+        // longest row and overflow accumulate over the builds since the host last cleared them (md_run checks
+        // once, after all its steps); the pair total is this build's
+        if (mx > flags[FLAG_MAXCOUNT]) flags[FLAG_MAXCOUNT] = mx;
+        if (mx > padding) flags[FLAG_OVERFLOW] = 1;
         *pair_total = hs;
         if (gate) flags[FLAG_REBUILDS] += 1;
     }
@@ -479,8 +489,10 @@ void list_prepare(md_list* l, md_state* s) {
 }
 
 // Histogram, scan, scatter and stable per-cell order of the current positions; with `permute` the state is
-// also gathered into its alt arrays in that order.
-static void cells_enqueue(md_list* l, md_state* s, double per_length, bool permute, const int* gate) {
+// also gathered into its alt arrays in that order (forces only with `carry_forces`: a rebuild that is followed by
+// a force evaluation leaves them behind).
+static void cells_enqueue(md_list* l, md_state* s, double per_length, bool permute, const int* gate,
+                          bool carry_forces) {
     hipStream_t st = l->ctx->stream;
     const size_t n = s->n;
     if (n == 0) return;
@@ -499,42 +511,43 @@ static void cells_enqueue(md_list* l, md_state* s, double per_length, bool permu
     if (permute) {
         pa.pos = s->pos;
         pa.vel = s->vel;
-        pa.frc = s->frc;
+        pa.frc = carry_forces ? s->frc : nullptr;
         pa.mass = s->mass;
         pa.id = s->id;
         pa.pos2 = s->pos_alt;
         pa.vel2 = s->vel_alt;
-        pa.frc2 = s->frc_alt;
+        pa.frc2 = carry_forces ? s->frc_alt : nullptr;
         pa.mass2 = s->mass_alt;
         pa.id2 = s->id_alt;
         pa.ref = l->ref;
         pa.cell_of2 = l->cell_of_alt;
     }
-    k_cell_sort<<<blocks_for(l->ncells * WAVE, 256), 256, 0, st>>>(l->cell_start, l->ncells, n, l->order_raw,
-                                                                   l->order, l->scan_tmp, l->cell_count, pa, gate);
+    const unsigned sort_blocks = std::min<unsigned>(blocks_for(l->ncells * WAVE, 256), SORT_BLOCKS);
+    k_cell_sort<<<sort_blocks, 256, 0, st>>>(l->cell_start, l->ncells, n, l->order_raw, l->order, l->scan_tmp,
+                                             l->cell_count, pa, gate);
     MD_HIP(hipGetLastError());
 }
 
-static void list_kernel_enqueue(md_list* l, md_state* s, bool from_alt, const int* gate) {
+static void list_kernel_enqueue(md_list* l, md_state* s, bool from_alt, const int* gate, bool carry_forces) {
     hipStream_t st = l->ctx->stream;
     const size_t n = s->n;
     CopyBackArgs cb{};
     cb.enabled = from_alt ? 1 : 0;
     if (from_alt) {
         cb.vel2 = s->vel_alt;
-        cb.frc2 = s->frc_alt;
+        cb.frc2 = carry_forces ? s->frc_alt : nullptr;
         cb.mass2 = s->mass_alt;
         cb.id2 = s->id_alt;
         cb.cell_of2 = l->cell_of_alt;
         cb.pos = s->pos;
         cb.vel = s->vel;
-        cb.frc = s->frc;
+        cb.frc = carry_forces ? s->frc : nullptr;
         cb.mass = s->mass;
         cb.id = s->id;
         cb.cell_of = l->cell_of;
     }
     if (tiling_enabled(n)) {
-        tiled_list_enqueue(l, s, from_alt, gate, !l->slots_only);
+        tiled_list_enqueue(l, s, from_alt, gate, !l->slots_only, carry_forces);
         l->tiled = true;
         if (!l->slots_only) l->rows32_valid = true;
         return;
@@ -558,16 +571,23 @@ void ensure_rows32(md_list* l) {
 
 void cells_build(md_list* l, md_state* s, double min_cell, int side) {
     (void)min_cell;
-    cells_enqueue(l, s, side / s->box, false, nullptr);
+    cells_enqueue(l, s, side / s->box, false, nullptr, true);
+}
+
+// Clears the accumulated longest row and overflow flag before a host-driven build.
+static void clear_build_flags(md_list* l) {
+    MD_HIP(hipMemsetAsync(l->flags + FLAG_OVERFLOW, 0, 2 * sizeof(int), l->ctx->stream));
+    static_assert(FLAG_MAXCOUNT == FLAG_OVERFLOW + 1, "the two words are cleared together");
 }
 
 void list_rebuild(md_list* l, md_state* s) {
     if (2.0 * l->radius > s->box) fail(MD_ERR_RUNTIME, "Cutoff + skin must be less than half the box length");
     list_prepare(l, s);
     hipStream_t st = l->ctx->stream;
+    clear_build_flags(l);
     const double per_length = l->side / s->box;
-    cells_enqueue(l, s, per_length, true, nullptr);
-    list_kernel_enqueue(l, s, true, nullptr);
+    cells_enqueue(l, s, per_length, true, nullptr, true);
+    list_kernel_enqueue(l, s, true, nullptr, true);
     int* hf = l->ctx->host_flags;
     for (int attempt = 0;; ++attempt) {
         MD_HIP(hipMemcpyAsync(hf, l->flags, sizeof(int) * FLAG_COUNT, hipMemcpyDeviceToHost, st));
@@ -583,7 +603,8 @@ void list_rebuild(md_list* l, md_state* s) {
         // grow the rows and rebuild the list from the (already re-sorted) state
         l->padding = (l->max_count + l->max_count / 4 + 16 + 7) / 8 * 8;
         list_prepare(l, s);
-        list_kernel_enqueue(l, s, false, nullptr);
+        clear_build_flags(l);
+        list_kernel_enqueue(l, s, false, nullptr, true);
     }
     MD_HIP(hipMemsetAsync(l->flags + FLAG_STALE, 0, sizeof(int), st));
     l->built = true;
@@ -603,8 +624,8 @@ void displacement_check(md_list* l, md_state* s) {
 void list_enqueue_conditional_rebuild(md_list* l, md_state* s) {
     const int* gate = l->flags + FLAG_STALE;
     const double per_length = l->side / s->box;
-    cells_enqueue(l, s, per_length, true, gate);
-    list_kernel_enqueue(l, s, true, gate);
+    cells_enqueue(l, s, per_length, true, gate, false);
+    list_kernel_enqueue(l, s, true, gate, false);
 }
 
 // Fixed-order list (no re-sort): candidates are the members of the 27 cells taken through the stable cell order,
@@ -664,7 +685,7 @@ __global__ void __launch_bounds__(256) k_list_build_fixed(const double4* __restr
 static void list_fixed_enqueue(md_list* l, md_state* s, const int* gate) {
     hipStream_t st = l->ctx->stream;
     const size_t n = s->n;
-    cells_enqueue(l, s, l->side / s->box, false, gate);
+    cells_enqueue(l, s, l->side / s->box, false, gate, true);
     l->tiled = false;
     l->rows32_valid = true;
     k_list_build_fixed<<<blocks_for(n ? n : 1, 256), 256, 0, st>>>(
@@ -680,6 +701,7 @@ void list_rebuild_fixed(md_list* l, md_state* s) {
     hipStream_t st = l->ctx->stream;
     int* hf = l->ctx->host_flags;
     for (int attempt = 0;; ++attempt) {
+        clear_build_flags(l);
         list_fixed_enqueue(l, s, nullptr);
         MD_HIP(hipMemcpyAsync(hf, l->flags, sizeof(int) * (FLAG_COUNT + 2), hipMemcpyDeviceToHost, st));
         MD_HIP(hipStreamSynchronize(st));

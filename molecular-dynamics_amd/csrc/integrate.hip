@@ -264,8 +264,10 @@ int md_run(md_state* s, md_list* l, const md_run_config* cfg, double* energies) 
         rb.draws = dev_alloc<double>(2 * static_cast<size_t>(steps));
         MD_HIP(hipMemcpyAsync(rb.draws, cfg->draws, 2 * sizeof(double) * steps, hipMemcpyHostToDevice, st));
     }
-    // clear the overflow word and the device rebuild counter before the run
-    MD_HIP(hipMemsetAsync(l->flags + FLAG_OVERFLOW, 0, sizeof(int), st));
+    // clear the overflow word, the longest row (both accumulate over the run's builds) and the device rebuild
+    // counter before the run
+    MD_HIP(hipMemsetAsync(l->flags + FLAG_OVERFLOW, 0, 2 * sizeof(int), st));
+    static_assert(FLAG_MAXCOUNT == FLAG_OVERFLOW + 1, "the two words are cleared together");
     MD_HIP(hipMemsetAsync(l->flags + FLAG_REBUILDS, 0, sizeof(int), st));
     MD_HIP(hipMemsetAsync(l->flags + FLAG_STALE, 0, sizeof(int), st));
     const double dof = 3.0 * static_cast<double>(s->n) - 3.0;

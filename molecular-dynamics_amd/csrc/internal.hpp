@@ -193,7 +193,7 @@ constexpr int TILE_PARTICLES = 128;
 constexpr int TILE_RANGES = 64;
 constexpr int TILE_HALO_MAX = 2560;  // LDS capacity bound of a tile's halo, in particles
 size_t tiled_rows16_entries(size_t n, int padding);
-void tiled_list_enqueue(md_list* l, md_state* s, bool from_alt, const int* gate, bool write32);
+void tiled_list_enqueue(md_list* l, md_state* s, bool from_alt, const int* gate, bool write32, bool carry_forces);
 void tiled_rows32_enqueue(md_list* l);                          // int32 rows from the slot rows
 void ensure_rows32(md_list* l);                                 // (cells.hip) materialise if stale
 // mixed-precision forces over the tiled rows

@@ -407,10 +407,9 @@ __global__ void __launch_bounds__(TILE_THREADS)
     if (cb.enabled && static_cast<size_t>(threadIdx.x) < i1 - i0) {  // move the re-sorted arrays into place
         const size_t i = i0 + threadIdx.x;
         cb.pos[i] = pos[i];
-        for (int c = 0; c < 3; ++c) {
-            cb.vel[c * n + i] = cb.vel2[c * n + i];
-            cb.frc[c * n + i] = cb.frc2[c * n + i];
-        }
+        for (int c = 0; c < 3; ++c) cb.vel[c * n + i] = cb.vel2[c * n + i];
+        if (cb.frc)
+            for (int c = 0; c < 3; ++c) cb.frc[c * n + i] = cb.frc2[c * n + i];
         cb.mass[i] = cb.mass2[i];
         cb.id[i] = cb.id2[i];
         cb.cell_of[i] = cb.cell_of2[i];
@@ -511,7 +510,8 @@ __global__ void __launch_bounds__(TILE_THREADS)
         if (L == WAVE - 1) s_lists_ok = incl <= LIST_CAP ? 1 : 0;
     }
     __syncthreads();
-    const bool lists_ok = use_lists && s_lists_ok;
+    // the walk keeps rows in its LDS row buffer; longer rows take the 4-lane scan, which stores directly
+    const bool lists_ok = use_lists && s_lists_ok && padding <= ROWBUF_CAP;
     // fill: each lane copies its segment's slots; the list's lanes then pad it to a multiple of 64 with the ghost
 #pragma unroll
     for (int u = 0; u < LIST_ROUNDS; ++u) {
@@ -537,7 +537,6 @@ __global__ void __launch_bounds__(TILE_THREADS)
             uint16_t* const row16 = rows16 + slot_row_base(i, row_len);
             int32_t* const row = rows + i;
             uint16_t* const rb = sm.rowbuf[wave];
-            const bool buffered = padding <= ROWBUF_CAP;
             int cnt = 0;
             // one batch of 64 candidates: test, rank the accepted ones over the ballot, store, advance the count
             auto batch = [&](int q, bool upper, auto interior_tag) {
@@ -557,10 +556,7 @@ __global__ void __launch_bounds__(TILE_THREADS)
                         static_cast<unsigned>(mask >> 32), __builtin_amdgcn_mbcnt_lo(static_cast<unsigned>(mask), 0u));
                     const unsigned at = static_cast<unsigned>(cnt) + rank;
                     if (at < static_cast<unsigned>(padding)) {
-                        if (buffered)
-                            rb[at] = static_cast<uint16_t>(sl);
-                        else
-                            row16[slot_entry(at)] = static_cast<uint16_t>(sl);
+                        rb[at] = static_cast<uint16_t>(sl);
                         if (write32) row[static_cast<uint64_t>(at) * n] = halo_particle(sl, hr, nr);
                     }
                 }
@@ -581,7 +577,7 @@ __global__ void __launch_bounds__(TILE_THREADS)
                 return h;
             };
             const int hcnt = interior ? both(std::true_type{}) : both(std::false_type{});
-            if (buffered) {
+            {
                 // the row leaves LDS as 16-byte chunk words: lane 4 c + w packs entries 32 c + w + 4 t, t = 0..7
                 // (the wavefront's LDS accesses complete in issue order, so every lane's entries are in place)
                 __builtin_amdgcn_wave_barrier();
@@ -877,7 +873,7 @@ size_t tiled_rows16_entries(size_t n, int padding) {
     return static_cast<size_t>(tile_count(n ? n : 1)) * TILE * slot_row_len(padding);
 }
 
-void tiled_list_enqueue(md_list* l, md_state* s, bool from_alt, const int* gate, bool write32) {
+void tiled_list_enqueue(md_list* l, md_state* s, bool from_alt, const int* gate, bool write32, bool carry_forces) {
     hipStream_t st = l->ctx->stream;
     const size_t n = s->n;
     if (n == 0) {
@@ -889,13 +885,13 @@ void tiled_list_enqueue(md_list* l, md_state* s, bool from_alt, const int* gate,
     cb.enabled = from_alt ? 1 : 0;
     if (from_alt) {
         cb.vel2 = s->vel_alt;
-        cb.frc2 = s->frc_alt;
+        cb.frc2 = carry_forces ? s->frc_alt : nullptr;
         cb.mass2 = s->mass_alt;
         cb.id2 = s->id_alt;
         cb.cell_of2 = l->cell_of_alt;
         cb.pos = s->pos;
         cb.vel = s->vel;
-        cb.frc = s->frc;
+        cb.frc = carry_forces ? s->frc : nullptr;
         cb.mass = s->mass;
         cb.id = s->id;
         cb.cell_of = l->cell_of;
