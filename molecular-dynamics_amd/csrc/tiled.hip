@@ -59,6 +59,7 @@ __device__ __forceinline__ int tile_range_count(int raw) { return raw < 0 ? -1 :
 __device__ __forceinline__ bool tile_is_flat(int raw) { return raw > 0 && (raw & TILE_FLAT) != 0; }
 static_assert(RMAX == WAVE, "one lane per halo range in k_tile_setup");
 static_assert(CHUNK_STRIDE == 4096, "slot_entry shifts the chunk index by 7 + 5 bits");
+static_assert(TILE == 2 * WAVE, "the force kernel's energy sum: one wavefront adds two particles per lane");
 
 // Three cell coordinates in increasing order, indexed without a private array (which would live in scratch).
 struct Sorted3 {
@@ -732,6 +733,7 @@ __global__ void __launch_bounds__(TILE_THREADS)
     double* zs = ldsd + 2 * TILE_HALO_MAX;
     __shared__ int ra[RMAX], rgs[RMAX], rofs[RMAX + 1];
     __shared__ int extent_bits;
+    __shared__ double e_part[3][TILE];  // per particle: U, W and inside count of its j > i row part
     const HaloRanges hr{ra, rgs, rofs};
     const int tile = xcd_item(static_cast<int>(blockIdx.x), fr.tiles);
     const int raw_nr = tile_nr[tile];
@@ -793,7 +795,7 @@ __global__ void __launch_bounds__(TILE_THREADS)
     __syncthreads();
     const float e = __int_as_float(extent_bits);
     const bool compact = staged && (2.0f * e + static_cast<float>(fr.reach) < static_cast<float>(fr.box) - 1.0f);
-    PairSums a;
+    PairSumsF a;
     if (active) {
         if (staged) {
             auto run = [&](auto wrap) {
@@ -811,7 +813,7 @@ __global__ void __launch_bounds__(TILE_THREADS)
                                 dy = fma(-fr.box, rint(dy * fr.inv_box), dy);
                                 dz = fma(-fr.box, rint(dz * fr.inv_box), dz);
                             }
-                            mixed_pair_term(dx, dy, dz, lj, k < h, a);
+                            mixed_pair_term_f(dx, dy, dz, lj, k < h, a);
                         }
                     }
                 };
@@ -826,24 +828,41 @@ __global__ void __launch_bounds__(TILE_THREADS)
         } else {
             const double4 ri = pos[i];
             for (int k = lane; k < m; k += LANES)
-                mixed_pair_gather(ri, pos[rows[static_cast<size_t>(k) * n + i]], fr.box, fr.inv_box, lj, k < h, a);
+                mixed_pair_gather_f(ri, pos[rows[static_cast<size_t>(k) * n + i]], fr.box, fr.inv_box, lj, k < h, a);
         }
     }
     // the particle's four lanes are adjacent in the wavefront
-    a.fx += __shfl_xor(a.fx, 1, WAVE);
-    a.fy += __shfl_xor(a.fy, 1, WAVE);
-    a.fz += __shfl_xor(a.fz, 1, WAVE);
-    a.fx += __shfl_xor(a.fx, 2, WAVE);
-    a.fy += __shfl_xor(a.fy, 2, WAVE);
-    a.fz += __shfl_xor(a.fz, 2, WAVE);
-    if (active && lane == 0) {
-        frc[i] = a.fx;
-        frc[n + i] = a.fy;
-        frc[2 * n + i] = a.fz;
+#pragma unroll
+    for (int off = 1; off < LANES; off <<= 1) {
+        a.fx += __shfl_xor(a.fx, off, WAVE);
+        a.fy += __shfl_xor(a.fy, off, WAVE);
+        a.fz += __shfl_xor(a.fz, off, WAVE);
+        a.u += __shfl_xor(a.u, off, WAVE);
+        a.w += __shfl_xor(a.w, off, WAVE);
+        a.c += __shfl_xor(a.c, off, WAVE);
+    }
+    const int pl = threadIdx.x >> 2;
+    if (lane == 0) {
+        if (active) {
+            frc[i] = a.fx;
+            frc[n + i] = a.fy;
+            frc[2 * n + i] = a.fz;
+        }
+        e_part[0][pl] = static_cast<double>(a.u);  // zero for the slots past n
+        e_part[1][pl] = a.w;
+        e_part[2][pl] = static_cast<double>(a.c);
     }
     if (clear_flag && blockIdx.x == 0 && threadIdx.x == 0) *clear_flag = 0;
-    double acc[3] = {a.u, a.w, a.c};
-    block_sum<3, TILE_THREADS>(acc, partials + static_cast<size_t>(tile) * 3);
+    __syncthreads();
+    // the tile's U, W and inside count: the 128 particle sums added in double by one wavefront, in a fixed order
+    if (threadIdx.x < WAVE) {
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            double v = e_part[q][threadIdx.x] + e_part[q][threadIdx.x + WAVE];
+            v = wave_sum(v);
+            if (threadIdx.x == 0) partials[static_cast<size_t>(tile) * 3 + q] = v;
+        }
+    }
 }
 
 bool tiling_enabled(size_t n) {
