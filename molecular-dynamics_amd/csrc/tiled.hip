@@ -363,16 +363,12 @@ struct LaneScan {
 // accepted ones (j > i from U_c, j < i from L_c, slot order being index order) are ranked by mbcnt over the
 // ballot, so the row comes out in the reference's order with no divergence between lanes. Other tiles use the
 // 4-lanes-per-particle scan with a binary search per stencil cell (LaneScan).
-__global__ void __launch_bounds__(TILE_THREADS)
-    k_list_build_tiled(const double4* __restrict__ pos, size_t n, const int* __restrict__ cell_of,
-                       const uint32_t* __restrict__ start, TileArgs ta, double box, double inv_box, double radius2,
-                       int padding, int write32, int row_len, int32_t* __restrict__ rows,
-                       uint16_t* __restrict__ rows16,
-                       int32_t* __restrict__ count, int32_t* __restrict__ half, int2* __restrict__ stats,
-                       const int4* __restrict__ tile_ranges,
-                       const int* __restrict__ tile_nr, TiledCopyBack cb, const int* gate) {
-    if (gate && *gate == 0) return;
-    extern __shared__ double lds[];
+__device__ __forceinline__ void list_build_tile(
+    int tile, double* lds, const double4* __restrict__ pos, size_t n, const int* __restrict__ cell_of,
+    const uint32_t* __restrict__ start, const TileArgs& ta, double box, double inv_box, double radius2, int padding,
+    int write32, int row_len, int32_t* __restrict__ rows, uint16_t* __restrict__ rows16, int32_t* __restrict__ count,
+    int32_t* __restrict__ half, int2* __restrict__ stats, const int4* __restrict__ tile_ranges,
+    const int* __restrict__ tile_nr, const TiledCopyBack& cb) {
     double* xs = lds;  // fixed strides: the three reads of a slot share one address plus immediate offsets
     double* ys = lds + TILE_HALO_MAX;
     double* zs = lds + 2 * TILE_HALO_MAX;
@@ -391,7 +387,6 @@ __global__ void __launch_bounds__(TILE_THREADS)
     __shared__ unsigned char cl_interior[CMAX];
     __shared__ int s_outside, s_lists_ok;
     const HaloRanges hr{ra, rgs, rofs};
-    const int tile = xcd_item(static_cast<int>(blockIdx.x), ta.tiles);
     const size_t i0 = static_cast<size_t>(tile) * TILE;
     const size_t i1 = (i0 + TILE) < n ? (i0 + TILE) : n;
     const int s = ta.side;
@@ -531,31 +526,32 @@ __global__ void __launch_bounds__(TILE_THREADS)
         const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
         for (size_t p = wave; p < i1 - i0; p += TILE_THREADS / WAVE) {
             const size_t i = i0 + p;
-            const int cl = cell_of[i] - c_first;
-            const int slot_i = cl_slot[cl] + (static_cast<int>(i) - cl_first[cl]);
+            // uniform over the wavefront: one particle per wavefront
+            const int cl = __builtin_amdgcn_readfirstlane(cell_of[i] - c_first);
+            const int slot_i = __builtin_amdgcn_readfirstlane(cl_slot[cl] + (static_cast<int>(i) - cl_first[cl]));
             const double xi = xs[slot_i], yi = ys[slot_i], zi = zs[slot_i];
             const bool interior = cl_interior[cl] && !s_outside;
             uint16_t* const row16 = rows16 + slot_row_base(i, row_len);
             int32_t* const row = rows + i;
             uint16_t* const rb = sm.rowbuf[wave];
             int cnt = 0;
-            // one batch of 64 candidates: test, rank the accepted ones over the ballot, store, advance the count
-            auto batch = [&](int q, bool upper, auto interior_tag) {
-                const int sl = lists[q];
+            // A batch is 64 candidates: each lane tests one, the accepted ones are ranked over the ballot (mbcnt
+            // counts the set bits below the lane, on top of the row's count so far) and stored. Two batches are
+            // loaded and tested before either stores, so their LDS reads overlap.
+            auto near = [&](int sl, auto interior_tag) {
                 double dx = xi - xs[sl], dy = yi - ys[sl], dz = zi - zs[sl];
                 if constexpr (!decltype(interior_tag)::value) {
                     dx = min_image(dx, box, inv_box);
                     dy = min_image(dy, box, inv_box);
                     dz = min_image(dz, box, inv_box);
                 }
-                const bool near = dx * dx + dy * dy + dz * dz < radius2;  // false for the ghost (NaN)
-                const bool order = upper ? sl > slot_i : sl < slot_i;
-                const bool accept = order & near;
-                const unsigned long long mask = __ballot(accept);
+                return dx * dx + dy * dy + dz * dz < radius2;  // false for the ghost (NaN)
+            };
+            auto commit = [&](int sl, bool accept, unsigned long long mask) {
                 if (accept) {
-                    const unsigned rank = __builtin_amdgcn_mbcnt_hi(
-                        static_cast<unsigned>(mask >> 32), __builtin_amdgcn_mbcnt_lo(static_cast<unsigned>(mask), 0u));
-                    const unsigned at = static_cast<unsigned>(cnt) + rank;
+                    const unsigned at = __builtin_amdgcn_mbcnt_hi(
+                        static_cast<unsigned>(mask >> 32),
+                        __builtin_amdgcn_mbcnt_lo(static_cast<unsigned>(mask), static_cast<unsigned>(cnt)));
                     if (at < static_cast<unsigned>(padding)) {
                         rb[at] = static_cast<uint16_t>(sl);
                         if (write32) row[static_cast<uint64_t>(at) * n] = halo_particle(sl, hr, nr);
@@ -565,16 +561,27 @@ __global__ void __launch_bounds__(TILE_THREADS)
             };
             auto walk = [&](int off, int len, bool upper, auto interior_tag) {
                 int q0 = 0;
-                for (; q0 + 2 * WAVE <= len; q0 += 2 * WAVE) {  // two batches in flight
-                    batch(off + q0 + lane, upper, interior_tag);
-                    batch(off + q0 + WAVE + lane, upper, interior_tag);
+                for (; q0 + 2 * WAVE <= len; q0 += 2 * WAVE) {
+                    const int sa = lists[off + q0 + lane], sb = lists[off + q0 + WAVE + lane];
+                    const bool na = near(sa, interior_tag), nb = near(sb, interior_tag);
+                    const bool oa = upper ? sa > slot_i : sa < slot_i, ob = upper ? sb > slot_i : sb < slot_i;
+                    // ballots of the two compares separately: each is the compare's own lane mask
+                    const unsigned long long ma = __ballot(na) & __ballot(oa), mb = __ballot(nb) & __ballot(ob);
+                    commit(sa, na && oa, ma);
+                    commit(sb, nb && ob, mb);
                 }
-                if (q0 < len) batch(off + q0 + lane, upper, interior_tag);
+                if (q0 < len) {
+                    const int sa = lists[off + q0 + lane];
+                    const bool na = near(sa, interior_tag), oa = upper ? sa > slot_i : sa < slot_i;
+                    commit(sa, na && oa, __ballot(na) & __ballot(oa));
+                }
             };
             auto both = [&](auto interior_tag) {
-                walk(lst[cl][0], lst[cl][1], true, interior_tag);
+                walk(__builtin_amdgcn_readfirstlane(lst[cl][0]), __builtin_amdgcn_readfirstlane(lst[cl][1]), true,
+                     interior_tag);
                 const int h = cnt;
-                walk(lst[cl][2], lst[cl][3], false, interior_tag);
+                walk(__builtin_amdgcn_readfirstlane(lst[cl][2]), __builtin_amdgcn_readfirstlane(lst[cl][3]), false,
+                     interior_tag);
                 return h;
             };
             const int hcnt = interior ? both(std::true_type{}) : both(std::false_type{});
@@ -679,6 +686,19 @@ __global__ void __launch_bounds__(TILE_THREADS)
         }
     }
     block_build_stat<TILE_THREADS>(ep_max, ep_half, stats + tile);
+}
+
+__global__ void __launch_bounds__(TILE_THREADS)
+    k_list_build_tiled(const double4* __restrict__ pos, size_t n, const int* __restrict__ cell_of,
+                       const uint32_t* __restrict__ start, TileArgs ta, double box, double inv_box, double radius2,
+                       int padding, int write32, int row_len, int32_t* __restrict__ rows,
+                       uint16_t* __restrict__ rows16, int32_t* __restrict__ count, int32_t* __restrict__ half,
+                       int2* __restrict__ stats, const int4* __restrict__ tile_ranges,
+                       const int* __restrict__ tile_nr, TiledCopyBack cb, const int* gate) {
+    if (gate && *gate == 0) return;
+    extern __shared__ double lds[];
+    list_build_tile(xcd_item(static_cast<int>(blockIdx.x), ta.tiles), lds, pos, n, cell_of, start, ta, box, inv_box,
+                    radius2, padding, write32, row_len, rows, rows16, count, half, stats, tile_ranges, tile_nr, cb);
 }
 
 // int32 rows from the 16-bit slot rows (builds that wrote only the slots): slot -> particle through the tile's
