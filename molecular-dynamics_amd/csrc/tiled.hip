@@ -385,6 +385,8 @@ __device__ __forceinline__ void list_build_tile(
     __shared__ int cl_first[CMAX];
     __shared__ int cl_slot[CMAX];  // LDS slot of the own cell's first particle
     __shared__ unsigned char cl_interior[CMAX];
+    __shared__ unsigned char p_cl[TILE];  // own cell of each of the tile's particles, counted from the first
+    __shared__ uint16_t p_slot[TILE];     // LDS slot of each of the tile's particles
     __shared__ int s_outside, s_lists_ok;
     const HaloRanges hr{ra, rgs, rofs};
     const size_t i0 = static_cast<size_t>(tile) * TILE;
@@ -396,6 +398,8 @@ __device__ __forceinline__ void list_build_tile(
     const int ncl = ta.use_cells ? cell_of[i1 - 1] - c_first + 1 : 0;
     const bool use_lists = staged && ta.use_cells && ncl <= CMAX;
     const uint32_t n32 = static_cast<uint32_t>(n);
+    // the cells of the tile's own particles, loaded together with the header so the walk reads them from LDS
+    const int my_cell = ta.use_cells && static_cast<size_t>(threadIdx.x) < i1 - i0 ? cell_of[i0 + threadIdx.x] : 0;
     if (threadIdx.x == 0) {
         s_outside = 0;
         s_lists_ok = 0;
@@ -440,6 +444,10 @@ __device__ __forceinline__ void list_build_tile(
         g.base = static_cast<uint16_t>(slot_base_of_cell(nc, hr, start, nr));
         g.len = static_cast<uint16_t>(start[nc + 1] - start[nc]);
         seg[t] = g;
+        if (st == OWN_STENCIL) {
+            cl_first[cl] = static_cast<int>(start[nc]);
+            cl_slot[cl] = g.base;
+        }
     }
     __syncthreads();
     // Candidate lists: list L = 2 cl + lower (U_c, then L_c). Half-wavefront h takes lists h, h + 16, h + 32; its
@@ -484,10 +492,13 @@ __device__ __forceinline__ void list_build_tile(
         list_used[u] = __shfl(incl, HALF - 1, HALF);
         if (hl == 0) lst[cl][lower ? 3 : 1] = (list_used[u] + WAVE - 1) / WAVE * WAVE;
     }
+    if (use_lists && static_cast<size_t>(threadIdx.x) < i1 - i0) {
+        const int cl = my_cell - c_first;
+        p_cl[threadIdx.x] = static_cast<unsigned char>(cl);
+        p_slot[threadIdx.x] = static_cast<uint16_t>(cl_slot[cl] + (static_cast<int>(i0 + threadIdx.x) - cl_first[cl]));
+    }
     if (use_lists && static_cast<int>(threadIdx.x) < ncl) {
         const int cl = threadIdx.x, c = c_first + cl;
-        cl_first[cl] = static_cast<int>(start[c]);
-        cl_slot[cl] = seg[cl * 27 + OWN_STENCIL].base;
         const int cx = c / (s * s), cy = (c / s) % s, cz = c % s;
         cl_interior[cl] = ta.interior_ok && cx >= 1 && cx <= s - 2 && cy >= 1 && cy <= s - 2 && cz >= 1 &&
                           cz <= s - 2;
@@ -527,8 +538,8 @@ __device__ __forceinline__ void list_build_tile(
         for (size_t p = wave; p < i1 - i0; p += TILE_THREADS / WAVE) {
             const size_t i = i0 + p;
             // uniform over the wavefront: one particle per wavefront
-            const int cl = __builtin_amdgcn_readfirstlane(cell_of[i] - c_first);
-            const int slot_i = __builtin_amdgcn_readfirstlane(cl_slot[cl] + (static_cast<int>(i) - cl_first[cl]));
+            const int cl = __builtin_amdgcn_readfirstlane(p_cl[p]);
+            const int slot_i = __builtin_amdgcn_readfirstlane(p_slot[p]);
             const double xi = xs[slot_i], yi = ys[slot_i], zi = zs[slot_i];
             const bool interior = cl_interior[cl] && !s_outside;
             uint16_t* const row16 = rows16 + slot_row_base(i, row_len);
