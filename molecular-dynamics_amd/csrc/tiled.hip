@@ -404,15 +404,30 @@ __device__ __forceinline__ void list_build_tile(
         s_outside = 0;
         s_lists_ok = 0;
     }
-    if (cb.enabled && static_cast<size_t>(threadIdx.x) < i1 - i0) {  // move the re-sorted arrays into place
-        const size_t i = i0 + threadIdx.x;
-        cb.pos[i] = pos[i];
-        for (int c = 0; c < 3; ++c) cb.vel[c * n + i] = cb.vel2[c * n + i];
-        if (cb.frc)
-            for (int c = 0; c < 3; ++c) cb.frc[c * n + i] = cb.frc2[c * n + i];
-        cb.mass[i] = cb.mass2[i];
-        cb.id[i] = cb.id2[i];
-        cb.cell_of[i] = cb.cell_of2[i];
+    // Moving the re-sorted arrays into place: each quarter of the block carries one group of fields of the tile's
+    // particles (positions; velocities; forces; mass, id and cell). The loads leave with the tile header; the stores
+    // wait until the tile is done (no thread of this kernel reads the destination arrays).
+    static_assert(TILE_THREADS == 4 * TILE, "one copy-back group per quarter of the block");
+    const int cb_group = threadIdx.x / TILE;
+    const size_t cb_i = i0 + threadIdx.x % TILE;
+    const bool cb_live = cb.enabled && cb_i < i1 && (cb_group != 2 || cb.frc != nullptr);
+    uint64_t cbv[4] = {0, 0, 0, 0};
+    if (cb_live) {
+        if (cb_group == 0) {
+            const double4 q = pos[cb_i];
+            cbv[0] = __double_as_longlong(q.x);
+            cbv[1] = __double_as_longlong(q.y);
+            cbv[2] = __double_as_longlong(q.z);
+            cbv[3] = __double_as_longlong(q.w);
+        } else if (cb_group == 1) {
+            for (int c = 0; c < 3; ++c) cbv[c] = __double_as_longlong(cb.vel2[c * n + cb_i]);
+        } else if (cb_group == 2) {
+            for (int c = 0; c < 3; ++c) cbv[c] = __double_as_longlong(cb.frc2[c * n + cb_i]);
+        } else {
+            cbv[0] = __double_as_longlong(cb.mass2[cb_i]);
+            cbv[1] = cb.id2[cb_i];
+            cbv[2] = static_cast<uint32_t>(cb.cell_of2[cb_i]);
+        }
     }
     if (staged) load_ranges(tile_ranges, tile, nr, hr);
     __syncthreads();
@@ -694,6 +709,20 @@ __device__ __forceinline__ void list_build_tile(
                 ep_half = static_cast<unsigned>(hcnt);
             }
             ep_max = sc.cnt;
+        }
+    }
+    if (cb_live) {
+        if (cb_group == 0) {
+            cb.pos[cb_i] = make_double4(__longlong_as_double(cbv[0]), __longlong_as_double(cbv[1]),
+                                        __longlong_as_double(cbv[2]), __longlong_as_double(cbv[3]));
+        } else if (cb_group == 1) {
+            for (int c = 0; c < 3; ++c) cb.vel[c * n + cb_i] = __longlong_as_double(cbv[c]);
+        } else if (cb_group == 2) {
+            for (int c = 0; c < 3; ++c) cb.frc[c * n + cb_i] = __longlong_as_double(cbv[c]);
+        } else {
+            cb.mass[cb_i] = __longlong_as_double(cbv[0]);
+            cb.id[cb_i] = cbv[1];
+            cb.cell_of[cb_i] = static_cast<int>(static_cast<uint32_t>(cbv[2]));
         }
     }
     block_build_stat<TILE_THREADS>(ep_max, ep_half, stats + tile);
