@@ -210,9 +210,8 @@ __device__ __forceinline__ int halo_particle(int q, HaloRanges h, int nr) {
     return h.rgs[lo] + (q - h.rofs[lo]);
 }
 
-// LDS slot of the first particle of cell `nc` (a cell inside the halo).
-__device__ __forceinline__ int slot_base_of_cell(int nc, HaloRanges h, const uint32_t* __restrict__ start,
-                                                 int nr) {
+// LDS slot of the first particle of cell `nc` (a cell inside the halo), given that particle's index `first`.
+__device__ __forceinline__ int slot_base_at(int nc, HaloRanges h, int first, int nr) {
     int lo = 0, hi = nr - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
@@ -221,7 +220,12 @@ __device__ __forceinline__ int slot_base_of_cell(int nc, HaloRanges h, const uin
         else
             hi = mid - 1;
     }
-    return h.rofs[lo] + static_cast<int>(start[nc]) - h.rgs[lo];
+    return h.rofs[lo] + first - h.rgs[lo];
+}
+
+__device__ __forceinline__ int slot_base_of_cell(int nc, HaloRanges h, const uint32_t* __restrict__ start,
+                                                 int nr) {
+    return slot_base_at(nc, h, static_cast<int>(start[nc]), nr);
 }
 
 // Stages the halo: thread q loads slots q, q + 512, ... (all loads issued before any is used), and hands each
@@ -398,6 +402,9 @@ __device__ __forceinline__ void list_build_tile(
     const int ncl = ta.use_cells ? cell_of[i1 - 1] - c_first + 1 : 0;
     const bool use_lists = staged && ta.use_cells && ncl <= CMAX;
     const uint32_t n32 = static_cast<uint32_t>(n);
+    // the halo ranges, loaded together with the header (every tile owns RMAX entries; those past nr are unused)
+    const int4 rr = static_cast<int>(threadIdx.x) < RMAX ? tile_ranges[static_cast<size_t>(tile) * RMAX + threadIdx.x]
+                                                         : make_int4(0, 0, 0, 0);
     // the cells of the tile's own particles, loaded together with the header so the walk reads them from LDS
     const int my_cell = ta.use_cells && static_cast<size_t>(threadIdx.x) < i1 - i0 ? cell_of[i0 + threadIdx.x] : 0;
     if (threadIdx.x == 0) {
@@ -429,7 +436,33 @@ __device__ __forceinline__ void list_build_tile(
             cbv[2] = static_cast<uint32_t>(cb.cell_of2[cb_i]);
         }
     }
-    if (staged) load_ranges(tile_ranges, tile, nr, hr);
+    if (staged && static_cast<int>(threadIdx.x) < nr) {
+        ra[threadIdx.x] = rr.x;
+        rgs[threadIdx.x] = rr.y;
+        rofs[threadIdx.x] = rr.z;
+        if (static_cast<int>(threadIdx.x) == nr - 1) rofs[nr] = rr.z + rr.w;
+    }
+    // the stencil cells of every own cell and their particle spans: loaded now, so that the loads are in flight
+    // while the halo is staged
+    constexpr int SEG_ROUNDS = (CMAX * 27 + TILE_THREADS - 1) / TILE_THREADS;
+    int seg_nc[SEG_ROUNDS];
+    uint32_t seg_b[SEG_ROUNDS], seg_e[SEG_ROUNDS];
+#pragma unroll
+    for (int u = 0; u < SEG_ROUNDS; ++u) {
+        const int t = threadIdx.x + u * TILE_THREADS;
+        seg_nc[u] = 0;
+        seg_b[u] = seg_e[u] = 0;
+        if (use_lists && t < ncl * 27) {
+            const int cl = t / 27, st = t % 27;
+            const int c = c_first + cl;
+            const int cx = c / (s * s), cy = (c / s) % s, cz = c % s;
+            const int nc = (wrap_index(cx + st / 9 - 1, s) * s + wrap_index(cy + (st / 3) % 3 - 1, s)) * s +
+                           wrap_index(cz + st % 3 - 1, s);
+            seg_nc[u] = nc;
+            seg_b[u] = start[nc];
+            seg_e[u] = start[nc + 1];
+        }
+    }
     __syncthreads();
     if (staged) {
         int outside = 0;
@@ -448,19 +481,18 @@ __device__ __forceinline__ void list_build_tile(
         }
     }
     // stencil segments of every own cell
-    for (int t = threadIdx.x; use_lists && t < ncl * 27; t += TILE_THREADS) {
+#pragma unroll
+    for (int u = 0; u < SEG_ROUNDS; ++u) {
+        const int t = threadIdx.x + u * TILE_THREADS;
+        if (!use_lists || t >= ncl * 27) continue;
         const int cl = t / 27, st = t % 27;
-        const int c = c_first + cl;
-        const int cx = c / (s * s), cy = (c / s) % s, cz = c % s;
-        const int nc = (wrap_index(cx + st / 9 - 1, s) * s + wrap_index(cy + (st / 3) % 3 - 1, s)) * s +
-                       wrap_index(cz + st % 3 - 1, s);
         Segment g;
-        g.nc = nc;
-        g.base = static_cast<uint16_t>(slot_base_of_cell(nc, hr, start, nr));
-        g.len = static_cast<uint16_t>(start[nc + 1] - start[nc]);
+        g.nc = seg_nc[u];
+        g.base = static_cast<uint16_t>(slot_base_at(seg_nc[u], hr, static_cast<int>(seg_b[u]), nr));
+        g.len = static_cast<uint16_t>(seg_e[u] - seg_b[u]);
         seg[t] = g;
         if (st == OWN_STENCIL) {
-            cl_first[cl] = static_cast<int>(start[nc]);
+            cl_first[cl] = static_cast<int>(seg_b[u]);
             cl_slot[cl] = g.base;
         }
     }
