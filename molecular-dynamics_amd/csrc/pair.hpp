@@ -74,36 +74,35 @@ __device__ __forceinline__ void mixed_pair_term(double dx, double dy, double dz,
     }
 }
 
-// Per-lane sums of the tiled mixed kernel: force and virial in double, the potential energy and the inside count
-// of the row's j > i part in float — for those two the reference's own precision (Forcefield.cpp:397-401:
-// energy_i and inside_i are float sums over the half row, added in double across particles). The virial, which
-// cancels strongly, stays in double (a double FMA costs the same issue slot as a float one).
+// Per-lane sums of the tiled mixed kernel over a particle's full row: force and virial in double, the potential
+// energy without its shift in float and the inside count as an integer. The reference sums energy_i and inside_i in
+// float over the half row j > i (Forcefield.cpp:397-401) and adds them in double across particles; here every pair
+// is met from both of its ends, and the caller takes the shift once per inside pair (U = sum 4 (s12 - s6) -
+// shift * count) and halves the tile's U, W and count: the same sums, with the float rounding taken in another
+// order. The virial, which cancels strongly, stays in double.
 struct PairSumsF {
     double fx = 0.0, fy = 0.0, fz = 0.0, w = 0.0;
-    float u = 0.0f, c = 0.0f;
+    float u = 0.0f;
+    int c = 0;
 };
 
-// mixed_pair_term with U and the inside count summed in float.
-__device__ __forceinline__ void mixed_pair_term_f(double dx, double dy, double dz, const LJConst& lj,
-                                                  bool count_energy, PairSumsF& a) {
+// One pair of a full row. Outside rc inv_r2 is zero, which makes the force and the energy term zero.
+__device__ __forceinline__ void mixed_pair_term_f(double dx, double dy, double dz, const LJConst& lj, PairSumsF& a) {
     const double r2d = fma(dx, dx, fma(dy, dy, dz * dz));
     const float r2 = static_cast<float>(r2d);
     const float q = __builtin_amdgcn_rcpf(r2);
-    const float inv_r2 = fmaf(q, fmaf(-r2, q, 1.0f), q);
+    const bool inside = r2 < lj.rc2f;
+    const float inv_r2 = inside ? fmaf(q, fmaf(-r2, q, 1.0f), q) : 0.0f;
     const float s2 = lj.sig2f * inv_r2;
     const float s6 = s2 * s2 * s2;
     const float s12 = s6 * s6;
-    const bool inside = r2 < lj.rc2f;
-    const float ff = inside ? lj.c24f * fmaf(2.0f, s12, -s6) * inv_r2 : 0.0f;
-    const double f = static_cast<double>(ff);
+    const double f = static_cast<double>(lj.c24f * fmaf(2.0f, s12, -s6) * inv_r2);
     a.fx = fma(f, dx, a.fx);
     a.fy = fma(f, dy, a.fy);
     a.fz = fma(f, dz, a.fz);
-    if (count_energy) {
-        a.u += inside ? fmaf(lj.c4f, s12 - s6, -lj.shiftf) : 0.0f;
-        a.w = fma(f, r2d, a.w);
-        a.c += inside ? 1.0f : 0.0f;
-    }
+    a.u = fmaf(lj.c4f, s12 - s6, a.u);
+    a.w = fma(f, r2d, a.w);
+    a.c += inside ? 1 : 0;
 }
 
 // Mixed term of the gather kernels: double minimum image, then mixed_pair_term.
@@ -117,12 +116,12 @@ __device__ __forceinline__ void mixed_pair_gather(const double4& ri, const doubl
 }
 
 __device__ __forceinline__ void mixed_pair_gather_f(const double4& ri, const double4& rj, double box, double inv_box,
-                                                    const LJConst& lj, bool count_energy, PairSumsF& a) {
+                                                    const LJConst& lj, PairSumsF& a) {
     double ex = ri.x - rj.x, ey = ri.y - rj.y, ez = ri.z - rj.z;
     ex = fma(-box, rint(ex * inv_box), ex);
     ey = fma(-box, rint(ey * inv_box), ey);
     ez = fma(-box, rint(ez * inv_box), ez);
-    mixed_pair_term_f(ex, ey, ez, lj, count_energy, a);
+    mixed_pair_term_f(ex, ey, ez, lj, a);
 }
 
 }  // namespace mdgpu

@@ -40,6 +40,11 @@ constexpr int CHUNK = 32;    // slot-row entries per particle and chunk
 constexpr int CHUNK_STRIDE = TILE * CHUNK;  // entries between a tile's chunks
 constexpr int GHOST_SLOT = TILE_HALO_MAX - 1;  // LDS slot holding NaN coordinates: never a neighbour
 constexpr int PRE_CHUNKS = 4;  // slot chunks the force kernel loads per lane ahead of its pair loop
+// The slot rows and the list build's candidate lists hold byte offsets of LDS slots (slot * 8, below 2^16), so that
+// the force kernel adds an entry to a fixed halo address folded into its LDS loads.
+constexpr int SLOT_BYTES = 8;
+static_assert(TILE_HALO_MAX * SLOT_BYTES <= 0x10000, "slot byte offsets fit 16 bits");
+
 
 // Position of a particle's first slot-row entry, and of entry k relative to it.
 __device__ __forceinline__ size_t slot_row_base(size_t i, int row_len) {
@@ -353,7 +358,7 @@ struct LaneScan {
                 const unsigned at = static_cast<unsigned>(cnt) + __popc(group & lower_mask);
                 if (at < static_cast<unsigned>(padding)) {
                     row[static_cast<uint64_t>(at) * n] = static_cast<int32_t>(j);
-                    row16[slot_entry(at)] = static_cast<uint16_t>(slot);
+                    row16[slot_entry(at)] = static_cast<uint16_t>(slot * SLOT_BYTES);
                 }
             }
             cnt += __popc(group);
@@ -384,13 +389,13 @@ __device__ __forceinline__ void list_build_tile(
         uint16_t rowbuf[TILE_THREADS / WAVE][ROWBUF_CAP];
     } sm;
     Segment* const seg = sm.seg;
-    __shared__ uint16_t lists[LIST_CAP];
+    __shared__ uint16_t lists[LIST_CAP];  // slot byte offsets
     __shared__ int lst[CMAX][4];  // U offset, U length, L offset, L length (lengths padded to 64 with the ghost)
     __shared__ int cl_first[CMAX];
     __shared__ int cl_slot[CMAX];  // LDS slot of the own cell's first particle
     __shared__ unsigned char cl_interior[CMAX];
     __shared__ unsigned char p_cl[TILE];  // own cell of each of the tile's particles, counted from the first
-    __shared__ uint16_t p_slot[TILE];     // LDS slot of each of the tile's particles
+    __shared__ uint16_t p_slot[TILE];     // LDS slot byte offset of each of the tile's particles
     __shared__ int s_outside, s_lists_ok;
     const HaloRanges hr{ra, rgs, rofs};
     const size_t i0 = static_cast<size_t>(tile) * TILE;
@@ -542,7 +547,8 @@ __device__ __forceinline__ void list_build_tile(
     if (use_lists && static_cast<size_t>(threadIdx.x) < i1 - i0) {
         const int cl = my_cell - c_first;
         p_cl[threadIdx.x] = static_cast<unsigned char>(cl);
-        p_slot[threadIdx.x] = static_cast<uint16_t>(cl_slot[cl] + (static_cast<int>(i0 + threadIdx.x) - cl_first[cl]));
+        p_slot[threadIdx.x] =
+            static_cast<uint16_t>((cl_slot[cl] + (static_cast<int>(i0 + threadIdx.x) - cl_first[cl])) * SLOT_BYTES);
     }
     if (use_lists && static_cast<int>(threadIdx.x) < ncl) {
         const int cl = threadIdx.x, c = c_first + cl;
@@ -573,9 +579,11 @@ __device__ __forceinline__ void list_build_tile(
         if (!lists_ok || L >= 2 * ncl) continue;
         const int cl = L >> 1, lower = L & 1;
         const int off = lst[cl][lower ? 2 : 0];
-        for (int m = 0; m < seg_len[u]; ++m) lists[off + seg_off[u] + m] = static_cast<uint16_t>(seg_base[u] + m);
+        for (int m = 0; m < seg_len[u]; ++m)
+            lists[off + seg_off[u] + m] = static_cast<uint16_t>((seg_base[u] + m) * SLOT_BYTES);
         const int padded = lst[cl][lower ? 3 : 1];
-        for (int q = list_used[u] + hl; q < padded; q += HALF) lists[off + q] = static_cast<uint16_t>(GHOST_SLOT);
+        for (int q = list_used[u] + hl; q < padded; q += HALF)
+            lists[off + q] = static_cast<uint16_t>(GHOST_SLOT * SLOT_BYTES);
     }
     __syncthreads();
     int ep_max = 0;      // longest row seen by this lane's wavefront / lane group
@@ -587,7 +595,10 @@ __device__ __forceinline__ void list_build_tile(
             // uniform over the wavefront: one particle per wavefront
             const int cl = __builtin_amdgcn_readfirstlane(p_cl[p]);
             const int slot_i = __builtin_amdgcn_readfirstlane(p_slot[p]);
-            const double xi = xs[slot_i], yi = ys[slot_i], zi = zs[slot_i];
+            // an entry is the byte offset of a slot: its x at xs + offset, y and z one and two arrays on
+            auto coords = [&](int off) { return reinterpret_cast<const double*>(reinterpret_cast<const char*>(xs) + off); };
+            const double* const ci = coords(slot_i);
+            const double xi = ci[0], yi = ci[TILE_HALO_MAX], zi = ci[2 * TILE_HALO_MAX];
             const bool interior = cl_interior[cl] && !s_outside;
             uint16_t* const row16 = rows16 + slot_row_base(i, row_len);
             int32_t* const row = rows + i;
@@ -597,7 +608,8 @@ __device__ __forceinline__ void list_build_tile(
             // counts the set bits below the lane, on top of the row's count so far) and stored. Two batches are
             // loaded and tested before either stores, so their LDS reads overlap.
             auto near = [&](int sl, auto interior_tag) {
-                double dx = xi - xs[sl], dy = yi - ys[sl], dz = zi - zs[sl];
+                const double* const cj = coords(sl);
+                double dx = xi - cj[0], dy = yi - cj[TILE_HALO_MAX], dz = zi - cj[2 * TILE_HALO_MAX];
                 if constexpr (!decltype(interior_tag)::value) {
                     dx = min_image(dx, box, inv_box);
                     dy = min_image(dy, box, inv_box);
@@ -612,7 +624,7 @@ __device__ __forceinline__ void list_build_tile(
                         __builtin_amdgcn_mbcnt_lo(static_cast<unsigned>(mask), static_cast<unsigned>(cnt)));
                     if (at < static_cast<unsigned>(padding)) {
                         rb[at] = static_cast<uint16_t>(sl);
-                        if (write32) row[static_cast<uint64_t>(at) * n] = halo_particle(sl, hr, nr);
+                        if (write32) row[static_cast<uint64_t>(at) * n] = halo_particle(sl / SLOT_BYTES, hr, nr);
                     }
                 }
                 cnt += __popcll(mask);
@@ -792,7 +804,7 @@ __global__ void __launch_bounds__(TILE_THREADS)
     const int m = count[i];
     const uint16_t* row16 = rows16 + slot_row_base(i, row_len);
     for (int k = lane; k < m; k += LANES)
-        rows[static_cast<size_t>(k) * n + i] = halo_particle(row16[slot_entry(static_cast<uint32_t>(k))], hr, nr);
+        rows[static_cast<size_t>(k) * n + i] = halo_particle(row16[slot_entry(static_cast<uint32_t>(k))] / SLOT_BYTES, hr, nr);
 }
 
 // Frame of the mixed force kernel: positions are staged relative to the tile's first particle (its nearest image),
@@ -816,16 +828,18 @@ __device__ __forceinline__ double frame_coord(double x, double ref, const TileFr
 __global__ void __launch_bounds__(TILE_THREADS)
     k_forces_tiled_mixed(const double4* __restrict__ pos, size_t n, const uint16_t* __restrict__ rows16,
                          const int32_t* __restrict__ rows, const int32_t* __restrict__ count,
-                         const int32_t* __restrict__ half, const int4* __restrict__ tile_ranges,
-                         const int* __restrict__ tile_nr, int row_len, LJConst lj, TileFrame fr,
-                         double* __restrict__ frc, double* __restrict__ partials, int* __restrict__ clear_flag) {
-    extern __shared__ double ldsd[];
-    double* xs = ldsd;
-    double* ys = ldsd + TILE_HALO_MAX;
-    double* zs = ldsd + 2 * TILE_HALO_MAX;
+                         const int4* __restrict__ tile_ranges, const int* __restrict__ tile_nr, int row_len,
+                         LJConst lj, TileFrame fr, double* __restrict__ frc, double* __restrict__ partials,
+                         int* __restrict__ clear_flag) {
+    // The halo is a static array: its address is known when the kernel is compiled, so a slot's byte offset, as the
+    // rows hold it, is the whole address computation.
+    __shared__ double halo[3][TILE_HALO_MAX];
+    double* xs = halo[0];
+    double* ys = halo[1];
+    double* zs = halo[2];
+    __shared__ double e_part[3][TILE];  // per particle: U, W and inside count of its row
     __shared__ int ra[RMAX], rgs[RMAX], rofs[RMAX + 1];
     __shared__ int extent_bits;
-    __shared__ double e_part[3][TILE];  // per particle: U, W and inside count of its j > i row part
     const HaloRanges hr{ra, rgs, rofs};
     const int tile = xcd_item(static_cast<int>(blockIdx.x), fr.tiles);
     const int raw_nr = tile_nr[tile];
@@ -836,11 +850,7 @@ __global__ void __launch_bounds__(TILE_THREADS)
     const int lane = threadIdx.x & (LANES - 1);
     const size_t i = i0 + (threadIdx.x >> 2);
     const bool active = i < n;
-    int m = 0, h = 0;  // row lengths, loaded first: the slot loads below depend on them
-    if (active) {
-        m = count[i];
-        h = half[i];
-    }
+    const int m = active ? count[i] : 0;  // row length, loaded first: the slot loads below depend on it
     const double4 ref = pos[i0];
     if (threadIdx.x == 0) extent_bits = 0;
     if (staged) load_ranges(tile_ranges, tile, nr, hr);
@@ -898,14 +908,15 @@ __global__ void __launch_bounds__(TILE_THREADS)
                     for (int t = 0; t < 8; ++t) {
                         const int k = c * CHUNK + t * LANES + lane;
                         if (k < m) {
-                            const int sl = static_cast<int>((words[t >> 1] >> ((t & 1) * 16)) & 0xFFFFu);
-                            double dx = xi - xs[sl], dy = yi - ys[sl], dz = zi - zs[sl];
+                            const unsigned off = __builtin_amdgcn_ubfe(words[t >> 1], (t & 1) * 16, 16);
+                            const double* xj = reinterpret_cast<const double*>(reinterpret_cast<const char*>(xs) + off);
+                            double dx = xi - xj[0], dy = yi - xj[TILE_HALO_MAX], dz = zi - xj[2 * TILE_HALO_MAX];
                             if constexpr (W) {
                                 dx = fma(-fr.box, rint(dx * fr.inv_box), dx);
                                 dy = fma(-fr.box, rint(dy * fr.inv_box), dy);
                                 dz = fma(-fr.box, rint(dz * fr.inv_box), dz);
                             }
-                            mixed_pair_term_f(dx, dy, dz, lj, k < h, a);
+                            mixed_pair_term_f(dx, dy, dz, lj, a);
                         }
                     }
                 };
@@ -920,7 +931,7 @@ __global__ void __launch_bounds__(TILE_THREADS)
         } else {
             const double4 ri = pos[i];
             for (int k = lane; k < m; k += LANES)
-                mixed_pair_gather_f(ri, pos[rows[static_cast<size_t>(k) * n + i]], fr.box, fr.inv_box, lj, k < h, a);
+                mixed_pair_gather_f(ri, pos[rows[static_cast<size_t>(k) * n + i]], fr.box, fr.inv_box, lj, a);
         }
     }
     // the particle's four lanes are adjacent in the wavefront
@@ -940,9 +951,11 @@ __global__ void __launch_bounds__(TILE_THREADS)
             frc[n + i] = a.fy;
             frc[2 * n + i] = a.fz;
         }
-        e_part[0][pl] = static_cast<double>(a.u);  // zero for the slots past n
-        e_part[1][pl] = a.w;
-        e_part[2][pl] = static_cast<double>(a.c);
+        // every pair was met from both ends: half of each full-row sum (zero for the slots past n); the potential's
+        // shift once per inside pair
+        e_part[0][pl] = 0.5 * (static_cast<double>(a.u) - static_cast<double>(lj.shiftf) * a.c);
+        e_part[1][pl] = 0.5 * a.w;
+        e_part[2][pl] = 0.5 * static_cast<double>(a.c);
     }
     if (clear_flag && blockIdx.x == 0 && threadIdx.x == 0) *clear_flag = 0;
     __syncthreads();
@@ -1038,9 +1051,8 @@ void tiled_forces_enqueue(md_state* s, md_list* l, const LJConst& lj, double cut
     fr.tiles = tiles;
     if (ev_start) MD_HIP(hipEventRecord(ev_start, st));
     if (n > 0) {
-        const size_t lds = static_cast<size_t>(TILE_HALO_MAX) * 3 * sizeof(double);
-        hipLaunchKernelGGL(k_forces_tiled_mixed, dim3(tiles), dim3(TILE_THREADS), lds, st, s->pos, n, l->rows16,
-                           l->rows, l->count, l->half, l->tile_ranges, l->tile_nr, slot_row_len(l->padding), lj,
+        hipLaunchKernelGGL(k_forces_tiled_mixed, dim3(tiles), dim3(TILE_THREADS), 0, st, s->pos, n,
+                           l->rows16, l->rows, l->count, l->tile_ranges, l->tile_nr, slot_row_len(l->padding), lj,
                            fr, s->frc, s->partials, clear_flag);
         MD_HIP(hipGetLastError());
     }
