@@ -475,7 +475,7 @@ void list_prepare(md_list* l, md_state* s) {
         l->tiles_cap = tiles;
     }
     // LDS room for a tile's halo: 1.5x the expected union of the 27-cell stencils of the cells a tile spans
-    // (x | y | z doubles, at most 2560 particles = 60 KB), or the whole system when that is smaller
+    // (x | y | z doubles, at most TILE_HALO_MAX particles), or the whole system when that is smaller
     {
         const double per_cell = static_cast<double>(s->n) / static_cast<double>(ncells);
         const double est = 1.5 * 9.0 * (TILE_PARTICLES / (per_cell > 0.5 ? per_cell : 0.5) + 3.0) * per_cell;

@@ -191,7 +191,10 @@ bool tiling_enabled(size_t n);
 int tile_count(size_t n);
 constexpr int TILE_PARTICLES = 128;
 constexpr int TILE_RANGES = 64;
-constexpr int TILE_HALO_MAX = 2560;  // LDS capacity bound of a tile's halo, in particles
+// LDS capacity bound of a tile's halo, in particles: x | y | z doubles of 2184 slots keep the force kernel within
+// 53248 B (13 x 4 KB) and so at three blocks per CU (160 KB of LDS); the few tiles whose halo is larger take the
+// unstaged path
+constexpr int TILE_HALO_MAX = 2184;
 size_t tiled_rows16_entries(size_t n, int padding);
 void tiled_list_enqueue(md_list* l, md_state* s, bool from_alt, const int* gate, bool write32, bool carry_forces);
 void tiled_rows32_enqueue(md_list* l);                          // int32 rows from the slot rows

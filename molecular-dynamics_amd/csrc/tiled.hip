@@ -837,7 +837,9 @@ __global__ void __launch_bounds__(TILE_THREADS)
     double* xs = halo[0];
     double* ys = halo[1];
     double* zs = halo[2];
-    __shared__ double e_part[3][TILE];  // per particle: U, W and inside count of its row
+    // per particle: U, W and inside count of its row, in the halo's memory once the pair loop is done
+    double(*e_part)[TILE] = reinterpret_cast<double(*)[TILE]>(&halo[0][0]);
+    static_assert(3 * TILE <= TILE_HALO_MAX, "the per-particle sums fit in the halo's x array");
     __shared__ int ra[RMAX], rgs[RMAX], rofs[RMAX + 1];
     __shared__ int extent_bits;
     const HaloRanges hr{ra, rgs, rofs};
@@ -945,6 +947,7 @@ __global__ void __launch_bounds__(TILE_THREADS)
         a.c += __shfl_xor(a.c, off, WAVE);
     }
     const int pl = threadIdx.x >> 2;
+    __syncthreads();  // every wavefront is done with the halo
     if (lane == 0) {
         if (active) {
             frc[i] = a.fx;
