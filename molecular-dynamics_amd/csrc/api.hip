@@ -95,6 +95,7 @@ void reduce_partials_enqueue(md_state* s, int blocks, int width, double* out_dev
     switch (width) {
         case 1: k_reduce_partials<1><<<RED_BLOCKS, RED_THREADS, 0, st>>>(partials, blocks, bs, tk, out_dev); break;
         case 3: k_reduce_partials<3><<<RED_BLOCKS, RED_THREADS, 0, st>>>(partials, blocks, bs, tk, out_dev); break;
+        case 4: k_reduce_partials<4><<<RED_BLOCKS, RED_THREADS, 0, st>>>(partials, blocks, bs, tk, out_dev); break;
         case 6: k_reduce_partials<6><<<RED_BLOCKS, RED_THREADS, 0, st>>>(partials, blocks, bs, tk, out_dev); break;
         default: fail(MD_ERR_ARG, "unsupported reduction width");
     }
@@ -317,6 +318,7 @@ int md_state_destroy(md_state* s) {
     dev_free(s->mass);
     dev_free(s->id);
     dev_free(s->pos_alt);
+    dev_free(s->pos_spare);
     dev_free(s->vel_alt);
     dev_free(s->frc_alt);
     dev_free(s->mass_alt);

@@ -621,11 +621,14 @@ void displacement_check(md_list* l, md_state* s) {
 
 // Device-flag-gated rebuild used inside md_run: every kernel reads flags[FLAG_STALE] and returns at once when it
 // is clear, so the same launch sequence serves steps with and without a rebuild.
-void list_enqueue_conditional_rebuild(md_list* l, md_state* s) {
-    const int* gate = l->flags + FLAG_STALE;
+void list_enqueue_conditional_rebuild(md_list* l, md_state* s, const int* gate) {
     const double per_length = l->side / s->box;
     cells_enqueue(l, s, per_length, true, gate, false);
     list_kernel_enqueue(l, s, true, gate, false);
+}
+
+void list_enqueue_conditional_rebuild(md_list* l, md_state* s) {
+    list_enqueue_conditional_rebuild(l, s, l->flags + FLAG_STALE);
 }
 
 // Fixed-order list (no re-sort): candidates are the members of the 27 cells taken through the stable cell order,

@@ -2,6 +2,7 @@
 // multi-step driver md_run (md_sim's production loop, main.cpp:272-304) that keeps the whole step on the GPU.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -270,13 +271,14 @@ int md_run(md_state* s, md_list* l, const md_run_config* cfg, double* energies) 
     static_assert(FLAG_MAXCOUNT == FLAG_OVERFLOW + 1, "the two words are cleared together");
     MD_HIP(hipMemsetAsync(l->flags + FLAG_REBUILDS, 0, sizeof(int), st));
     MD_HIP(hipMemsetAsync(l->flags + FLAG_STALE, 0, sizeof(int), st));
+    MD_HIP(hipMemsetAsync(l->flags + FLAG_STALE_ODD, 0, sizeof(int), st));
     const double dof = 3.0 * static_cast<double>(s->n) - 3.0;
     const double kbar = 0.5 * dof * cfg->temperature;
     const double c = cfg->thermostat ? std::exp(-cfg->dt / cfg->tau) : 0.0;
     const size_t nvel = 3 * s->n;
     // reserve the partials once, before any capture (no allocation may happen inside it): per-block rows of the
     // 256-thread kernels, or per-tile rows of the tiled force kernel, whichever is more
-    state_reserve_partials(s, static_cast<size_t>(std::max<size_t>(blocks_for(s->n, 256), tile_count(s->n))) * 3 + 8);
+    state_reserve_partials(s, static_cast<size_t>(std::max<size_t>(blocks_for(s->n, 256), tile_count(s->n))) * 4 + 8);
 
     // optional force-kernel timing: one event pair per step, eager launches only
     const bool timed = cfg->force_kernel_ms != nullptr && !cfg->use_graph;
@@ -330,11 +332,92 @@ int md_run(md_state* s, md_list* l, const md_run_config* cfg, double* energies) 
     };
 
     const bool fused = !cfg->thermostat;
-    if (fused) {  // head: the first step's kick-drift and forces
+    // Mixed NVE on the tiled rows goes further: the force kernel itself carries each step (closing kick, kinetic
+    // energy, opening kick and drift of the next step) for its own particles, reading the positions from one
+    // buffer and writing the next step's into the other. Step k (k = 1 .. steps - 1) reads buffer (k - 1) % 2 and
+    // is gated by the stale word of that parity, which its force kernel clears while setting the other one. The
+    // head is the first kick-drift; the tail is the last step's forces, closing kick and record.
+    const char* unfolded = std::getenv("MDGPU_UNFOLDED");  // 1: keep the separate kick-kick-drift kernel
+    const bool fold = fused && cfg->precision == MD_MIXED && tiling_enabled(s->n) && l->tiled && steps >= 2 &&
+                      !(unfolded && unfolded[0] == '1');
+    double4* bufs[2] = {s->pos, nullptr};
+    if (fold) {
+        if (!s->pos_spare) s->pos_spare = dev_alloc<double4>(s->n);
+        bufs[1] = s->pos_spare;
+    }
+    struct SpareGuard {  // however the run ends, the spare is the buffer that s->pos is not
+        md_state* s;
+        double4* b0;
+        double4* b1;
+        bool on;
+        ~SpareGuard() {
+            if (on) s->pos_spare = s->pos == b0 ? b1 : b0;
+        }
+    } spare_guard{s, bufs[0], bufs[1], fold};
+    auto enqueue_fold_body = [&](int p) {
+        s->pos = bufs[p];
+        int* gate = l->flags + (p ? FLAG_STALE_ODD : FLAG_STALE);
+        int* next = l->flags + (p ? FLAG_STALE : FLAG_STALE_ODD);
+        list_enqueue_conditional_rebuild(l, s, gate);
+        hipEvent_t e0 = timed ? events[2 * step_index] : nullptr;
+        hipEvent_t e1 = timed ? events[2 * step_index + 1] : nullptr;
+        ++step_index;
+        tiled_forces_step_enqueue(s, l, cfg->lj, cfg->dt, bufs[1 - p], gate, next, s->scalars, e0, e1);
+        k_record<<<1, 1, 0, st>>>(s->scalars, 3, rb.energies, rb.counters);
+        MD_HIP(hipGetLastError());
+    };
+    if (fold) {
+        kick_drift_enqueue(s, cfg->dt, l);
+        const int bodies = steps - 1;
+        if (cfg->use_graph && bodies >= 2) {
+            // one graph holds an even and an odd step (their buffers differ)
+            hipGraph_t graph = nullptr;
+            hipGraphExec_t exec = nullptr;
+            MD_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            try {
+                enqueue_fold_body(0);
+                enqueue_fold_body(1);
+            } catch (...) {
+                (void)hipStreamEndCapture(st, &graph);
+                if (graph) (void)hipGraphDestroy(graph);
+                s->pos = bufs[0];
+                throw;
+            }
+            MD_HIP(hipStreamEndCapture(st, &graph));
+            hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+            if (e != hipSuccess) {
+                (void)hipGraphDestroy(graph);
+                s->pos = bufs[0];
+                hip_check(e, "hipGraphInstantiate");
+            }
+            for (int k = 0; k < bodies / 2; ++k) {
+                e = hipGraphLaunch(exec, st);
+                if (e != hipSuccess) break;
+            }
+            (void)hipStreamSynchronize(st);
+            (void)hipGraphExecDestroy(exec);
+            (void)hipGraphDestroy(graph);
+            if (e != hipSuccess) s->pos = bufs[0];
+            hip_check(e, "hipGraphLaunch");
+            if (bodies % 2) enqueue_fold_body(0);
+        } else {
+            for (int k = 1; k <= bodies; ++k) enqueue_fold_body((k - 1) % 2);
+        }
+        // tail: the last step's forces (stored), closing kick and record
+        const int p = (steps - 1) % 2;
+        s->pos = bufs[p];
+        int* gate = l->flags + (p ? FLAG_STALE_ODD : FLAG_STALE);
+        list_enqueue_conditional_rebuild(l, s, gate);
+        hipEvent_t e0 = timed ? events[2 * step_index] : nullptr;
+        hipEvent_t e1 = timed ? events[2 * step_index + 1] : nullptr;
+        ++step_index;
+        forces_enqueue(s, l, cfg->lj, cfg->precision, s->scalars, gate, e0, e1);
+    }
+    const int replays = fold ? 0 : (fused ? steps - 1 : steps);
+    if (fused && !fold) {  // head: the first step's kick-drift and forces
         kick_drift_enqueue(s, cfg->dt, l);
         enqueue_forces();
     }
-    const int replays = fused ? steps - 1 : steps;
     if (cfg->use_graph && replays > 0) {
         hipGraph_t graph = nullptr;
         hipGraphExec_t exec = nullptr;

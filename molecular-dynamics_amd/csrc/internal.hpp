@@ -63,6 +63,7 @@ enum Flag : int {
     FLAG_MAXCOUNT = 2,  // longest row seen by the last build
     FLAG_REBUILDS = 3,  // rebuilds performed on the device
     FLAG_FORCE = 4,     // host-requested rebuild (first build, n changed)
+    FLAG_STALE_ODD = 5, // FLAG_STALE of the odd steps of a fused md_run (each step's kernel sets the next one's)
     FLAG_COUNT = 8
 };
 
@@ -106,6 +107,8 @@ struct md_state {
     double* frc_alt = nullptr;
     double* mass_alt = nullptr;
     uint64_t* id_alt = nullptr;
+    // positions after the drift of a fused md_run step (allocated by the first such run)
+    double4* pos_spare = nullptr;
     // reductions: per-block partials and the device result words
     double* partials = nullptr;
     size_t partials_cap = 0;
@@ -162,6 +165,7 @@ namespace mdgpu {
 void cells_build(md_list* l, md_state* s, double min_cell, int side);   // count, scan, scatter, stable order
 void list_rebuild(md_list* l, md_state* s);                            // full host-driven rebuild + read-back
 void list_enqueue_conditional_rebuild(md_list* l, md_state* s);       // device-flag-gated rebuild (md_run)
+void list_enqueue_conditional_rebuild(md_list* l, md_state* s, const int* gate);  // gated by another flag word
 void list_prepare(md_list* l, md_state* s);                            // grid/buffers for the state's size
 void displacement_check(md_list* l, md_state* s);                      // sets FLAG_STALE
 void list_rebuild_fixed(md_list* l, md_state* s);                      // list without re-sorting (host-driven)
@@ -203,6 +207,10 @@ void ensure_rows32(md_list* l);                                 // (cells.hip) m
 struct LJConst;
 void tiled_forces_enqueue(md_state* s, md_list* l, const LJConst& lj, double cutoff, double* out_dev,
                           int* clear_flag, hipEvent_t ev_start, hipEvent_t ev_stop);
+// the same with the NVE step of the tile's particles folded in: U, W, inside count and KE to out_dev[0..3], new
+// positions to pos_next, the next displacement test to *stale_next; the forces are not stored
+void tiled_forces_step_enqueue(md_state* s, md_list* l, const md_lj& p, double dt, double4* pos_next, int* clear_flag,
+                               int* stale_next, double* out_dev, hipEvent_t ev_start, hipEvent_t ev_stop);
 
 // integrate.hip
 void kick_drift_enqueue(md_state* s, double dt, md_list* check_list);

@@ -815,6 +815,16 @@ struct TileFrame {
     int tiles;
 };
 
+// Arguments of the step that the force kernel can carry out for its own particles (see k_forces_tiled_mixed).
+struct ForceStep {
+    double4* pos_next;     // positions after the drift (a buffer other than the one being read)
+    double* vel;           // velocities, x | y | z blocks of n
+    const double* mass;
+    const double4* ref;    // positions at the last list build
+    double half_dt, dt, max_disp2;
+    int* stale_next;       // set when a particle has moved more than half the skin
+};
+
 __device__ __forceinline__ double frame_coord(double x, double ref, const TileFrame& fr) {
     const double t = x - ref;
     return fma(-fr.box, rint(t * fr.inv_box), t);
@@ -825,21 +835,30 @@ __device__ __forceinline__ double frame_coord(double x, double ref, const TileFr
 // evaluated) and the frame positions from LDS; unstaged tiles gather positions through the int32 rows. In a
 // compact tile (its own particles within e of the reference particle, with 2e + reach < L - 1) every listed
 // separation is already the minimum image; otherwise it is taken per pair.
+//
+// STEP: the velocity-Verlet step around the force evaluation, done for the tile's own particles as soon as their
+// forces are summed (NVE md_run, mixed): the closing kick of this step and its kinetic energy, the opening kick and
+// drift of the next step, and the displacement test of the next list check (the arithmetic of k_kick_kick_drift).
+// The new positions go to a second buffer, since other tiles' halos still read this step's; the forces themselves
+// are not stored.
+template <bool STEP>
 __global__ void __launch_bounds__(TILE_THREADS)
     k_forces_tiled_mixed(const double4* __restrict__ pos, size_t n, const uint16_t* __restrict__ rows16,
                          const int32_t* __restrict__ rows, const int32_t* __restrict__ count,
                          const int4* __restrict__ tile_ranges, const int* __restrict__ tile_nr, int row_len,
                          LJConst lj, TileFrame fr, double* __restrict__ frc, double* __restrict__ partials,
-                         int* __restrict__ clear_flag) {
+                         int* __restrict__ clear_flag, ForceStep fs) {
     // The halo is a static array: its address is known when the kernel is compiled, so a slot's byte offset, as the
     // rows hold it, is the whole address computation.
     __shared__ double halo[3][TILE_HALO_MAX];
     double* xs = halo[0];
     double* ys = halo[1];
     double* zs = halo[2];
-    // per particle: U, W and inside count of its row, in the halo's memory once the pair loop is done
+    // per particle: U, W and inside count of its row (and with STEP its kinetic energy), in the halo's memory once
+    // the pair loop is done
+    constexpr int SUMS = STEP ? 4 : 3;
     double(*e_part)[TILE] = reinterpret_cast<double(*)[TILE]>(&halo[0][0]);
-    static_assert(3 * TILE <= TILE_HALO_MAX, "the per-particle sums fit in the halo's x array");
+    static_assert(4 * TILE <= TILE_HALO_MAX, "the per-particle sums fit in the halo's x array");
     __shared__ int ra[RMAX], rgs[RMAX], rofs[RMAX + 1];
     __shared__ int extent_bits;
     const HaloRanges hr{ra, rgs, rofs};
@@ -947,9 +966,35 @@ __global__ void __launch_bounds__(TILE_THREADS)
         a.c += __shfl_xor(a.c, off, WAVE);
     }
     const int pl = threadIdx.x >> 2;
+    double ke = 0.0;
+    if constexpr (STEP) {
+        if (lane == 0 && active) {
+            const double m = fs.mass[i];
+            const double f[3] = {a.fx, a.fy, a.fz};
+            double v[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) v[c] = fs.vel[c * n + i] + fs.half_dt * f[c] / m;
+            ke = 0.5 * m * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                v[c] = v[c] + fs.half_dt * f[c] / m;
+                fs.vel[c * n + i] = v[c];
+            }
+            double4 r = pos[i];
+            r.x = wrap_coord(r.x + fs.dt * v[0], fr.box, fr.inv_box);
+            r.y = wrap_coord(r.y + fs.dt * v[1], fr.box, fr.inv_box);
+            r.z = wrap_coord(r.z + fs.dt * v[2], fr.box, fr.inv_box);
+            fs.pos_next[i] = r;
+            const double4 q = fs.ref[i];
+            const double dx = min_image(r.x - q.x, fr.box, fr.inv_box);
+            const double dy = min_image(r.y - q.y, fr.box, fr.inv_box);
+            const double dz = min_image(r.z - q.z, fr.box, fr.inv_box);
+            if (dx * dx + dy * dy + dz * dz > fs.max_disp2) *fs.stale_next = 1;
+        }
+    }
     __syncthreads();  // every wavefront is done with the halo
     if (lane == 0) {
-        if (active) {
+        if (active && !STEP) {
             frc[i] = a.fx;
             frc[n + i] = a.fy;
             frc[2 * n + i] = a.fz;
@@ -959,16 +1004,18 @@ __global__ void __launch_bounds__(TILE_THREADS)
         e_part[0][pl] = 0.5 * (static_cast<double>(a.u) - static_cast<double>(lj.shiftf) * a.c);
         e_part[1][pl] = 0.5 * a.w;
         e_part[2][pl] = 0.5 * static_cast<double>(a.c);
+        if constexpr (STEP) e_part[3][pl] = ke;
     }
     if (clear_flag && blockIdx.x == 0 && threadIdx.x == 0) *clear_flag = 0;
     __syncthreads();
-    // the tile's U, W and inside count: the 128 particle sums added in double by one wavefront, in a fixed order
+    // the tile's U, W and inside count (and KE): the 128 particle sums added in double by one wavefront, in a fixed
+    // order
     if (threadIdx.x < WAVE) {
 #pragma unroll
-        for (int q = 0; q < 3; ++q) {
+        for (int q = 0; q < SUMS; ++q) {
             double v = e_part[q][threadIdx.x] + e_part[q][threadIdx.x + WAVE];
             v = wave_sum(v);
-            if (threadIdx.x == 0) partials[static_cast<size_t>(tile) * 3 + q] = v;
+            if (threadIdx.x == 0) partials[static_cast<size_t>(tile) * SUMS + q] = v;
         }
     }
 }
@@ -1041,12 +1088,13 @@ void tiled_list_enqueue(md_list* l, md_state* s, bool from_alt, const int* gate,
     MD_HIP(hipGetLastError());
 }
 
-void tiled_forces_enqueue(md_state* s, md_list* l, const LJConst& lj, double cutoff, double* out_dev,
-                          int* clear_flag, hipEvent_t ev_start, hipEvent_t ev_stop) {
+static void tiled_forces_launch(md_state* s, md_list* l, const LJConst& lj, double cutoff, double* out_dev,
+                                int* clear_flag, const ForceStep* step, hipEvent_t ev_start, hipEvent_t ev_stop) {
     hipStream_t st = s->ctx->stream;
     const size_t n = s->n;
     const int tiles = tile_count(n ? n : 1);
-    state_reserve_partials(s, static_cast<size_t>(tiles) * 3);
+    const int sums = step ? 4 : 3;
+    state_reserve_partials(s, static_cast<size_t>(tiles) * sums);
     TileFrame fr;
     fr.box = s->box;
     fr.inv_box = s->inv_box;
@@ -1054,13 +1102,37 @@ void tiled_forces_enqueue(md_state* s, md_list* l, const LJConst& lj, double cut
     fr.tiles = tiles;
     if (ev_start) MD_HIP(hipEventRecord(ev_start, st));
     if (n > 0) {
-        hipLaunchKernelGGL(k_forces_tiled_mixed, dim3(tiles), dim3(TILE_THREADS), 0, st, s->pos, n,
-                           l->rows16, l->rows, l->count, l->tile_ranges, l->tile_nr, slot_row_len(l->padding), lj,
-                           fr, s->frc, s->partials, clear_flag);
+        if (step)
+            hipLaunchKernelGGL(k_forces_tiled_mixed<true>, dim3(tiles), dim3(TILE_THREADS), 0, st, s->pos, n,
+                               l->rows16, l->rows, l->count, l->tile_ranges, l->tile_nr, slot_row_len(l->padding),
+                               lj, fr, s->frc, s->partials, clear_flag, *step);
+        else
+            hipLaunchKernelGGL(k_forces_tiled_mixed<false>, dim3(tiles), dim3(TILE_THREADS), 0, st, s->pos, n,
+                               l->rows16, l->rows, l->count, l->tile_ranges, l->tile_nr, slot_row_len(l->padding),
+                               lj, fr, s->frc, s->partials, clear_flag, ForceStep{});
         MD_HIP(hipGetLastError());
     }
     if (ev_stop) MD_HIP(hipEventRecord(ev_stop, st));
-    reduce_partials_enqueue(s, n > 0 ? tiles : 0, 3, out_dev);
+    reduce_partials_enqueue(s, n > 0 ? tiles : 0, sums, out_dev);
+}
+
+void tiled_forces_enqueue(md_state* s, md_list* l, const LJConst& lj, double cutoff, double* out_dev,
+                          int* clear_flag, hipEvent_t ev_start, hipEvent_t ev_stop) {
+    tiled_forces_launch(s, l, lj, cutoff, out_dev, clear_flag, nullptr, ev_start, ev_stop);
+}
+
+void tiled_forces_step_enqueue(md_state* s, md_list* l, const md_lj& p, double dt, double4* pos_next, int* clear_flag,
+                               int* stale_next, double* out_dev, hipEvent_t ev_start, hipEvent_t ev_stop) {
+    ForceStep fs;
+    fs.pos_next = pos_next;
+    fs.vel = s->vel;
+    fs.mass = s->mass;
+    fs.ref = l->ref;
+    fs.half_dt = 0.5 * dt;
+    fs.dt = dt;
+    fs.max_disp2 = l->max_disp2;
+    fs.stale_next = stale_next;
+    tiled_forces_launch(s, l, make_lj(p), p.cutoff, out_dev, clear_flag, &fs, ev_start, ev_stop);
 }
 
 }  // namespace mdgpu

@@ -219,6 +219,36 @@ def test_mixed_graph_run_matches_eager_run(ctx):
     assert ga.state_hash() == gb.state_hash()
 
 
+@pytest.mark.parametrize("steps,use_graph", [(2, False), (151, False), (150, True), (151, True)])
+def test_mixed_run_folded_step_matches_separate_kernels(ctx, monkeypatch, steps, use_graph):
+    """Mixed NVE md_run carries each step inside the force kernel, with the positions in two alternating buffers.
+    The run with the separate kick-kick-drift kernel (MDGPU_UNFOLDED=1) does the same arithmetic: positions,
+    velocities and the final forces agree bit for bit, the list rebuilds at the same steps, and the recorded U, W
+    and inside count are equal; the kinetic energy differs only by its summation order."""
+    s = O.make_liquid(10, 40, skin=0.3)
+    ga, gb = to_gpu(ctx, s), to_gpu(ctx, s)
+    la, lb = mdgpu.NeighbourList(ctx, 2.5, 0.3), mdgpu.NeighbourList(ctx, 2.5, 0.3)
+    for g, nl in ((ga, la), (gb, lb)):
+        nl.build(g)
+        nl.forces(g, LJ, mdgpu.MD_MIXED)
+    monkeypatch.setenv("MDGPU_UNFOLDED", "1")
+    ea = la.run(ga, LJ, 0.005, steps, precision=mdgpu.MD_MIXED, use_graph=use_graph)
+    monkeypatch.delenv("MDGPU_UNFOLDED")
+    eb = lb.run(gb, LJ, 0.005, steps, precision=mdgpu.MD_MIXED, use_graph=use_graph)
+    assert lb.info().tiled == 1
+    assert la.info().rebuilds == lb.info().rebuilds and (steps < 100 or lb.info().rebuilds > 2)
+    for field in (mdgpu.FIELD_POS, mdgpu.FIELD_VEL, mdgpu.FIELD_FORCE, mdgpu.FIELD_ID):
+        assert np.array_equal(ga.download(field), gb.download(field))
+    assert np.array_equal(ea[:, [0, 2, 3]], eb[:, [0, 2, 3]])
+    np.testing.assert_allclose(eb[:, 1], ea[:, 1], rtol=1e-12, atol=0)
+    assert ga.state_hash() == gb.state_hash()
+    # a second run continues from the state the first left (positions in whichever buffer)
+    ea2 = la.run(ga, LJ, 0.005, 31, precision=mdgpu.MD_MIXED, use_graph=False)
+    eb2 = lb.run(gb, LJ, 0.005, 31, precision=mdgpu.MD_MIXED, use_graph=False)
+    assert np.array_equal(ga.download(mdgpu.FIELD_POS), gb.download(mdgpu.FIELD_POS))
+    assert np.array_equal(ea2[:, 0], eb2[:, 0])
+
+
 def test_int32_rows_rebuilt_after_mixed_run(ctx):
     """A mixed md_run rebuilds only the 16-bit slot rows on the device; the int32 rows are materialised from them
     when the FP64 kernel needs them, and hold every pair within the cutoff."""
