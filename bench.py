@@ -296,11 +296,17 @@ def main():
     all_en = en if warm is None else np.vstack([warm, en])
     drift1000, max_drift = drift_stats(all_en, e0, n)
     k_full = 2.0 * info.half_pairs / n
-    # force kernel: per atom it reads its position (32 B), its two row lengths (8 B) and its row (4 B per
-    # full-list neighbour as int32, 2 B as the tiled kernel's 16-bit LDS slots), and writes its force (24 B);
-    # neighbour positions are re-reads of the same array
+    # force kernel: per atom it reads its position (32 B), its row length(s) (4 B, 8 B with the half-row length
+    # of the untiled kernels) and its row (4 B per full-list neighbour as int32, 2 B as the tiled kernel's 16-bit
+    # LDS slots), and writes its force (24 B); neighbour positions are re-reads of the same array. In a mixed NVE
+    # run the tiled kernel also carries the step for all but the last one: instead of writing the force it reads
+    # the velocity (24 B), mass (8 B) and list-build position (32 B) and writes the velocity (24 B) and the next
+    # position (32 B)
     tiled = bool(info.tiled) and prec == mdgpu.MD_MIXED
-    bytes_per_atom = 32.0 + 8.0 + (2.0 if tiled else 4.0) * k_full + 24.0
+    plain = 32.0 + (4.0 if tiled else 8.0) + (2.0 if tiled else 4.0) * k_full + 24.0
+    folded = 32.0 + 4.0 + 2.0 * k_full + 24.0 + 8.0 + 32.0 + 24.0 + 32.0
+    fold = tiled and os.environ.get("MDGPU_UNFOLDED") != "1" and args.steps >= 2
+    bytes_per_atom = ((args.steps - 1) * folded + plain) / args.steps if fold else plain
     roofline = None
     if timing:
         avg_ms = timing[0] / args.steps
@@ -313,8 +319,8 @@ def main():
                 traffic = tr.get("hbm_bytes_per_launch")
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                    "kernel": ("k_forces_tiled_mixed" if tiled else "k_forces_mixed") if prec == mdgpu.MD_MIXED
-                    else "k_forces_fp64",
+                    "kernel": ("k_forces_tiled_mixed<true>" if fold else "k_forces_tiled_mixed" if tiled
+                               else "k_forces_mixed") if prec == mdgpu.MD_MIXED else "k_forces_fp64",
                     "avg_launch_ms": round(avg_ms, 5), "algorithmic_bytes_per_launch": round(bytes_per_atom * n)}
     cpu = None
     if not args.no_cpu:
