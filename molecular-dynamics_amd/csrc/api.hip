@@ -41,7 +41,7 @@ template <int W>
 __global__ void __launch_bounds__(RED_THREADS) k_reduce_partials(const double* __restrict__ partials, int rows,
                                                                  double* __restrict__ block_sums,
                                                                  unsigned* __restrict__ ticket,
-                                                                 double* __restrict__ out) {
+                                                                 double* __restrict__ out, StepRecord rec) {
     __shared__ double lds[RED_THREADS / WAVE][W];
     __shared__ double fin[RED_BLOCKS][W];
     __shared__ int s_last;
@@ -78,28 +78,45 @@ __global__ void __launch_bounds__(RED_THREADS) k_reduce_partials(const double* _
     }
     __syncthreads();
     if (threadIdx.x == 0) {
+        double sum[W];
         for (int k = 0; k < W; ++k) {
             double t = 0.0;
             for (int g = 0; g < RED_BLOCKS; ++g) t += fin[g][k];
             out[k] = t;
+            sum[k] = t;
+        }
+        if constexpr (W == 4) {
+            if (rec.energies) {  // the run's record of this step from (U, W, inside count, KE): U, KE, W, count
+                const int step = *rec.step;
+                rec.energies[4 * step + 0] = sum[0];
+                rec.energies[4 * step + 1] = sum[3];
+                rec.energies[4 * step + 2] = sum[1];
+                rec.energies[4 * step + 3] = sum[2];
+                *rec.step = step + 1;
+            }
         }
         *ticket = 0u;
     }
 }
 
-void reduce_partials_enqueue(md_state* s, int blocks, int width, double* out_dev) {
+void reduce_partials_enqueue(md_state* s, int blocks, int width, double* out_dev, StepRecord rec) {
     hipStream_t st = s->ctx->stream;
     const double* partials = s->partials;
     double* bs = s->red_blocks;
     unsigned* tk = s->red_ticket;
+    if (rec.energies && width != 4) fail(MD_ERR_ARG, "a step record needs the four-sum reduction");
     switch (width) {
-        case 1: k_reduce_partials<1><<<RED_BLOCKS, RED_THREADS, 0, st>>>(partials, blocks, bs, tk, out_dev); break;
-        case 3: k_reduce_partials<3><<<RED_BLOCKS, RED_THREADS, 0, st>>>(partials, blocks, bs, tk, out_dev); break;
-        case 4: k_reduce_partials<4><<<RED_BLOCKS, RED_THREADS, 0, st>>>(partials, blocks, bs, tk, out_dev); break;
-        case 6: k_reduce_partials<6><<<RED_BLOCKS, RED_THREADS, 0, st>>>(partials, blocks, bs, tk, out_dev); break;
+        case 1: k_reduce_partials<1><<<RED_BLOCKS, RED_THREADS, 0, st>>>(partials, blocks, bs, tk, out_dev, rec); break;
+        case 3: k_reduce_partials<3><<<RED_BLOCKS, RED_THREADS, 0, st>>>(partials, blocks, bs, tk, out_dev, rec); break;
+        case 4: k_reduce_partials<4><<<RED_BLOCKS, RED_THREADS, 0, st>>>(partials, blocks, bs, tk, out_dev, rec); break;
+        case 6: k_reduce_partials<6><<<RED_BLOCKS, RED_THREADS, 0, st>>>(partials, blocks, bs, tk, out_dev, rec); break;
         default: fail(MD_ERR_ARG, "unsupported reduction width");
     }
     MD_HIP(hipGetLastError());
+}
+
+void reduce_partials_enqueue(md_state* s, int blocks, int width, double* out_dev) {
+    reduce_partials_enqueue(s, blocks, width, out_dev, StepRecord{});
 }
 
 // Kinetic energy in the reference's order (Parallel.hpp:32-57): block b of 1024 particles is summed left to

@@ -362,9 +362,8 @@ int md_run(md_state* s, md_list* l, const md_run_config* cfg, double* energies) 
         hipEvent_t e0 = timed ? events[2 * step_index] : nullptr;
         hipEvent_t e1 = timed ? events[2 * step_index + 1] : nullptr;
         ++step_index;
-        tiled_forces_step_enqueue(s, l, cfg->lj, cfg->dt, bufs[1 - p], gate, next, s->scalars, e0, e1);
-        k_record<<<1, 1, 0, st>>>(s->scalars, 3, rb.energies, rb.counters);
-        MD_HIP(hipGetLastError());
+        tiled_forces_step_enqueue(s, l, cfg->lj, cfg->dt, bufs[1 - p], gate, next, s->scalars,
+                                  StepRecord{rb.energies, rb.counters}, e0, e1);
     };
     if (fold) {
         kick_drift_enqueue(s, cfg->dt, l);

@@ -161,6 +161,13 @@ struct md_list {
 
 namespace mdgpu {
 
+// An md_run step's record, written by the four-sum reduction (U, W, inside count, KE) when `energies` is set:
+// energies[4 k ..] = U, KE, W, count for k = *step, then *step = k + 1.
+struct StepRecord {
+    double* energies = nullptr;
+    int* step = nullptr;
+};
+
 // cells.hip
 void cells_build(md_list* l, md_state* s, double min_cell, int side);   // count, scan, scatter, stable order
 void list_rebuild(md_list* l, md_state* s);                            // full host-driven rebuild + read-back
@@ -207,10 +214,11 @@ void ensure_rows32(md_list* l);                                 // (cells.hip) m
 struct LJConst;
 void tiled_forces_enqueue(md_state* s, md_list* l, const LJConst& lj, double cutoff, double* out_dev,
                           int* clear_flag, hipEvent_t ev_start, hipEvent_t ev_stop);
-// the same with the NVE step of the tile's particles folded in: U, W, inside count and KE to out_dev[0..3], new
-// positions to pos_next, the next displacement test to *stale_next; the forces are not stored
+// the same with the NVE step of the tile's particles folded in: U, W, inside count and KE to out_dev[0..3] (and the
+// step's record), new positions to pos_next, the next displacement test to *stale_next; the forces are not stored
 void tiled_forces_step_enqueue(md_state* s, md_list* l, const md_lj& p, double dt, double4* pos_next, int* clear_flag,
-                               int* stale_next, double* out_dev, hipEvent_t ev_start, hipEvent_t ev_stop);
+                               int* stale_next, double* out_dev, StepRecord rec, hipEvent_t ev_start,
+                               hipEvent_t ev_stop);
 
 // integrate.hip
 void kick_drift_enqueue(md_state* s, double dt, md_list* check_list);
@@ -218,5 +226,6 @@ void kick_enqueue(md_state* s, double dt, double* ke_dev);
 
 // Final fixed-order reduction of `blocks` rows of `width` partials (s->partials) into out_dev[0..width).
 void reduce_partials_enqueue(md_state* s, int blocks, int width, double* out_dev);
+void reduce_partials_enqueue(md_state* s, int blocks, int width, double* out_dev, StepRecord rec);
 
 }  // namespace mdgpu

@@ -1089,7 +1089,8 @@ void tiled_list_enqueue(md_list* l, md_state* s, bool from_alt, const int* gate,
 }
 
 static void tiled_forces_launch(md_state* s, md_list* l, const LJConst& lj, double cutoff, double* out_dev,
-                                int* clear_flag, const ForceStep* step, hipEvent_t ev_start, hipEvent_t ev_stop) {
+                                int* clear_flag, const ForceStep* step, StepRecord rec, hipEvent_t ev_start,
+                                hipEvent_t ev_stop) {
     hipStream_t st = s->ctx->stream;
     const size_t n = s->n;
     const int tiles = tile_count(n ? n : 1);
@@ -1113,16 +1114,17 @@ static void tiled_forces_launch(md_state* s, md_list* l, const LJConst& lj, doub
         MD_HIP(hipGetLastError());
     }
     if (ev_stop) MD_HIP(hipEventRecord(ev_stop, st));
-    reduce_partials_enqueue(s, n > 0 ? tiles : 0, sums, out_dev);
+    reduce_partials_enqueue(s, n > 0 ? tiles : 0, sums, out_dev, rec);
 }
 
 void tiled_forces_enqueue(md_state* s, md_list* l, const LJConst& lj, double cutoff, double* out_dev,
                           int* clear_flag, hipEvent_t ev_start, hipEvent_t ev_stop) {
-    tiled_forces_launch(s, l, lj, cutoff, out_dev, clear_flag, nullptr, ev_start, ev_stop);
+    tiled_forces_launch(s, l, lj, cutoff, out_dev, clear_flag, nullptr, StepRecord{}, ev_start, ev_stop);
 }
 
 void tiled_forces_step_enqueue(md_state* s, md_list* l, const md_lj& p, double dt, double4* pos_next, int* clear_flag,
-                               int* stale_next, double* out_dev, hipEvent_t ev_start, hipEvent_t ev_stop) {
+                               int* stale_next, double* out_dev, StepRecord rec, hipEvent_t ev_start,
+                               hipEvent_t ev_stop) {
     ForceStep fs;
     fs.pos_next = pos_next;
     fs.vel = s->vel;
@@ -1132,7 +1134,7 @@ void tiled_forces_step_enqueue(md_state* s, md_list* l, const md_lj& p, double d
     fs.dt = dt;
     fs.max_disp2 = l->max_disp2;
     fs.stale_next = stale_next;
-    tiled_forces_launch(s, l, make_lj(p), p.cutoff, out_dev, clear_flag, &fs, ev_start, ev_stop);
+    tiled_forces_launch(s, l, make_lj(p), p.cutoff, out_dev, clear_flag, &fs, rec, ev_start, ev_stop);
 }
 
 }  // namespace mdgpu
