@@ -71,8 +71,8 @@ __global__ void __launch_bounds__(256) k_cell_count(const double4* __restrict__ 
     if (run.head) atomicAdd(&counts[c], static_cast<uint32_t>(run.len));
 }
 
-// Three-phase exclusive scan of the cell histogram, SCAN_ITEMS cells per block: block totals, a scan of the
-// totals, then each block rescans its cells from its offset. start[ncells] receives the total.
+// Two-phase exclusive scan of the cell histogram, SCAN_ITEMS cells per block: block totals, then each block adds
+// the totals before it and scans its own cells. start[ncells] receives the total.
 constexpr int SCAN_THREADS = 256;
 constexpr int SCAN_PER_THREAD = 8;
 constexpr int SCAN_ITEMS = SCAN_THREADS * SCAN_PER_THREAD;
@@ -97,48 +97,19 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_totals(const uint32_t* __
     }
 }
 
-__global__ void __launch_bounds__(1024) k_scan_top(uint32_t* __restrict__ totals, int blocks,
-                                                    uint32_t* __restrict__ grand, const int* gate) {
-    if (gated_off(gate)) return;
-    __shared__ uint32_t wsum[1024 / WAVE];
-    __shared__ uint32_t carry;
-    if (threadIdx.x == 0) carry = 0;
-    __syncthreads();
-    const int lane = threadIdx.x % WAVE, wave = threadIdx.x / WAVE;
-    for (int b0 = 0; b0 < blocks; b0 += 1024) {
-        const int b = b0 + threadIdx.x;
-        const uint32_t v = b < blocks ? totals[b] : 0u;
-        uint32_t incl = v;
-#pragma unroll
-        for (int off = 1; off < WAVE; off <<= 1) {
-            const uint32_t o = __shfl_up(incl, off, WAVE);
-            if (lane >= off) incl += o;
-        }
-        if (lane == WAVE - 1) wsum[wave] = incl;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            uint32_t acc = 0;
-            for (int w = 0; w < 1024 / WAVE; ++w) {
-                const uint32_t t = wsum[w];
-                wsum[w] = acc;
-                acc += t;
-            }
-        }
-        __syncthreads();
-        const uint32_t excl = carry + wsum[wave] + incl - v;
-        if (b < blocks) totals[b] = excl;
-        __syncthreads();
-        if (threadIdx.x == 1023) carry = excl + v;
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) *grand = carry;
-}
-
 __global__ void __launch_bounds__(SCAN_THREADS) k_scan_apply(const uint32_t* __restrict__ counts, size_t ncells,
                                                               const uint32_t* __restrict__ totals,
                                                               uint32_t* __restrict__ start, const int* gate) {
     if (gated_off(gate)) return;
     __shared__ uint32_t wsum[SCAN_THREADS / WAVE];
+    __shared__ uint32_t prior[SCAN_THREADS / WAVE];
+    {  // the particles of the blocks before this one
+        uint32_t t = 0;
+        for (unsigned b = threadIdx.x; b < blockIdx.x; b += SCAN_THREADS) t += totals[b];
+#pragma unroll
+        for (int off = WAVE / 2; off > 0; off >>= 1) t += __shfl_xor(t, off, WAVE);
+        if (threadIdx.x % WAVE == 0) prior[threadIdx.x / WAVE] = t;
+    }
     const size_t base = static_cast<size_t>(blockIdx.x) * SCAN_ITEMS + threadIdx.x * SCAN_PER_THREAD;
     uint32_t v[SCAN_PER_THREAD];
     uint32_t s = 0;
@@ -156,7 +127,8 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_apply(const uint32_t* __r
     }
     if (lane == WAVE - 1) wsum[wave] = incl;
     __syncthreads();
-    uint32_t before = totals[blockIdx.x];
+    uint32_t before = 0;
+    for (int w = 0; w < SCAN_THREADS / WAVE; ++w) before += prior[w];
     for (int w = 0; w < wave; ++w) before += wsum[w];
     uint32_t run = before + incl - s;
 #pragma unroll
@@ -164,6 +136,7 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_apply(const uint32_t* __r
         if (base + k < ncells) start[base + k] = run;
         run += v[k];
     }
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == SCAN_THREADS - 1) start[ncells] = run;
 }
 
 __global__ void __launch_bounds__(256) k_cell_scatter(const int* __restrict__ cell_of, size_t n,
@@ -501,7 +474,6 @@ static void cells_enqueue(md_list* l, md_state* s, double per_length, bool permu
     {
         const int nb = static_cast<int>((l->ncells + SCAN_ITEMS - 1) / SCAN_ITEMS);
         k_scan_totals<<<nb, SCAN_THREADS, 0, st>>>(l->cell_count, l->ncells, l->scan_totals, gate);
-        k_scan_top<<<1, 1024, 0, st>>>(l->scan_totals, nb, l->cell_start + l->ncells, gate);
         k_scan_apply<<<nb, SCAN_THREADS, 0, st>>>(l->cell_count, l->ncells, l->scan_totals, l->cell_start, gate);
     }
     k_cell_scatter<<<blocks_for(n, 256), 256, 0, st>>>(l->cell_of, n, l->cell_start, l->scan_tmp, l->order_raw,
