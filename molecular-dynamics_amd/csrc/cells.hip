@@ -54,6 +54,11 @@ struct CopyBackArgs {
 
 __device__ __forceinline__ bool gated_off(const int* gate) { return gate && *gate == 0; }
 
+// the fixed-order build: lanes per particle and block shape
+constexpr int FIXED_LANES = 16;
+constexpr int FIXED_THREADS = 256;
+constexpr int FIXED_PER_BLOCK = FIXED_THREADS / FIXED_LANES;
+
 // Cell of every particle and the cell histogram, one atomic per run of equal cells.
 __global__ void __launch_bounds__(256) k_cell_count(const double4* __restrict__ pos, size_t n, double per_length,
                                                      int side, int* __restrict__ cell_of,
@@ -434,7 +439,7 @@ void list_prepare(md_list* l, md_state* s) {
         l->rows16_cap = need16;
     }
     const size_t tiles = static_cast<size_t>(tile_count(s->n ? s->n : 1));
-    const size_t stat_blocks = std::max<size_t>(tiles, blocks_for(s->n ? s->n : 1, 256));
+    const size_t stat_blocks = std::max<size_t>(tiles, blocks_for(s->n ? s->n : 1, FIXED_PER_BLOCK));
     if (stat_blocks > l->stats_cap || !l->build_stats) {
         dev_free(l->build_stats);
         l->build_stats = dev_alloc<int2>(stat_blocks);
@@ -606,55 +611,93 @@ void list_enqueue_conditional_rebuild(md_list* l, md_state* s) {
 // Fixed-order list (no re-sort): candidates are the members of the 27 cells taken through the stable cell order,
 // so particle indices never change. Rows hold the j > i neighbours first, then the j < i ones. Also records the
 // build-time positions. Used by the adjoint run, whose reverse sweep needs the same indices at every step.
-__global__ void __launch_bounds__(256) k_list_build_fixed(const double4* __restrict__ pos, size_t n,
-                                                           const int* __restrict__ cell_of,
-                                                           const uint32_t* __restrict__ start,
-                                                           const uint32_t* __restrict__ order, int side, double box,
-                                                           double inv_box, double radius2, int use_cells, int padding,
-                                                           int32_t* __restrict__ rows, int32_t* __restrict__ count,
-                                                           int32_t* __restrict__ half, int2* __restrict__ stats,
-                                                           double4* __restrict__ ref, const int* gate) {
+// FIXED_LANES lanes per particle, one pass over the 27 cells in stencil order (each cell's members in index
+// order, through the stable cell order): the j > i neighbours go straight to the row, ranked within the lane
+// group over the ballot; the j < i ones, in the same visiting order, wait in LDS and follow them.
+
+__global__ void __launch_bounds__(FIXED_THREADS) k_list_build_fixed(const double4* __restrict__ pos, size_t n,
+                                                                    const int* __restrict__ cell_of,
+                                                                    const uint32_t* __restrict__ start,
+                                                                    const uint32_t* __restrict__ order, int side,
+                                                                    double box, double inv_box, double radius2,
+                                                                    int use_cells, int padding,
+                                                                    int32_t* __restrict__ rows,
+                                                                    int32_t* __restrict__ count,
+                                                                    int32_t* __restrict__ half,
+                                                                    int2* __restrict__ stats,
+                                                                    double4* __restrict__ ref, const int* gate) {
     if (gated_off(gate)) return;
-    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    int cnt = 0, hcnt = 0;
+    extern __shared__ int32_t lower_rows[];  // FIXED_PER_BLOCK rows of `padding` entries
+    const int g = threadIdx.x / FIXED_LANES, lane = threadIdx.x % FIXED_LANES;
+    const size_t i = static_cast<size_t>(blockIdx.x) * FIXED_PER_BLOCK + g;
+    const unsigned shift = (threadIdx.x % WAVE) & ~(FIXED_LANES - 1);  // the group's first lane in the wavefront
+    const unsigned below = (1u << lane) - 1u;
+    int32_t* const low = lower_rows + static_cast<size_t>(g) * padding;
+    int cnt = 0, lcnt = 0;  // uniform over the group
     if (i < n) {
         const double4 ri = pos[i];
-        ref[i] = ri;
+        if (lane == 0) ref[i] = ri;
         const uint32_t self = static_cast<uint32_t>(i);
-        auto consider = [&](uint32_t j) {
-            const double4 rj = pos[j];
-            const double dx = min_image(ri.x - rj.x, box, inv_box);
-            const double dy = min_image(ri.y - rj.y, box, inv_box);
-            const double dz = min_image(ri.z - rj.z, box, inv_box);
-            if (dx * dx + dy * dy + dz * dz < radius2) {
-                if (cnt < padding) rows[static_cast<size_t>(cnt) * n + i] = static_cast<int32_t>(j);
-                ++cnt;
+        // one candidate per lane (valid lanes only), classified and ranked
+        auto chunk = [&](uint32_t j, bool valid) {
+            bool near = false;
+            if (valid && j != self) {
+                const double4 rj = pos[j];
+                const double dx = min_image(ri.x - rj.x, box, inv_box);
+                const double dy = min_image(ri.y - rj.y, box, inv_box);
+                const double dz = min_image(ri.z - rj.z, box, inv_box);
+                near = dx * dx + dy * dy + dz * dz < radius2;
             }
+            const bool up = near && j > self, lo = near && j < self;
+            const unsigned mu = static_cast<unsigned>(__ballot(up) >> shift) & 0xFFFFu;
+            const unsigned ml = static_cast<unsigned>(__ballot(lo) >> shift) & 0xFFFFu;
+            if (up) {
+                const int at = cnt + __popc(mu & below);
+                if (at < padding) rows[static_cast<size_t>(at) * n + i] = static_cast<int32_t>(j);
+            }
+            if (lo) {
+                const int at = lcnt + __popc(ml & below);
+                if (at < padding) low[at] = static_cast<int32_t>(j);
+            }
+            cnt += __popc(mu);
+            lcnt += __popc(ml);
         };
-        for (int pass = 0; pass < 2; ++pass) {
-            const bool upper = pass == 0;
-            if (use_cells) {
-                const int c = cell_of[i];
-                const int cx = c / (side * side), cy = (c / side) % side, cz = c % side;
-                for (int ox = -1; ox <= 1; ++ox)
-                    for (int oy = -1; oy <= 1; ++oy)
-                        for (int oz = -1; oz <= 1; ++oz) {
-                            const int nc = (wrap_index(cx + ox, side) * side + wrap_index(cy + oy, side)) * side +
-                                           wrap_index(cz + oz, side);
-                            for (uint32_t k = start[nc]; k < start[nc + 1]; ++k) {
-                                const uint32_t j = order[k];
-                                if (upper ? j > self : j < self) consider(j);
-                            }
+        if (use_cells) {
+            const int c = cell_of[i];
+            const int cx = c / (side * side), cy = (c / side) % side, cz = c % side;
+            for (int ox = -1; ox <= 1; ++ox)
+                for (int oy = -1; oy <= 1; ++oy)
+                    for (int oz = -1; oz <= 1; ++oz) {
+                        const int nc = (wrap_index(cx + ox, side) * side + wrap_index(cy + oy, side)) * side +
+                                       wrap_index(cz + oz, side);
+                        const uint32_t e = start[nc + 1];
+                        for (uint32_t k0 = start[nc]; k0 < e; k0 += FIXED_LANES) {
+                            const uint32_t k = k0 + static_cast<uint32_t>(lane);
+                            chunk(k < e ? order[k] : 0u, k < e);
                         }
-            } else {
-                for (uint32_t j = upper ? self + 1 : 0; j < (upper ? n : self); ++j) consider(j);
+                    }
+        } else {
+            // every particle is a candidate: j > i in increasing j, then j < i in increasing j
+            for (uint32_t j0 = 0; j0 < n; j0 += FIXED_LANES) {
+                const uint32_t j = j0 + static_cast<uint32_t>(lane);
+                chunk(j, j < n);
             }
-            if (upper) hcnt = cnt;
         }
-        count[i] = cnt < padding ? cnt : padding;
-        half[i] = hcnt < padding ? hcnt : padding;
     }
-    block_build_stat<256>(cnt, static_cast<unsigned>(hcnt), stats + blockIdx.x);
+    __syncthreads();  // the group's LDS row is complete (written by its own lanes)
+    const int total = cnt + lcnt;
+    if (i < n) {
+        for (int k = lane; k < lcnt; k += FIXED_LANES) {
+            const int at = cnt + k;
+            if (at < padding) rows[static_cast<size_t>(at) * n + i] = low[k];
+        }
+        if (lane == 0) {
+            count[i] = total < padding ? total : padding;
+            half[i] = cnt < padding ? cnt : padding;
+        }
+    }
+    block_build_stat<FIXED_THREADS>(lane == 0 ? total : 0, lane == 0 ? static_cast<unsigned>(cnt) : 0u,
+                                    stats + blockIdx.x);
 }
 
 static void list_fixed_enqueue(md_list* l, md_state* s, const int* gate) {
@@ -663,11 +706,15 @@ static void list_fixed_enqueue(md_list* l, md_state* s, const int* gate) {
     cells_enqueue(l, s, l->side / s->box, false, gate, true);
     l->tiled = false;
     l->rows32_valid = true;
-    k_list_build_fixed<<<blocks_for(n ? n : 1, 256), 256, 0, st>>>(
+    // each block keeps FIXED_PER_BLOCK rows of j < i neighbours in LDS (64 KB at most)
+    if (static_cast<size_t>(FIXED_PER_BLOCK) * l->padding * sizeof(int32_t) > 65536)
+        fail(MD_ERR_STATE, "fixed-order neighbour list: rows longer than 1024 entries are not supported");
+    const unsigned blocks = blocks_for(n ? n : 1, FIXED_PER_BLOCK);
+    k_list_build_fixed<<<blocks, FIXED_THREADS, sizeof(int32_t) * FIXED_PER_BLOCK * l->padding, st>>>(
         s->pos, n, l->cell_of, l->cell_start, l->order, l->side, s->box, s->inv_box, l->radius2,
         l->side >= 3 ? 1 : 0, l->padding, l->rows, l->count, l->half, l->build_stats, l->ref, gate);
     MD_HIP(hipGetLastError());
-    finish_build_enqueue(l, static_cast<int>(blocks_for(n ? n : 1, 256)), gate);
+    finish_build_enqueue(l, static_cast<int>(blocks), gate);
 }
 
 void list_rebuild_fixed(md_list* l, md_state* s) {
