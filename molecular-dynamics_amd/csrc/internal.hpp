@@ -147,7 +147,7 @@ struct md_list {
     int* flags = nullptr;            // FLAG_COUNT ints on the device
     int2* build_stats = nullptr;     // per build block: longest row, half-row sum (reduced by k_finish_build)
     size_t stats_cap = 0;
-    // tiled view of the same rows (tiled.hip): LDS slots of each tile's staged halo
+    // tiled view of the same rows (tiled.hip): LDS slot byte offsets into each tile's staged halo
     uint16_t* rows16 = nullptr;      // ceil(padding / 32) chunks of 32 entries per particle, chunk-major
     size_t rows16_cap = 0;
     int4* tile_ranges = nullptr;     // tiles * TILE_RANGES (first cell, first particle, first slot, count)

@@ -1,23 +1,24 @@
 // Tiled neighbour list and mixed-precision force kernel: the list build and the force evaluation stage each
 // tile's halo positions in LDS once, instead of gathering every neighbour from L2/MALL.
 //
-// A tile is TILE = 128 consecutive particles of the cell-sorted state, worked by 512 threads: LANES = 4 lanes per
-// particle. Its halo is the union of the 27-cell stencils of the cells it spans, kept as at most RMAX ranges of
-// whole cells (each a contiguous run of particle indices, since particles are sorted by cell). At rho* = 0.8442
-// and rc + skin = 2.8 a halo holds ~1500 particles (~2400 when the tile crosses a cell column); the capacity
-// `hmax` (<= TILE_HALO_MAX = 2560, 24 B each = 60 KB) is set from the density, and a tile that does not fit is
-// marked tile_nr = -1 and handled through global gathers.
+// A tile is TILE = 128 consecutive particles of the cell-sorted state, worked by 512 threads. Its halo is the union
+// of the 27-cell stencils of the cells it spans, kept as at most RMAX ranges of whole cells (each a contiguous run
+// of particle indices, since particles are sorted by cell). At rho* = 0.8442 and rc + skin = 2.8 a halo holds ~1700
+// particles (~2300 when the tile crosses a cell column); the capacity `hmax` (<= TILE_HALO_MAX - 8, x | y | z
+// doubles) is set from the density, and a tile that does not fit is marked tile_nr = -1 and handled through global
+// gathers.
 //
 // A rebuild runs two kernels: k_tile_setup (one wavefront per tile) finds the halo ranges and their LDS slots;
-// k_list_build_tiled stages the halo, tabulates the 27 stencil cells of every cell the tile spans, and tests the
-// candidates. It writes each row twice: as global indices (int32, column-major: what the reference's half-list
-// view, the RDF, the adjoint and the FP64 force kernel read) and as LDS slots of the tile's halo (uint16, tile-major:
-// each tile owns row_len / 32 chunks of 128 particles x 32 entries; entry k of the tile's particle p sits at chunk
-// k / 32, offset 32 p + 8 (k % 4) + (k / 4) % 8, so each of the force kernel's 4 lanes fetches its 8 entries
-// k = lane, lane + 4, ... of a chunk with one 16-byte load, a wavefront's 16 particles read 1 KB contiguously, and
-// chunks are a constant 4096 entries apart). Rows keep the reference's
-// order (j > i in 27-cell stencil order, then j < i increasing): the 4 lanes test 4 consecutive candidates and a
-// wavefront ballot compacts the accepted ones in order.
+// k_list_build_tiled stages the halo, builds per own cell two flat candidate lists (the stencil cells above and
+// below it) and walks them one wavefront per particle, 64 candidates per step, ranking the accepted ones over the
+// ballot so that rows keep the reference's order (j > i in 27-cell stencil order, then j < i increasing). It writes
+// each row as LDS slot byte offsets of the tile's halo (uint16, tile-major: each tile owns row_len / 32 chunks of
+// 128 particles x 32 entries; entry k of the tile's particle p sits at chunk k / 32, offset 32 p + 8 (k % 4) +
+// (k / 4) % 8, so each of the force kernel's 4 lanes fetches its 8 entries k = lane, lane + 4, ... of a chunk with
+// one 16-byte load, a wavefront's 16 particles read 1 KB contiguously, and chunks are a constant 4096 entries
+// apart) and, when a caller needs them, as global indices (int32, column-major: what the reference's half-list
+// view, the RDF, the adjoint and the FP64 force kernel read). k_forces_tiled_mixed walks the slot rows; its STEP
+// form also carries the velocity-Verlet step of the tile's particles.
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
