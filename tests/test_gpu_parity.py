@@ -249,6 +249,31 @@ def test_mixed_run_folded_step_matches_separate_kernels(ctx, monkeypatch, steps,
     assert np.array_equal(ea2[:, 0], eb2[:, 0])
 
 
+def test_mixed_run_folded_step_with_unstaged_tiles(ctx, monkeypatch):
+    """The folded step where some tiles' halos exceed the LDS capacity (108k atoms, skin 0.8: cells of 3.36 sigma hold
+    ~32 particles, so a tile that crosses a cell column has a halo of ~2700 > 2184 slots and takes the unstaged
+    gather path): still bit-equal to the separate-kernel run."""
+    runs = []
+    for unfolded in (True, False):
+        if unfolded:
+            monkeypatch.setenv("MDGPU_UNFOLDED", "1")
+        else:
+            monkeypatch.delenv("MDGPU_UNFOLDED")
+        g = mdgpu.fcc_system(ctx, 30, 0.8442, 0.722, 1)
+        nl = mdgpu.NeighbourList(ctx, 2.5, 0.8)
+        nl.build(g)
+        nl.forces(g, LJ, mdgpu.MD_MIXED)
+        en = nl.run(g, LJ, 0.005, 40, precision=mdgpu.MD_MIXED, use_graph=False)
+        assert nl.info().tiled == 1
+        runs.append((g, nl, en))
+    (ga, la, ea), (gb, lb, eb) = runs
+    assert la.info().rebuilds == lb.info().rebuilds
+    for field in (mdgpu.FIELD_POS, mdgpu.FIELD_VEL, mdgpu.FIELD_FORCE):
+        assert np.array_equal(ga.download(field), gb.download(field))
+    assert np.array_equal(ea[:, [0, 2, 3]], eb[:, [0, 2, 3]])
+    np.testing.assert_allclose(eb[:, 1], ea[:, 1], rtol=1e-12, atol=0)
+
+
 def test_int32_rows_rebuilt_after_mixed_run(ctx):
     """A mixed md_run rebuilds only the 16-bit slot rows on the device; the int32 rows are materialised from them
     when the FP64 kernel needs them, and hold every pair within the cutoff."""
