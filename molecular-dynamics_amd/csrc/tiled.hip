@@ -326,17 +326,18 @@ struct LaneScan {
     uint16_t* row16;  // the particle's slot_row_base
     int cnt;
 
-    template <bool STAGED, bool INTERIOR>
+    // STAGED: positions from the LDS halo, else from global memory; SLOTS: the slot entries (base + offset in the
+    // cell) are written too, which a staged tile needs however its positions were read
+    template <bool STAGED, bool INTERIOR, bool SLOTS = STAGED>
     __device__ __forceinline__ void scan(uint32_t b, uint32_t e, int base) {
         for (uint32_t jb = b; jb < e; jb += LANES) {
             // past the cell's end a lane re-reads its first candidate and rejects it
             const uint32_t j = jb + lane;
             const bool valid = j < e;
             const uint32_t jj = valid ? j : b;
-            int slot = 0;
+            const int slot = SLOTS ? base + static_cast<int>(jj - b) : 0;
             double xj, yj, zj;
             if constexpr (STAGED) {
-                slot = base + static_cast<int>(jj - b);
                 xj = xs[slot];
                 yj = ys[slot];
                 zj = zs[slot];
@@ -373,6 +374,13 @@ struct LaneScan {
 // accepted ones (j > i from U_c, j < i from L_c, slot order being index order) are ranked by mbcnt over the
 // ballot, so the row comes out in the reference's order with no divergence between lanes. Other tiles use the
 // 4-lanes-per-particle scan with a binary search per stencil cell (LaneScan).
+//
+// FH (float halo): the halo is staged as float coordinates in the frame of the tile's first particle (half the LDS
+// of doubles, three blocks per CU). A candidate is accepted when its float r^2 lies below the list radius by more
+// than the rounding bound `band` of the frame, rejected when above it by more, and otherwise tested again exactly in
+// double from global memory with the reference's minimum image; so the rows are those of the double walk. Tiles
+// whose frame is not compact (2 e + 4 cells + 1 >= L, e the largest own frame coordinate) take the unstaged scan.
+template <bool FH>
 __device__ __forceinline__ void list_build_tile(
     int tile, double* lds, const double4* __restrict__ pos, size_t n, const int* __restrict__ cell_of,
     const uint32_t* __restrict__ start, const TileArgs& ta, double box, double inv_box, double radius2, int padding,
@@ -382,6 +390,10 @@ __device__ __forceinline__ void list_build_tile(
     double* xs = lds;  // fixed strides: the three reads of a slot share one address plus immediate offsets
     double* ys = lds + TILE_HALO_MAX;
     double* zs = lds + 2 * TILE_HALO_MAX;
+    float* xf = reinterpret_cast<float*>(lds);  // FH: the same slots as floats
+    float* yf = xf + TILE_HALO_MAX;
+    float* zf = xf + 2 * TILE_HALO_MAX;
+    __shared__ int s_extent;  // FH: largest own frame coordinate (float bits)
     __shared__ int ra[RMAX], rgs[RMAX], rofs[RMAX + 1];
     // the stencil segments are dead once the candidate lists are filled; the walk then keeps each wavefront's
     // current row in the same memory
@@ -416,7 +428,13 @@ __device__ __forceinline__ void list_build_tile(
     if (threadIdx.x == 0) {
         s_outside = 0;
         s_lists_ok = 0;
+        s_extent = 0;
     }
+    const double4 fref = FH ? pos[i0] : make_double4(0.0, 0.0, 0.0, 0.0);  // FH: the frame's origin
+    auto fcoord = [&](double x, double r) {
+        const double t = x - r;
+        return static_cast<float>(fma(-box, rint(t * inv_box), t));
+    };
     // Moving the re-sorted arrays into place: each quarter of the block carries one group of fields of the tile's
     // particles (positions; velocities; forces; mass, id and cell). The loads leave with the tile header; the stores
     // wait until the tile is done (no thread of this kernel reads the destination arrays).
@@ -470,7 +488,19 @@ __device__ __forceinline__ void list_build_tile(
         }
     }
     __syncthreads();
-    if (staged) {
+    if (staged && FH) {
+        stage_halo(pos, hr, nr, rofs[nr], [&](int q, const double4& p) {
+            xf[q] = fcoord(p.x, fref.x);
+            yf[q] = fcoord(p.y, fref.y);
+            zf[q] = fcoord(p.z, fref.z);
+        });
+        if (threadIdx.x == 0) {
+            const float ghost = __builtin_inff();  // its r^2 is above every bound
+            xf[GHOST_SLOT] = ghost;
+            yf[GHOST_SLOT] = ghost;
+            zf[GHOST_SLOT] = ghost;
+        }
+    } else if (staged) {
         int outside = 0;
         stage_halo(pos, hr, nr, rofs[nr], [&](int q, const double4& p) {
             xs[q] = p.x;
@@ -547,9 +577,13 @@ __device__ __forceinline__ void list_build_tile(
     }
     if (use_lists && static_cast<size_t>(threadIdx.x) < i1 - i0) {
         const int cl = my_cell - c_first;
+        const int slot = cl_slot[cl] + (static_cast<int>(i0 + threadIdx.x) - cl_first[cl]);
         p_cl[threadIdx.x] = static_cast<unsigned char>(cl);
-        p_slot[threadIdx.x] =
-            static_cast<uint16_t>((cl_slot[cl] + (static_cast<int>(i0 + threadIdx.x) - cl_first[cl])) * SLOT_BYTES);
+        p_slot[threadIdx.x] = static_cast<uint16_t>(slot * SLOT_BYTES);
+        if constexpr (FH) {  // non-negative floats order as their bit patterns
+            const float e = fmaxf(fabsf(xf[slot]), fmaxf(fabsf(yf[slot]), fabsf(zf[slot])));
+            atomicMax(&s_extent, __float_as_int(e));
+        }
     }
     if (use_lists && static_cast<int>(threadIdx.x) < ncl) {
         const int cl = threadIdx.x, c = c_first + cl;
@@ -572,7 +606,17 @@ __device__ __forceinline__ void list_build_tile(
     }
     __syncthreads();
     // the walk keeps rows in its LDS row buffer; longer rows take the 4-lane scan, which stores directly
-    const bool lists_ok = use_lists && s_lists_ok && padding <= ROWBUF_CAP;
+    // FH: the frame holds minimum images when every own particle lies within e of the origin and 2 e + 4 cells + 1
+    // < L (a candidate is within 2 cells of its own particle); then a float r^2 is within `band` of the exact one,
+    // band bounding the rounding of frame coordinates up to M = e + 2 cells + 1 (ulp(M) / 2 <= M 2^-24 each)
+    const double cell = box / s;
+    const double fe = FH ? static_cast<double>(__int_as_float(s_extent)) : 0.0;
+    const bool compact = !FH || 2.0 * fe + 4.0 * cell + 1.0 < box;
+    const double big_m = fe + 2.0 * cell + 1.0, r_l = sqrt(radius2);
+    const double band = 4.0 * (2.0 * 1.7320508075688772 * r_l * (2.0 * big_m + 3.0 + r_l) + 3.0 * radius2) *
+                        5.9604644775390625e-08;  // 2^-24
+    const float lo = static_cast<float>(radius2 - band), hi = static_cast<float>(radius2 + band);
+    const bool lists_ok = use_lists && s_lists_ok && padding <= ROWBUF_CAP && compact;
     // fill: each lane copies its segment's slots; the list's lanes then pad it to a multiple of 64 with the ghost
 #pragma unroll
     for (int u = 0; u < LIST_ROUNDS; ++u) {
@@ -596,10 +640,27 @@ __device__ __forceinline__ void list_build_tile(
             // uniform over the wavefront: one particle per wavefront
             const int cl = __builtin_amdgcn_readfirstlane(p_cl[p]);
             const int slot_i = __builtin_amdgcn_readfirstlane(p_slot[p]);
-            // an entry is the byte offset of a slot: its x at xs + offset, y and z one and two arrays on
+            // an entry is the byte offset of a slot: its x at xs + offset, y and z one and two arrays on (FH: half the
+            // offset into the float arrays)
             auto coords = [&](int off) { return reinterpret_cast<const double*>(reinterpret_cast<const char*>(xs) + off); };
-            const double* const ci = coords(slot_i);
-            const double xi = ci[0], yi = ci[TILE_HALO_MAX], zi = ci[2 * TILE_HALO_MAX];
+            auto coords_f = [&](int off) {
+                return reinterpret_cast<const float*>(reinterpret_cast<const char*>(xf) + (off >> 1));
+            };
+            double xi = 0.0, yi = 0.0, zi = 0.0;
+            float xif = 0.0f, yif = 0.0f, zif = 0.0f;
+            double4 ri = make_double4(0.0, 0.0, 0.0, 0.0);
+            if constexpr (FH) {
+                const float* const ci = coords_f(slot_i);
+                xif = ci[0];
+                yif = ci[TILE_HALO_MAX];
+                zif = ci[2 * TILE_HALO_MAX];
+                ri = pos[i];
+            } else {
+                const double* const ci = coords(slot_i);
+                xi = ci[0];
+                yi = ci[TILE_HALO_MAX];
+                zi = ci[2 * TILE_HALO_MAX];
+            }
             const bool interior = cl_interior[cl] && !s_outside;
             uint16_t* const row16 = rows16 + slot_row_base(i, row_len);
             int32_t* const row = rows + i;
@@ -609,14 +670,32 @@ __device__ __forceinline__ void list_build_tile(
             // counts the set bits below the lane, on top of the row's count so far) and stored. Two batches are
             // loaded and tested before either stores, so their LDS reads overlap.
             auto near = [&](int sl, auto interior_tag) {
-                const double* const cj = coords(sl);
-                double dx = xi - cj[0], dy = yi - cj[TILE_HALO_MAX], dz = zi - cj[2 * TILE_HALO_MAX];
-                if constexpr (!decltype(interior_tag)::value) {
-                    dx = min_image(dx, box, inv_box);
-                    dy = min_image(dy, box, inv_box);
-                    dz = min_image(dz, box, inv_box);
+                if constexpr (FH) {
+                    const float* const cj = coords_f(sl);
+                    const float dx = xif - cj[0], dy = yif - cj[TILE_HALO_MAX], dz = zif - cj[2 * TILE_HALO_MAX];
+                    const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+                    bool in = r2 < lo;  // the ghost's r^2 is infinite
+                    const bool unsure = !in && !(r2 > hi);
+                    if (__ballot(unsure)) {  // rare: within rounding of the cut-off, the exact test in double
+                        if (unsure) {
+                            const double4 rj = pos[halo_particle(sl / SLOT_BYTES, hr, nr)];
+                            const double ex = min_image(ri.x - rj.x, box, inv_box);
+                            const double ey = min_image(ri.y - rj.y, box, inv_box);
+                            const double ez = min_image(ri.z - rj.z, box, inv_box);
+                            in = ex * ex + ey * ey + ez * ez < radius2;
+                        }
+                    }
+                    return in;
+                } else {
+                    const double* const cj = coords(sl);
+                    double dx = xi - cj[0], dy = yi - cj[TILE_HALO_MAX], dz = zi - cj[2 * TILE_HALO_MAX];
+                    if constexpr (!decltype(interior_tag)::value) {
+                        dx = min_image(dx, box, inv_box);
+                        dy = min_image(dy, box, inv_box);
+                        dz = min_image(dz, box, inv_box);
+                    }
+                    return dx * dx + dy * dy + dz * dz < radius2;  // false for the ghost (NaN)
                 }
-                return dx * dx + dy * dy + dz * dz < radius2;  // false for the ghost (NaN)
             };
             auto commit = [&](int sl, bool accept, unsigned long long mask) {
                 if (accept) {
@@ -655,7 +734,7 @@ __device__ __forceinline__ void list_build_tile(
                      interior_tag);
                 return h;
             };
-            const int hcnt = interior ? both(std::true_type{}) : both(std::false_type{});
+            const int hcnt = FH || interior ? both(std::true_type{}) : both(std::false_type{});
             {
                 // the row leaves LDS as 16-byte chunk words: lane 4 c + w packs entries 32 c + w + 4 t, t = 0..7
                 // (the wavefront's LDS accesses complete in issue order, so every lane's entries are in place)
@@ -704,10 +783,12 @@ __device__ __forceinline__ void list_build_tile(
             sc.cnt = 0;
             const uint32_t self = static_cast<uint32_t>(i);
             int hcnt = 0;
-            auto run = [&](auto staged_tag) {
+            // STAGED: positions from the LDS halo; SLOTS: a staged tile, whose slot rows are written too
+            auto run = [&](auto staged_tag, auto slots_tag) {
                 constexpr bool STAGED = decltype(staged_tag)::value;
+                constexpr bool SLOTS = decltype(slots_tag)::value;
                 auto base_of = [&](int nc) {
-                    if constexpr (STAGED)
+                    if constexpr (SLOTS)
                         return slot_base_of_cell(nc, hr, start, nr);
                     else
                         return 0;
@@ -723,9 +804,9 @@ __device__ __forceinline__ void list_build_tile(
                         const int base = base_of(nc);
                         const uint32_t b = start[nc], e = start[nc + 1];
                         if (nc == c)
-                            sc.scan<STAGED, false>(self + 1, e, base + static_cast<int>(self + 1 - b));
+                            sc.scan<STAGED, false, SLOTS>(self + 1, e, base + static_cast<int>(self + 1 - b));
                         else
-                            sc.scan<STAGED, false>(b, e, base);
+                            sc.scan<STAGED, false, SLOTS>(b, e, base);
                     }
                     hcnt = sc.cnt;
                     const Sorted3 xo(wrap_index(cx - 1, s), cx, wrap_index(cx + 1, s));
@@ -735,19 +816,21 @@ __device__ __forceinline__ void list_build_tile(
                     for (int st = 0; st < 27; ++st) {  // j < i: cells in increasing id
                         const int nc = (xo[st / 9] * s + yo[(st / 3) % 3]) * s + zo[st % 3];
                         if (nc > c) break;
-                        sc.scan<STAGED, false>(start[nc], nc == c ? self : start[nc + 1], base_of(nc));
+                        sc.scan<STAGED, false, SLOTS>(start[nc], nc == c ? self : start[nc + 1], base_of(nc));
                     }
                 } else {
                     // every particle is in the single staged range, at slot = its index
-                    sc.scan<STAGED, false>(self + 1, n32, static_cast<int>(self + 1));
+                    sc.scan<STAGED, false, SLOTS>(self + 1, n32, static_cast<int>(self + 1));
                     hcnt = sc.cnt;
-                    sc.scan<STAGED, false>(0, self, 0);
+                    sc.scan<STAGED, false, SLOTS>(0, self, 0);
                 }
             };
-            if (staged)
-                run(std::true_type{});
+            if (staged && !FH)
+                run(std::true_type{}, std::true_type{});
+            else if (staged)  // FH: the halo holds floats; positions from global memory, slots from the ranges
+                run(std::false_type{}, std::true_type{});
             else
-                run(std::false_type{});
+                run(std::false_type{}, std::false_type{});
             if (lane == 0) {
                 count[i] = sc.cnt < padding ? sc.cnt : padding;
                 half[i] = hcnt < padding ? hcnt : padding;
@@ -773,7 +856,9 @@ __device__ __forceinline__ void list_build_tile(
     block_build_stat<TILE_THREADS>(ep_max, ep_half, stats + tile);
 }
 
-__global__ void __launch_bounds__(TILE_THREADS)
+// six wavefronts per SIMD (three blocks per CU, which the float halo's LDS allows) cap the registers at 80
+template <bool FH>
+__global__ void __launch_bounds__(TILE_THREADS, 6)
     k_list_build_tiled(const double4* __restrict__ pos, size_t n, const int* __restrict__ cell_of,
                        const uint32_t* __restrict__ start, TileArgs ta, double box, double inv_box, double radius2,
                        int padding, int write32, int row_len, int32_t* __restrict__ rows,
@@ -782,8 +867,9 @@ __global__ void __launch_bounds__(TILE_THREADS)
                        const int* __restrict__ tile_nr, TiledCopyBack cb, const int* gate) {
     if (gate && *gate == 0) return;
     extern __shared__ double lds[];
-    list_build_tile(xcd_item(static_cast<int>(blockIdx.x), ta.tiles), lds, pos, n, cell_of, start, ta, box, inv_box,
-                    radius2, padding, write32, row_len, rows, rows16, count, half, stats, tile_ranges, tile_nr, cb);
+    list_build_tile<FH>(xcd_item(static_cast<int>(blockIdx.x), ta.tiles), lds, pos, n, cell_of, start, ta, box,
+                        inv_box, radius2, padding, write32, row_len, rows, rows16, count, half, stats, tile_ranges,
+                        tile_nr, cb);
 }
 
 // int32 rows from the 16-bit slot rows (builds that wrote only the slots): slot -> particle through the tile's
@@ -1079,9 +1165,15 @@ void tiled_list_enqueue(md_list* l, md_state* s, bool from_alt, const int* gate,
     hipLaunchKernelGGL(k_tile_setup, dim3((tiles + SETUP_TILES - 1) / SETUP_TILES), dim3(SETUP_TILES * WAVE), 0, st,
                        n, cells, l->cell_start, ta.side, ta.use_cells, ta.hmax, tiles, flat_cells, l->tile_ranges,
                        l->tile_nr, gate);
-    hipLaunchKernelGGL(k_list_build_tiled, dim3(tiles), dim3(TILE_THREADS),
-                       static_cast<size_t>(TILE_HALO_MAX) * 3 * sizeof(double), st, from_alt ? s->pos_alt : s->pos,
-                       n, cells, l->cell_start, ta, s->box, s->inv_box, l->radius2, l->padding, write32 ? 1 : 0,
+    // the float halo where the grid is wide enough for compact frames and the cells dense enough for the
+    // candidate lists (MDGPU_BUILD_F64=1 keeps the double halo)
+    const char* f64 = std::getenv("MDGPU_BUILD_F64");
+    const bool fh = l->side >= 20 && static_cast<double>(n) >= 6.0 * static_cast<double>(l->ncells) &&
+                    !(f64 && f64[0] == '1');
+    auto kernel = fh ? k_list_build_tiled<true> : k_list_build_tiled<false>;
+    const size_t lds = static_cast<size_t>(TILE_HALO_MAX) * 3 * (fh ? sizeof(float) : sizeof(double));
+    hipLaunchKernelGGL(kernel, dim3(tiles), dim3(TILE_THREADS), lds, st, from_alt ? s->pos_alt : s->pos, n, cells,
+                       l->cell_start, ta, s->box, s->inv_box, l->radius2, l->padding, write32 ? 1 : 0,
                        slot_row_len(l->padding), l->rows, l->rows16, l->count, l->half, l->build_stats,
                        l->tile_ranges, l->tile_nr, cb, gate);
     MD_HIP(hipGetLastError());

@@ -274,6 +274,34 @@ def test_mixed_run_folded_step_with_unstaged_tiles(ctx, monkeypatch):
     np.testing.assert_allclose(eb[:, 1], ea[:, 1], rtol=1e-12, atol=0)
 
 
+def test_float_halo_build_matches_double_build(ctx, monkeypatch):
+    """A 256k-atom perturbed FCC state has 24 cells per side, so its tiled build stages the halo as float frame
+    coordinates and re-tests in double the candidates within rounding of the list radius. Its lists equal those of the
+    double-halo build (MDGPU_BUILD_F64=1) entry for entry; so do the mixed forces read through the slot rows and a
+    60-step mixed run with its device-side rebuilds."""
+    out = []
+    for f64 in ("1", "0"):
+        monkeypatch.setenv("MDGPU_BUILD_F64", f64)
+        g = mdgpu.fcc_system(ctx, 40, 0.8442, 0.722, 1)
+        pos = g.download(mdgpu.FIELD_POS)
+        pos[:, :3] = np.mod(pos[:, :3] + np.random.default_rng(7).normal(scale=0.06, size=(len(pos), 3)), g.box)
+        g.upload(mdgpu.FIELD_POS, pos)
+        nl = mdgpu.NeighbourList(ctx, 2.5, 0.3)
+        nl.build(g)
+        assert nl.info().cells_per_side >= 20 and nl.info().tiled == 1
+        counts, half, rows = nl.full()
+        r = nl.forces(g, LJ, mdgpu.MD_MIXED)
+        f = g.download(mdgpu.FIELD_FORCE)
+        en = nl.run(g, LJ, 0.005, 60, precision=mdgpu.MD_MIXED, use_graph=False)
+        out.append((counts, half, rows, f, r.potential, en, g.download(mdgpu.FIELD_POS), nl.info().rebuilds))
+    a, b = out
+    for k in range(3):
+        assert np.array_equal(a[k], b[k])
+    assert np.array_equal(a[3], b[3]) and a[4] == b[4]
+    assert a[7] == b[7] and a[7] > 1
+    assert np.array_equal(a[5], b[5]) and np.array_equal(a[6], b[6])
+
+
 def test_int32_rows_rebuilt_after_mixed_run(ctx):
     """A mixed md_run rebuilds only the 16-bit slot rows on the device; the int32 rows are materialised from them
     when the FP64 kernel needs them, and hold every pair within the cutoff."""
