@@ -951,6 +951,9 @@ __global__ void __launch_bounds__(TILE_THREADS)
     const HaloRanges hr{ra, rgs, rofs};
     const int tile = xcd_item(static_cast<int>(blockIdx.x), fr.tiles);
     const int raw_nr = tile_nr[tile];
+    // the halo ranges, loaded together with the header (every tile owns RMAX entries; those past nr are unused)
+    const int4 rr = static_cast<int>(threadIdx.x) < RMAX ? tile_ranges[static_cast<size_t>(tile) * RMAX + threadIdx.x]
+                                                         : make_int4(0, 0, 0, 0);
     const int nr = tile_range_count(raw_nr);
     const bool staged = nr > 0;
     const bool flat = tile_is_flat(raw_nr);
@@ -960,8 +963,27 @@ __global__ void __launch_bounds__(TILE_THREADS)
     const bool active = i < n;
     const int m = active ? count[i] : 0;  // row length, loaded first: the slot loads below depend on it
     const double4 ref = pos[i0];
+    // the particle's own frame coordinates, its position loaded with the header
+    double xi = 0.0, yi = 0.0, zi = 0.0;
+    if (active) {
+        const double4 ri = pos[i];
+        if (flat) {
+            xi = ri.x - ref.x;
+            yi = ri.y - ref.y;
+            zi = ri.z - ref.z;
+        } else {
+            xi = frame_coord(ri.x, ref.x, fr);
+            yi = frame_coord(ri.y, ref.y, fr);
+            zi = frame_coord(ri.z, ref.z, fr);
+        }
+    }
     if (threadIdx.x == 0) extent_bits = 0;
-    if (staged) load_ranges(tile_ranges, tile, nr, hr);
+    if (staged && static_cast<int>(threadIdx.x) < nr) {
+        ra[threadIdx.x] = rr.x;
+        rgs[threadIdx.x] = rr.y;
+        rofs[threadIdx.x] = rr.z;
+        if (static_cast<int>(threadIdx.x) == nr - 1) rofs[nr] = rr.z + rr.w;
+    }
     __syncthreads();
     if (staged && flat) {
         stage_halo(pos, hr, nr, rofs[nr], [&](int q, const double4& p) {
@@ -985,22 +1007,9 @@ __global__ void __launch_bounds__(TILE_THREADS)
 #pragma unroll
     for (int c = 0; c < PRE_CHUNKS; ++c)
         pre[c] = staged && c < chunks ? rp[c * stride] : make_uint4(0u, 0u, 0u, 0u);
-    double xi = 0.0, yi = 0.0, zi = 0.0;
-    if (active) {
-        const double4 ri = pos[i];
-        if (flat) {
-            xi = ri.x - ref.x;
-            yi = ri.y - ref.y;
-            zi = ri.z - ref.z;
-        } else {
-            xi = frame_coord(ri.x, ref.x, fr);
-            yi = frame_coord(ri.y, ref.y, fr);
-            zi = frame_coord(ri.z, ref.z, fr);
-        }
-        if (lane == 0) {  // how far the tile's own particles lie from the reference particle
-            const float e = static_cast<float>(fmax(fabs(xi), fmax(fabs(yi), fabs(zi))));
-            atomicMax(&extent_bits, __float_as_int(e));
-        }
+    if (active && lane == 0) {  // how far the tile's own particles lie from the reference particle
+        const float e = static_cast<float>(fmax(fabs(xi), fmax(fabs(yi), fabs(zi))));
+        atomicMax(&extent_bits, __float_as_int(e));
     }
     __syncthreads();
     const float e = __int_as_float(extent_bits);
