@@ -615,7 +615,7 @@ __device__ __forceinline__ void list_build_tile(
     const double big_m = fe + 2.0 * cell + 1.0, r_l = sqrt(radius2);
     const double band = 4.0 * (2.0 * 1.7320508075688772 * r_l * (2.0 * big_m + 3.0 + r_l) + 3.0 * radius2) *
                         5.9604644775390625e-08;  // 2^-24
-    const float lo = static_cast<float>(radius2 - band), hi = static_cast<float>(radius2 + band);
+    const float r2c = static_cast<float>(radius2), bandf = static_cast<float>(band);
     const bool lists_ok = use_lists && s_lists_ok && padding <= ROWBUF_CAP && compact;
     // fill: each lane copies its segment's slots; the list's lanes then pad it to a multiple of 64 with the ghost
 #pragma unroll
@@ -669,33 +669,36 @@ __device__ __forceinline__ void list_build_tile(
             // A batch is 64 candidates: each lane tests one, the accepted ones are ranked over the ballot (mbcnt
             // counts the set bits below the lane, on top of the row's count so far) and stored. Two batches are
             // loaded and tested before either stores, so their LDS reads overlap.
-            auto near = [&](int sl, auto interior_tag) {
-                if constexpr (FH) {
-                    const float* const cj = coords_f(sl);
-                    const float dx = xif - cj[0], dy = yif - cj[TILE_HALO_MAX], dz = zif - cj[2 * TILE_HALO_MAX];
-                    const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-                    bool in = r2 < lo;  // the ghost's r^2 is infinite
-                    const bool unsure = !in && !(r2 > hi);
-                    if (__ballot(unsure)) {  // rare: within rounding of the cut-off, the exact test in double
-                        if (unsure) {
-                            const double4 rj = pos[halo_particle(sl / SLOT_BYTES, hr, nr)];
-                            const double ex = min_image(ri.x - rj.x, box, inv_box);
-                            const double ey = min_image(ri.y - rj.y, box, inv_box);
-                            const double ez = min_image(ri.z - rj.z, box, inv_box);
-                            in = ex * ex + ey * ey + ez * ez < radius2;
-                        }
-                    }
-                    return in;
-                } else {
-                    const double* const cj = coords(sl);
-                    double dx = xi - cj[0], dy = yi - cj[TILE_HALO_MAX], dz = zi - cj[2 * TILE_HALO_MAX];
-                    if constexpr (!decltype(interior_tag)::value) {
-                        dx = min_image(dx, box, inv_box);
-                        dy = min_image(dy, box, inv_box);
-                        dz = min_image(dz, box, inv_box);
-                    }
-                    return dx * dx + dy * dy + dz * dz < radius2;  // false for the ghost (NaN)
+            // FH: d = float r^2 minus the list radius squared (infinite for the ghost); a candidate is inside when
+            // d < -band, unsure when |d| <= band (both from the same d, so every candidate is one of inside, unsure,
+            // outside), and an unsure one is decided by the exact test in double
+            auto dist_f = [&](int sl) {
+                const float* const cj = coords_f(sl);
+                const float dx = xif - cj[0], dy = yif - cj[TILE_HALO_MAX], dz = zif - cj[2 * TILE_HALO_MAX];
+                return fmaf(dx, dx, fmaf(dy, dy, dz * dz)) - r2c;
+            };
+            auto unsure_f = [&](float d) { return fabsf(d) <= bandf; };
+            auto inside_f = [&](float d) { return d < -bandf; };
+            auto resolve_f = [&](int sl, float d, bool unsure) {
+                bool in = inside_f(d);
+                if (unsure) {
+                    const double4 rj = pos[halo_particle(sl / SLOT_BYTES, hr, nr)];
+                    const double ex = min_image(ri.x - rj.x, box, inv_box);
+                    const double ey = min_image(ri.y - rj.y, box, inv_box);
+                    const double ez = min_image(ri.z - rj.z, box, inv_box);
+                    in = ex * ex + ey * ey + ez * ez < radius2;
                 }
+                return in;
+            };
+            auto near = [&](int sl, auto interior_tag) {  // the double halo
+                const double* const cj = coords(sl);
+                double dx = xi - cj[0], dy = yi - cj[TILE_HALO_MAX], dz = zi - cj[2 * TILE_HALO_MAX];
+                if constexpr (!decltype(interior_tag)::value) {
+                    dx = min_image(dx, box, inv_box);
+                    dy = min_image(dy, box, inv_box);
+                    dz = min_image(dz, box, inv_box);
+                }
+                return dx * dx + dy * dy + dz * dz < radius2;  // false for the ghost (NaN)
             };
             auto commit = [&](int sl, bool accept, unsigned long long mask) {
                 if (accept) {
@@ -711,6 +714,36 @@ __device__ __forceinline__ void list_build_tile(
             };
             auto walk = [&](int off, int len, bool upper, auto interior_tag) {
                 int q0 = 0;
+                if constexpr (FH) {
+                    // A pair of batches takes the plain path unless a lane is within rounding of the cut-off
+                    // (a few in a thousand batches), which re-tests those lanes exactly.
+                    for (; q0 + 2 * WAVE <= len; q0 += 2 * WAVE) {
+                        const int sa = lists[off + q0 + lane], sb = lists[off + q0 + WAVE + lane];
+                        const float ra = dist_f(sa), rb2 = dist_f(sb);
+                        const bool ua = unsure_f(ra), ub = unsure_f(rb2);
+                        const bool oa = upper ? sa > slot_i : sa < slot_i, ob = upper ? sb > slot_i : sb < slot_i;
+                        if ((__ballot(ua) | __ballot(ub)) == 0) {
+                            const bool na = inside_f(ra), nb = inside_f(rb2);
+                            const unsigned long long ma = __ballot(na) & __ballot(oa);
+                            const unsigned long long mb = __ballot(nb) & __ballot(ob);
+                            commit(sa, na && oa, ma);
+                            commit(sb, nb && ob, mb);
+                        } else {
+                            const bool na = resolve_f(sa, ra, ua), nb = resolve_f(sb, rb2, ub);
+                            const unsigned long long ma = __ballot(na && oa), mb = __ballot(nb && ob);
+                            commit(sa, na && oa, ma);
+                            commit(sb, nb && ob, mb);
+                        }
+                    }
+                    if (q0 < len) {
+                        const int sa = lists[off + q0 + lane];
+                        const float ra = dist_f(sa);
+                        const bool ua = unsure_f(ra), oa = upper ? sa > slot_i : sa < slot_i;
+                        const bool na = __ballot(ua) == 0 ? inside_f(ra) : resolve_f(sa, ra, ua);
+                        commit(sa, na && oa, __ballot(na && oa));
+                    }
+                    return;
+                }
                 for (; q0 + 2 * WAVE <= len; q0 += 2 * WAVE) {
                     const int sa = lists[off + q0 + lane], sb = lists[off + q0 + WAVE + lane];
                     const bool na = near(sa, interior_tag), nb = near(sb, interior_tag);
