@@ -394,6 +394,9 @@ __device__ __forceinline__ void list_build_tile(
     float* yf = xf + TILE_HALO_MAX;
     float* zf = xf + 2 * TILE_HALO_MAX;
     __shared__ int s_extent;  // FH: largest own frame coordinate (float bits)
+    // the candidate lists, the own slots and the row buffer hold byte offsets into the staged halo: of floats with
+    // FH (converted to the slot rows' double offsets when a row is written out), of doubles otherwise
+    constexpr int LIST_BYTES = FH ? 4 : SLOT_BYTES;
     __shared__ int ra[RMAX], rgs[RMAX], rofs[RMAX + 1];
     // the stencil segments are dead once the candidate lists are filled; the walk then keeps each wavefront's
     // current row in the same memory
@@ -579,7 +582,7 @@ __device__ __forceinline__ void list_build_tile(
         const int cl = my_cell - c_first;
         const int slot = cl_slot[cl] + (static_cast<int>(i0 + threadIdx.x) - cl_first[cl]);
         p_cl[threadIdx.x] = static_cast<unsigned char>(cl);
-        p_slot[threadIdx.x] = static_cast<uint16_t>(slot * SLOT_BYTES);
+        p_slot[threadIdx.x] = static_cast<uint16_t>(slot * LIST_BYTES);
         if constexpr (FH) {  // non-negative floats order as their bit patterns
             const float e = fmaxf(fabsf(xf[slot]), fmaxf(fabsf(yf[slot]), fabsf(zf[slot])));
             atomicMax(&s_extent, __float_as_int(e));
@@ -625,10 +628,10 @@ __device__ __forceinline__ void list_build_tile(
         const int cl = L >> 1, lower = L & 1;
         const int off = lst[cl][lower ? 2 : 0];
         for (int m = 0; m < seg_len[u]; ++m)
-            lists[off + seg_off[u] + m] = static_cast<uint16_t>((seg_base[u] + m) * SLOT_BYTES);
+            lists[off + seg_off[u] + m] = static_cast<uint16_t>((seg_base[u] + m) * LIST_BYTES);
         const int padded = lst[cl][lower ? 3 : 1];
         for (int q = list_used[u] + hl; q < padded; q += HALF)
-            lists[off + q] = static_cast<uint16_t>(GHOST_SLOT * SLOT_BYTES);
+            lists[off + q] = static_cast<uint16_t>(GHOST_SLOT * LIST_BYTES);
     }
     __syncthreads();
     int ep_max = 0;      // longest row seen by this lane's wavefront / lane group
@@ -644,7 +647,7 @@ __device__ __forceinline__ void list_build_tile(
             // offset into the float arrays)
             auto coords = [&](int off) { return reinterpret_cast<const double*>(reinterpret_cast<const char*>(xs) + off); };
             auto coords_f = [&](int off) {
-                return reinterpret_cast<const float*>(reinterpret_cast<const char*>(xf) + (off >> 1));
+                return reinterpret_cast<const float*>(reinterpret_cast<const char*>(xf) + off);
             };
             double xi = 0.0, yi = 0.0, zi = 0.0;
             float xif = 0.0f, yif = 0.0f, zif = 0.0f;
@@ -682,7 +685,7 @@ __device__ __forceinline__ void list_build_tile(
             auto resolve_f = [&](int sl, float d, bool unsure) {
                 bool in = inside_f(d);
                 if (unsure) {
-                    const double4 rj = pos[halo_particle(sl / SLOT_BYTES, hr, nr)];
+                    const double4 rj = pos[halo_particle(sl / LIST_BYTES, hr, nr)];
                     const double ex = min_image(ri.x - rj.x, box, inv_box);
                     const double ey = min_image(ri.y - rj.y, box, inv_box);
                     const double ez = min_image(ri.z - rj.z, box, inv_box);
@@ -707,7 +710,7 @@ __device__ __forceinline__ void list_build_tile(
                         __builtin_amdgcn_mbcnt_lo(static_cast<unsigned>(mask), static_cast<unsigned>(cnt)));
                     if (at < static_cast<unsigned>(padding)) {
                         rb[at] = static_cast<uint16_t>(sl);
-                        if (write32) row[static_cast<uint64_t>(at) * n] = halo_particle(sl / SLOT_BYTES, hr, nr);
+                        if (write32) row[static_cast<uint64_t>(at) * n] = halo_particle(sl / LIST_BYTES, hr, nr);
                     }
                 }
                 cnt += __popcll(mask);
@@ -781,6 +784,13 @@ __device__ __forceinline__ void list_build_tile(
                     v.y = static_cast<unsigned>(e[8]) | (static_cast<unsigned>(e[12]) << 16);
                     v.z = static_cast<unsigned>(e[16]) | (static_cast<unsigned>(e[20]) << 16);
                     v.w = static_cast<unsigned>(e[24]) | (static_cast<unsigned>(e[28]) << 16);
+                    if constexpr (FH) {  // float offsets to double offsets: both halves doubled (each < 2^15)
+                        static_assert(TILE_HALO_MAX * 4 < 0x8000, "a doubled float offset stays in its half");
+                        v.x <<= 1;
+                        v.y <<= 1;
+                        v.z <<= 1;
+                        v.w <<= 1;
+                    }
                     *reinterpret_cast<uint4*>(row16 + c * CHUNK_STRIDE + 8 * w) = v;
                 }
                 __builtin_amdgcn_wave_barrier();
