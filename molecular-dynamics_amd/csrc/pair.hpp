@@ -74,19 +74,19 @@ __device__ __forceinline__ void mixed_pair_term(double dx, double dy, double dz,
     }
 }
 
-// Per-lane sums of the tiled mixed kernel over a particle's full row: force and virial in double, the potential
-// energy without its shift in float and the inside count as an integer. The reference sums energy_i and inside_i in
-// float over the half row j > i (Forcefield.cpp:397-401) and adds them in double across particles; here every pair
-// is met from both of its ends, and the caller takes the shift once per inside pair (U = sum 4 (s12 - s6) -
-// shift * count) and halves the tile's U, W and count: the same sums, with the float rounding taken in another
-// order. The virial, which cancels strongly, stays in double.
+// Per-lane sums of the tiled mixed kernel over a particle's full row, with the constant factors left for the end:
+// g = sum t inv_r2 d (in double), T = sum t and S = sum s12 (in float), where t = 2 s12 - s6, and the inside count
+// as an integer. Then F = 24 eps g, W = sum f r^2 = 24 eps T and U = 4 eps (T - S) - shift * count. The reference
+// sums energy_i, virial_i and inside_i in float over the half row j > i (Forcefield.cpp:397-401) and adds them in
+// double across particles; here every pair is met from both of its ends, so the caller halves the tile's U, W and
+// count: the same sums, with the float rounding taken in another order.
 struct PairSumsF {
-    double fx = 0.0, fy = 0.0, fz = 0.0, w = 0.0;
-    float u = 0.0f;
+    double gx = 0.0, gy = 0.0, gz = 0.0;
+    float t = 0.0f, s = 0.0f;
     int c = 0;
 };
 
-// One pair of a full row. Outside rc inv_r2 is zero, which makes the force and the energy term zero.
+// One pair of a full row. Outside rc inv_r2 is zero, which makes the force and the energy terms zero.
 __device__ __forceinline__ void mixed_pair_term_f(double dx, double dy, double dz, const LJConst& lj, PairSumsF& a) {
     const double r2d = fma(dx, dx, fma(dy, dy, dz * dz));
     const float r2 = static_cast<float>(r2d);
@@ -96,12 +96,13 @@ __device__ __forceinline__ void mixed_pair_term_f(double dx, double dy, double d
     const float s2 = lj.sig2f * inv_r2;
     const float s6 = s2 * s2 * s2;
     const float s12 = s6 * s6;
-    const double f = static_cast<double>(lj.c24f * fmaf(2.0f, s12, -s6) * inv_r2);
-    a.fx = fma(f, dx, a.fx);
-    a.fy = fma(f, dy, a.fy);
-    a.fz = fma(f, dz, a.fz);
-    a.u = fmaf(lj.c4f, s12 - s6, a.u);
-    a.w = fma(f, r2d, a.w);
+    const float t = fmaf(2.0f, s12, -s6);
+    const double g = static_cast<double>(t * inv_r2);
+    a.gx = fma(g, dx, a.gx);
+    a.gy = fma(g, dy, a.gy);
+    a.gz = fma(g, dz, a.gz);
+    a.t += t;
+    a.s += s12;
     a.c += inside ? 1 : 0;
 }
 

@@ -1064,10 +1064,10 @@ __global__ void __launch_bounds__(TILE_THREADS)
                 constexpr bool W = decltype(wrap)::value;
                 auto chunk_pairs = [&](const uint4& cw, int c) {
                     const unsigned words[4] = {cw.x, cw.y, cw.z, cw.w};
+                    const int rem = m - c * CHUNK - lane;  // entry c * CHUNK + t * LANES + lane exists iff t * LANES < rem
 #pragma unroll
                     for (int t = 0; t < 8; ++t) {
-                        const int k = c * CHUNK + t * LANES + lane;
-                        if (k < m) {
+                        if (t * LANES < rem) {
                             const unsigned off = __builtin_amdgcn_ubfe(words[t >> 1], (t & 1) * 16, 16);
                             const double* xj = reinterpret_cast<const double*>(reinterpret_cast<const char*>(xs) + off);
                             double dx = xi - xj[0], dy = yi - xj[TILE_HALO_MAX], dz = zi - xj[2 * TILE_HALO_MAX];
@@ -1097,26 +1097,27 @@ __global__ void __launch_bounds__(TILE_THREADS)
     // the particle's four lanes are adjacent in the wavefront
 #pragma unroll
     for (int off = 1; off < LANES; off <<= 1) {
-        a.fx += __shfl_xor(a.fx, off, WAVE);
-        a.fy += __shfl_xor(a.fy, off, WAVE);
-        a.fz += __shfl_xor(a.fz, off, WAVE);
-        a.u += __shfl_xor(a.u, off, WAVE);
-        a.w += __shfl_xor(a.w, off, WAVE);
+        a.gx += __shfl_xor(a.gx, off, WAVE);
+        a.gy += __shfl_xor(a.gy, off, WAVE);
+        a.gz += __shfl_xor(a.gz, off, WAVE);
+        a.t += __shfl_xor(a.t, off, WAVE);
+        a.s += __shfl_xor(a.s, off, WAVE);
         a.c += __shfl_xor(a.c, off, WAVE);
     }
+    const double c24 = static_cast<double>(lj.c24f);
+    const double fi[3] = {c24 * a.gx, c24 * a.gy, c24 * a.gz};
     const int pl = threadIdx.x >> 2;
     double ke = 0.0;
     if constexpr (STEP) {
         if (lane == 0 && active) {
             const double m = fs.mass[i];
-            const double f[3] = {a.fx, a.fy, a.fz};
             double v[3];
 #pragma unroll
-            for (int c = 0; c < 3; ++c) v[c] = fs.vel[c * n + i] + fs.half_dt * f[c] / m;
+            for (int c = 0; c < 3; ++c) v[c] = fs.vel[c * n + i] + fs.half_dt * fi[c] / m;
             ke = 0.5 * m * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                v[c] = v[c] + fs.half_dt * f[c] / m;
+                v[c] = v[c] + fs.half_dt * fi[c] / m;
                 fs.vel[c * n + i] = v[c];
             }
             double4 r = pos[i];
@@ -1134,14 +1135,15 @@ __global__ void __launch_bounds__(TILE_THREADS)
     __syncthreads();  // every wavefront is done with the halo
     if (lane == 0) {
         if (active && !STEP) {
-            frc[i] = a.fx;
-            frc[n + i] = a.fy;
-            frc[2 * n + i] = a.fz;
+            frc[i] = fi[0];
+            frc[n + i] = fi[1];
+            frc[2 * n + i] = fi[2];
         }
         // every pair was met from both ends: half of each full-row sum (zero for the slots past n); the potential's
         // shift once per inside pair
-        e_part[0][pl] = 0.5 * (static_cast<double>(a.u) - static_cast<double>(lj.shiftf) * a.c);
-        e_part[1][pl] = 0.5 * a.w;
+        const double u = static_cast<double>(lj.c4f) * (static_cast<double>(a.t) - static_cast<double>(a.s));
+        e_part[0][pl] = 0.5 * (u - static_cast<double>(lj.shiftf) * a.c);
+        e_part[1][pl] = 0.5 * c24 * static_cast<double>(a.t);
         e_part[2][pl] = 0.5 * static_cast<double>(a.c);
         if constexpr (STEP) e_part[3][pl] = ke;
     }
