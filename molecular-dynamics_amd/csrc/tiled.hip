@@ -1104,41 +1104,44 @@ __global__ void __launch_bounds__(TILE_THREADS)
         a.s += __shfl_xor(a.s, off, WAVE);
         a.c += __shfl_xor(a.c, off, WAVE);
     }
+    // every lane now holds the particle's sums; lane c (c < 3) takes force component c (lane 3 repeats z)
+    const int comp = lane < 3 ? lane : 2;
     const double c24 = static_cast<double>(lj.c24f);
-    const double fi[3] = {c24 * a.gx, c24 * a.gy, c24 * a.gz};
+    const double fc = c24 * (comp == 0 ? a.gx : (comp == 1 ? a.gy : a.gz));
     const int pl = threadIdx.x >> 2;
     double ke = 0.0;
     if constexpr (STEP) {
-        if (lane == 0 && active) {
-            const double m = fs.mass[i];
-            double v[3];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) v[c] = fs.vel[c * n + i] + fs.half_dt * fi[c] / m;
-            ke = 0.5 * m * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                v[c] = v[c] + fs.half_dt * fi[c] / m;
-                fs.vel[c * n + i] = v[c];
+        // the arithmetic of k_kick_kick_drift, one component per lane; the kinetic energy and the displacement test
+        // gather the three components on lane 0 and add them in x, y, z order
+        double m = 0.0, kick = 0.0, v = 0.0, d = 0.0, r = 0.0;
+        const double* pi = reinterpret_cast<const double*>(pos + i);
+        if (active) {
+            m = fs.mass[i];
+            kick = fs.half_dt * fc / m;
+            v = fs.vel[comp * n + i] + kick;  // closing kick of this step
+        }
+        const double vy = __shfl_xor(v, 1, WAVE), vz = __shfl_xor(v, 2, WAVE);
+        if (active) {
+            ke = 0.5 * m * (v * v + vy * vy + vz * vz);  // lane 0's is the particle's
+            v = v + kick;                                 // opening kick of the next step
+            r = wrap_coord(pi[comp] + fs.dt * v, fr.box, fr.inv_box);
+            d = min_image(r - reinterpret_cast<const double*>(fs.ref + i)[comp], fr.box, fr.inv_box);
+        }
+        const double dy = __shfl_xor(d, 1, WAVE), dz = __shfl_xor(d, 2, WAVE);
+        if (active) {
+            double* next = reinterpret_cast<double*>(fs.pos_next + i);
+            if (lane < 3) {
+                fs.vel[comp * n + i] = v;
+                next[lane] = r;
+            } else {
+                next[3] = pi[3];
             }
-            double4 r = pos[i];
-            r.x = wrap_coord(r.x + fs.dt * v[0], fr.box, fr.inv_box);
-            r.y = wrap_coord(r.y + fs.dt * v[1], fr.box, fr.inv_box);
-            r.z = wrap_coord(r.z + fs.dt * v[2], fr.box, fr.inv_box);
-            fs.pos_next[i] = r;
-            const double4 q = fs.ref[i];
-            const double dx = min_image(r.x - q.x, fr.box, fr.inv_box);
-            const double dy = min_image(r.y - q.y, fr.box, fr.inv_box);
-            const double dz = min_image(r.z - q.z, fr.box, fr.inv_box);
-            if (dx * dx + dy * dy + dz * dz > fs.max_disp2) *fs.stale_next = 1;
+            if (lane == 0 && d * d + dy * dy + dz * dz > fs.max_disp2) *fs.stale_next = 1;
         }
     }
+    if (active && !STEP && lane < 3) frc[lane * n + i] = fc;
     __syncthreads();  // every wavefront is done with the halo
     if (lane == 0) {
-        if (active && !STEP) {
-            frc[i] = fi[0];
-            frc[n + i] = fi[1];
-            frc[2 * n + i] = fi[2];
-        }
         // every pair was met from both ends: half of each full-row sum (zero for the slots past n); the potential's
         // shift once per inside pair
         const double u = static_cast<double>(lj.c4f) * (static_cast<double>(a.t) - static_cast<double>(a.s));
