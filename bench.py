@@ -5,7 +5,8 @@ A "step" is one velocity-Verlet step of the whole system (kick-drift, displaceme
 when stale, forces, kick), driven on the device by md_run. Inputs are generated on the host by the library's own
 createFCCLattice / initializeVelocities, uploaded, and the first list built before the timed region.
 
-Printed: one JSON line with the throughput, the force kernel's HBM roofline (event-timed inside the timed region)
+Printed: one JSON line with the throughput, the force kernel's HBM roofline (event-timed inside the timed region;
+with the kernel's VALU issue floor beside it, from the instruction count profiles/force_kernel_traffic.json holds)
 and the reference's own OpenMP path (oracle/_ref/libmdref_fast.so) timed on this host on a bounded sample.
 """
 from __future__ import annotations
@@ -312,16 +313,24 @@ def main():
         avg_ms = timing[0] / args.steps
         achieved = bytes_per_atom * n / (avg_ms * 1e-3) / 1e9
         traffic = None
+        valu = None
         if os.path.exists(TRAFFIC_FILE):
             with open(TRAFFIC_FILE) as f:
                 tr = json.load(f)
             if tr.get("workload") == args.workload:
                 traffic = tr.get("hbm_bytes_per_launch")
+                valu = tr.get("valu_instructions_per_launch")
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                     "kernel": ("k_forces_tiled_mixed<true>" if fold else "k_forces_tiled_mixed" if tiled
                                else "k_forces_mixed") if prec == mdgpu.MD_MIXED else "k_forces_fp64",
                     "avg_launch_ms": round(avg_ms, 5), "algorithmic_bytes_per_launch": round(bytes_per_atom * n)}
+        if valu and fold:
+            # the kernel's issue floor: its wavefront VALU instructions (rocprofv3 SQ_INSTS_VALU of the same
+            # workload) at 4 cycles each on 256 CUs x 4 SIMDs at the 2.4 GHz peak engine clock
+            floor_ms = valu * 4.0 / (1024 * 2.4e9) * 1e3
+            roofline["valu_issue"] = {"instructions": valu, "floor_ms": round(floor_ms, 4),
+                                      "frac": round(floor_ms / avg_ms, 4)}
     cpu = None
     if not args.no_cpu:
         log("timing the reference's CPU path ...")
