@@ -133,8 +133,97 @@ __device__ int tile_cell_ranges(int c_first, int c_last, int s, int2* cr) {
     return out <= RMAX ? out : -1;
 }
 
-// Halo ranges of every tile: one wavefront per tile. Lane 0 merges the cell ranges (a short serial pass; the
-// launch keeps 32 tiles per CU in flight to hide it), then lane r reads the particle span of range r and a
+// The same ranges computed by the whole wavefront, for tiles whose cells span at most three cell columns (every
+// lane calls it, with the same arguments): lane l < 9 nseg takes segment l / 9 and neighbour column l % 9, the one
+// or two cell runs of each lane are compacted into cr, ranked by first cell, and runs that overlap or touch are
+// merged (a run starts a merged range when its first cell lies past the prefix maximum of the earlier ends + 1).
+__device__ int tile_cell_ranges_wave(int c_first, int c_last, int s, int2* cr, int lane) {
+    const int col0 = c_first / s, col1 = c_last / s;
+    const int nseg = col1 - col0 + 1;
+    int a0 = 0, b0 = -1, a1 = 0, b1 = -1;  // b < a: no run
+    if (lane < 9 * nseg) {
+        const int k = lane / 9, nb = lane % 9;
+        const int col = col0 + k;
+        const int x = col / s, y = col % s;
+        const int z0 = k == 0 ? c_first % s : 0;
+        const int z1 = col == col1 ? c_last % s : s - 1;
+        const int base = (wrap_index(x + nb / 3 - 1, s) * s + wrap_index(y + nb % 3 - 1, s)) * s;
+        const int za = z0 - 1, zb = z1 + 1;
+        if (zb - za + 1 >= s) {
+            a0 = base;
+            b0 = base + s - 1;
+        } else if (za < 0) {
+            a0 = base + za + s;
+            b0 = base + s - 1;
+            a1 = base;
+            b1 = base + zb;
+        } else if (zb >= s) {
+            a0 = base + za;
+            b0 = base + s - 1;
+            a1 = base;
+            b1 = base + zb - s;
+        } else {
+            a0 = base + za;
+            b0 = base + zb;
+        }
+    }
+    const int cnt = (b0 >= a0 ? 1 : 0) + (b1 >= a1 ? 1 : 0);
+    int incl = cnt;
+#pragma unroll
+    for (int d = 1; d < WAVE; d <<= 1) {
+        const int o = __shfl_up(incl, d, WAVE);
+        if (lane >= d) incl += o;
+    }
+    const int total = __shfl(incl, WAVE - 1, WAVE);  // at most 2 x 9 x 3 = 54 runs
+    if (cnt > 0) cr[incl - cnt] = make_int2(a0, b0);
+    if (cnt > 1) cr[incl - cnt + 1] = make_int2(a1, b1);
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    // rank of run `lane` by (first cell, position)
+    int2 mine = make_int2(0, -1);
+    int rank = 0;
+    if (lane < total) {
+        mine = cr[lane];
+        for (int j = 0; j < total; ++j) {
+            const int aj = cr[j].x;
+            rank += (aj < mine.x || (aj == mine.x && j < lane)) ? 1 : 0;
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    if (lane < total) cr[rank] = mine;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    const bool in = lane < total;
+    const int2 r = in ? cr[lane] : make_int2(0x7fffffff, -1);
+    int pmax = r.y;  // inclusive prefix maximum of the run ends
+#pragma unroll
+    for (int d = 1; d < WAVE; d <<= 1) {
+        const int o = __shfl_up(pmax, d, WAVE);
+        if (lane >= d && o > pmax) pmax = o;
+    }
+    int before = __shfl_up(pmax, 1, WAVE);
+    if (lane == 0) before = -2;
+    const bool head = in && r.x > before + 1;
+    const unsigned long long heads = __ballot(head);
+    const int nr = __popcll(heads);
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    if (in && nr <= RMAX) {
+        const int g = static_cast<int>(__builtin_amdgcn_mbcnt_hi(static_cast<unsigned>(heads >> 32),
+                                                                 __builtin_amdgcn_mbcnt_lo(static_cast<unsigned>(heads), 0u))) -
+                      (head ? 0 : 1);
+        const bool last = lane == total - 1 || ((heads >> (lane + 1)) & 1ull) != 0;
+        if (head) cr[g].x = r.x;
+        if (last) cr[g].y = pmax;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    return nr <= RMAX ? nr : -1;
+}
+
+// Halo ranges of every tile: one wavefront per tile. The wavefront merges the cell ranges (tiles spanning more than
+// three cell columns, on small grids: lane 0 alone), then lane r reads the particle span of range r and a
 // wavefront scan assigns the LDS slots. tile_nr = -1 marks a tile whose halo does not fit in `hmax` slots.
 __global__ void __launch_bounds__(SETUP_TILES * WAVE)
     k_tile_setup(size_t n, const int* __restrict__ cell_of, const uint32_t* __restrict__ start, int side,
@@ -148,22 +237,26 @@ __global__ void __launch_bounds__(SETUP_TILES * WAVE)
     const int t = blockIdx.x * SETUP_TILES + w;
     const bool live = t < tiles;
     int2* cr = cr_all[w];
-    if (live && lane == 0) {
+    if (live) {
         const size_t i0 = static_cast<size_t>(t) * TILE;
         const size_t i1 = (i0 + TILE) < n ? (i0 + TILE) : n;
-        int nr, flat = 0;
         if (use_cells) {
             const int c0 = cell_of[i0], c1 = cell_of[i1 - 1];
-            nr = tile_cell_ranges(c0, c1, side, cr);
-            const int col = c0 / side, x = col / side, y = col % side, z0 = c0 % side, z1 = c1 % side;
-            flat = col == c1 / side && x >= 2 && x <= side - 3 && y >= 2 && y <= side - 3 && z0 >= 2 &&
-                   z1 <= side - 3 && static_cast<double>(c1 - c0 + 3) < flat_cells;
-        } else {
+            const bool wave_path = c1 / side - c0 / side < 3;
+            int nr = 0;
+            if (wave_path) nr = tile_cell_ranges_wave(c0, c1, side, cr, lane);
+            if (lane == 0) {
+                if (!wave_path) nr = tile_cell_ranges(c0, c1, side, cr);
+                const int col = c0 / side, x = col / side, y = col % side, z0 = c0 % side, z1 = c1 % side;
+                nr_all[w] = nr;
+                flat_all[w] = col == c1 / side && x >= 2 && x <= side - 3 && y >= 2 && y <= side - 3 && z0 >= 2 &&
+                              z1 <= side - 3 && static_cast<double>(c1 - c0 + 3) < flat_cells;
+            }
+        } else if (lane == 0) {
             cr[0] = make_int2(0, side * side * side - 1);  // every particle
-            nr = 1;
+            nr_all[w] = 1;
+            flat_all[w] = 0;
         }
-        nr_all[w] = nr;
-        flat_all[w] = flat;
     }
     __syncthreads();
     if (!live) return;
