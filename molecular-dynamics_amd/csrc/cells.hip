@@ -21,22 +21,6 @@
 
 namespace mdgpu {
 
-struct PermuteArgs {
-    int enabled;
-    const double4* pos;
-    const double* vel;
-    const double* frc;
-    const double* mass;
-    const uint64_t* id;
-    double4* pos2;
-    double* vel2;
-    double* frc2;
-    double* mass2;
-    uint64_t* id2;
-    double4* ref;
-    int* cell_of2;
-};
-
 struct CopyBackArgs {
     int enabled;
     const double* vel2;
@@ -185,17 +169,8 @@ __device__ __forceinline__ void sort_cell(size_t cell, int lane, const uint32_t*
         if (valid) {
             const uint32_t k = b + rank;
             order[k] = v;
-            if (pa.enabled) {
-                const double4 r = pa.pos[v];
-                pa.pos2[k] = r;
-                pa.ref[k] = r;
-                for (int c = 0; c < 3; ++c) pa.vel2[c * n + k] = pa.vel[c * n + v];
-                if (pa.frc)
-                    for (int c = 0; c < 3; ++c) pa.frc2[c * n + k] = pa.frc[c * n + v];
-                pa.mass2[k] = pa.mass[v];
-                pa.id2[k] = pa.id[v];
-                pa.cell_of2[k] = static_cast<int>(cell);
-            }
+            if (pa.enabled) pa.cell_of2[k] = static_cast<int>(cell);
+            if (pa.enabled == PERMUTE_STATE) permute_member(pa, k, v, n);
         }
     }
 }
@@ -466,6 +441,24 @@ void list_prepare(md_list* l, md_state* s) {
     l->n = s->n;
 }
 
+PermuteArgs permute_args(md_list* l, md_state* s, bool carry_forces) {
+    PermuteArgs pa{};
+    pa.enabled = PERMUTE_STATE;
+    pa.pos = s->pos;
+    pa.vel = s->vel;
+    pa.frc = carry_forces ? s->frc : nullptr;
+    pa.mass = s->mass;
+    pa.id = s->id;
+    pa.pos2 = s->pos_alt;
+    pa.vel2 = s->vel_alt;
+    pa.frc2 = carry_forces ? s->frc_alt : nullptr;
+    pa.mass2 = s->mass_alt;
+    pa.id2 = s->id_alt;
+    pa.ref = l->ref;
+    pa.cell_of2 = l->cell_of_alt;
+    return pa;
+}
+
 // Histogram, scan, scatter and stable per-cell order of the current positions; with `permute` the state is
 // also gathered into its alt arrays in that order (forces only with `carry_forces`: a rebuild that is followed by
 // a force evaluation leaves them behind).
@@ -483,21 +476,11 @@ static void cells_enqueue(md_list* l, md_state* s, double per_length, bool permu
     }
     k_cell_scatter<<<blocks_for(n, 256), 256, 0, st>>>(l->cell_of, n, l->cell_start, l->scan_tmp, l->order_raw,
                                                        gate);
+    // the tiled build's tile setup gathers the state itself (one lane per destination particle), after this sort
     PermuteArgs pa{};
-    pa.enabled = permute ? 1 : 0;
     if (permute) {
-        pa.pos = s->pos;
-        pa.vel = s->vel;
-        pa.frc = carry_forces ? s->frc : nullptr;
-        pa.mass = s->mass;
-        pa.id = s->id;
-        pa.pos2 = s->pos_alt;
-        pa.vel2 = s->vel_alt;
-        pa.frc2 = carry_forces ? s->frc_alt : nullptr;
-        pa.mass2 = s->mass_alt;
-        pa.id2 = s->id_alt;
-        pa.ref = l->ref;
-        pa.cell_of2 = l->cell_of_alt;
+        pa = permute_args(l, s, carry_forces);
+        if (tiling_enabled(n)) pa.enabled = PERMUTE_ORDER;
     }
     const unsigned sort_blocks = std::min<unsigned>(blocks_for(l->ncells * WAVE, 256), SORT_BLOCKS);
     k_cell_sort<<<sort_blocks, 256, 0, st>>>(l->cell_start, l->ncells, n, l->order_raw, l->order, l->scan_tmp,

@@ -168,7 +168,43 @@ struct StepRecord {
     int* step = nullptr;
 };
 
+// Re-sort of the state into its alt arrays: source and destination arrays (frc only when forces are carried), the
+// positions at the build (ref) and the particles' cells in the new order.
+enum PermuteMode : int {
+    PERMUTE_NONE = 0,
+    PERMUTE_STATE = 1,  // the order, the cells and the state
+    PERMUTE_ORDER = 2   // the order and the cells only (the state follows in the tile setup)
+};
+struct PermuteArgs {
+    int enabled;
+    const double4* pos;
+    const double* vel;
+    const double* frc;
+    const double* mass;
+    const uint64_t* id;
+    double4* pos2;
+    double* vel2;
+    double* frc2;
+    double* mass2;
+    uint64_t* id2;
+    double4* ref;
+    int* cell_of2;
+};
+
+// Member v of the old order goes to position k of the new one.
+__device__ __forceinline__ void permute_member(const PermuteArgs& pa, size_t k, size_t v, size_t n) {
+    const double4 r = pa.pos[v];
+    pa.pos2[k] = r;
+    pa.ref[k] = r;
+    for (int c = 0; c < 3; ++c) pa.vel2[c * n + k] = pa.vel[c * n + v];
+    if (pa.frc)
+        for (int c = 0; c < 3; ++c) pa.frc2[c * n + k] = pa.frc[c * n + v];
+    pa.mass2[k] = pa.mass[v];
+    pa.id2[k] = pa.id[v];
+}
+
 // cells.hip
+PermuteArgs permute_args(md_list* l, md_state* s, bool carry_forces);  // mode PERMUTE_STATE
 void cells_build(md_list* l, md_state* s, double min_cell, int side);   // count, scan, scatter, stable order
 void list_rebuild(md_list* l, md_state* s);                            // full host-driven rebuild + read-back
 void list_enqueue_conditional_rebuild(md_list* l, md_state* s);       // device-flag-gated rebuild (md_run)

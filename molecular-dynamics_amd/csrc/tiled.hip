@@ -228,7 +228,7 @@ __device__ int tile_cell_ranges_wave(int c_first, int c_last, int s, int2* cr, i
 __global__ void __launch_bounds__(SETUP_TILES * WAVE)
     k_tile_setup(size_t n, const int* __restrict__ cell_of, const uint32_t* __restrict__ start, int side,
                  int use_cells, int hmax, int tiles, double flat_cells, int4* __restrict__ tile_ranges,
-                 int* __restrict__ tile_nr, const int* gate) {
+                 int* __restrict__ tile_nr, const uint32_t* __restrict__ order, PermuteArgs pa, const int* gate) {
     if (gate && *gate == 0) return;
     __shared__ int2 cr_all[SETUP_TILES][CR_CAP];
     __shared__ int nr_all[SETUP_TILES];
@@ -277,6 +277,13 @@ __global__ void __launch_bounds__(SETUP_TILES * WAVE)
     if (lane < nr)
         tile_ranges[static_cast<size_t>(t) * RMAX + lane] = make_int4(first_cell, first, incl - cnt, cnt);
     if (lane == 0) tile_nr[t] = (nr > 0 && total <= hmax) ? nr + (flat_all[w] ? TILE_FLAT : 0) : -1;
+    // after a re-sort: the state of the tile's particles gathered into the alt arrays in the new order
+    if (pa.enabled == PERMUTE_STATE) {
+        for (int q = lane; q < TILE; q += WAVE) {
+            const size_t k = static_cast<size_t>(t) * TILE + q;
+            if (k < n) permute_member(pa, k, order[k], n);
+        }
+    }
 }
 
 // Tile header shared by the build and the force kernel: the tile's halo ranges in LDS.
@@ -1314,7 +1321,7 @@ void tiled_list_enqueue(md_list* l, md_state* s, bool from_alt, const int* gate,
     const double flat_cells = (0.5 * s->box - l->skin) / (s->box / l->side);
     hipLaunchKernelGGL(k_tile_setup, dim3((tiles + SETUP_TILES - 1) / SETUP_TILES), dim3(SETUP_TILES * WAVE), 0, st,
                        n, cells, l->cell_start, ta.side, ta.use_cells, ta.hmax, tiles, flat_cells, l->tile_ranges,
-                       l->tile_nr, gate);
+                       l->tile_nr, l->order, from_alt ? permute_args(l, s, carry_forces) : PermuteArgs{}, gate);
     // the float halo where the grid is wide enough for compact frames and the cells dense enough for the
     // candidate lists (MDGPU_BUILD_F64=1 keeps the double halo)
     const char* f64 = std::getenv("MDGPU_BUILD_F64");
