@@ -137,7 +137,7 @@ __device__ int tile_cell_ranges(int c_first, int c_last, int s, int2* cr) {
 // lane calls it, with the same arguments): lane l < 9 nseg takes segment l / 9 and neighbour column l % 9, the one
 // or two cell runs of each lane are compacted into cr, ranked by first cell, and runs that overlap or touch are
 // merged (a run starts a merged range when its first cell lies past the prefix maximum of the earlier ends + 1).
-__device__ int tile_cell_ranges_wave(int c_first, int c_last, int s, int2* cr, int lane) {
+__device__ __forceinline__ int tile_cell_ranges_wave(int c_first, int c_last, int s, int2* cr, int lane) {
     const int col0 = c_first / s, col1 = c_last / s;
     const int nseg = col1 - col0 + 1;
     int a0 = 0, b0 = -1, a1 = 0, b1 = -1;  // b < a: no run
@@ -222,6 +222,56 @@ __device__ int tile_cell_ranges_wave(int c_first, int c_last, int s, int2* cr, i
     return nr <= RMAX ? nr : -1;
 }
 
+// A tile's share of the re-sort, TILE / WAVE particles per lane: every load issued before any store (the state's
+// arrays are not marked as unaliased, so a store would otherwise wait for nothing and order the next loads).
+constexpr int GATHER_PER_LANE = TILE / WAVE;
+struct GatheredTile {  // plain doubles: a private array of double4 is not kept in registers
+    double r[GATHER_PER_LANE][4];
+    double v[GATHER_PER_LANE][3], f[GATHER_PER_LANE][3], m[GATHER_PER_LANE];
+    uint64_t id[GATHER_PER_LANE];
+};
+
+__device__ __forceinline__ void load_gathered(const PermuteArgs& pa, const uint32_t* __restrict__ order,
+                                              GatheredTile& g, int t, int lane, size_t n) {
+#pragma unroll
+    for (int u = 0; u < GATHER_PER_LANE; ++u) {
+        const size_t k = static_cast<size_t>(t) * TILE + u * WAVE + lane;
+        if (k >= n) continue;
+        const size_t v = order[k];
+        const double4 p = pa.pos[v];
+        g.r[u][0] = p.x;
+        g.r[u][1] = p.y;
+        g.r[u][2] = p.z;
+        g.r[u][3] = p.w;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) g.v[u][c] = pa.vel[c * n + v];
+        if (pa.frc)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) g.f[u][c] = pa.frc[c * n + v];
+        g.m[u] = pa.mass[v];
+        g.id[u] = pa.id[v];
+    }
+}
+
+__device__ __forceinline__ void store_gathered(const PermuteArgs& pa, const GatheredTile& g, int t, int lane,
+                                               size_t n) {
+#pragma unroll
+    for (int u = 0; u < GATHER_PER_LANE; ++u) {
+        const size_t k = static_cast<size_t>(t) * TILE + u * WAVE + lane;
+        if (k >= n) continue;
+        const double4 p = make_double4(g.r[u][0], g.r[u][1], g.r[u][2], g.r[u][3]);
+        pa.pos2[k] = p;
+        pa.ref[k] = p;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) pa.vel2[c * n + k] = g.v[u][c];
+        if (pa.frc)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) pa.frc2[c * n + k] = g.f[u][c];
+        pa.mass2[k] = g.m[u];
+        pa.id2[k] = g.id[u];
+    }
+}
+
 // Halo ranges of every tile: one wavefront per tile. The wavefront merges the cell ranges (tiles spanning more than
 // three cell columns, on small grids: lane 0 alone), then lane r reads the particle span of range r and a
 // wavefront scan assigns the LDS slots. tile_nr = -1 marks a tile whose halo does not fit in `hmax` slots.
@@ -279,10 +329,9 @@ __global__ void __launch_bounds__(SETUP_TILES * WAVE)
     if (lane == 0) tile_nr[t] = (nr > 0 && total <= hmax) ? nr + (flat_all[w] ? TILE_FLAT : 0) : -1;
     // after a re-sort: the state of the tile's particles gathered into the alt arrays in the new order
     if (pa.enabled == PERMUTE_STATE) {
-        for (int q = lane; q < TILE; q += WAVE) {
-            const size_t k = static_cast<size_t>(t) * TILE + q;
-            if (k < n) permute_member(pa, k, order[k], n);
-        }
+        GatheredTile g;
+        load_gathered(pa, order, g, t, lane, n);
+        store_gathered(pa, g, t, lane, n);
     }
 }
 
