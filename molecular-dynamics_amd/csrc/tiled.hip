@@ -287,6 +287,11 @@ __global__ void __launch_bounds__(SETUP_TILES * WAVE)
     const int t = blockIdx.x * SETUP_TILES + w;
     const bool live = t < tiles;
     int2* cr = cr_all[w];
+    // after a re-sort: the state of the tile's particles, loaded in the new order while the ranges are merged and
+    // stored into the alt arrays at the end
+    const bool gather = live && pa.enabled == PERMUTE_STATE;
+    GatheredTile g;
+    if (gather) load_gathered(pa, order, g, t, lane, n);
     if (live) {
         const size_t i0 = static_cast<size_t>(t) * TILE;
         const size_t i1 = (i0 + TILE) < n ? (i0 + TILE) : n;
@@ -327,12 +332,7 @@ __global__ void __launch_bounds__(SETUP_TILES * WAVE)
     if (lane < nr)
         tile_ranges[static_cast<size_t>(t) * RMAX + lane] = make_int4(first_cell, first, incl - cnt, cnt);
     if (lane == 0) tile_nr[t] = (nr > 0 && total <= hmax) ? nr + (flat_all[w] ? TILE_FLAT : 0) : -1;
-    // after a re-sort: the state of the tile's particles gathered into the alt arrays in the new order
-    if (pa.enabled == PERMUTE_STATE) {
-        GatheredTile g;
-        load_gathered(pa, order, g, t, lane, n);
-        store_gathered(pa, g, t, lane, n);
-    }
+    if (gather) store_gathered(pa, g, t, lane, n);
 }
 
 // Tile header shared by the build and the force kernel: the tile's halo ranges in LDS.
