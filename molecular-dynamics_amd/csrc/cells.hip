@@ -6,8 +6,9 @@
 //   k_scan_*       exclusive prefix sum of the histogram -> cell start offsets (CellList.cpp:41-44)
 //   k_cell_scatter particle index into its cell's range, unordered (atomics)
 //   k_cell_sort    one wavefront per cell ranks its members by index, restoring the counting sort's stable order
-//                  (CellList.cpp:47-51); when re-sorting, it also gathers every particle array into the new order
-//                  (System::reorder) and records the build-time positions (NeighbourList.cpp:129-132)
+//                  (CellList.cpp:47-51); when re-sorting, it records each particle's cell in the new order and
+//                  gathers every particle array into that order (System::reorder) with the build-time positions
+//                  (NeighbourList.cpp:129-132) — on the tiled path k_tile_setup does the gather instead
 //   k_list_build   one thread per particle: the j > i candidates of the 27 cells in the reference's visit order
 //                  (its half-list row), then the j < i candidates in increasing index; r^2 < (rc+skin)^2
 //                  strictly (NeighbourList.cpp:75-98); also moves the re-sorted arrays back into place.

@@ -8,7 +8,8 @@
 // doubles) is set from the density, and a tile that does not fit is marked tile_nr = -1 and handled through global
 // gathers.
 //
-// A rebuild runs two kernels: k_tile_setup (one wavefront per tile) finds the halo ranges and their LDS slots;
+// A rebuild runs two kernels: k_tile_setup (one wavefront per tile) finds the halo ranges and their LDS slots (and,
+// after a re-sort, gathers the tile's particles into the new order);
 // k_list_build_tiled stages the halo, builds per own cell two flat candidate lists (the stencil cells above and
 // below it) and walks them one wavefront per particle, 64 candidates per step, ranking the accepted ones over the
 // ballot so that rows keep the reference's order (j > i in 27-cell stencil order, then j < i increasing). It writes
