@@ -1087,6 +1087,8 @@ __global__ void __launch_bounds__(TILE_THREADS)
         rows[static_cast<size_t>(k) * n + i] = halo_particle(row16[slot_entry(static_cast<uint32_t>(k))] / SLOT_BYTES, hr, nr);
 }
 
+constexpr int UNSTAGED_BATCH = 2;  // row entries per lane in flight on the force kernel's unstaged path
+
 // Frame of the mixed force kernel: positions are staged relative to the tile's first particle (its nearest image),
 // in double, so a separation is one double subtraction.
 struct TileFrame {
@@ -1239,9 +1241,31 @@ __global__ void __launch_bounds__(TILE_THREADS)
             else
                 run(std::true_type{});
         } else {
+            // unstaged tile: neighbours gathered through the int32 rows, UNSTAGED_BATCH entries of the lane at a time
+            // (their row words, then their positions, then their terms in entry order), so the two dependent loads
+            // of an entry overlap with those of the others
             const double4 ri = pos[i];
-            for (int k = lane; k < m; k += LANES)
-                mixed_pair_gather_f(ri, pos[rows[static_cast<size_t>(k) * n + i]], fr.box, fr.inv_box, lj, a);
+            for (int k0 = lane; k0 < m; k0 += UNSTAGED_BATCH * LANES) {
+                int j[UNSTAGED_BATCH];
+#pragma unroll
+                for (int b = 0; b < UNSTAGED_BATCH; ++b) {
+                    const int k = k0 + b * LANES;
+                    j[b] = k < m ? rows[static_cast<size_t>(k) * n + i] : -1;
+                }
+                double xj[UNSTAGED_BATCH], yj[UNSTAGED_BATCH], zj[UNSTAGED_BATCH];  // plain doubles stay in registers
+#pragma unroll
+                for (int b = 0; b < UNSTAGED_BATCH; ++b) {
+                    if (j[b] < 0) continue;
+                    const double4 r = pos[j[b]];
+                    xj[b] = r.x;
+                    yj[b] = r.y;
+                    zj[b] = r.z;
+                }
+#pragma unroll
+                for (int b = 0; b < UNSTAGED_BATCH; ++b)
+                    if (j[b] >= 0)
+                        mixed_pair_gather_f(ri, make_double4(xj[b], yj[b], zj[b], 0.0), fr.box, fr.inv_box, lj, a);
+            }
         }
     }
     // the particle's four lanes are adjacent in the wavefront
