@@ -261,7 +261,7 @@ __device__ __forceinline__ void store_gathered(const PermuteArgs& pa, const Gath
         const size_t k = static_cast<size_t>(t) * TILE + u * WAVE + lane;
         if (k >= n) continue;
         const double4 p = make_double4(g.r[u][0], g.r[u][1], g.r[u][2], g.r[u][3]);
-        pa.pos2[k] = p;
+        if (pa.pos2) pa.pos2[k] = p;
         pa.ref[k] = p;
 #pragma unroll
         for (int c = 0; c < 3; ++c) pa.vel2[c * n + k] = g.v[u][c];
@@ -1390,12 +1390,20 @@ void tiled_list_enqueue(md_list* l, md_state* s, bool from_alt, const int* gate,
     }
     const int tiles = tile_count(n);
     TileArgs ta{l->side, l->side >= 3 ? 1 : 0, l->hmax, tiles, l->side >= 7 ? 1 : 0};
+    // after a re-sort the tile setup gathers the state into the alt arrays, the positions only into `ref` (the
+    // build-time positions, which they are): the build reads them there and copies them back into place
+    PermuteArgs gather{};
+    if (from_alt) {
+        gather = permute_args(l, s, carry_forces);
+        gather.pos2 = nullptr;
+    }
+    const double4* build_pos = from_alt ? l->ref : s->pos;
     const int* cells = from_alt ? l->cell_of_alt : l->cell_of;
     // cells that a tile and its halo may span with every coordinate inside half a box (skin: the largest drift)
     const double flat_cells = (0.5 * s->box - l->skin) / (s->box / l->side);
     hipLaunchKernelGGL(k_tile_setup, dim3((tiles + SETUP_TILES - 1) / SETUP_TILES), dim3(SETUP_TILES * WAVE), 0, st,
                        n, cells, l->cell_start, ta.side, ta.use_cells, ta.hmax, tiles, flat_cells, l->tile_ranges,
-                       l->tile_nr, l->order, from_alt ? permute_args(l, s, carry_forces) : PermuteArgs{}, gate);
+                       l->tile_nr, l->order, gather, gate);
     // the float halo where the grid is wide enough for compact frames and the cells dense enough for the
     // candidate lists (MDGPU_BUILD_F64=1 keeps the double halo)
     const char* f64 = std::getenv("MDGPU_BUILD_F64");
@@ -1403,7 +1411,7 @@ void tiled_list_enqueue(md_list* l, md_state* s, bool from_alt, const int* gate,
                     !(f64 && f64[0] == '1');
     auto kernel = fh ? k_list_build_tiled<true> : k_list_build_tiled<false>;
     const size_t lds = static_cast<size_t>(TILE_HALO_MAX) * 3 * (fh ? sizeof(float) : sizeof(double));
-    hipLaunchKernelGGL(kernel, dim3(tiles), dim3(TILE_THREADS), lds, st, from_alt ? s->pos_alt : s->pos, n, cells,
+    hipLaunchKernelGGL(kernel, dim3(tiles), dim3(TILE_THREADS), lds, st, build_pos, n, cells,
                        l->cell_start, ta, s->box, s->inv_box, l->radius2, l->padding, write32 ? 1 : 0,
                        slot_row_len(l->padding), l->rows, l->rows16, l->count, l->half, l->build_stats,
                        l->tile_ranges, l->tile_nr, cb, gate);
