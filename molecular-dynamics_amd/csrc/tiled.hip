@@ -289,7 +289,7 @@ __global__ void __launch_bounds__(SETUP_TILES * WAVE)
     const bool live = t < tiles;
     int2* cr = cr_all[w];
     // after a re-sort: the state of the tile's particles, loaded in the new order while the ranges are merged and
-    // stored into the alt arrays at the end
+    // stored at the end (positions into ref, the rest into the alt arrays)
     const bool gather = live && pa.enabled == PERMUTE_STATE;
     GatheredTile g;
     if (gather) load_gathered(pa, order, g, t, lane, n);
