@@ -33,6 +33,10 @@ WORKLOADS = {
                "LJ NVE N=256,000 (FCC 40^3x4) rho*=0.8442 T*=1.0 seed 1 rc=2.5 skin=0.3 dt=0.005 mixed"),
     "lj4m_fp64": (100, 0.8442, 0.722, 1, 0.3, mdgpu.MD_FP64,
                   "LJ NVE N=4,000,000 (FCC 100^3x4) rho*=0.8442 T*=0.722 seed 1 rc=2.5 skin=0.3 dt=0.005 fp64"),
+    "lj32m": (200, 0.8442, 0.722, 1, 0.3, mdgpu.MD_MIXED,
+              "LJ NVE N=32,000,000 (FCC 200^3x4) rho*=0.8442 T*=0.722 seed 1 rc=2.5 skin=0.3 dt=0.005 mixed"),
+    "lj108m": (300, 0.8442, 0.722, 1, 0.3, mdgpu.MD_MIXED,
+               "LJ NVE N=108,000,000 (FCC 300^3x4) rho*=0.8442 T*=0.722 seed 1 rc=2.5 skin=0.3 dt=0.005 mixed"),
     "lj2048": (8, 0.8442, 0.722, 1, 0.3, mdgpu.MD_FP64,
                "LJ NVE N=2048 (FCC 8^3x4) rho*=0.8442 T*=0.722 seed 1 rc=2.5 skin=0.3 dt=0.005 fp64"),
 }
