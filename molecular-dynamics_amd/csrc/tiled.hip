@@ -41,7 +41,7 @@ constexpr int STAGE_ROUNDS = (TILE_HALO_MAX + TILE_THREADS - 1) / TILE_THREADS;
 constexpr int CHUNK = 32;    // slot-row entries per particle and chunk
 constexpr int CHUNK_STRIDE = TILE * CHUNK;  // entries between a tile's chunks
 constexpr int GHOST_SLOT = TILE_HALO_MAX - 1;  // LDS slot holding NaN coordinates: never a neighbour
-constexpr int PRE_CHUNKS = 4;  // slot chunks the force kernel loads per lane ahead of its pair loop
+constexpr int PRE_CHUNKS = 3;  // slot chunks the force kernel loads per lane ahead of its pair loop
 // The slot rows and the list build's candidate lists hold byte offsets of LDS slots (slot * 8, below 2^16), so that
 // the force kernel adds an entry to a fixed halo address folded into its LDS loads.
 constexpr int SLOT_BYTES = 8;
