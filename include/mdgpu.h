@@ -31,7 +31,8 @@ typedef enum md_status {
     MD_ERR_ARG = 1,      /* bad argument or handle */
     MD_ERR_HIP = 2,      /* HIP runtime failure */
     MD_ERR_RUNTIME = 3,  /* the reference's std::runtime_error cases */
-    MD_ERR_OVERFLOW = 4, /* a padded list row overflowed inside md_run and the state was restored */
+    MD_ERR_OVERFLOW = 4, /* list rows kept overflowing after repeated regrowth (md_run, md_batch_run and
+                            md_adjoint_run grow the rows and resume by themselves) */
     MD_ERR_STATE = 5     /* call order violated (e.g. forces before any list build) */
 } md_status;
 
@@ -74,6 +75,9 @@ typedef struct md_list_info {
     int padding;           /* allocated row length of the padded full list */
     size_t particles;      /* particle count of the last build (rows of the list) */
     int tiled;             /* 1 when the rows also carry the 16-bit LDS-slot view read by the tiled kernels */
+    int tiles;             /* tiles of 128 particles (tiled rows only) */
+    int unstaged_tiles;    /* tiles whose halo exceeds the LDS capacity (their forces gather through the int32
+                              rows) */
 } md_list_info;
 
 /* ---- host-side input generation (Initialisation.hpp:9-12) --------------------------------------------- */
@@ -124,6 +128,9 @@ int md_list_build(md_list* l, md_state* s);
 /* NeighbourList::update: rebuild if n changed or any particle moved > skin/2 (NeighbourList.cpp:11-36). */
 int md_list_update(md_list* l, md_state* s, int* rebuilt);
 int md_list_get_info(const md_list* l, md_list_info* info);
+/* NeighbourList copy construction / assignment (NeighbourList.hpp:12-76, implicitly defaulted there): dst takes a
+   deep copy of src's parameters, cell grid and rows, so forces through either list give the same results. */
+int md_list_copy(md_list* dst, const md_list* src);
 /* The half list (j > i) in CSR, as neighboursOf / firstPairOf expose it (NeighbourList.hpp:25-31):
    offsets[n+1], neighbours[half_pairs]; either may be NULL. */
 int md_list_half(md_list* l, uint64_t* offsets, uint64_t* neighbours);
@@ -182,7 +189,10 @@ typedef struct md_run_config {
 /* Advances `steps` velocity-Verlet steps entirely on the device: the list update decision, re-sort and rebuild
    are driven by a device flag. energies (host, may be NULL) receives 4 doubles per step: potential, kinetic,
    virial, inside pairs. The kinetic energy is a fixed-order tree sum after the closing half kick in NVE, and
-   the thermostat's new kinetic energy K' (eq. A7) with the Bussi thermostat on. */
+   the thermostat's new kinetic energy K' (eq. A7) with the Bussi thermostat on. A rebuild whose rows outgrow the
+   padding halts every later kernel of the run, so the state stops after that step's opening kick and drift; the
+   host then grows the rows and resumes from that rebuild (NeighbourList.cpp:11-36 regrows the same way), and the
+   trajectory equals the one an ample padding gives. */
 int md_run(md_state* s, md_list* l, const md_run_config* cfg, double* energies);
 
 /* ---- Adjoint (no counterpart in the reference) ----------------------------------------------------------- */

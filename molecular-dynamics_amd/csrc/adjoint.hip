@@ -231,7 +231,6 @@ extern "C" int md_adjoint_run(md_state* src, md_list* l, const md_adjoint_config
     double* obs = buf.get<double>(3 * static_cast<size_t>(K + 1));
     double* point = buf.get<double>(ADJ_PARTS);
     double* theta = buf.get<double>(2);
-    MD_HIP(hipMemsetAsync(theta, 0, 2 * sizeof(double), st));
 
     auto save_checkpoint = [&](int c) {
         k_copy_pos<<<bn, ADJ_THREADS, 0, st>>>(s->pos, ck_pos + static_cast<size_t>(c) * n, n);
@@ -264,91 +263,111 @@ extern "C" int md_adjoint_run(md_state* src, md_list* l, const md_adjoint_config
         }
     } evg{e0, e1, e2};
 
-    // ---- forward -------------------------------------------------------------------------------------------
-    MD_HIP(hipEventRecord(e0, st));
-    list_rebuild_fixed(l, s);
-    forces_enqueue(s, l, cfg->lj, MD_FP64, s->scalars, nullptr, nullptr, nullptr);
-    save_checkpoint(0);
-    for (int k = 1; k <= K; ++k) {
-        step();
-        k_record_obs<<<1, 1, 0, st>>>(s->scalars, obs, k);
-        if (k % M == 0 && k < K) {
-            list_rebuild_fixed(l, s);  // the recompute starts from a fresh list here too
-            save_checkpoint(k / M);
-        }
-    }
-    MD_HIP(hipEventRecord(e1, st));
-    std::vector<double> h_obs(3 * static_cast<size_t>(K + 1));
-    MD_HIP(hipMemcpyAsync(h_obs.data(), obs, sizeof(double) * h_obs.size(), hipMemcpyDeviceToHost, st));
-    MD_HIP(hipStreamSynchronize(st));
-    double ubar = 0.0, pbar = 0.0;
-    for (int k = 1; k <= K; ++k) {
-        ubar += h_obs[3 * k];
-        pbar += (2.0 * h_obs[3 * k + 2] + h_obs[3 * k + 1]) / (3.0 * volume);
-    }
-    ubar /= K;
-    pbar /= K;
+    // A device-gated rebuild whose rows overflow leaves truncated rows behind for the rest of its segment: the
+    // whole computation is then repeated from the caller's state with longer rows.
+    int* hf = ctx->host_flags;
     const double target = cfg->pressure_target;
-    const double g = 2.0 * (pbar - target);
-    const double cu = 1.0 / K, cw = g / (3.0 * volume * K), cke = 2.0 * g / (3.0 * volume * K);
-
-    // ---- reverse ---------------------------------------------------------------------------------------------
-    AdjLJ alj;
-    alj.eps = cfg->lj.epsilon;
-    alj.sigma = cfg->lj.sigma;
-    alj.sig2 = cfg->lj.sigma * cfg->lj.sigma;
-    alj.rc2 = cfg->lj.cutoff * cfg->lj.cutoff;
-    {
-        const double q6 = std::pow(cfg->lj.sigma / cfg->lj.cutoff, 6);
-        alj.shift1 = 4.0 * (q6 * q6 - q6);
-        alj.dshift_sig = cfg->lj.epsilon * 4.0 * (12.0 * q6 * q6 - 6.0 * q6) / cfg->lj.sigma;
-    }
-    auto pair_at = [&](int slot, double coef_u, double coef_w) {
-        // positions of the point into the work state; keep the list valid for them
-        k_copy_pos<<<bn, ADJ_THREADS, 0, st>>>(seg_pos + static_cast<size_t>(slot) * n, s->pos, n);
-        displacement_check(l, s);
-        list_enqueue_conditional_rebuild_fixed(l, s);
-        k_adj_pair<<<bn, ADJ_THREADS, 0, st>>>(s->pos, n, l->rows, l->count, l->half, s->box, s->inv_box, alj, w, hw,
-                                               gu, gw, s->partials);
-        reduce_partials_enqueue(s, static_cast<int>(bn), ADJ_PARTS, point);
-        k_adj_theta<<<1, 1, 0, st>>>(point, theta, coef_u, coef_w, dt);
-        // the stale flag is left set only by a check that rebuilt; clear it for the next point
-        MD_HIP(hipMemsetAsync(l->flags + FLAG_STALE, 0, sizeof(int), st));
-    };
-
-    for (int seg = nck - 1; seg >= 0; --seg) {
-        const int c = seg * M;
-        const int e = std::min(K, c + M);
-        load_checkpoint(seg);
+    double ubar = 0.0, pbar = 0.0;
+    double h_theta[2] = {0.0, 0.0};
+    for (int attempt = 0;; ++attempt) {
+        if (attempt > 0) {
+            const int rc = md_state_copy(s, src);
+            if (rc != MD_OK) return rc;
+        }
+        MD_HIP(hipMemsetAsync(theta, 0, 2 * sizeof(double), st));
+        // ---- forward -------------------------------------------------------------------------------------------
+        MD_HIP(hipEventRecord(e0, st));
         list_rebuild_fixed(l, s);
-        k_copy_pos<<<bn, ADJ_THREADS, 0, st>>>(s->pos, seg_pos, n);
-        k_copy_f64<<<b3, 256, 0, st>>>(s->vel, seg_vel, 3 * n);
-        for (int k = c + 1; k <= e; ++k) {
+        // from here on the device-gated rebuilds report an overflow through the sticky words
+        MD_HIP(hipMemsetAsync(l->flags + FLAG_OVERFLOW_STICKY, 0, 2 * sizeof(int), st));
+        static_assert(FLAG_MAXCOUNT_STICKY == FLAG_OVERFLOW_STICKY + 1, "the two words are cleared together");
+        forces_enqueue(s, l, cfg->lj, MD_FP64, s->scalars, nullptr, nullptr, nullptr);
+        save_checkpoint(0);
+        for (int k = 1; k <= K; ++k) {
             step();
-            k_copy_pos<<<bn, ADJ_THREADS, 0, st>>>(s->pos, seg_pos + static_cast<size_t>(k - c) * n, n);
-            k_copy_f64<<<b3, 256, 0, st>>>(s->vel, seg_vel + static_cast<size_t>(k - c) * 3 * n, 3 * n);
+            k_record_obs<<<1, 1, 0, st>>>(s->scalars, obs, k);
+            if (k % M == 0 && k < K) {
+                list_rebuild_fixed(l, s);  // the recompute starts from a fresh list here too
+                save_checkpoint(k / M);
+            }
         }
-        for (int k = e; k >= c + 1; --k) {
-            if (k == K)
-                k_adj_start<<<b3, 256, 0, st>>>(n, lr, lv, pend, w, s->mass,
-                                                seg_vel + static_cast<size_t>(k - c) * 3 * n, cke);
-            pair_at(k - c, cu, cw);
-            k_adj_update<<<b3, 256, 0, st>>>(n, lr, lv, pend, w, hw, gu, gw, s->mass,
-                                             seg_vel + static_cast<size_t>(k - 1 - c) * 3 * n, cu, cw, cke, dt,
-                                             k - 1 == 0 ? 1 : 0);
+        MD_HIP(hipEventRecord(e1, st));
+        std::vector<double> h_obs(3 * static_cast<size_t>(K + 1));
+        MD_HIP(hipMemcpyAsync(h_obs.data(), obs, sizeof(double) * h_obs.size(), hipMemcpyDeviceToHost, st));
+        MD_HIP(hipStreamSynchronize(st));
+        ubar = 0.0;
+        pbar = 0.0;
+        for (int k = 1; k <= K; ++k) {
+            ubar += h_obs[3 * k];
+            pbar += (2.0 * h_obs[3 * k + 2] + h_obs[3 * k + 1]) / (3.0 * volume);
         }
-        if (c == 0) {
-            pair_at(0, 0.0, 0.0);
-            k_adj_finish<<<b3, 256, 0, st>>>(n, lr, hw, dt);
+        ubar /= K;
+        pbar /= K;
+        const double g = 2.0 * (pbar - target);
+        const double cu = 1.0 / K, cw = g / (3.0 * volume * K), cke = 2.0 * g / (3.0 * volume * K);
+
+        // ---- reverse ---------------------------------------------------------------------------------------------
+        AdjLJ alj;
+        alj.eps = cfg->lj.epsilon;
+        alj.sigma = cfg->lj.sigma;
+        alj.sig2 = cfg->lj.sigma * cfg->lj.sigma;
+        alj.rc2 = cfg->lj.cutoff * cfg->lj.cutoff;
+        {
+            const double q6 = std::pow(cfg->lj.sigma / cfg->lj.cutoff, 6);
+            alj.shift1 = 4.0 * (q6 * q6 - q6);
+            alj.dshift_sig = cfg->lj.epsilon * 4.0 * (12.0 * q6 * q6 - 6.0 * q6) / cfg->lj.sigma;
         }
-        MD_HIP(hipGetLastError());
+        auto pair_at = [&](int slot, double coef_u, double coef_w) {
+            // positions of the point into the work state; keep the list valid for them
+            k_copy_pos<<<bn, ADJ_THREADS, 0, st>>>(seg_pos + static_cast<size_t>(slot) * n, s->pos, n);
+            displacement_check(l, s);
+            list_enqueue_conditional_rebuild_fixed(l, s);
+            k_adj_pair<<<bn, ADJ_THREADS, 0, st>>>(s->pos, n, l->rows, l->count, l->half, s->box, s->inv_box, alj, w, hw,
+                                                   gu, gw, s->partials);
+            reduce_partials_enqueue(s, static_cast<int>(bn), ADJ_PARTS, point);
+            k_adj_theta<<<1, 1, 0, st>>>(point, theta, coef_u, coef_w, dt);
+            // the stale flag is left set only by a check that rebuilt; clear it for the next point
+            MD_HIP(hipMemsetAsync(l->flags + FLAG_STALE, 0, sizeof(int), st));
+        };
+
+        for (int seg = nck - 1; seg >= 0; --seg) {
+            const int c = seg * M;
+            const int e = std::min(K, c + M);
+            load_checkpoint(seg);
+            list_rebuild_fixed(l, s);
+            k_copy_pos<<<bn, ADJ_THREADS, 0, st>>>(s->pos, seg_pos, n);
+            k_copy_f64<<<b3, 256, 0, st>>>(s->vel, seg_vel, 3 * n);
+            for (int k = c + 1; k <= e; ++k) {
+                step();
+                k_copy_pos<<<bn, ADJ_THREADS, 0, st>>>(s->pos, seg_pos + static_cast<size_t>(k - c) * n, n);
+                k_copy_f64<<<b3, 256, 0, st>>>(s->vel, seg_vel + static_cast<size_t>(k - c) * 3 * n, 3 * n);
+            }
+            for (int k = e; k >= c + 1; --k) {
+                if (k == K)
+                    k_adj_start<<<b3, 256, 0, st>>>(n, lr, lv, pend, w, s->mass,
+                                                    seg_vel + static_cast<size_t>(k - c) * 3 * n, cke);
+                pair_at(k - c, cu, cw);
+                k_adj_update<<<b3, 256, 0, st>>>(n, lr, lv, pend, w, hw, gu, gw, s->mass,
+                                                 seg_vel + static_cast<size_t>(k - 1 - c) * 3 * n, cu, cw, cke, dt,
+                                                 k - 1 == 0 ? 1 : 0);
+            }
+            if (c == 0) {
+                pair_at(0, 0.0, 0.0);
+                k_adj_finish<<<b3, 256, 0, st>>>(n, lr, hw, dt);
+            }
+            MD_HIP(hipGetLastError());
+        }
+        MD_HIP(hipEventRecord(e2, st));
+        MD_HIP(hipMemcpyAsync(h_theta, theta, sizeof(h_theta), hipMemcpyDeviceToHost, st));
+        MD_HIP(hipMemcpyAsync(hf, l->flags, sizeof(int) * FLAG_COUNT, hipMemcpyDeviceToHost, st));
+        if (grad_positions) MD_HIP(hipMemcpyAsync(grad_positions, lr, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, st));
+        if (grad_velocities) MD_HIP(hipMemcpyAsync(grad_velocities, lv, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, st));
+        MD_HIP(hipStreamSynchronize(st));
+        if (!hf[FLAG_OVERFLOW_STICKY]) break;
+        if (attempt >= 4) fail(MD_ERR_OVERFLOW, "neighbour-list rows keep overflowing during md_adjoint_run");
+        l->padding = (hf[FLAG_MAXCOUNT_STICKY] + hf[FLAG_MAXCOUNT_STICKY] / 4 + 16 + 7) / 8 * 8;
+        l->built = false;
     }
-    MD_HIP(hipEventRecord(e2, st));
-    double h_theta[2];
-    MD_HIP(hipMemcpyAsync(h_theta, theta, sizeof(h_theta), hipMemcpyDeviceToHost, st));
-    if (grad_positions) MD_HIP(hipMemcpyAsync(grad_positions, lr, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, st));
-    if (grad_velocities) MD_HIP(hipMemcpyAsync(grad_velocities, lv, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, st));
-    MD_HIP(hipStreamSynchronize(st));
     // the list now describes the work state, not the caller's: force a rebuild on its next use
     l->built = false;
     float fwd_ms = 0.f, bwd_ms = 0.f;

@@ -41,7 +41,9 @@ template <int W>
 __global__ void __launch_bounds__(RED_THREADS) k_reduce_partials(const double* __restrict__ partials, int rows,
                                                                  double* __restrict__ block_sums,
                                                                  unsigned* __restrict__ ticket,
-                                                                 double* __restrict__ out, StepRecord rec) {
+                                                                 double* __restrict__ out, StepRecord rec,
+                                                                 const int* halt) {
+    if (halted(halt)) return;
     __shared__ double lds[RED_THREADS / WAVE][W];
     __shared__ double fin[RED_BLOCKS][W];
     __shared__ int s_last;
@@ -104,12 +106,13 @@ void reduce_partials_enqueue(md_state* s, int blocks, int width, double* out_dev
     const double* partials = s->partials;
     double* bs = s->red_blocks;
     unsigned* tk = s->red_ticket;
+    const int* h = s->halt;
     if (rec.energies && width != 4) fail(MD_ERR_ARG, "a step record needs the four-sum reduction");
     switch (width) {
-        case 1: k_reduce_partials<1><<<RED_BLOCKS, RED_THREADS, 0, st>>>(partials, blocks, bs, tk, out_dev, rec); break;
-        case 3: k_reduce_partials<3><<<RED_BLOCKS, RED_THREADS, 0, st>>>(partials, blocks, bs, tk, out_dev, rec); break;
-        case 4: k_reduce_partials<4><<<RED_BLOCKS, RED_THREADS, 0, st>>>(partials, blocks, bs, tk, out_dev, rec); break;
-        case 6: k_reduce_partials<6><<<RED_BLOCKS, RED_THREADS, 0, st>>>(partials, blocks, bs, tk, out_dev, rec); break;
+        case 1: k_reduce_partials<1><<<RED_BLOCKS, RED_THREADS, 0, st>>>(partials, blocks, bs, tk, out_dev, rec, h); break;
+        case 3: k_reduce_partials<3><<<RED_BLOCKS, RED_THREADS, 0, st>>>(partials, blocks, bs, tk, out_dev, rec, h); break;
+        case 4: k_reduce_partials<4><<<RED_BLOCKS, RED_THREADS, 0, st>>>(partials, blocks, bs, tk, out_dev, rec, h); break;
+        case 6: k_reduce_partials<6><<<RED_BLOCKS, RED_THREADS, 0, st>>>(partials, blocks, bs, tk, out_dev, rec, h); break;
         default: fail(MD_ERR_ARG, "unsupported reduction width");
     }
     MD_HIP(hipGetLastError());
@@ -124,7 +127,8 @@ void reduce_partials_enqueue(md_state* s, int blocks, int width, double* out_dev
 constexpr int KE_BLOCK = 1024;
 
 __global__ void __launch_bounds__(256) k_ke_blocks(const double* __restrict__ vel, const double* __restrict__ mass,
-                                                    size_t n, double* __restrict__ partial) {
+                                                    size_t n, double* __restrict__ partial, const int* halt) {
+    if (halted(halt)) return;
     __shared__ double terms[KE_BLOCK];
     const size_t base = static_cast<size_t>(blockIdx.x) * KE_BLOCK;
     for (int t = threadIdx.x; t < KE_BLOCK; t += 256) {
@@ -147,7 +151,8 @@ __global__ void __launch_bounds__(256) k_ke_blocks(const double* __restrict__ ve
 }
 
 __global__ void __launch_bounds__(256) k_ke_final(const double* __restrict__ partial, int blocks,
-                                                   double* __restrict__ out) {
+                                                   double* __restrict__ out, const int* halt) {
+    if (halted(halt)) return;
     __shared__ double stage[2048];
     double total = 0.0;
     for (int base = 0; base < blocks; base += 2048) {
@@ -179,8 +184,8 @@ void kinetic_exact_enqueue(md_state* s, double* out_dev) {
         return;
     }
     state_reserve_partials(s, blocks);
-    k_ke_blocks<<<blocks, 256, 0, st>>>(s->vel, s->mass, s->n, s->partials);
-    k_ke_final<<<1, 256, 0, st>>>(s->partials, blocks, out_dev);
+    k_ke_blocks<<<blocks, 256, 0, st>>>(s->vel, s->mass, s->n, s->partials, s->halt);
+    k_ke_final<<<1, 256, 0, st>>>(s->partials, blocks, out_dev, s->halt);
     MD_HIP(hipGetLastError());
 }
 

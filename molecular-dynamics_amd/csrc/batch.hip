@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -71,7 +72,8 @@ __global__ void __launch_bounds__(BT)
     k_b_kick_drift(const Rep* __restrict__ reps, const uint16_t* __restrict__ block_rep, double4* __restrict__ pos,
                    double* __restrict__ vel, const double* __restrict__ frc, const double* __restrict__ mass,
                    size_t slots, double half_dt, double dt, const double4* __restrict__ ref, double max_disp2,
-                   int* __restrict__ stale) {
+                   int* __restrict__ stale, const int* halt) {
+    if (halted(halt)) return;
     const int r = block_rep[blockIdx.x];
     const Rep rp = reps[r];
     const size_t i = static_cast<size_t>(blockIdx.x) * BT + threadIdx.x;
@@ -100,7 +102,8 @@ __global__ void __launch_bounds__(BT)
 __global__ void __launch_bounds__(BT)
     k_b_kick(const Rep* __restrict__ reps, const uint16_t* __restrict__ block_rep, double* __restrict__ vel,
              const double* __restrict__ frc, const double* __restrict__ mass, size_t slots, double half_dt,
-             double* __restrict__ partials) {
+             double* __restrict__ partials, const int* halt) {
+    if (halted(halt)) return;
     const Rep rp = reps[block_rep[blockIdx.x]];
     const size_t i = static_cast<size_t>(blockIdx.x) * BT + threadIdx.x;
     double ke[1] = {0.0};
@@ -121,7 +124,8 @@ __global__ void __launch_bounds__(BT)
 
 // Per replica, in block order: the kick's kinetic-energy partials -> scalars[3].
 __global__ void k_b_reduce_ke(const Rep* __restrict__ reps, int count, const double* __restrict__ partials,
-                              double* __restrict__ scalars) {
+                              double* __restrict__ scalars, const int* halt) {
+    if (halted(halt)) return;
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= count) return;
     const Rep rp = reps[r];
@@ -136,7 +140,8 @@ __global__ void k_b_reduce_ke(const Rep* __restrict__ reps, int count, const dou
 __global__ void __launch_bounds__(BT)
     k_b_ke_blocks(const Rep* __restrict__ reps, const uint16_t* __restrict__ block_rep,
                   const uint32_t* __restrict__ ke_start, const double* __restrict__ vel,
-                  const double* __restrict__ mass, size_t slots, double* __restrict__ ke_part) {
+                  const double* __restrict__ mass, size_t slots, double* __restrict__ ke_part, const int* halt) {
+    if (halted(halt)) return;
     __shared__ double terms[KE_TERMS];
     const uint32_t base = ke_start[blockIdx.x];
     const Rep rp = reps[block_rep[base / BT]];
@@ -161,7 +166,8 @@ __global__ void __launch_bounds__(BT)
 }
 
 __global__ void k_b_ke_final(const Rep* __restrict__ reps, int count, const double* __restrict__ ke_part,
-                             double* __restrict__ scalars) {
+                             double* __restrict__ scalars, const int* halt) {
+    if (halted(halt)) return;
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= count) return;
     const Rep rp = reps[r];
@@ -172,7 +178,8 @@ __global__ void k_b_ke_final(const Rep* __restrict__ reps, int count, const doub
 
 // k_bussi per replica: eq. A7 from the replica's KE and its own draw stream (consumed only when KE > 0).
 __global__ void k_b_bussi(int count, double* __restrict__ scalars, const double* __restrict__ draws, int steps,
-                          int* __restrict__ draw_index, const double2* __restrict__ dof_kbar, double c) {
+                          int* __restrict__ draw_index, const double2* __restrict__ dof_kbar, double c, const int* halt) {
+    if (halted(halt)) return;
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= count) return;
     double* sc = scalars + r * SCALARS;
@@ -196,7 +203,8 @@ __global__ void k_b_bussi(int count, double* __restrict__ scalars, const double*
 
 __global__ void __launch_bounds__(BT)
     k_b_scale(const Rep* __restrict__ reps, const uint16_t* __restrict__ block_rep, double* __restrict__ vel,
-              size_t slots, const double* __restrict__ scalars) {
+              size_t slots, const double* __restrict__ scalars, const int* halt) {
+    if (halted(halt)) return;
     const int r = block_rep[blockIdx.x];
     const Rep rp = reps[r];
     const size_t i = static_cast<size_t>(blockIdx.x) * BT + threadIdx.x;
@@ -210,7 +218,8 @@ __global__ void __launch_bounds__(BT)
 // One block; flags[2] is the step counter.
 __global__ void __launch_bounds__(1024)
     k_b_record(int count, const double* __restrict__ scalars, int ke_slot, double* __restrict__ energies,
-               int* __restrict__ stale, int* __restrict__ rebuilds, int* __restrict__ flags) {
+               int* __restrict__ stale, int* __restrict__ rebuilds, int* __restrict__ flags, const int* halt) {
+    if (halted(halt)) return;
     const int k = flags[2];
     for (int r = threadIdx.x; r < count; r += blockDim.x) {
         const double* sc = scalars + r * SCALARS;
@@ -231,7 +240,8 @@ __global__ void __launch_bounds__(1024)
 __global__ void __launch_bounds__(BT)
     k_b_cell_count(const Rep* __restrict__ reps, const uint16_t* __restrict__ block_rep,
                    const double4* __restrict__ pos, int* __restrict__ cell_of, uint32_t* __restrict__ counts,
-                   const int* __restrict__ stale) {
+                   const int* __restrict__ stale, const int* halt) {
+    if (halted(halt)) return;
     const int r = block_rep[blockIdx.x];
     if (!stale[r]) return;
     const Rep rp = reps[r];
@@ -250,7 +260,8 @@ __global__ void __launch_bounds__(BT)
 // One block per replica: exclusive scan of its histogram into batch slots; also clears its pair total.
 __global__ void __launch_bounds__(BT)
     k_b_scan(const Rep* __restrict__ reps, const uint32_t* __restrict__ counts, uint32_t* __restrict__ start,
-             const int* __restrict__ stale, unsigned long long* __restrict__ pairs) {
+             const int* __restrict__ stale, unsigned long long* __restrict__ pairs, const int* halt) {
+    if (halted(halt)) return;
     const int r = blockIdx.x;
     if (!stale[r]) return;
     const Rep rp = reps[r];
@@ -286,7 +297,8 @@ __global__ void __launch_bounds__(BT)
 __global__ void __launch_bounds__(BT)
     k_b_scatter(const Rep* __restrict__ reps, const uint16_t* __restrict__ block_rep,
                 const int* __restrict__ cell_of, const uint32_t* __restrict__ start, uint32_t* __restrict__ fill,
-                uint32_t* __restrict__ order_raw, const int* __restrict__ stale) {
+                uint32_t* __restrict__ order_raw, const int* __restrict__ stale, const int* halt) {
+    if (halted(halt)) return;
     const int r = block_rep[blockIdx.x];
     if (!stale[r]) return;
     const Rep rp = reps[r];
@@ -305,7 +317,8 @@ __global__ void __launch_bounds__(BT)
 __global__ void __launch_bounds__(BT)
     k_b_sort(const Rep* __restrict__ reps, const uint16_t* __restrict__ cell_rep, size_t cells_total, size_t slots,
              const uint32_t* __restrict__ start, const uint32_t* __restrict__ order_raw, uint32_t* __restrict__ fill,
-             uint32_t* __restrict__ counts, BatchPermute pa, const int* __restrict__ stale) {
+             uint32_t* __restrict__ counts, BatchPermute pa, const int* __restrict__ stale, const int* halt) {
+    if (halted(halt)) return;
     const size_t g = (static_cast<size_t>(blockIdx.x) * BT + threadIdx.x) / WAVE;
     const int lane = threadIdx.x % WAVE;
     if (g >= cells_total) return;
@@ -352,7 +365,8 @@ __global__ void __launch_bounds__(BT)
                    const double4* __restrict__ pos, size_t slots, const int* __restrict__ cell_of,
                    const uint32_t* __restrict__ start, double radius2, int padding, int32_t* __restrict__ rows,
                    int32_t* __restrict__ count, int32_t* __restrict__ half, int2* __restrict__ bstats,
-                   BatchCopyBack cb, const int* __restrict__ stale) {
+                   BatchCopyBack cb, const int* __restrict__ stale, const int* halt) {
+    if (halted(halt)) return;
     const int r = block_rep[blockIdx.x];
     if (!stale[r]) return;
     const Rep rp = reps[r];
@@ -431,7 +445,8 @@ __global__ void __launch_bounds__(BT)
 // and the longest row and overflow over the stale replicas into flags[1] and flags[0].
 __global__ void k_b_build_totals(const Rep* __restrict__ reps, int count, const int2* __restrict__ bstats,
                                  int padding, int* __restrict__ flags, unsigned long long* __restrict__ pairs,
-                                 const int* __restrict__ stale) {
+                                 const int* __restrict__ stale, const int* halt) {
+    if (halted(halt)) return;
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= count || !stale[r]) return;
     const Rep rp = reps[r];
@@ -456,7 +471,8 @@ __global__ void __launch_bounds__(BT)
     k_b_forces(const Rep* __restrict__ reps, const uint16_t* __restrict__ block_rep, const double4* __restrict__ pos,
                size_t slots, const int32_t* __restrict__ rows, const int32_t* __restrict__ count,
                const int32_t* __restrict__ half, LJConst lj, double* __restrict__ frc,
-               double* __restrict__ partials) {
+               double* __restrict__ partials, const int* halt) {
+    if (halted(halt)) return;
     constexpr int PB = BT / LANES;
     const size_t p0 = static_cast<size_t>(blockIdx.x) * PB;
     const Rep rp = reps[block_rep[p0 / BT]];
@@ -493,7 +509,8 @@ __global__ void __launch_bounds__(BT)
 
 // Per replica, in block order: potential, virial, inside pairs -> scalars[0..2].
 __global__ void k_b_reduce_forces(const Rep* __restrict__ reps, int count, int lanes,
-                                  const double* __restrict__ partials, double* __restrict__ scalars) {
+                                  const double* __restrict__ partials, double* __restrict__ scalars, const int* halt) {
+    if (halted(halt)) return;
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= count) return;
     const Rep rp = reps[r];
@@ -563,6 +580,9 @@ struct md_batch {
     double* partials = nullptr; // force partials (up to blocks * 4 lanes * 3), then the kick's (blocks)
     double* ke_part = nullptr;
     double* scalars = nullptr;
+    // During md_batch_run: the overflow word. The run's kernels exit at once while it is set, so a replica's rebuild
+    // that overflows its rows freezes every replica after that step's opening kick and drift.
+    const int* halt = nullptr;
 };
 
 namespace mdgpu {
@@ -672,21 +692,21 @@ void batch_alloc(md_batch* b) {
 void enqueue_rebuild(md_batch* b) {
     hipStream_t st = b->ctx->stream;
     const size_t S = b->slots;
-    k_b_cell_count<<<b->blocks, BT, 0, st>>>(b->d_reps, b->block_rep, b->pos, b->cell_of, b->counts, b->stale);
-    k_b_scan<<<b->count, BT, 0, st>>>(b->d_reps, b->counts, b->start, b->stale, b->pairs);
+    k_b_cell_count<<<b->blocks, BT, 0, st>>>(b->d_reps, b->block_rep, b->pos, b->cell_of, b->counts, b->stale, b->halt);
+    k_b_scan<<<b->count, BT, 0, st>>>(b->d_reps, b->counts, b->start, b->stale, b->pairs, b->halt);
     k_b_scatter<<<b->blocks, BT, 0, st>>>(b->d_reps, b->block_rep, b->cell_of, b->start, b->fill, b->order_raw,
-                                          b->stale);
+                                          b->stale, b->halt);
     BatchPermute pa{b->pos, b->vel, b->frc, b->mass, b->id, b->pos_alt, b->vel_alt,
                     b->frc_alt, b->mass_alt, b->id_alt, b->ref, b->cell_of_alt};
     k_b_sort<<<blocks_for(b->cells_total * WAVE, BT), BT, 0, st>>>(b->d_reps, b->cell_rep, b->cells_total, S,
                                                                    b->start, b->order_raw, b->fill, b->counts, pa,
-                                                                   b->stale);
+                                                                   b->stale, b->halt);
     BatchCopyBack cb{b->vel_alt, b->frc_alt, b->mass_alt, b->id_alt, b->pos, b->vel, b->frc, b->mass, b->id};
     k_b_list_build<<<b->blocks, BT, 0, st>>>(b->d_reps, b->block_rep, b->pos_alt, S, b->cell_of_alt, b->start,
                                              b->radius2, b->padding, b->rows, b->rcount, b->half, b->bstats, cb,
-                                             b->stale);
+                                             b->stale, b->halt);
     k_b_build_totals<<<blocks_for(b->count, 256), 256, 0, st>>>(b->d_reps, b->count, b->bstats, b->padding,
-                                                                 b->flags, b->pairs, b->stale);
+                                                                 b->flags, b->pairs, b->stale, b->halt);
     MD_HIP(hipGetLastError());
 }
 
@@ -698,7 +718,7 @@ void enqueue_forces(md_batch* b, const LJConst& lj, int precision, int lanes, hi
     if (e0) MD_HIP(hipEventRecord(e0, st));
 #define MD_B_FORCES(M, L)                                                                                  \
     k_b_forces<M, L><<<grid, BT, 0, st>>>(b->d_reps, b->block_rep, b->pos, S, b->rows, b->rcount, b->half, lj, \
-                                          b->frc, b->partials)
+                                          b->frc, b->partials, b->halt)
     if (precision == MD_MIXED) {
         if (lanes == 4)
             MD_B_FORCES(true, 4);
@@ -713,7 +733,7 @@ void enqueue_forces(md_batch* b, const LJConst& lj, int precision, int lanes, hi
     MD_HIP(hipGetLastError());
     if (e1) MD_HIP(hipEventRecord(e1, st));
     k_b_reduce_forces<<<blocks_for(b->count, 256), 256, 0, st>>>(b->d_reps, b->count, lanes, b->partials,
-                                                                 b->scalars);
+                                                                 b->scalars, b->halt);
     MD_HIP(hipGetLastError());
 }
 
@@ -807,6 +827,12 @@ int md_batch_create(md_ctx* ctx, md_state* const* states, int count, double cuto
     b->blocks = static_cast<int>(slot / BT);
     b->ke_total = static_cast<int>(ke);
     b->padding = padding;
+    // MDGPU_BATCH_PADDING=<rows> replaces the density-based initial row length (a short one exercises the regrowth
+    // inside md_batch_run)
+    if (const char* v = std::getenv("MDGPU_BATCH_PADDING")) {
+        const int p = std::atoi(v);
+        if (p > 0) b->padding = p;
+    }
     batch_alloc(b);
     hipStream_t st = ctx->stream;
     for (int r = 0; r < count; ++r) {
@@ -959,44 +985,64 @@ int md_batch_run(md_batch* b, const md_batch_config* cfg, double* energies) {
             for (hipEvent_t e : ev) (void)hipEventDestroy(e);
         }
     } eguard{events};
-    if (timed) {
-        events.resize(2 * static_cast<size_t>(steps), nullptr);
-        for (auto& e : events) MD_HIP(hipEventCreate(&e));
-    }
-    int step_index = 0;
     const double half_dt = 0.5 * cfg->dt;
-    auto enqueue_step = [&]() {
-        k_b_kick_drift<<<b->blocks, BT, 0, st>>>(b->d_reps, b->block_rep, b->pos, b->vel, b->frc, b->mass, S, half_dt,
-                                                 cfg->dt, b->ref, b->max_disp2, b->stale);
+    struct HaltGuard {
+        md_batch* b;
+        ~HaltGuard() { b->halt = nullptr; }
+    } halt_guard{b};
+    b->halt = b->flags;
+    auto enqueue_step = [&](bool kick_drift) {
+        if (kick_drift)
+            k_b_kick_drift<<<b->blocks, BT, 0, st>>>(b->d_reps, b->block_rep, b->pos, b->vel, b->frc, b->mass, S,
+                                                     half_dt, cfg->dt, b->ref, b->max_disp2, b->stale, b->halt);
         enqueue_rebuild(b);
-        hipEvent_t e0 = timed ? events[2 * step_index] : nullptr;
-        hipEvent_t e1 = timed ? events[2 * step_index + 1] : nullptr;
-        ++step_index;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (timed) {  // one pair per force launch (a resumed step launches twice)
+            MD_HIP(hipEventCreate(&e0));
+            events.push_back(e0);
+            MD_HIP(hipEventCreate(&e1));
+            events.push_back(e1);
+        }
         enqueue_forces(b, lj, cfg->precision, lanes, e0, e1);
         if (cfg->thermostat) {
-            k_b_kick<<<b->blocks, BT, 0, st>>>(b->d_reps, b->block_rep, b->vel, b->frc, b->mass, S, half_dt, nullptr);
+            k_b_kick<<<b->blocks, BT, 0, st>>>(b->d_reps, b->block_rep, b->vel, b->frc, b->mass, S, half_dt, nullptr,
+                                               b->halt);
             k_b_ke_blocks<<<b->ke_total, BT, 0, st>>>(b->d_reps, b->block_rep, b->ke_start, b->vel, b->mass, S,
-                                                      b->ke_part);
-            k_b_ke_final<<<blocks_for(count, 256), 256, 0, st>>>(b->d_reps, count, b->ke_part, b->scalars);
+                                                      b->ke_part, b->halt);
+            k_b_ke_final<<<blocks_for(count, 256), 256, 0, st>>>(b->d_reps, count, b->ke_part, b->scalars, b->halt);
             k_b_bussi<<<blocks_for(count, 256), 256, 0, st>>>(count, b->scalars, rb.draws, steps, rb.draw_index,
-                                                              rb.dof_kbar, c);
-            k_b_scale<<<b->blocks, BT, 0, st>>>(b->d_reps, b->block_rep, b->vel, S, b->scalars);
-            k_b_record<<<1, 1024, 0, st>>>(count, b->scalars, 6, rb.energies, b->stale, b->rebuilds, b->flags);
+                                                              rb.dof_kbar, c, b->halt);
+            k_b_scale<<<b->blocks, BT, 0, st>>>(b->d_reps, b->block_rep, b->vel, S, b->scalars, b->halt);
+            k_b_record<<<1, 1024, 0, st>>>(count, b->scalars, 6, rb.energies, b->stale, b->rebuilds, b->flags,
+                                           b->halt);
         } else {
             double* kick_partials = b->partials + static_cast<size_t>(b->blocks) * 4 * 3;
             k_b_kick<<<b->blocks, BT, 0, st>>>(b->d_reps, b->block_rep, b->vel, b->frc, b->mass, S, half_dt,
-                                               kick_partials);
-            k_b_reduce_ke<<<blocks_for(count, 256), 256, 0, st>>>(b->d_reps, count, kick_partials, b->scalars);
-            k_b_record<<<1, 1024, 0, st>>>(count, b->scalars, 3, rb.energies, b->stale, b->rebuilds, b->flags);
+                                               kick_partials, b->halt);
+            k_b_reduce_ke<<<blocks_for(count, 256), 256, 0, st>>>(b->d_reps, count, kick_partials, b->scalars, b->halt);
+            k_b_record<<<1, 1024, 0, st>>>(count, b->scalars, 3, rb.energies, b->stale, b->rebuilds, b->flags,
+                                           b->halt);
         }
         MD_HIP(hipGetLastError());
     };
-    if (cfg->use_graph) {
+    // `n_steps` steps; with `resume` the first one's opening kick and drift are already done
+    auto run_steps = [&](int n_steps, bool resume) {
+        int first = 0;
+        if (resume) {
+            enqueue_step(false);
+            first = 1;
+        }
+        const int replays = n_steps - first;
+        if (replays <= 0) return;
+        if (!cfg->use_graph) {
+            for (int k = 0; k < replays; ++k) enqueue_step(true);
+            return;
+        }
         hipGraph_t graph = nullptr;
         hipGraphExec_t exec = nullptr;
         MD_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
         try {
-            enqueue_step();
+            enqueue_step(true);
         } catch (...) {
             (void)hipStreamEndCapture(st, &graph);
             if (graph) (void)hipGraphDestroy(graph);
@@ -1008,40 +1054,54 @@ int md_batch_run(md_batch* b, const md_batch_config* cfg, double* energies) {
             (void)hipGraphDestroy(graph);
             hip_check(e, "hipGraphInstantiate");
         }
-        for (int k = 0; k < steps; ++k) {
+        for (int k = 0; k < replays; ++k) {
             e = hipGraphLaunch(exec, st);
             if (e != hipSuccess) break;
         }
-        (void)hipStreamSynchronize(st);
+        const hipError_t sync = hipStreamSynchronize(st);
         (void)hipGraphExecDestroy(exec);
         (void)hipGraphDestroy(graph);
         hip_check(e, "hipGraphLaunch");
-    } else {
-        for (int k = 0; k < steps; ++k) enqueue_step();
-    }
+        hip_check(sync, "hipStreamSynchronize");
+    };
     int* hf = b->ctx->host_flags;
-    MD_HIP(hipMemcpyAsync(hf, b->flags, sizeof(int) * 4, hipMemcpyDeviceToHost, st));
+    for (int attempt = 0, done = 0;; ++attempt) {
+        run_steps(steps - done, attempt > 0);
+        MD_HIP(hipMemcpyAsync(hf, b->flags, sizeof(int) * 4, hipMemcpyDeviceToHost, st));
+        MD_HIP(hipStreamSynchronize(st));
+        if (!hf[0]) break;
+        if (attempt >= 8) fail(MD_ERR_OVERFLOW, "neighbour-list rows keep overflowing during md_batch_run");
+        // Frozen at the first unrecorded step, after its opening kick and drift and the rebuilds of its stale
+        // replicas (whose stale flags are still set). Longer rows keep every other replica's entries where they
+        // were (column-major, stride = slots); the resumed step repeats the stale replicas' rebuilds, re-sorting
+        // nothing (they are sorted already).
+        done = hf[2];
+        const int longest = hf[1];
+        const int grown = (longest + longest / 4 + 16 + 7) / 8 * 8;
+        int32_t* rows = dev_alloc<int32_t>(static_cast<size_t>(grown) * b->slots);
+        const hipError_t ce = hipMemcpyAsync(rows, b->rows, sizeof(int32_t) * b->padding * b->slots,
+                                             hipMemcpyDeviceToDevice, st);
+        if (ce != hipSuccess) {
+            dev_free(rows);
+            hip_check(ce, "hipMemcpyAsync");
+        }
+        MD_HIP(hipStreamSynchronize(st));
+        dev_free(b->rows);
+        b->rows = rows;
+        b->padding = grown;
+        MD_HIP(hipMemsetAsync(b->flags, 0, sizeof(int), st));
+    }
     if (energies)
         MD_HIP(hipMemcpyAsync(energies, rb.energies, sizeof(double) * steps * count * 4, hipMemcpyDeviceToHost, st));
     MD_HIP(hipStreamSynchronize(st));
     if (timed) {
         double total_ms = 0.0;
-        for (int k = 0; k < steps; ++k) {
+        for (size_t k = 0; k + 1 < events.size(); k += 2) {
             float ms = 0.0f;
-            MD_HIP(hipEventElapsedTime(&ms, events[2 * k], events[2 * k + 1]));
+            MD_HIP(hipEventElapsedTime(&ms, events[k], events[k + 1]));
             total_ms += ms;
         }
         *cfg->force_kernel_ms = total_ms;
-    }
-    if (hf[0]) {
-        const int longest = hf[1];
-        b->padding = (longest + longest / 4 + 16 + 7) / 8 * 8;
-        dev_free(b->rows);
-        b->rows = dev_alloc<int32_t>(static_cast<size_t>(b->padding) * b->slots);
-        b->forces_valid = false;
-        k_b_fill<<<blocks_for(count, 256), 256, 0, st>>>(b->stale, count, 1);
-        MD_HIP(hipStreamSynchronize(st));
-        fail(MD_ERR_OVERFLOW, "a neighbour-list row outgrew its padding during md_batch_run; state is invalid");
     }
     MD_API_END
 }

@@ -13,6 +13,7 @@
 //                  (its half-list row), then the j < i candidates in increasing index; r^2 < (rc+skin)^2
 //                  strictly (NeighbourList.cpp:75-98); also moves the re-sorted arrays back into place.
 #include <algorithm>
+#include <type_traits>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -37,8 +38,6 @@ struct CopyBackArgs {
     int* cell_of;
 };
 
-__device__ __forceinline__ bool gated_off(const int* gate) { return gate && *gate == 0; }
-
 // the fixed-order build: lanes per particle and block shape
 constexpr int FIXED_LANES = 16;
 constexpr int FIXED_THREADS = 256;
@@ -47,8 +46,9 @@ constexpr int FIXED_PER_BLOCK = FIXED_THREADS / FIXED_LANES;
 // Cell of every particle and the cell histogram, one atomic per run of equal cells.
 __global__ void __launch_bounds__(256) k_cell_count(const double4* __restrict__ pos, size_t n, double per_length,
                                                      int side, int* __restrict__ cell_of,
-                                                     uint32_t* __restrict__ counts, const int* gate) {
-    if (gated_off(gate)) return;
+                                                     uint32_t* __restrict__ counts, const int* gate,
+                                                     const int* halt) {
+    if (gated_off(gate, halt)) return;
     const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     const bool valid = i < n;
     int c = -1;
@@ -68,8 +68,9 @@ constexpr int SCAN_PER_THREAD = 8;
 constexpr int SCAN_ITEMS = SCAN_THREADS * SCAN_PER_THREAD;
 
 __global__ void __launch_bounds__(SCAN_THREADS) k_scan_totals(const uint32_t* __restrict__ counts, size_t ncells,
-                                                               uint32_t* __restrict__ totals, const int* gate) {
-    if (gated_off(gate)) return;
+                                                               uint32_t* __restrict__ totals, const int* gate,
+                                                     const int* halt) {
+    if (gated_off(gate, halt)) return;
     __shared__ uint32_t wsum[SCAN_THREADS / WAVE];
     const size_t base = static_cast<size_t>(blockIdx.x) * SCAN_ITEMS + threadIdx.x * SCAN_PER_THREAD;
     uint32_t s = 0;
@@ -89,8 +90,9 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_totals(const uint32_t* __
 
 __global__ void __launch_bounds__(SCAN_THREADS) k_scan_apply(const uint32_t* __restrict__ counts, size_t ncells,
                                                               const uint32_t* __restrict__ totals,
-                                                              uint32_t* __restrict__ start, const int* gate) {
-    if (gated_off(gate)) return;
+                                                              uint32_t* __restrict__ start, const int* gate,
+                                                     const int* halt) {
+    if (gated_off(gate, halt)) return;
     __shared__ uint32_t wsum[SCAN_THREADS / WAVE];
     __shared__ uint32_t prior[SCAN_THREADS / WAVE];
     {  // the particles of the blocks before this one
@@ -132,8 +134,8 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_apply(const uint32_t* __r
 __global__ void __launch_bounds__(256) k_cell_scatter(const int* __restrict__ cell_of, size_t n,
                                                        const uint32_t* __restrict__ start,
                                                        uint32_t* __restrict__ fill, uint32_t* __restrict__ order_raw,
-                                                       const int* gate) {
-    if (gated_off(gate)) return;
+                                                       const int* gate, const int* halt) {
+    if (gated_off(gate, halt)) return;
     const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     const bool valid = i < n;
     const int c = valid ? cell_of[i] : -1;
@@ -181,8 +183,9 @@ constexpr unsigned SORT_BLOCKS = 2048;
 __global__ void __launch_bounds__(256) k_cell_sort(const uint32_t* __restrict__ start, size_t ncells, size_t n,
                                                     const uint32_t* __restrict__ order_raw,
                                                     uint32_t* __restrict__ order, uint32_t* __restrict__ fill,
-                                                    uint32_t* __restrict__ counts, PermuteArgs pa, const int* gate) {
-    if (gated_off(gate)) return;
+                                                    uint32_t* __restrict__ counts, PermuteArgs pa, const int* gate,
+                                                    const int* halt) {
+    if (gated_off(gate, halt)) return;
     const int lane = threadIdx.x % WAVE;
     const size_t waves = static_cast<size_t>(gridDim.x) * (blockDim.x / WAVE);
     for (size_t cell = (static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x) / WAVE; cell < ncells;
@@ -203,8 +206,9 @@ __global__ void __launch_bounds__(256) k_list_build(const double4* __restrict__ 
                                                      double inv_box, double radius2, int use_cells, int padding,
                                                      int32_t* __restrict__ rows, int32_t* __restrict__ count,
                                                      int32_t* __restrict__ half, int2* __restrict__ stats,
-                                                     CopyBackArgs cb, const int* gate) {
-    if (gated_off(gate)) return;
+                                                     CopyBackArgs cb, const int* gate,
+                                                     const int* halt) {
+    if (gated_off(gate, halt)) return;
     const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     int cnt = 0, hcnt = 0;
     if (i < n) {
@@ -295,8 +299,8 @@ constexpr int FINISH_THREADS = 1024;
 __global__ void __launch_bounds__(FINISH_THREADS) k_finish_build(const int2* __restrict__ stats, int blocks,
                                                                  int padding, int* __restrict__ flags,
                                                                  unsigned long long* __restrict__ pair_total,
-                                                                 const int* gate) {
-    if (gated_off(gate)) return;
+                                                                 const int* gate, const int* halt) {
+    if (gated_off(gate, halt)) return;
     __shared__ int s_mx[FINISH_THREADS / WAVE];
     __shared__ unsigned long long s_hs[FINISH_THREADS / WAVE];
     int mx = 0;
@@ -322,12 +326,19 @@ __global__ void __launch_bounds__(FINISH_THREADS) k_finish_build(const int2* __r
             mx = s_mx[w] > mx ? s_mx[w] : mx;
             hs += s_hs[w];
         }
-        // longest row and overflow accumulate over the builds since the host last cleared them (md_run checks
-        // once, after all its steps); the pair total is this build's
+        // longest row and overflow accumulate over the builds since the host last cleared them (md_run reads
+        // them after its steps, and its kernels halt on the overflow word); the sticky pair records device-gated
+        // builds only and survives host-driven ones (the adjoint reads it once per call); the pair total is this
+        // build's. A device-gated build that overflowed is not counted: md_run repeats it with longer rows.
         if (mx > flags[FLAG_MAXCOUNT]) flags[FLAG_MAXCOUNT] = mx;
-        if (mx > padding) flags[FLAG_OVERFLOW] = 1;
+        if (gate && mx > flags[FLAG_MAXCOUNT_STICKY]) flags[FLAG_MAXCOUNT_STICKY] = mx;
+        if (mx > padding) {
+            flags[FLAG_OVERFLOW] = 1;
+            if (gate) flags[FLAG_OVERFLOW_STICKY] = 1;
+        } else if (gate) {
+            flags[FLAG_REBUILDS] += 1;
+        }
         *pair_total = hs;
-        if (gate) flags[FLAG_REBUILDS] += 1;
     }
 }
 
@@ -355,9 +366,9 @@ static unsigned long long* pair_total_of(md_list* l) {
     return reinterpret_cast<unsigned long long*>(l->flags + FLAG_COUNT);
 }
 
-void finish_build_enqueue(md_list* l, int blocks, const int* gate) {
+void finish_build_enqueue(md_list* l, int blocks, const int* gate, const int* halt) {
     k_finish_build<<<1, FINISH_THREADS, 0, l->ctx->stream>>>(l->build_stats, blocks, l->padding, l->flags,
-                                                             pair_total_of(l), gate);
+                                                             pair_total_of(l), gate, halt);
     MD_HIP(hipGetLastError());
 }
 
@@ -469,14 +480,15 @@ static void cells_enqueue(md_list* l, md_state* s, double per_length, bool permu
     const size_t n = s->n;
     if (n == 0) return;
     k_cell_count<<<blocks_for(n, 256), 256, 0, st>>>(s->pos, n, per_length, l->side, l->cell_of, l->cell_count,
-                                                     gate);
+                                                     gate, s->halt);
     {
         const int nb = static_cast<int>((l->ncells + SCAN_ITEMS - 1) / SCAN_ITEMS);
-        k_scan_totals<<<nb, SCAN_THREADS, 0, st>>>(l->cell_count, l->ncells, l->scan_totals, gate);
-        k_scan_apply<<<nb, SCAN_THREADS, 0, st>>>(l->cell_count, l->ncells, l->scan_totals, l->cell_start, gate);
+        k_scan_totals<<<nb, SCAN_THREADS, 0, st>>>(l->cell_count, l->ncells, l->scan_totals, gate, s->halt);
+        k_scan_apply<<<nb, SCAN_THREADS, 0, st>>>(l->cell_count, l->ncells, l->scan_totals, l->cell_start, gate,
+                                                  s->halt);
     }
     k_cell_scatter<<<blocks_for(n, 256), 256, 0, st>>>(l->cell_of, n, l->cell_start, l->scan_tmp, l->order_raw,
-                                                       gate);
+                                                       gate, s->halt);
     // the tiled build's tile setup gathers the state itself (one lane per destination particle), after this sort
     PermuteArgs pa{};
     if (permute) {
@@ -485,13 +497,14 @@ static void cells_enqueue(md_list* l, md_state* s, double per_length, bool permu
     }
     const unsigned sort_blocks = std::min<unsigned>(blocks_for(l->ncells * WAVE, 256), SORT_BLOCKS);
     k_cell_sort<<<sort_blocks, 256, 0, st>>>(l->cell_start, l->ncells, n, l->order_raw, l->order, l->scan_tmp,
-                                             l->cell_count, pa, gate);
+                                             l->cell_count, pa, gate, s->halt);
     MD_HIP(hipGetLastError());
 }
 
 static void list_kernel_enqueue(md_list* l, md_state* s, bool from_alt, const int* gate, bool carry_forces) {
     hipStream_t st = l->ctx->stream;
     const size_t n = s->n;
+    l->unstaged = -1;
     CopyBackArgs cb{};
     cb.enabled = from_alt ? 1 : 0;
     if (from_alt) {
@@ -516,9 +529,9 @@ static void list_kernel_enqueue(md_list* l, md_state* s, bool from_alt, const in
     k_list_build<<<blocks_for(n ? n : 1, 256), 256, 0, st>>>(
         from_alt ? s->pos_alt : s->pos, n, from_alt ? l->cell_of_alt : l->cell_of, l->cell_start, l->side, s->box,
         s->inv_box, l->radius2, l->side >= 3 ? 1 : 0, l->padding, l->rows, l->count, l->half, l->build_stats, cb,
-        gate);
+        gate, s->halt);
     MD_HIP(hipGetLastError());
-    finish_build_enqueue(l, static_cast<int>(blocks_for(n ? n : 1, 256)), gate);
+    finish_build_enqueue(l, static_cast<int>(blocks_for(n ? n : 1, 256)), gate, s->halt);
     l->tiled = false;
     l->rows32_valid = true;
 }
@@ -593,12 +606,13 @@ void list_enqueue_conditional_rebuild(md_list* l, md_state* s) {
 }
 
 // Fixed-order list (no re-sort): candidates are the members of the 27 cells taken through the stable cell order,
-// so particle indices never change. Rows hold the j > i neighbours first, then the j < i ones. Also records the
-// build-time positions. Used by the adjoint run, whose reverse sweep needs the same indices at every step.
-// FIXED_LANES lanes per particle, one pass over the 27 cells in stencil order (each cell's members in index
-// order, through the stable cell order): the j > i neighbours go straight to the row, ranked within the lane
-// group over the ballot; the j < i ones, in the same visiting order, wait in LDS and follow them.
-
+// so particle indices never change. Each row holds the j > i neighbours in increasing j, then the j < i ones in
+// increasing j, so a particle's force, summed along its row, depends only on the pairs within the cut-off (the
+// other entries add exact zeros) and not on when the list was built: the adjoint's forward run and its segment
+// recomputations are the same trajectory whatever the checkpoint spacing. Also records the build-time
+// positions. FIXED_LANES lanes per particle make one pass over the 27 cells in stencil order; the accepted
+// neighbours of each part are gathered in LDS over the ballot, then each is written at its rank within its part
+// (the number of smaller indices there).
 __global__ void __launch_bounds__(FIXED_THREADS) k_list_build_fixed(const double4* __restrict__ pos, size_t n,
                                                                     const int* __restrict__ cell_of,
                                                                     const uint32_t* __restrict__ start,
@@ -609,20 +623,22 @@ __global__ void __launch_bounds__(FIXED_THREADS) k_list_build_fixed(const double
                                                                     int32_t* __restrict__ count,
                                                                     int32_t* __restrict__ half,
                                                                     int2* __restrict__ stats,
-                                                                    double4* __restrict__ ref, const int* gate) {
-    if (gated_off(gate)) return;
-    extern __shared__ int32_t lower_rows[];  // FIXED_PER_BLOCK rows of `padding` entries
+                                                                    double4* __restrict__ ref, const int* gate,
+                                                                    const int* halt) {
+    if (gated_off(gate, halt)) return;
+    extern __shared__ int32_t part_rows[];  // per particle of the block: `padding` j > i, then `padding` j < i
     const int g = threadIdx.x / FIXED_LANES, lane = threadIdx.x % FIXED_LANES;
     const size_t i = static_cast<size_t>(blockIdx.x) * FIXED_PER_BLOCK + g;
     const unsigned shift = (threadIdx.x % WAVE) & ~(FIXED_LANES - 1);  // the group's first lane in the wavefront
     const unsigned below = (1u << lane) - 1u;
-    int32_t* const low = lower_rows + static_cast<size_t>(g) * padding;
+    int32_t* const upper = part_rows + static_cast<size_t>(g) * 2 * padding;
+    int32_t* const lower = upper + padding;
     int cnt = 0, lcnt = 0;  // uniform over the group
     if (i < n) {
         const double4 ri = pos[i];
         if (lane == 0) ref[i] = ri;
         const uint32_t self = static_cast<uint32_t>(i);
-        // one candidate per lane (valid lanes only), classified and ranked
+        // one candidate per lane (valid lanes only), classified and gathered
         auto chunk = [&](uint32_t j, bool valid) {
             bool near = false;
             if (valid && j != self) {
@@ -637,11 +653,11 @@ __global__ void __launch_bounds__(FIXED_THREADS) k_list_build_fixed(const double
             const unsigned ml = static_cast<unsigned>(__ballot(lo) >> shift) & 0xFFFFu;
             if (up) {
                 const int at = cnt + __popc(mu & below);
-                if (at < padding) rows[static_cast<size_t>(at) * n + i] = static_cast<int32_t>(j);
+                if (at < padding) upper[at] = static_cast<int32_t>(j);
             }
             if (lo) {
                 const int at = lcnt + __popc(ml & below);
-                if (at < padding) low[at] = static_cast<int32_t>(j);
+                if (at < padding) lower[at] = static_cast<int32_t>(j);
             }
             cnt += __popc(mu);
             lcnt += __popc(ml);
@@ -661,19 +677,26 @@ __global__ void __launch_bounds__(FIXED_THREADS) k_list_build_fixed(const double
                         }
                     }
         } else {
-            // every particle is a candidate: j > i in increasing j, then j < i in increasing j
             for (uint32_t j0 = 0; j0 < n; j0 += FIXED_LANES) {
                 const uint32_t j = j0 + static_cast<uint32_t>(lane);
                 chunk(j, j < n);
             }
         }
     }
-    __syncthreads();  // the group's LDS row is complete (written by its own lanes)
+    __syncthreads();  // the group's LDS parts are complete (written by its own lanes)
     const int total = cnt + lcnt;
     if (i < n) {
-        for (int k = lane; k < lcnt; k += FIXED_LANES) {
-            const int at = cnt + k;
-            if (at < padding) rows[static_cast<size_t>(at) * n + i] = low[k];
+        // each stored entry goes to its rank among the stored entries of its part (indices are distinct)
+        const int su = cnt < padding ? cnt : padding, sl = lcnt < padding ? lcnt : padding;
+        for (int e = lane; e < su + sl; e += FIXED_LANES) {
+            const bool is_up = e < su;
+            const int32_t* part = is_up ? upper : lower;
+            const int m = is_up ? su : sl;
+            const int32_t j = part[is_up ? e : e - su];
+            int rank = 0;
+            for (int q = 0; q < m; ++q) rank += part[q] < j ? 1 : 0;
+            const int at = (is_up ? 0 : cnt) + rank;
+            if (at < padding) rows[static_cast<size_t>(at) * n + i] = j;
         }
         if (lane == 0) {
             count[i] = total < padding ? total : padding;
@@ -690,15 +713,15 @@ static void list_fixed_enqueue(md_list* l, md_state* s, const int* gate) {
     cells_enqueue(l, s, l->side / s->box, false, gate, true);
     l->tiled = false;
     l->rows32_valid = true;
-    // each block keeps FIXED_PER_BLOCK rows of j < i neighbours in LDS (64 KB at most)
-    if (static_cast<size_t>(FIXED_PER_BLOCK) * l->padding * sizeof(int32_t) > 65536)
-        fail(MD_ERR_STATE, "fixed-order neighbour list: rows longer than 1024 entries are not supported");
+    // each block keeps both parts of FIXED_PER_BLOCK rows in LDS (64 KB at most)
+    const size_t lds = static_cast<size_t>(FIXED_PER_BLOCK) * 2 * l->padding * sizeof(int32_t);
+    if (lds > 65536) fail(MD_ERR_STATE, "fixed-order neighbour list: rows longer than 512 entries are not supported");
     const unsigned blocks = blocks_for(n ? n : 1, FIXED_PER_BLOCK);
-    k_list_build_fixed<<<blocks, FIXED_THREADS, sizeof(int32_t) * FIXED_PER_BLOCK * l->padding, st>>>(
+    k_list_build_fixed<<<blocks, FIXED_THREADS, lds, st>>>(
         s->pos, n, l->cell_of, l->cell_start, l->order, l->side, s->box, s->inv_box, l->radius2,
-        l->side >= 3 ? 1 : 0, l->padding, l->rows, l->count, l->half, l->build_stats, l->ref, gate);
+        l->side >= 3 ? 1 : 0, l->padding, l->rows, l->count, l->half, l->build_stats, l->ref, gate, s->halt);
     MD_HIP(hipGetLastError());
-    finish_build_enqueue(l, static_cast<int>(blocks), gate);
+    finish_build_enqueue(l, static_cast<int>(blocks), gate, s->halt);
 }
 
 void list_rebuild_fixed(md_list* l, md_state* s) {
@@ -829,6 +852,53 @@ int md_list_destroy(md_list* l) {
     return MD_OK;
 }
 
+int md_list_copy(md_list* dst, const md_list* src) {
+    MD_API_BEGIN
+    if (!dst || !src) fail(MD_ERR_ARG, "list is NULL");
+    if (dst == src) return MD_OK;
+    if (dst->ctx != src->ctx) fail(MD_ERR_ARG, "both lists must belong to one context");
+    hipStream_t st = src->ctx->stream;
+    MD_HIP(hipStreamSynchronize(st));
+    release_list_buffers(dst);
+    // every scalar member, then each device buffer at the source's capacity
+    md_ctx* const ctx = dst->ctx;
+    md_list copy = *src;
+    copy.ctx = ctx;
+    auto clone = [&](auto*& out, auto* in, size_t count) {
+        using T = std::remove_const_t<std::remove_pointer_t<std::remove_reference_t<decltype(out)>>>;
+        out = nullptr;
+        if (!in) return;
+        out = dev_alloc<T>(count);
+        MD_HIP(hipMemcpyAsync(out, in, sizeof(T) * count, hipMemcpyDeviceToDevice, st));
+    };
+    const size_t cells = src->ncells_cap, nc = src->n_cap ? src->n_cap : 1;
+    try {
+        clone(copy.cell_count, src->cell_count, cells);
+        clone(copy.cell_start, src->cell_start, cells);
+        clone(copy.scan_tmp, src->scan_tmp, cells);
+        clone(copy.scan_totals, src->scan_totals, (cells ? cells - 1 : 0) / 2048 + 2);  // as list_prepare sizes it
+        clone(copy.cell_of, src->cell_of, nc);
+        clone(copy.cell_of_alt, src->cell_of_alt, nc);
+        clone(copy.order_raw, src->order_raw, nc);
+        clone(copy.order, src->order, nc);
+        clone(copy.ref, src->ref, nc);
+        clone(copy.count, src->count, nc);
+        clone(copy.half, src->half, nc);
+        clone(copy.rows, src->rows, src->rows_cap);
+        clone(copy.flags, src->flags, FLAG_COUNT + 4);
+        clone(copy.build_stats, src->build_stats, src->stats_cap);
+        clone(copy.rows16, src->rows16, src->rows16_cap);
+        clone(copy.tile_ranges, src->tile_ranges, src->tiles_cap * TILE_RANGES);
+        clone(copy.tile_nr, src->tile_nr, src->tiles_cap);
+        MD_HIP(hipStreamSynchronize(st));
+    } catch (...) {
+        release_list_buffers(&copy);
+        throw;
+    }
+    *dst = copy;
+    MD_API_END
+}
+
 int md_list_build(md_list* l, md_state* s) {
     MD_API_BEGIN
     if (!l || !s) fail(MD_ERR_ARG, "list/state is NULL");
@@ -863,6 +933,20 @@ int md_list_get_info(const md_list* l, md_list_info* info) {
     info->padding = l->padding;
     info->particles = l->built ? l->n : 0;
     info->tiled = l->built && l->tiled ? 1 : 0;
+    info->tiles = 0;
+    info->unstaged_tiles = 0;
+    if (info->tiled && l->n > 0) {
+        const int tiles = tile_count(l->n);
+        info->tiles = tiles;
+        if (l->unstaged < 0) {
+            std::vector<int> nr(static_cast<size_t>(tiles));
+            hipStream_t st = l->ctx->stream;
+            MD_HIP(hipMemcpyAsync(nr.data(), l->tile_nr, sizeof(int) * nr.size(), hipMemcpyDeviceToHost, st));
+            MD_HIP(hipStreamSynchronize(st));
+            l->unstaged = static_cast<int>(std::count(nr.begin(), nr.end(), -1));
+        }
+        info->unstaged_tiles = l->unstaged;
+    }
     MD_API_END
 }
 
@@ -874,9 +958,10 @@ int md_list_half(md_list* l, uint64_t* offsets, uint64_t* neighbours) {
     hipStream_t st = l->ctx->stream;
     std::vector<int32_t> half(n), rows(neighbours ? static_cast<size_t>(l->padding) * n : 0);
     if (n) MD_HIP(hipMemcpyAsync(half.data(), l->half, sizeof(int32_t) * n, hipMemcpyDeviceToHost, st));
-    if (neighbours && n)
+    if (neighbours && n) {
         ensure_rows32(l);
         MD_HIP(hipMemcpyAsync(rows.data(), l->rows, sizeof(int32_t) * rows.size(), hipMemcpyDeviceToHost, st));
+    }
     MD_HIP(hipStreamSynchronize(st));
     uint64_t at = 0;
     for (size_t i = 0; i < n; ++i) {
@@ -898,9 +983,10 @@ int md_list_full(md_list* l, int32_t* counts, int32_t* half_counts, int32_t* row
     if (counts && n) MD_HIP(hipMemcpyAsync(counts, l->count, sizeof(int32_t) * n, hipMemcpyDeviceToHost, st));
     if (half_counts && n) MD_HIP(hipMemcpyAsync(half_counts, l->half, sizeof(int32_t) * n, hipMemcpyDeviceToHost, st));
     std::vector<int32_t> rows(rows_out ? static_cast<size_t>(l->padding) * n : 0);
-    if (rows_out && n)
+    if (rows_out && n) {
         ensure_rows32(l);
         MD_HIP(hipMemcpyAsync(rows.data(), l->rows, sizeof(int32_t) * rows.size(), hipMemcpyDeviceToHost, st));
+    }
     MD_HIP(hipStreamSynchronize(st));
     if (rows_out)
         for (size_t i = 0; i < n; ++i)

@@ -40,7 +40,9 @@ constexpr int FORCE_THREADS = 256;
 __global__ void __launch_bounds__(FORCE_THREADS)
     k_forces_fp64(const double4* __restrict__ pos, size_t n, const int32_t* __restrict__ rows,
                   const int32_t* __restrict__ count, const int32_t* __restrict__ half, double box, double inv_box,
-                  LJConst lj, double* __restrict__ frc, double* __restrict__ partials, int* __restrict__ clear_flag) {
+                  LJConst lj, double* __restrict__ frc, double* __restrict__ partials, int* __restrict__ clear_flag,
+                  const int* halt) {
+    if (halted(halt)) return;
     const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     PairSums a;
     if (i < n) {
@@ -60,7 +62,9 @@ __global__ void __launch_bounds__(FORCE_THREADS)
 __global__ void __launch_bounds__(FORCE_THREADS)
     k_forces_mixed(const double4* __restrict__ pos, size_t n, const int32_t* __restrict__ rows,
                    const int32_t* __restrict__ count, const int32_t* __restrict__ half, double box, double inv_box,
-                   LJConst lj, double* __restrict__ frc, double* __restrict__ partials, int* __restrict__ clear_flag) {
+                   LJConst lj, double* __restrict__ frc, double* __restrict__ partials, int* __restrict__ clear_flag,
+                   const int* halt) {
+    if (halted(halt)) return;
     const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     PairSums a;
     if (i < n) {
@@ -149,10 +153,10 @@ void forces_enqueue(md_state* s, md_list* l, const md_lj& p, int precision, doub
     if (ev_start) MD_HIP(hipEventRecord(ev_start, st));
     if (precision == MD_MIXED)
         k_forces_mixed<<<blocks, FORCE_THREADS, 0, st>>>(s->pos, n, l->rows, l->count, l->half, s->box, s->inv_box,
-                                                          lj, s->frc, s->partials, clear_flag);
+                                                          lj, s->frc, s->partials, clear_flag, s->halt);
     else
         k_forces_fp64<<<blocks, FORCE_THREADS, 0, st>>>(s->pos, n, l->rows, l->count, l->half, s->box, s->inv_box,
-                                                         lj, s->frc, s->partials, clear_flag);
+                                                         lj, s->frc, s->partials, clear_flag, s->halt);
     MD_HIP(hipGetLastError());
     if (ev_stop) MD_HIP(hipEventRecord(ev_stop, st));
     reduce_partials_enqueue(s, static_cast<int>(blocks), 3, out_dev);

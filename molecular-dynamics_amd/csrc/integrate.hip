@@ -18,7 +18,8 @@ constexpr int VV_THREADS = 256;
 __global__ void __launch_bounds__(VV_THREADS)
     k_kick_drift(double4* __restrict__ pos, double* __restrict__ vel, const double* __restrict__ frc,
                  const double* __restrict__ mass, size_t n, double half_dt, double dt, double box, double inv_box,
-                 const double4* __restrict__ ref, double max_disp2, int* __restrict__ stale) {
+                 const double4* __restrict__ ref, double max_disp2, int* __restrict__ stale, const int* halt) {
+    if (halted(halt)) return;
     const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const double m = mass[i];
@@ -46,7 +47,8 @@ __global__ void __launch_bounds__(VV_THREADS)
 // v += (dt/2 f) / m (Integrator.cpp:55-71); with `partials`, each block also sums (0.5 m) v^2 in a fixed order.
 __global__ void __launch_bounds__(VV_THREADS)
     k_kick(double* __restrict__ vel, const double* __restrict__ frc, const double* __restrict__ mass, size_t n,
-           double half_dt, double* __restrict__ partials) {
+           double half_dt, double* __restrict__ partials, const int* halt) {
+    if (halted(halt)) return;
     const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     double ke[1] = {0.0};
     if (i < n) {
@@ -71,7 +73,8 @@ __global__ void __launch_bounds__(VV_THREADS)
     k_kick_kick_drift(double4* __restrict__ pos, double* __restrict__ vel, const double* __restrict__ frc,
                       const double* __restrict__ mass, size_t n, double half_dt, double dt, double box,
                       double inv_box, const double4* __restrict__ ref, double max_disp2, int* __restrict__ stale,
-                      double* __restrict__ partials) {
+                      double* __restrict__ partials, const int* halt) {
+    if (halted(halt)) return;
     const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     double ke[1] = {0.0};
     if (i < n) {
@@ -109,7 +112,8 @@ __global__ void __launch_bounds__(VV_THREADS)
 // (Thermostat.cpp:41-57). Writes the velocity scale to scalars[5] and the new kinetic energy to scalars[6].
 // Draws are consumed only when KE > 0, as the reference draws only then.
 __global__ void k_bussi(double* __restrict__ scalars, const double* __restrict__ draws, int* __restrict__ draw_index,
-                        double dof, double kbar, double c) {
+                        double dof, double kbar, double c, const int* halt) {
+    if (halted(halt)) return;
     const double ke = scalars[3];
     if (!(ke > 0.0)) {
         scalars[5] = 1.0;
@@ -126,14 +130,17 @@ __global__ void k_bussi(double* __restrict__ scalars, const double* __restrict__
     scalars[6] = ke_new;
 }
 
-__global__ void k_scale_dev(double* __restrict__ vel, size_t count, const double* __restrict__ scale) {
+__global__ void k_scale_dev(double* __restrict__ vel, size_t count, const double* __restrict__ scale,
+                            const int* halt) {
+    if (halted(halt)) return;
     const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < count) vel[i] = vel[i] * *scale;
 }
 
 // Per-step record: potential, kinetic, virial, inside pairs -> energies[4 * step].
 __global__ void k_record(const double* __restrict__ scalars, int ke_slot, double* __restrict__ energies,
-                         int* __restrict__ step) {
+                         int* __restrict__ step, const int* halt) {
+    if (halted(halt)) return;
     const int k = *step;
     energies[4 * k + 0] = scalars[0];
     energies[4 * k + 1] = scalars[ke_slot];
@@ -147,7 +154,7 @@ void kick_drift_enqueue(md_state* s, double dt, md_list* check_list) {
     k_kick_drift<<<blocks_for(s->n, VV_THREADS), VV_THREADS, 0, s->ctx->stream>>>(
         s->pos, s->vel, s->frc, s->mass, s->n, 0.5 * dt, dt, s->box, s->inv_box,
         check_list ? check_list->ref : nullptr, check_list ? check_list->max_disp2 : 0.0,
-        check_list ? check_list->flags + FLAG_STALE : nullptr);
+        check_list ? check_list->flags + FLAG_STALE : nullptr, s->halt);
     MD_HIP(hipGetLastError());
 }
 
@@ -156,7 +163,7 @@ void kick_enqueue(md_state* s, double dt, double* ke_dev) {
     const unsigned blocks = blocks_for(s->n, VV_THREADS);
     if (ke_dev) state_reserve_partials(s, blocks);
     k_kick<<<blocks, VV_THREADS, 0, s->ctx->stream>>>(s->vel, s->frc, s->mass, s->n, 0.5 * dt,
-                                                      ke_dev ? s->partials : nullptr);
+                                                      ke_dev ? s->partials : nullptr, s->halt);
     MD_HIP(hipGetLastError());
     if (ke_dev) reduce_partials_enqueue(s, static_cast<int>(blocks), 1, ke_dev);
 }
@@ -280,7 +287,7 @@ int md_run(md_state* s, md_list* l, const md_run_config* cfg, double* energies) 
     // 256-thread kernels, or per-tile rows of the tiled force kernel, whichever is more
     state_reserve_partials(s, static_cast<size_t>(std::max<size_t>(blocks_for(s->n, 256), tile_count(s->n))) * 4 + 8);
 
-    // optional force-kernel timing: one event pair per step, eager launches only
+    // optional force-kernel timing: one event pair per force launch, eager launches only
     const bool timed = cfg->force_kernel_ms != nullptr && !cfg->use_graph;
     std::vector<hipEvent_t> events;
     struct EventGuard {
@@ -289,17 +296,28 @@ int md_run(md_state* s, md_list* l, const md_run_config* cfg, double* energies) 
             for (hipEvent_t e : ev) (void)hipEventDestroy(e);
         }
     } guard{events};
-    if (timed) {
-        events.resize(2 * static_cast<size_t>(steps), nullptr);
-        for (auto& e : events) MD_HIP(hipEventCreate(&e));
-    }
-    int step_index = 0;
+    auto event_pair = [&](hipEvent_t& e0, hipEvent_t& e1) {
+        e0 = e1 = nullptr;
+        if (!timed) return;
+        MD_HIP(hipEventCreate(&e0));
+        events.push_back(e0);
+        MD_HIP(hipEventCreate(&e1));
+        events.push_back(e1);
+    };
+
+    // Every kernel of the run exits at once after a rebuild has overflowed its rows (md_state::halt): the state
+    // then stands after the opening kick and drift of the step whose rebuild overflowed, and the steps before it
+    // are recorded. The host grows the rows and resumes from there, repeating that rebuild.
+    struct HaltGuard {
+        md_state* s;
+        ~HaltGuard() { s->halt = nullptr; }
+    } halt_guard{s};
+    s->halt = l->flags + FLAG_OVERFLOW;
 
     auto enqueue_forces = [&]() {
         list_enqueue_conditional_rebuild(l, s);
-        hipEvent_t e0 = timed ? events[2 * step_index] : nullptr;
-        hipEvent_t e1 = timed ? events[2 * step_index + 1] : nullptr;
-        ++step_index;
+        hipEvent_t e0, e1;
+        event_pair(e0, e1);
         forces_enqueue(s, l, cfg->lj, cfg->precision, s->scalars, l->flags + FLAG_STALE, e0, e1);
     };
     // NVE: steps k and k+1 meet in one fused kernel (closing kick of k, its kinetic energy, opening kick-drift of
@@ -309,123 +327,39 @@ int md_run(md_state* s, md_list* l, const md_run_config* cfg, double* energies) 
     auto enqueue_nve_body = [&]() {
         k_kick_kick_drift<<<vv_blocks, VV_THREADS, 0, st>>>(s->pos, s->vel, s->frc, s->mass, s->n, 0.5 * cfg->dt,
                                                            cfg->dt, s->box, s->inv_box, l->ref, l->max_disp2,
-                                                           l->flags + FLAG_STALE, s->partials);
+                                                           l->flags + FLAG_STALE, s->partials, s->halt);
         reduce_partials_enqueue(s, static_cast<int>(vv_blocks), 1, s->scalars + 3);
-        k_record<<<1, 1, 0, st>>>(s->scalars, 3, rb.energies, rb.counters);
+        k_record<<<1, 1, 0, st>>>(s->scalars, 3, rb.energies, rb.counters, s->halt);
         enqueue_forces();
         MD_HIP(hipGetLastError());
     };
-    auto enqueue_step = [&]() {
-        kick_drift_enqueue(s, cfg->dt, l);
+    auto enqueue_step = [&](bool kick_drift) {
+        if (kick_drift) kick_drift_enqueue(s, cfg->dt, l);
         enqueue_forces();
         if (cfg->thermostat) {
             kick_enqueue(s, cfg->dt, nullptr);
             kinetic_exact_enqueue(s, s->scalars + 3);
-            k_bussi<<<1, 1, 0, st>>>(s->scalars, rb.draws, rb.counters + 1, dof, kbar, c);
-            k_scale_dev<<<blocks_for(nvel, 256), 256, 0, st>>>(s->vel, nvel, s->scalars + 5);
-            k_record<<<1, 1, 0, st>>>(s->scalars, 6, rb.energies, rb.counters);
+            k_bussi<<<1, 1, 0, st>>>(s->scalars, rb.draws, rb.counters + 1, dof, kbar, c, s->halt);
+            k_scale_dev<<<blocks_for(nvel, 256), 256, 0, st>>>(s->vel, nvel, s->scalars + 5, s->halt);
+            k_record<<<1, 1, 0, st>>>(s->scalars, 6, rb.energies, rb.counters, s->halt);
         } else {
             kick_enqueue(s, cfg->dt, s->scalars + 3);
-            k_record<<<1, 1, 0, st>>>(s->scalars, 3, rb.energies, rb.counters);
+            k_record<<<1, 1, 0, st>>>(s->scalars, 3, rb.energies, rb.counters, s->halt);
         }
         MD_HIP(hipGetLastError());
     };
-
-    const bool fused = !cfg->thermostat;
-    // Mixed NVE on the tiled rows goes further: the force kernel itself carries each step (closing kick, kinetic
-    // energy, opening kick and drift of the next step) for its own particles, reading the positions from one
-    // buffer and writing the next step's into the other. Step k (k = 1 .. steps - 1) reads buffer (k - 1) % 2 and
-    // is gated by the stale word of that parity, which its force kernel clears while setting the other one. The
-    // head is the first kick-drift; the tail is the last step's forces, closing kick and record.
-    const char* unfolded = std::getenv("MDGPU_UNFOLDED");  // 1: keep the separate kick-kick-drift kernel
-    const bool fold = fused && cfg->precision == MD_MIXED && tiling_enabled(s->n) && l->tiled && steps >= 2 &&
-                      !(unfolded && unfolded[0] == '1');
-    double4* bufs[2] = {s->pos, nullptr};
-    if (fold) {
-        if (!s->pos_spare) s->pos_spare = dev_alloc<double4>(s->n);
-        bufs[1] = s->pos_spare;
-    }
-    struct SpareGuard {  // however the run ends, the spare is the buffer that s->pos is not
-        md_state* s;
-        double4* b0;
-        double4* b1;
-        bool on;
-        ~SpareGuard() {
-            if (on) s->pos_spare = s->pos == b0 ? b1 : b0;
+    // `body` enqueued `replays` times: captured once and replayed as a hipGraph, or launched eagerly
+    auto repeat = [&](int replays, auto&& body) {
+        if (replays <= 0) return;
+        if (!cfg->use_graph) {
+            for (int k = 0; k < replays; ++k) body();
+            return;
         }
-    } spare_guard{s, bufs[0], bufs[1], fold};
-    auto enqueue_fold_body = [&](int p) {
-        s->pos = bufs[p];
-        int* gate = l->flags + (p ? FLAG_STALE_ODD : FLAG_STALE);
-        int* next = l->flags + (p ? FLAG_STALE : FLAG_STALE_ODD);
-        list_enqueue_conditional_rebuild(l, s, gate);
-        hipEvent_t e0 = timed ? events[2 * step_index] : nullptr;
-        hipEvent_t e1 = timed ? events[2 * step_index + 1] : nullptr;
-        ++step_index;
-        tiled_forces_step_enqueue(s, l, cfg->lj, cfg->dt, bufs[1 - p], gate, next, s->scalars,
-                                  StepRecord{rb.energies, rb.counters}, e0, e1);
-    };
-    if (fold) {
-        kick_drift_enqueue(s, cfg->dt, l);
-        const int bodies = steps - 1;
-        if (cfg->use_graph && bodies >= 2) {
-            // one graph holds an even and an odd step (their buffers differ)
-            hipGraph_t graph = nullptr;
-            hipGraphExec_t exec = nullptr;
-            MD_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-            try {
-                enqueue_fold_body(0);
-                enqueue_fold_body(1);
-            } catch (...) {
-                (void)hipStreamEndCapture(st, &graph);
-                if (graph) (void)hipGraphDestroy(graph);
-                s->pos = bufs[0];
-                throw;
-            }
-            MD_HIP(hipStreamEndCapture(st, &graph));
-            hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-            if (e != hipSuccess) {
-                (void)hipGraphDestroy(graph);
-                s->pos = bufs[0];
-                hip_check(e, "hipGraphInstantiate");
-            }
-            for (int k = 0; k < bodies / 2; ++k) {
-                e = hipGraphLaunch(exec, st);
-                if (e != hipSuccess) break;
-            }
-            (void)hipStreamSynchronize(st);
-            (void)hipGraphExecDestroy(exec);
-            (void)hipGraphDestroy(graph);
-            if (e != hipSuccess) s->pos = bufs[0];
-            hip_check(e, "hipGraphLaunch");
-            if (bodies % 2) enqueue_fold_body(0);
-        } else {
-            for (int k = 1; k <= bodies; ++k) enqueue_fold_body((k - 1) % 2);
-        }
-        // tail: the last step's forces (stored), closing kick and record
-        const int p = (steps - 1) % 2;
-        s->pos = bufs[p];
-        int* gate = l->flags + (p ? FLAG_STALE_ODD : FLAG_STALE);
-        list_enqueue_conditional_rebuild(l, s, gate);
-        hipEvent_t e0 = timed ? events[2 * step_index] : nullptr;
-        hipEvent_t e1 = timed ? events[2 * step_index + 1] : nullptr;
-        ++step_index;
-        forces_enqueue(s, l, cfg->lj, cfg->precision, s->scalars, gate, e0, e1);
-    }
-    const int replays = fold ? 0 : (fused ? steps - 1 : steps);
-    if (fused && !fold) {  // head: the first step's kick-drift and forces
-        kick_drift_enqueue(s, cfg->dt, l);
-        enqueue_forces();
-    }
-    if (cfg->use_graph && replays > 0) {
         hipGraph_t graph = nullptr;
         hipGraphExec_t exec = nullptr;
         MD_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
         try {
-            if (fused)
-                enqueue_nve_body();
-            else
-                enqueue_step();
+            body();
         } catch (...) {
             (void)hipStreamEndCapture(st, &graph);
             if (graph) (void)hipGraphDestroy(graph);
@@ -441,33 +375,130 @@ int md_run(md_state* s, md_list* l, const md_run_config* cfg, double* energies) 
             e = hipGraphLaunch(exec, st);
             if (e != hipSuccess) break;
         }
-        (void)hipStreamSynchronize(st);
+        const hipError_t sync = hipStreamSynchronize(st);
         (void)hipGraphExecDestroy(exec);
         (void)hipGraphDestroy(graph);
         hip_check(e, "hipGraphLaunch");
-    } else {
-        for (int k = 0; k < replays; ++k) {
-            if (fused)
-                enqueue_nve_body();
-            else
-                enqueue_step();
+        hip_check(sync, "hipStreamSynchronize");
+    };
+
+    const bool fused = !cfg->thermostat;
+    // Mixed NVE on the tiled rows goes further: the force kernel itself carries each step (closing kick, kinetic
+    // energy, opening kick and drift of the next step) for its own particles, reading the positions from one
+    // buffer and writing the next step's into the other. Step k (k = 1 .. steps - 1) reads buffer (k - 1) % 2 and
+    // is gated by the stale word of that parity, which its force kernel clears while setting the other one. The
+    // head is the first kick-drift; the tail is the last step's forces, closing kick and record.
+    const char* unfolded = std::getenv("MDGPU_UNFOLDED");  // 1: keep the separate kick-kick-drift kernel
+    const bool can_fold = fused && cfg->precision == MD_MIXED && tiling_enabled(s->n) && l->tiled &&
+                          !(unfolded && unfolded[0] == '1');
+    if (can_fold && steps >= 2 && !s->pos_spare) s->pos_spare = dev_alloc<double4>(s->n);
+    double4* bufs[2] = {nullptr, nullptr};
+    struct SpareGuard {  // on an exception the spare is whichever buffer s->pos is not
+        md_state* s;
+        double4** b;
+        bool on = false;
+        ~SpareGuard() {
+            if (on) s->pos_spare = s->pos == b[0] ? b[1] : b[0];
         }
-    }
-    if (fused) {  // tail: the last step's closing kick and record
-        kick_enqueue(s, cfg->dt, s->scalars + 3);
-        k_record<<<1, 1, 0, st>>>(s->scalars, 3, rb.energies, rb.counters);
-        MD_HIP(hipGetLastError());
-    }
+    } spare_guard{s, bufs};
+    auto enqueue_fold_body = [&](int p) {
+        s->pos = bufs[p];
+        int* gate = l->flags + (p ? FLAG_STALE_ODD : FLAG_STALE);
+        int* next = l->flags + (p ? FLAG_STALE : FLAG_STALE_ODD);
+        list_enqueue_conditional_rebuild(l, s, gate);
+        hipEvent_t e0, e1;
+        event_pair(e0, e1);
+        tiled_forces_step_enqueue(s, l, cfg->lj, cfg->dt, bufs[1 - p], gate, next, s->scalars,
+                                  StepRecord{rb.energies, rb.counters}, e0, e1);
+    };
+
+    // One stretch of `n_steps` steps. With `resume` the first step's opening kick and drift are already done (the
+    // state a halted stretch left). Returns true when it ran folded (positions in the two buffers).
+    auto run_steps = [&](int n_steps, bool resume) -> bool {
+        const bool fold = can_fold && n_steps >= 2;
+        if (fold) {
+            bufs[0] = s->pos;
+            bufs[1] = s->pos_spare;
+            spare_guard.on = true;
+            if (!resume) kick_drift_enqueue(s, cfg->dt, l);
+            const int bodies = n_steps - 1;
+            if (cfg->use_graph && bodies >= 2) {
+                // one graph holds an even and an odd step (their buffers differ)
+                repeat(bodies / 2, [&]() {
+                    enqueue_fold_body(0);
+                    enqueue_fold_body(1);
+                });
+                if (bodies % 2) enqueue_fold_body(0);
+            } else {
+                for (int k = 1; k <= bodies; ++k) enqueue_fold_body((k - 1) % 2);
+            }
+            // tail: the last step's forces (stored), closing kick and record
+            const int p = (n_steps - 1) % 2;
+            s->pos = bufs[p];
+            int* gate = l->flags + (p ? FLAG_STALE_ODD : FLAG_STALE);
+            list_enqueue_conditional_rebuild(l, s, gate);
+            hipEvent_t e0, e1;
+            event_pair(e0, e1);
+            forces_enqueue(s, l, cfg->lj, cfg->precision, s->scalars, gate, e0, e1);
+            kick_enqueue(s, cfg->dt, s->scalars + 3);
+            k_record<<<1, 1, 0, st>>>(s->scalars, 3, rb.energies, rb.counters, s->halt);
+            MD_HIP(hipGetLastError());
+            s->pos_spare = bufs[1 - p];
+            spare_guard.on = false;
+            return true;
+        }
+        if (fused) {
+            // head: the first step's kick-drift and forces; bodies; tail: the last step's closing kick and record
+            if (!resume) kick_drift_enqueue(s, cfg->dt, l);
+            enqueue_forces();
+            repeat(n_steps - 1, enqueue_nve_body);
+            kick_enqueue(s, cfg->dt, s->scalars + 3);
+            k_record<<<1, 1, 0, st>>>(s->scalars, 3, rb.energies, rb.counters, s->halt);
+            MD_HIP(hipGetLastError());
+            return false;
+        }
+        int first = 0;
+        if (resume) {
+            enqueue_step(false);
+            first = 1;
+        }
+        repeat(n_steps - first, [&]() { enqueue_step(true); });
+        return false;
+    };
+
     int* hf = s->ctx->host_flags;
-    MD_HIP(hipMemcpyAsync(hf, l->flags, sizeof(int) * (FLAG_COUNT + 2), hipMemcpyDeviceToHost, st));
+    int done = 0;  // steps recorded before the current stretch
+    for (int attempt = 0;; ++attempt) {
+        const bool folded = run_steps(steps - done, attempt > 0);
+        MD_HIP(hipMemcpyAsync(hf, l->flags, sizeof(int) * (FLAG_COUNT + 2), hipMemcpyDeviceToHost, st));
+        MD_HIP(hipMemcpyAsync(hf + FLAG_COUNT + 2, rb.counters, sizeof(int), hipMemcpyDeviceToHost, st));
+        MD_HIP(hipStreamSynchronize(st));
+        if (!hf[FLAG_OVERFLOW]) break;
+        if (attempt >= 8) fail(MD_ERR_OVERFLOW, "neighbour-list rows keep overflowing during md_run");
+        // the stretch froze at its step j (1-based) = the first one without a record, after that step's opening
+        // kick and drift and its re-sort; folded, the positions are in the buffer step j reads
+        const int recorded = hf[FLAG_COUNT + 2];
+        const int j = recorded - done + 1;
+        if (folded) {
+            s->pos = bufs[(j - 1) % 2];
+            s->pos_spare = bufs[j % 2];
+        }
+        done = recorded;
+        // longer rows (their contents are rebuilt), then the frozen step again from its rebuild: its gate is set
+        l->padding = (hf[FLAG_MAXCOUNT] + hf[FLAG_MAXCOUNT] / 4 + 16 + 7) / 8 * 8;
+        list_prepare(l, s);
+        MD_HIP(hipMemsetAsync(l->flags + FLAG_OVERFLOW, 0, sizeof(int), st));
+        MD_HIP(hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(l->flags + FLAG_STALE), 1, 1, st));
+        MD_HIP(hipMemsetAsync(l->flags + FLAG_STALE_ODD, 0, sizeof(int), st));
+    }
     if (energies)
         MD_HIP(hipMemcpyAsync(energies, rb.energies, 4 * sizeof(double) * steps, hipMemcpyDeviceToHost, st));
     MD_HIP(hipStreamSynchronize(st));
     if (timed) {
         double total_ms = 0.0;
-        for (int k = 0; k < steps; ++k) {
+        for (size_t k = 0; k + 1 < events.size(); k += 2) {
             float ms = 0.0f;
-            MD_HIP(hipEventElapsedTime(&ms, events[2 * k], events[2 * k + 1]));
+            MD_HIP(hipEventElapsedTime(&ms, events[k], events[k + 1]));
             total_ms += ms;
         }
         *cfg->force_kernel_ms = total_ms;
@@ -480,11 +511,6 @@ int md_run(md_state* s, md_list* l, const md_run_config* cfg, double* energies) 
         std::memcpy(&total, hf + FLAG_COUNT, sizeof(total));
         l->half_pairs = static_cast<size_t>(total);
         l->max_count = hf[FLAG_MAXCOUNT];
-    }
-    if (hf[FLAG_OVERFLOW]) {
-        l->built = false;  // rows were truncated: the next call rebuilds with more padding
-        l->padding = (hf[FLAG_MAXCOUNT] + hf[FLAG_MAXCOUNT] / 4 + 16 + 7) / 8 * 8;
-        fail(MD_ERR_OVERFLOW, "a neighbour-list row outgrew its padding during md_run; state is invalid");
     }
     MD_API_END
 }

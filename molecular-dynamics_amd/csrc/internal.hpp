@@ -64,8 +64,17 @@ enum Flag : int {
     FLAG_REBUILDS = 3,  // rebuilds performed on the device
     FLAG_FORCE = 4,     // host-requested rebuild (first build, n changed)
     FLAG_STALE_ODD = 5, // FLAG_STALE of the odd steps of a fused md_run (each step's kernel sets the next one's)
+    FLAG_OVERFLOW_STICKY = 6,  // FLAG_OVERFLOW and FLAG_MAXCOUNT of the device-gated builds only, cleared only by
+    FLAG_MAXCOUNT_STICKY = 7,  // the caller that reads them (host-driven builds clear the other two)
     FLAG_COUNT = 8
 };
+
+// True inside a kernel of a halted md_run (see md_state::halt).
+__device__ __forceinline__ bool halted(const int* halt) { return halt && *halt != 0; }
+// A device-gated rebuild kernel runs only when its gate word is set and the run is not halted.
+__device__ __forceinline__ bool gated_off(const int* gate, const int* halt) {
+    return (gate && *gate == 0) || halted(halt);
+}
 
 }  // namespace mdgpu
 
@@ -115,6 +124,9 @@ struct md_state {
     double* scalars = nullptr;  // [0..2] U, W, inside; [3] KE; [4] KE fast; [5..7] scratch
     double* red_blocks = nullptr;   // per-block sums of the partials reduction
     unsigned* red_ticket = nullptr; // its completion counter (zero between launches)
+    // During md_run: the list's overflow word. Every kernel the run enqueues on this state exits at once while it
+    // is set, so a rebuild that overflows its rows freezes the state after that step's opening kick and drift.
+    const int* halt = nullptr;
 };
 
 struct md_list {
@@ -157,6 +169,7 @@ struct md_list {
     bool tiled = false;              // rows16 / tile_* describe the current rows
     bool rows32_valid = false;       // `rows` (int32) matches the current list
     bool slots_only = false;         // device-gated builds may write only rows16 (mixed md_run)
+    mutable int unstaged = -1;       // tiles with tile_nr = -1 as of the last build (-1: not counted yet)
 };
 
 namespace mdgpu {
@@ -214,7 +227,7 @@ void displacement_check(md_list* l, md_state* s);                      // sets F
 void list_rebuild_fixed(md_list* l, md_state* s);                      // list without re-sorting (host-driven)
 void list_enqueue_conditional_rebuild_fixed(md_list* l, md_state* s); // same, gated by FLAG_STALE
 // build totals (FLAG_MAXCOUNT, FLAG_OVERFLOW, half-pair total, FLAG_REBUILDS when gated) from `blocks` records
-void finish_build_enqueue(md_list* l, int blocks, const int* gate);
+void finish_build_enqueue(md_list* l, int blocks, const int* gate, const int* halt);
 
 // api.hip
 void state_reserve_partials(md_state* s, size_t count);

@@ -50,7 +50,10 @@ __device__ __forceinline__ void fp64_pair(const double4& ri, const double4& rj, 
 // Mixed term from a double separation d (already the minimum image): r^2 in double rounded once to float, the
 // LJ factor F/r in float (Newton-refined hardware reciprocal), then F/r * d accumulated in double. Rounding r^2
 // once keeps the relative error of the steep r^-14 force at a few float ulps; the reference's float path
-// (r^2 from the narrowed d) can lose about twice that.
+// (r^2 from the narrowed d) can lose about twice that. The cut-off test is taken on the double r^2, so exactly
+// the FP64 path's pairs interact: the shifted potential's force jumps by 24 eps (2 (s/rc)^12 - (s/rc)^6) / rc
+// (0.039 at rc = 2.5) at the cut-off, and a test on the float r^2 (the reference's, Forcefield.cpp:395-402)
+// moves the pairs within float rounding of rc^2 across it (about ten of them at 4M atoms).
 __device__ __forceinline__ void mixed_pair_term(double dx, double dy, double dz, const LJConst& lj,
                                                 bool count_energy, PairSums& a) {
     const double r2d = fma(dx, dx, fma(dy, dy, dz * dz));
@@ -60,7 +63,7 @@ __device__ __forceinline__ void mixed_pair_term(double dx, double dy, double dz,
     const float s2 = lj.sig2f * inv_r2;
     const float s6 = s2 * s2 * s2;
     const float s12 = s6 * s6;
-    const bool inside = r2 < lj.rc2f;
+    const bool inside = r2d < lj.rc2;
     const float ff = inside ? lj.c24f * fmaf(2.0f, s12, -s6) * inv_r2 : 0.0f;
     const double f = static_cast<double>(ff);
     a.fx = fma(f, dx, a.fx);
@@ -86,12 +89,13 @@ struct PairSumsF {
     int c = 0;
 };
 
-// One pair of a full row. Outside rc inv_r2 is zero, which makes the force and the energy terms zero.
+// One pair of a full row. Outside rc inv_r2 is zero, which makes the force and the energy terms zero; the cut-off
+// test is on the double r^2, as in mixed_pair_term.
 __device__ __forceinline__ void mixed_pair_term_f(double dx, double dy, double dz, const LJConst& lj, PairSumsF& a) {
     const double r2d = fma(dx, dx, fma(dy, dy, dz * dz));
     const float r2 = static_cast<float>(r2d);
     const float q = __builtin_amdgcn_rcpf(r2);
-    const bool inside = r2 < lj.rc2f;
+    const bool inside = r2d < lj.rc2;
     const float inv_r2 = inside ? fmaf(q, fmaf(-r2, q, 1.0f), q) : 0.0f;
     const float s2 = lj.sig2f * inv_r2;
     const float s6 = s2 * s2 * s2;

@@ -279,8 +279,9 @@ __device__ __forceinline__ void store_gathered(const PermuteArgs& pa, const Gath
 __global__ void __launch_bounds__(SETUP_TILES * WAVE)
     k_tile_setup(size_t n, const int* __restrict__ cell_of, const uint32_t* __restrict__ start, int side,
                  int use_cells, int hmax, int tiles, double flat_cells, int4* __restrict__ tile_ranges,
-                 int* __restrict__ tile_nr, const uint32_t* __restrict__ order, PermuteArgs pa, const int* gate) {
-    if (gate && *gate == 0) return;
+                 int* __restrict__ tile_nr, const uint32_t* __restrict__ order, PermuteArgs pa, const int* gate,
+                 const int* halt) {
+    if (gated_off(gate, halt)) return;
     __shared__ int2 cr_all[SETUP_TILES][CR_CAP];
     __shared__ int nr_all[SETUP_TILES];
     __shared__ int flat_all[SETUP_TILES];
@@ -1057,8 +1058,8 @@ __global__ void __launch_bounds__(TILE_THREADS, 6)
                        int padding, int write32, int row_len, int32_t* __restrict__ rows,
                        uint16_t* __restrict__ rows16, int32_t* __restrict__ count, int32_t* __restrict__ half,
                        int2* __restrict__ stats, const int4* __restrict__ tile_ranges,
-                       const int* __restrict__ tile_nr, TiledCopyBack cb, const int* gate) {
-    if (gate && *gate == 0) return;
+                       const int* __restrict__ tile_nr, TiledCopyBack cb, const int* gate, const int* halt) {
+    if (gated_off(gate, halt)) return;
     extern __shared__ double lds[];
     list_build_tile<FH>(xcd_item(static_cast<int>(blockIdx.x), ta.tiles), lds, pos, n, cell_of, start, ta, box,
                         inv_box, radius2, padding, write32, row_len, rows, rows16, count, half, stats, tile_ranges,
@@ -1129,7 +1130,8 @@ __global__ void __launch_bounds__(TILE_THREADS)
                          const int32_t* __restrict__ rows, const int32_t* __restrict__ count,
                          const int4* __restrict__ tile_ranges, const int* __restrict__ tile_nr, int row_len,
                          LJConst lj, TileFrame fr, double* __restrict__ frc, double* __restrict__ partials,
-                         int* __restrict__ clear_flag, ForceStep fs) {
+                         int* __restrict__ clear_flag, ForceStep fs, const int* halt) {
+    if (halted(halt)) return;
     // The halo is a static array: its address is known when the kernel is compiled, so a slot's byte offset, as the
     // rows hold it, is the whole address computation.
     __shared__ double halo[3][TILE_HALO_MAX];
@@ -1369,7 +1371,7 @@ void tiled_list_enqueue(md_list* l, md_state* s, bool from_alt, const int* gate,
     hipStream_t st = l->ctx->stream;
     const size_t n = s->n;
     if (n == 0) {
-        finish_build_enqueue(l, 0, gate);
+        finish_build_enqueue(l, 0, gate, s->halt);
         return;
     }
     if (l->hmax > TILE_HALO_MAX) fail(MD_ERR_STATE, "tile halo capacity exceeds the LDS bound");
@@ -1403,7 +1405,7 @@ void tiled_list_enqueue(md_list* l, md_state* s, bool from_alt, const int* gate,
     const double flat_cells = (0.5 * s->box - l->skin) / (s->box / l->side);
     hipLaunchKernelGGL(k_tile_setup, dim3((tiles + SETUP_TILES - 1) / SETUP_TILES), dim3(SETUP_TILES * WAVE), 0, st,
                        n, cells, l->cell_start, ta.side, ta.use_cells, ta.hmax, tiles, flat_cells, l->tile_ranges,
-                       l->tile_nr, l->order, gather, gate);
+                       l->tile_nr, l->order, gather, gate, s->halt);
     // the float halo where the grid is wide enough for compact frames and the cells dense enough for the
     // candidate lists (MDGPU_BUILD_F64=1 keeps the double halo)
     const char* f64 = std::getenv("MDGPU_BUILD_F64");
@@ -1414,9 +1416,9 @@ void tiled_list_enqueue(md_list* l, md_state* s, bool from_alt, const int* gate,
     hipLaunchKernelGGL(kernel, dim3(tiles), dim3(TILE_THREADS), lds, st, build_pos, n, cells,
                        l->cell_start, ta, s->box, s->inv_box, l->radius2, l->padding, write32 ? 1 : 0,
                        slot_row_len(l->padding), l->rows, l->rows16, l->count, l->half, l->build_stats,
-                       l->tile_ranges, l->tile_nr, cb, gate);
+                       l->tile_ranges, l->tile_nr, cb, gate, s->halt);
     MD_HIP(hipGetLastError());
-    finish_build_enqueue(l, tiles, gate);
+    finish_build_enqueue(l, tiles, gate, s->halt);
     MD_HIP(hipGetLastError());
 }
 
@@ -1438,11 +1440,11 @@ static void tiled_forces_launch(md_state* s, md_list* l, const LJConst& lj, doub
         if (step)
             hipLaunchKernelGGL(k_forces_tiled_mixed<true>, dim3(tiles), dim3(TILE_THREADS), 0, st, s->pos, n,
                                l->rows16, l->rows, l->count, l->tile_ranges, l->tile_nr, slot_row_len(l->padding),
-                               lj, fr, s->frc, s->partials, clear_flag, *step);
+                               lj, fr, s->frc, s->partials, clear_flag, *step, s->halt);
         else
             hipLaunchKernelGGL(k_forces_tiled_mixed<false>, dim3(tiles), dim3(TILE_THREADS), 0, st, s->pos, n,
                                l->rows16, l->rows, l->count, l->tile_ranges, l->tile_nr, slot_row_len(l->padding),
-                               lj, fr, s->frc, s->partials, clear_flag, ForceStep{});
+                               lj, fr, s->frc, s->partials, clear_flag, ForceStep{}, s->halt);
         MD_HIP(hipGetLastError());
     }
     if (ev_stop) MD_HIP(hipEventRecord(ev_stop, st));

@@ -15,7 +15,7 @@ struct md_list;
 class NeighbourList {
 public:
     NeighbourList(double cutOff, double skin, bool reverseIndex = false);
-    // A copy keeps the parameters and the counters; its device list is built on the copy's next build/update.
+    // A copy is deep: parameters, counters and the device rows (md_list_copy), usable for forces at once.
     NeighbourList(const NeighbourList& other);
     NeighbourList& operator=(const NeighbourList& other);
     NeighbourList(NeighbourList&& other) noexcept;

@@ -12,7 +12,16 @@ NeighbourList::NeighbourList(double cutOff, double skin, bool reverseIndex)
 
 NeighbourList::NeighbourList(const NeighbourList& other)
     : cutOff_(other.cutOff_), skin_(other.skin_), reverseIndex_(other.reverseIndex_),
-      rebuildsBefore_(other.getRebuildCount()) {
+      rebuildsBefore_(other.rebuildsBefore_) {
+    if (other.device_) {
+        ensureDevice();
+        try {
+            check(md_list_copy(device_, other.device_));
+        } catch (...) {
+            md_list_destroy(device_);
+            throw;
+        }
+    }
 }
 
 NeighbourList& NeighbourList::operator=(const NeighbourList& other) {

@@ -37,7 +37,7 @@ class ForceResult(C.Structure):
 class ListInfo(C.Structure):
     _fields_ = [("half_pairs", C.c_size_t), ("rebuilds", C.c_int), ("cells_per_side", C.c_int),
                 ("max_neighbours", C.c_int), ("padding", C.c_int), ("particles", C.c_size_t),
-                ("tiled", C.c_int)]
+                ("tiled", C.c_int), ("tiles", C.c_int), ("unstaged_tiles", C.c_int)]
 
 
 class RunConfig(C.Structure):
@@ -99,6 +99,7 @@ _SIGS = {
     "md_list_build": (C.c_int, [_P, _P]),
     "md_list_update": (C.c_int, [_P, _P, _IP]),
     "md_list_get_info": (C.c_int, [_P, C.POINTER(ListInfo)]),
+    "md_list_copy": (C.c_int, [_P, _P]),
     "md_list_half": (C.c_int, [_P, _U64P, _U64P]),
     "md_list_full": (C.c_int, [_P, _I32P, _I32P, _I32P]),
     "md_forces": (C.c_int, [_P, _P, C.POINTER(LJ), C.c_int, C.POINTER(ForceResult)]),
@@ -168,6 +169,17 @@ def fcc_system(ctx, cells: int, density: float, temperature: float, seed: int) -
     pos = fcc_lattice(cells, a)
     vel = maxwell_boltzmann(pos.shape[0], temperature, seed)
     return State.from_arrays(ctx, cells * a, pos, vel=vel)
+
+def batched_replica_specs(count: int, base_seed: int = 1):
+    """The batched workload's replicas: replica r has FCC n_r ~ U{5..20} cells (N = 4 n^3), rho ~ U[0.70, 0.90],
+    T ~ U[0.7, 1.5] and seed base + r, drawn from numpy's PCG64 seeded with the base seed. Returns a list of
+    (cells, density, temperature, seed)."""
+    rng = np.random.default_rng(base_seed)
+    cells = rng.integers(5, 21, size=count)
+    rho = rng.uniform(0.70, 0.90, size=count)
+    temp = rng.uniform(0.7, 1.5, size=count)
+    return [(int(cells[r]), float(rho[r]), float(temp[r]), base_seed + r) for r in range(count)]
+
 
 _lib = None
 
@@ -371,6 +383,11 @@ class NeighbourList:
         check(lib().md_list_update(self.h, state.h, C.byref(r)))
         self.n = state.n
         return bool(r.value)
+
+    def copy_from(self, other: "NeighbourList"):
+        """md_list_copy: a deep copy of other's parameters, grid and rows."""
+        check(lib().md_list_copy(self.h, other.h))
+        self.n = other.n
 
     def info(self) -> ListInfo:
         i = ListInfo()

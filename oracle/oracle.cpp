@@ -1,4 +1,4 @@
-// CPU ORACLE — test infrastructure only. A clean-room, serial restatement of the reference engine's hot path
+// CPU ORACLE — test infrastructure only. A clean-room restatement of the reference engine's hot path
 // (nhantran1711/molecular-dynamics), written from its semantics; used by tests/, smoke() and bench.py's
 // cpu_baseline leg as the checker, never by the product. Compile with -ffp-contract=off: every expression
 // below is evaluated in the reference's source order with one rounding per operation, which is what makes the
@@ -12,6 +12,16 @@
 // One deliberate difference: the reference's default list path sums per-thread force arrays and uses SIMD
 // reductions whose order depends on the thread count and the vector width (Forcefield.cpp:105-207); this
 // restatement runs it serially in list order, so it matches the reference there to rounding (~1e-15), not bits.
+//
+// Threads (OpenMP) only split loops whose results do not depend on the split: per-particle list rows (counted,
+// then filled at prefix-sum offsets), per-pair and per-particle force terms, element-wise kicks and drifts,
+// integer counts, and the 1024-term blocks of blocked_sum (whose block sums are still added in order). Every
+// result is therefore the same for any thread count, and equal to the serial evaluation.
+//
+// Two additions without a reference function, for checking the GPU library's own extensions:
+//  - a fixed-order list (no re-sort; each full row holds the j > i neighbours, then the j < i ones, each part in
+//    increasing j, forces summed along the row): the arithmetic of the adjoint's forward run;
+//  - the exact pair-distance histogram over every pair of the box (O(N^2)), the check of the cell-pass g(r).
 
 #include "orc_capi.h"
 
@@ -20,6 +30,7 @@
 #include <cmath>
 #include <cstring>
 #include <numbers>
+#include <omp.h>
 #include <random>
 #include <stdexcept>
 #include <string>
@@ -75,11 +86,13 @@ inline double wrap(const Particles& p, double x) { return x - p.box * std::floor
 void permute(Particles& p, const std::uint64_t* order) {
     const std::size_t n = p.n();
     std::vector<double> pos(4 * n);
+#pragma omp parallel for schedule(static)
     for (std::size_t k = 0; k < n; ++k)
         for (int c = 0; c < 4; ++c) pos[4 * k + c] = p.pos[4 * order[k] + c];
     p.pos.swap(pos);
     auto gather = [&](auto& arr) {
         auto out = arr;
+#pragma omp parallel for schedule(static)
         for (std::size_t k = 0; k < n; ++k) out[k] = arr[order[k]];
         arr.swap(out);
     };
@@ -96,13 +109,17 @@ void permute(Particles& p, const std::uint64_t* order) {
 template <typename Term>
 double blocked_sum(std::size_t count, Term term) {
     constexpr std::size_t block = 1024;
-    double total = 0.0;
-    for (std::size_t b = 0; b * block < count; ++b) {
+    const std::size_t blocks = (count + block - 1) / block;
+    std::vector<double> parts(blocks);
+#pragma omp parallel for schedule(static)
+    for (std::size_t b = 0; b < blocks; ++b) {
         double part = 0.0;
         const std::size_t end = std::min(count, (b + 1) * block);
         for (std::size_t i = b * block; i < end; ++i) part += term(i);
-        total += part;
+        parts[b] = part;
     }
+    double total = 0.0;
+    for (std::size_t b = 0; b < blocks; ++b) total += parts[b];
     return total;
 }
 
@@ -188,12 +205,17 @@ struct Grid {
 };
 
 // ---------------------------------------------------------------------------------------------------------
-// Half Verlet list (NeighbourList.cpp:5-134).
+// Half Verlet list (NeighbourList.cpp:5-134). With `fixed` set (no reference function: the GPU adjoint's list)
+// the state is not re-sorted and full rows are kept: row i holds its j > i neighbours in increasing j, then its
+// j < i neighbours in increasing j; the half list (offsets, neigh) is then the j > i part of those rows.
 struct HalfList {
     double radius = 0.0, radius2 = 0.0, max_disp2 = 0.0;
     bool reverse = false;
+    bool fixed = false;
     Grid grid;
     std::vector<std::uint64_t> offsets, neigh, roffsets, rpairs;
+    std::vector<std::uint64_t> full_offsets, full; // fixed mode: full rows in CSR
+    std::vector<std::uint32_t> half_len;           // fixed mode: j > i entries at the start of each row
     std::vector<double> ref; // 3 per particle: positions at the last build
     int rebuilds = 0;
 
@@ -205,30 +227,80 @@ struct HalfList {
         grid.min_cell = rc + skin;
     }
 
+    // Candidates of particle i in the reference's visiting order: the 27 stencil cells in order, each cell's
+    // members in increasing index; every particle in increasing index on grids too small for a distinct stencil
+    // (NeighbourList.cpp:46-50, 75-98).
+    template <typename Visit>
+    void visit(const Particles& p, std::size_t i, bool cells, Visit&& v) const {
+        if (cells) {
+            for (int c : grid.stencil(grid.cell_of[i]))
+                for (std::uint64_t k = grid.start[c]; k < grid.start[c + 1]; ++k) v(grid.order[k]);
+        } else {
+            for (std::size_t j = 0; j < p.n(); ++j) v(j);
+        }
+    }
+
+    bool near(const Particles& p, std::size_t i, std::size_t j) const {
+        double d[3];
+        min_image(p, &p.pos[4 * i], &p.pos[4 * j], d);
+        return norm2(d) < radius2;
+    }
+
     void build(Particles& p) {
         if (2.0 * radius > p.box) throw std::runtime_error("Cutoff + skin must be less than half the box length");
         const std::size_t n = p.n();
         grid.build(p);
-        permute(p, grid.order.data());
-        grid.build(p);
+        if (!fixed) {
+            permute(p, grid.order.data());
+            grid.build(p);
+        }
         const bool cells = grid.side >= 3; // smaller grids would repeat cells in the stencil
-        offsets.assign(n + 1, 0);
-        neigh.clear();
+        // rows are counted, then written at their prefix-sum offsets: the result does not depend on the threads
+        std::vector<std::uint64_t> up(n + 1, 0), all(n + 1, 0);
+#pragma omp parallel for schedule(dynamic, 256)
         for (std::size_t i = 0; i < n; ++i) {
-            const double* ri = &p.pos[4 * i];
-            auto consider = [&](std::size_t j) {
-                if (j <= i) return;
-                double d[3];
-                min_image(p, ri, &p.pos[4 * j], d);
-                if (norm2(d) < radius2) neigh.push_back(j);
-            };
-            if (cells) {
-                for (int c : grid.stencil(grid.cell_of[i]))
-                    for (std::uint64_t k = grid.start[c]; k < grid.start[c + 1]; ++k) consider(grid.order[k]);
-            } else {
-                for (std::size_t j = 0; j < n; ++j) consider(j);
+            std::uint64_t a = 0, u = 0;
+            visit(p, i, cells, [&](std::size_t j) {
+                if (j == i || (!fixed && j < i) || !near(p, i, j)) return;
+                ++a;
+                u += j > i ? 1 : 0;
+            });
+            up[i + 1] = u;
+            all[i + 1] = a;
+        }
+        for (std::size_t i = 0; i < n; ++i) {
+            up[i + 1] += up[i];
+            all[i + 1] += all[i];
+        }
+        offsets = up;
+        neigh.assign(up[n], 0);
+        if (fixed) {
+            full_offsets = all;
+            full.assign(all[n], 0);
+            half_len.assign(n, 0);
+        }
+#pragma omp parallel for schedule(dynamic, 256)
+        for (std::size_t i = 0; i < n; ++i) {
+            if (!fixed) {
+                std::uint64_t at = offsets[i];
+                visit(p, i, cells, [&](std::size_t j) {
+                    if (j > i && near(p, i, j)) neigh[at++] = j;
+                });
+                continue;
             }
-            offsets[i + 1] = neigh.size();
+            const std::uint64_t b = full_offsets[i];
+            const std::uint32_t h = static_cast<std::uint32_t>(offsets[i + 1] - offsets[i]);
+            std::uint64_t hi = b, lo = b + h;
+            visit(p, i, cells, [&](std::size_t j) {
+                if (j == i || !near(p, i, j)) return;
+                full[j > i ? hi++ : lo++] = j;
+            });
+            std::sort(full.begin() + static_cast<std::ptrdiff_t>(b), full.begin() + static_cast<std::ptrdiff_t>(b + h));
+            std::sort(full.begin() + static_cast<std::ptrdiff_t>(b + h),
+                      full.begin() + static_cast<std::ptrdiff_t>(full_offsets[i + 1]));
+            half_len[i] = h;
+            std::copy(full.begin() + static_cast<std::ptrdiff_t>(b), full.begin() + static_cast<std::ptrdiff_t>(b + h),
+                      neigh.begin() + static_cast<std::ptrdiff_t>(offsets[i]));
         }
         if (reverse) {
             roffsets.assign(n + 1, 0);
@@ -239,6 +311,7 @@ struct HalfList {
             for (std::size_t k = 0; k < neigh.size(); ++k) rpairs[next[neigh[k]]++] = k;
         }
         ref.resize(3 * n);
+#pragma omp parallel for schedule(static)
         for (std::size_t i = 0; i < n; ++i)
             for (int k = 0; k < 3; ++k) ref[3 * i + k] = p.pos[4 * i + k];
         ++rebuilds;
@@ -252,14 +325,15 @@ struct HalfList {
             build(p);
             return true;
         }
-        bool stale = false;
-        for (std::size_t i = 0; i < n && !stale; ++i) {
+        int stale = 0;
+#pragma omp parallel for schedule(static) reduction(| : stale)
+        for (std::size_t i = 0; i < n; ++i) {
             double d[3];
             min_image(p, &p.pos[4 * i], &ref[3 * i], d);
-            stale = norm2(d) > max_disp2;
+            stale |= norm2(d) > max_disp2 ? 1 : 0;
         }
         if (stale) build(p);
-        return stale;
+        return stale != 0;
     }
 };
 
@@ -327,6 +401,7 @@ struct LJ {
 
     double list(Particles& p, const HalfList& nl) {
         check(p);
+        if (nl.fixed) return list_fixed(p, nl);
         if (det) return list_deterministic(p, nl);
         if (mixed) return list_mixed(p, nl);
         return list_double(p, nl);
@@ -377,7 +452,8 @@ struct LJ {
             throw std::runtime_error("Deterministic forces need a NeighbourList built with reverseIndex = true");
         const std::size_t n = p.n(), pairs = nl.neigh.size();
         std::vector<double> fx(pairs), fy(pairs), fz(pairs), e(pairs), w(pairs);
-        double cnt = 0.0;
+        long long cnt = 0;  // an integer count: exact in any order
+#pragma omp parallel for schedule(dynamic, 256) reduction(+ : cnt)
         for (std::size_t i = 0; i < n; ++i) {
             for (std::uint64_t k = nl.offsets[i]; k < nl.offsets[i + 1]; ++k) {
                 double d[3];
@@ -392,9 +468,10 @@ struct LJ {
                 fz[k] = f * d[2];
                 e[k] = in ? en : 0.0;
                 w[k] = f * r2;
-                cnt += in ? 1.0 : 0.0;
+                cnt += in ? 1 : 0;
             }
         }
+#pragma omp parallel for schedule(static)
         for (std::size_t i = 0; i < n; ++i) {
             double s[3] = {0.0, 0.0, 0.0};
             for (std::uint64_t k = nl.offsets[i]; k < nl.offsets[i + 1]; ++k) {
@@ -411,8 +488,43 @@ struct LJ {
             for (int c = 0; c < 3; ++c) p.frc[c][i] = s[c];
         }
         virial = blocked_sum(pairs, [&](std::size_t k) { return w[k]; });
-        inside = cnt;
+        inside = static_cast<double>(cnt);
         return blocked_sum(pairs, [&](std::size_t k) { return e[k]; });
+    }
+
+    // Fixed-order list (no reference function; the GPU adjoint's forward run): particle i sums F/r d over its
+    // full row in row order (j > i part, then j < i part, each in increasing j). U and W are taken over the j > i
+    // part of each row, per particle, then in blocked order over particles.
+    double list_fixed(Particles& p, const HalfList& nl) {
+        const std::size_t n = p.n();
+        std::vector<double> ui(n), wi(n);
+        long long cnt = 0;
+#pragma omp parallel for schedule(dynamic, 256) reduction(+ : cnt)
+        for (std::size_t i = 0; i < n; ++i) {
+            double s[3] = {0.0, 0.0, 0.0}, u = 0.0, w = 0.0;
+            const std::uint64_t b = nl.full_offsets[i];
+            for (std::uint64_t k = b; k < nl.full_offsets[i + 1]; ++k) {
+                double d[3];
+                min_image(p, &p.pos[4 * i], &p.pos[4 * nl.full[k]], d);
+                const double r2 = norm2(d);
+                double f, en;
+                pair(r2, f, en);
+                const bool in = r2 < rc2;
+                f = in ? f : 0.0;
+                for (int c = 0; c < 3; ++c) s[c] += f * d[c];
+                if (k - b < nl.half_len[i]) {
+                    u += in ? en : 0.0;
+                    w += f * r2;
+                    cnt += in ? 1 : 0;
+                }
+            }
+            for (int c = 0; c < 3; ++c) p.frc[c][i] = s[c];
+            ui[i] = u;
+            wi[i] = w;
+        }
+        virial = blocked_sum(n, [&](std::size_t i) { return wi[i]; });
+        inside = static_cast<double>(cnt);
+        return blocked_sum(n, [&](std::size_t i) { return ui[i]; });
     }
 
     // Mixed path: double minimum image narrowed to float, float pair maths and float per-row sums, double
@@ -464,18 +576,27 @@ struct LJ {
 // Velocity Verlet (Integrator.cpp:7-71): v += (dt/2 f) / m; r = wrap(r + dt v); forces; v += (dt/2 f) / m.
 void half_kick(Particles& p, double dt) {
     const double h = 0.5 * dt;
-    for (int k = 0; k < 3; ++k)
-        for (std::size_t i = 0; i < p.n(); ++i) p.vel[k][i] += h * p.frc[k][i] / p.mass[i];
+    const std::size_t n = p.n();
+    for (int k = 0; k < 3; ++k) {
+#pragma omp parallel for schedule(static)
+        for (std::size_t i = 0; i < n; ++i) p.vel[k][i] += h * p.frc[k][i] / p.mass[i];
+    }
 }
 
 void drift(Particles& p, double dt) {
-    for (int k = 0; k < 3; ++k)
-        for (std::size_t i = 0; i < p.n(); ++i) p.pos[4 * i + k] = wrap(p, p.pos[4 * i + k] + dt * p.vel[k][i]);
+    const std::size_t n = p.n();
+    for (int k = 0; k < 3; ++k) {
+#pragma omp parallel for schedule(static)
+        for (std::size_t i = 0; i < n; ++i) p.pos[4 * i + k] = wrap(p, p.pos[4 * i + k] + dt * p.vel[k][i]);
+    }
 }
 
 void scale_velocities(Particles& p, double s) {
-    for (int k = 0; k < 3; ++k)
-        for (double& v : p.vel[k]) v *= s;
+    const std::size_t n = p.n();
+    for (int k = 0; k < 3; ++k) {
+#pragma omp parallel for schedule(static)
+        for (std::size_t i = 0; i < n; ++i) p.vel[k][i] *= s;
+    }
 }
 
 // s = sqrt(T0 / T), skipped when T <= 0 (Thermostat.cpp:17-30).
@@ -600,8 +721,8 @@ struct orc_sim {
 extern "C" {
 
 const char* orc_last_error(void) { return g_error.c_str(); }
-void orc_set_threads(int) {}
-int orc_get_threads(void) { return 1; }
+void orc_set_threads(int threads) { omp_set_num_threads(threads > 0 ? threads : 1); }
+int orc_get_threads(void) { return omp_get_max_threads(); }
 
 orc_sim* orc_create(size_t n, double box_length, const double* pos, const double* vel, const double* mass) {
     auto* s = new orc_sim;
@@ -713,6 +834,36 @@ int orc_forces_list(orc_sim* s, double* u) { ORC_TRY(*u = s->lj.list(s->p, s->nl
 void orc_list(orc_sim* s, double cutoff, double skin, int reverse) {
     s->nl = HalfList{};
     s->nl.init(cutoff, skin, reverse != 0);
+}
+void orc_list_fixed(orc_sim* s, int fixed) { s->nl.fixed = fixed != 0; }
+
+int orc_pair_histogram(const orc_sim* s, double rmax, int bins, uint64_t* counts) {
+    if (bins <= 0 || !(rmax > 0.0)) {
+        g_error = "bins and rmax must be positive";
+        return -1;
+    }
+    const Particles& p = s->p;
+    const std::size_t n = p.n();
+    const double rmax2 = rmax * rmax, width = rmax / bins;
+    std::vector<std::uint64_t> total(static_cast<std::size_t>(bins), 0);
+#pragma omp parallel
+    {
+        std::vector<std::uint64_t> local(static_cast<std::size_t>(bins), 0);
+#pragma omp for schedule(dynamic, 64)
+        for (std::size_t i = 0; i < n; ++i)
+            for (std::size_t j = i + 1; j < n; ++j) {
+                double d[3];
+                min_image(p, &p.pos[4 * i], &p.pos[4 * j], d);
+                const double r2 = norm2(d);
+                if (!(r2 < rmax2)) continue;
+                const std::size_t b = static_cast<std::size_t>(std::sqrt(r2) / width);
+                if (b < local.size()) ++local[b];
+            }
+#pragma omp critical
+        for (std::size_t b = 0; b < local.size(); ++b) total[b] += local[b];
+    }
+    std::copy(total.begin(), total.end(), counts);
+    return 0;
 }
 int orc_list_build(orc_sim* s) { ORC_TRY(s->nl.build(s->p)) }
 int orc_list_update(orc_sim* s, int* rebuilt) { ORC_TRY(*rebuilt = s->nl.update(s->p) ? 1 : 0) }

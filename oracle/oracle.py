@@ -69,6 +69,11 @@ _SIGS = {
     "orc_rdf_sample": (None, [_P]),
     "orc_rdf_values": (None, [_P, _DP, _DP]),
 }
+# exported by liboracle.so only (no reference function): the fixed-order list and the brute-force pair histogram
+_SIGS_ORACLE = {
+    "orc_list_fixed": (None, [_P, C.c_int]),
+    "orc_pair_histogram": (C.c_int, [_P, _D, C.c_int, _U64P]),
+}
 
 
 def available(kind: str) -> bool:
@@ -81,7 +86,7 @@ def lib(kind: str = "oracle") -> C.CDLL:
         if not os.path.exists(path):
             raise FileNotFoundError(f"{path} is not built (run `make -C oracle`)")
         h = C.CDLL(path)
-        for name, (res, args) in _SIGS.items():
+        for name, (res, args) in list(_SIGS.items()) + (list(_SIGS_ORACLE.items()) if kind == "oracle" else []):
             fn = getattr(h, name)
             fn.restype = res
             fn.argtypes = args
@@ -243,8 +248,14 @@ class Sim:
         return self.L.orc_energy_shift(self.h)
 
     # neighbour list ---------------------------------------------------------------------------------
-    def neighbour_list(self, cutoff=2.5, skin=0.2, reverse=False):
+    def neighbour_list(self, cutoff=2.5, skin=0.2, reverse=False, fixed_order=False):
+        """NeighbourList(cutoff, skin, reverse). fixed_order (oracle only): no re-sort, full rows with the j > i
+        part then the j < i part, each in increasing j, and list forces summed along them."""
         self.L.orc_list(self.h, cutoff, skin, int(reverse))
+        if fixed_order:
+            if self.kind != "oracle":
+                raise OracleError("the fixed-order list is an oracle-only mode")
+            self.L.orc_list_fixed(self.h, 1)
 
     def list_build(self):
         self._check(self.L.orc_list_build(self.h))
@@ -301,6 +312,15 @@ class Sim:
 
     def rdf_sample(self):
         self.L.orc_rdf_sample(self.h)
+
+    def pair_histogram(self, rmax: float, bins: int) -> np.ndarray:
+        """Counts of every pair i < j with r^2 < rmax^2 in bin size_t(sqrt(r^2) / (rmax / bins)), by brute force
+        (oracle only)."""
+        if self.kind != "oracle":
+            raise OracleError("the brute-force pair histogram is an oracle-only function")
+        out = np.zeros(bins, dtype=np.uint64)
+        self._check(self.L.orc_pair_histogram(self.h, rmax, bins, _u64p(out)))
+        return out
 
     def rdf_values(self):
         r = np.empty(self._bins)

@@ -75,6 +75,14 @@ void orc_list_half(const orc_sim* s, uint64_t* offsets, uint64_t* neigh);
 /* reverse CSR (NeighbourList.hpp:38-41); roffsets[n+1], rpairs[pairs]; returns -1 without a reverse index. */
 int orc_list_reverse(const orc_sim* s, uint64_t* roffsets, uint64_t* rpairs);
 
+/* Oracle only (no reference function; oracle/_ref does not export these):
+   orc_list_fixed: after orc_list, builds keep the particle order and full rows (j > i part, then j < i part,
+   each in increasing j), and the list forces sum along those rows (the GPU adjoint's forward arithmetic).
+   orc_pair_histogram: counts[bins] of every pair i < j of the box with r^2 < rmax^2, in bin
+   size_t(sqrt(r^2) / (rmax / bins)), by brute force. */
+void orc_list_fixed(orc_sim* s, int fixed);
+int orc_pair_histogram(const orc_sim* s, double rmax, int bins, uint64_t* counts);
+
 /* VelocityVerletIntegrator(dt).step(...) (Integrator.cpp:7-30). */
 int orc_step_list(orc_sim* s, double dt, double* potential);
 int orc_step_allpairs(orc_sim* s, double dt, double* potential);

@@ -250,9 +250,10 @@ def test_mixed_run_folded_step_matches_separate_kernels(ctx, monkeypatch, steps,
 
 
 def test_mixed_run_folded_step_with_unstaged_tiles(ctx, monkeypatch):
-    """The folded step where some tiles' halos exceed the LDS capacity (108k atoms, skin 0.8: cells of 3.36 sigma hold
-    ~32 particles, so a tile that crosses a cell column has a halo of ~2700 > 2184 slots and takes the unstaged
-    gather path): still bit-equal to the separate-kernel run."""
+    """The folded step where some tiles' halos exceed the LDS capacity (108k atoms, skin 0.9: 14 cells of 3.6 sigma
+    per side hold ~39 particles each, so a tile that crosses a cell column has a halo of up to ~2950 > 2176 slots;
+    about two thirds of the tiles take the unstaged gather path, the rest stage their halo): mixed forces within
+    1e-6 of FP64 on both paths, and the folded run bit-equal to the separate-kernel run."""
     runs = []
     for unfolded in (True, False):
         if unfolded:
@@ -260,11 +261,17 @@ def test_mixed_run_folded_step_with_unstaged_tiles(ctx, monkeypatch):
         else:
             monkeypatch.delenv("MDGPU_UNFOLDED")
         g = mdgpu.fcc_system(ctx, 30, 0.8442, 0.722, 1)
-        nl = mdgpu.NeighbourList(ctx, 2.5, 0.8)
+        nl = mdgpu.NeighbourList(ctx, 2.5, 0.9)
         nl.build(g)
+        nl.run(g, LJ, 0.005, 30, precision=mdgpu.MD_FP64, use_graph=False, record=False)
+        info = nl.info()
+        assert 0 < info.unstaged_tiles < info.tiles  # both paths are taken
+        nl.forces(g, LJ, mdgpu.MD_FP64)
+        f64 = g.download(mdgpu.FIELD_FORCE)
         nl.forces(g, LJ, mdgpu.MD_MIXED)
+        assert np.abs(g.download(mdgpu.FIELD_FORCE) - f64).max() <= 1e-6 * np.abs(f64).max()
         en = nl.run(g, LJ, 0.005, 40, precision=mdgpu.MD_MIXED, use_graph=False)
-        assert nl.info().tiled == 1
+        assert nl.info().tiled == 1 and nl.info().unstaged_tiles > 0
         runs.append((g, nl, en))
     (ga, la, ea), (gb, lb, eb) = runs
     assert la.info().rebuilds == lb.info().rebuilds

@@ -367,3 +367,84 @@ def test_oracle_reproduces_golden_bussi(golden):
         s.step(0.005)
         s.bussi_apply(0.005)
     assert f"{s.state_hash():016x}" == golden["bussi_det"]["hash"]
+
+
+# ---- the oracle's two additions without a reference function ------------------------------------------------
+
+def test_fixed_order_list_pairs_forces_and_rebuild_independence():
+    """The fixed-order mode keeps the particle order, lists exactly the reference's pair set, and gives forces
+    equal to the deterministic mode's to rounding. Its per-particle sums run over the pairs in increasing j
+    (j > i part, then j < i part) and entries beyond rc add exact zeros, so a trajectory does not depend on when
+    the list was rebuilt: forcing a rebuild after every step leaves the state hash unchanged."""
+    base = O.make_liquid(6, 60, skin=0.3)
+    ref = base.clone()
+    ref.forcefield(1.0, 1.0, 2.5, deterministic=True)
+    ref.neighbour_list(2.5, 0.3, reverse=True)
+    ref.list_build()
+    ref.forces_list()
+    fx = base.clone()
+    fx.forcefield(1.0, 1.0, 2.5)
+    fx.neighbour_list(2.5, 0.3, fixed_order=True)
+    ids_before = fx.state()["ids"].copy()
+    fx.list_build()
+    assert np.array_equal(fx.state()["ids"], ids_before)  # no re-sort
+    offs, nb = fx.half_list()
+    owner = np.repeat(np.arange(fx.n), np.diff(offs).astype(np.int64))
+    ids = fx.state()["ids"]
+    pairs_fixed = {(min(a, b), max(a, b)) for a, b in zip(ids[owner], ids[nb.astype(np.int64)])}
+    ro, rn = ref.half_list()
+    rown = np.repeat(np.arange(ref.n), np.diff(ro).astype(np.int64))
+    rids = ref.state()["ids"]
+    pairs_ref = {(min(a, b), max(a, b)) for a, b in zip(rids[rown], rids[rn.astype(np.int64)])}
+    assert pairs_fixed == pairs_ref
+    for i in range(fx.n):  # j > i part in increasing j
+        row = nb[offs[i]:offs[i + 1]]
+        assert np.all(row > i) and np.all(np.diff(row.astype(np.int64)) > 0)
+    u = fx.forces_list()
+    f = fx.state()["force"][:, np.argsort(ids)]
+    fr = ref.state()["force"][:, np.argsort(rids)]
+    assert np.abs(f - fr).max() <= 1e-13 * np.abs(fr).max()
+    assert u == pytest.approx(ref.forces_list(), rel=1e-13)
+    a, b = fx.clone(), fx.clone()
+    for _ in range(80):
+        a.step(0.005)
+        b.step(0.005)
+        b.list_build()
+    assert a.rebuilds < b.rebuilds and a.state_hash() == b.state_hash()
+
+
+def test_pair_histogram_matches_reference_rdf_counts_and_fcc_shells():
+    """Brute-force pair histogram: below the list radius it equals the reference's list-pair counts right after
+    a build (every pair within rmax is listed then); on a perfect FCC lattice it counts the coordination shells
+    12, 6, 24, 12, 24, 8, 48 (N / 2 pairs per neighbour)."""
+    kind = "ref" if HAS_REF else "oracle"
+    s = O.make_liquid(6, 40, kind=kind, skin=0.3)
+    s.neighbour_list(2.5, 0.3)
+    s.list_build()
+    bins, rmax = 140, 2.8
+    s.rdf(rmax, bins)
+    s.rdf_sample()
+    _, g = s.rdf_values()
+    st = s.state()
+    o = O.Sim.from_arrays(st["pos"][:, :3], s.box, vel=st["vel"].T)
+    counts = o.pair_histogram(rmax, bins)
+    width = rmax / bins
+    n = s.n
+    density = n / s.box ** 3
+    for b in range(bins):  # the reference's normalisation (RadialDistribution.cpp:45-60), one sample
+        lo = b * width
+        hi = lo + width
+        shell = 4.0 / 3.0 * math.pi * (hi * hi * hi - lo * lo * lo)
+        assert float(counts[b]) / (1 * 0.5 * n * density * shell) == g[b]
+    assert counts.sum() > 0
+    a = (4.0 / 0.8442) ** (1.0 / 3.0)
+    lat = O.Sim.lattice(6, a, 6 * a)
+    h = lat.pair_histogram(1.95 * a, 390)
+    radii = (np.arange(390) + 0.5) * (1.95 * a / 390)
+    got = {}
+    for b in np.nonzero(h)[0]:
+        got[round(radii[b] / a, 2)] = got.get(round(radii[b] / a, 2), 0) + int(h[b])
+    shells = {0.71: 12, 1.0: 6, 1.22: 24, 1.41: 12, 1.58: 24, 1.73: 8, 1.87: 48}
+    assert sorted(got) == sorted(shells)
+    for r, c in shells.items():
+        assert got[r] == c * lat.n // 2
