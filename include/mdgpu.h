@@ -167,9 +167,13 @@ int md_bussi_rng_destroy(md_rng* r);
 int md_bussi_rng_draws(md_rng* r, size_t n_particles, int count, double* out);
 
 /* ---- RadialDistribution (RadialDistribution.hpp:8-29) --------------------------------------------------- */
-/* RadialDistribution::sample: adds the half-list pair counts with r^2 < rmax^2 to counts[bins]
-   (RadialDistribution.cpp:11-30). */
+/* RadialDistribution::sample: adds the half-list pair counts with r^2 < rmax^2, bin size_t(sqrt(r^2) / (rmax /
+   bins)), to counts[bins] (RadialDistribution.cpp:11-30). With rmax beyond the list radius, where the
+   reference's list holds only part of the pairs, it counts every pair within rmax (md_rdf_sample_all). */
 int md_rdf_sample(md_state* s, md_list* l, double rmax, int bins, uint64_t* counts);
+/* No counterpart in the reference: the same histogram over every pair of the box within rmax (a cell pass with
+   cells at least rmax wide), e.g. a g(r) to 5 sigma next to a 2.8 sigma list; rmax <= L / 2. */
+int md_rdf_sample_all(md_state* s, double rmax, int bins, uint64_t* counts);
 
 /* ---- Multi-step driver (md_sim's production loop, main.cpp:272-304, without host round trips) ---------- */
 typedef struct md_run_config {
@@ -184,6 +188,14 @@ typedef struct md_run_config {
     int use_graph;        /* capture one step as a hipGraph and replay it */
     double* force_kernel_ms; /* optional out (eager runs only): summed duration of the force kernel launches,
                                 measured with HIP events on the context's stream */
+    /* RadialDistribution::sample every rdf_every steps inside the run (main.cpp:277-279 samples the same way),
+       over every pair within rdf_rmax (md_rdf_sample_all): rdf_counts[rdf_bins] (host) and *rdf_samples (host)
+       receive the run's counts and number of samples, added to what they hold. rdf_every = 0: no sampling. */
+    int rdf_every;
+    int rdf_bins;
+    double rdf_rmax;
+    uint64_t* rdf_counts;
+    int* rdf_samples;
 } md_run_config;
 
 /* Advances `steps` velocity-Verlet steps entirely on the device: the list update decision, re-sort and rebuild

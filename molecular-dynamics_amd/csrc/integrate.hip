@@ -305,6 +305,23 @@ int md_run(md_state* s, md_list* l, const md_run_config* cfg, double* energies) 
         events.push_back(e1);
     };
 
+    // the pair histogram every rdf_every steps (after the step's record, on the step's final positions)
+    RdfPass* rdf = nullptr;
+    struct RdfGuard {
+        RdfPass*& p;
+        ~RdfGuard() {
+            if (p) rdf_pass_destroy(p);
+        }
+    } rdf_guard{rdf};
+    if (cfg->rdf_every > 0) {
+        if (!cfg->rdf_counts) fail(MD_ERR_ARG, "rdf_every needs rdf_counts");
+        rdf = rdf_pass_create(s, cfg->rdf_rmax, cfg->rdf_bins);
+    }
+    // a sample of s->pos when (records done + offset) is a multiple of rdf_every
+    auto enqueue_rdf = [&](int offset) {
+        if (rdf) rdf_pass_enqueue(rdf, s, rb.counters, offset, cfg->rdf_every);
+    };
+
     // Every kernel of the run exits at once after a rebuild has overflowed its rows (md_state::halt): the state
     // then stands after the opening kick and drift of the step whose rebuild overflowed, and the steps before it
     // are recorded. The host grows the rows and resumes from there, repeating that rebuild.
@@ -325,6 +342,7 @@ int md_run(md_state* s, md_list* l, const md_run_config* cfg, double* energies) 
     // forces of step k+1) and a tail (closing kick and record of the last step)
     const unsigned vv_blocks = blocks_for(s->n, VV_THREADS);
     auto enqueue_nve_body = [&]() {
+        enqueue_rdf(1);  // the positions of the step whose record comes next
         k_kick_kick_drift<<<vv_blocks, VV_THREADS, 0, st>>>(s->pos, s->vel, s->frc, s->mass, s->n, 0.5 * cfg->dt,
                                                            cfg->dt, s->box, s->inv_box, l->ref, l->max_disp2,
                                                            l->flags + FLAG_STALE, s->partials, s->halt);
@@ -347,6 +365,7 @@ int md_run(md_state* s, md_list* l, const md_run_config* cfg, double* energies) 
             k_record<<<1, 1, 0, st>>>(s->scalars, 3, rb.energies, rb.counters, s->halt);
         }
         MD_HIP(hipGetLastError());
+        enqueue_rdf(0);
     };
     // `body` enqueued `replays` times: captured once and replayed as a hipGraph, or launched eagerly
     auto repeat = [&](int replays, auto&& body) {
@@ -410,6 +429,7 @@ int md_run(md_state* s, md_list* l, const md_run_config* cfg, double* energies) 
         event_pair(e0, e1);
         tiled_forces_step_enqueue(s, l, cfg->lj, cfg->dt, bufs[1 - p], gate, next, s->scalars,
                                   StepRecord{rb.energies, rb.counters}, e0, e1);
+        enqueue_rdf(0);  // this step's positions, still in bufs[p]
     };
 
     // One stretch of `n_steps` steps. With `resume` the first step's opening kick and drift are already done (the
@@ -443,6 +463,7 @@ int md_run(md_state* s, md_list* l, const md_run_config* cfg, double* energies) 
             kick_enqueue(s, cfg->dt, s->scalars + 3);
             k_record<<<1, 1, 0, st>>>(s->scalars, 3, rb.energies, rb.counters, s->halt);
             MD_HIP(hipGetLastError());
+            enqueue_rdf(0);
             s->pos_spare = bufs[1 - p];
             spare_guard.on = false;
             return true;
@@ -455,6 +476,7 @@ int md_run(md_state* s, md_list* l, const md_run_config* cfg, double* energies) 
             kick_enqueue(s, cfg->dt, s->scalars + 3);
             k_record<<<1, 1, 0, st>>>(s->scalars, 3, rb.energies, rb.counters, s->halt);
             MD_HIP(hipGetLastError());
+            enqueue_rdf(0);
             return false;
         }
         int first = 0;
@@ -502,6 +524,10 @@ int md_run(md_state* s, md_list* l, const md_run_config* cfg, double* energies) 
             total_ms += ms;
         }
         *cfg->force_kernel_ms = total_ms;
+    }
+    if (rdf) {
+        const unsigned samples = rdf_pass_read(rdf, s, cfg->rdf_counts);
+        if (cfg->rdf_samples) *cfg->rdf_samples += static_cast<int>(samples);
     }
     l->rebuilds += hf[FLAG_REBUILDS];
     slots_guard.rebuilt = hf[FLAG_REBUILDS] > 0;

@@ -273,6 +273,14 @@ void tiled_forces_step_enqueue(md_state* s, md_list* l, const md_lj& p, double d
 void kick_drift_enqueue(md_state* s, double dt, md_list* check_list);
 void kick_enqueue(md_state* s, double dt, double* ke_dev);
 
+// rdf.hip: the cell pass over every pair within rmax (buffers for one state size, made before a run)
+struct RdfPass;
+RdfPass* rdf_pass_create(md_state* s, double rmax, int bins);
+void rdf_pass_destroy(RdfPass* p);
+// a sample of s's positions; inside md_run only when (*counter + offset) % every == 0 (counter may be NULL)
+void rdf_pass_enqueue(RdfPass* p, md_state* s, const int* counter, int offset, int every);
+unsigned rdf_pass_read(RdfPass* p, md_state* s, uint64_t* counts);
+
 // Final fixed-order reduction of `blocks` rows of `width` partials (s->partials) into out_dev[0..width).
 void reduce_partials_enqueue(md_state* s, int blocks, int width, double* out_dev);
 void reduce_partials_enqueue(md_state* s, int blocks, int width, double* out_dev, StepRecord rec);

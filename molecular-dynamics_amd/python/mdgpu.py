@@ -44,7 +44,9 @@ class RunConfig(C.Structure):
     _fields_ = [("dt", C.c_double), ("precision", C.c_int), ("steps", C.c_int), ("lj", LJ),
                 ("thermostat", C.c_int), ("temperature", C.c_double), ("tau", C.c_double),
                 ("draws", C.POINTER(C.c_double)), ("use_graph", C.c_int),
-                ("force_kernel_ms", C.POINTER(C.c_double))]
+                ("force_kernel_ms", C.POINTER(C.c_double)), ("rdf_every", C.c_int), ("rdf_bins", C.c_int),
+                ("rdf_rmax", C.c_double), ("rdf_counts", C.POINTER(C.c_uint64)),
+                ("rdf_samples", C.POINTER(C.c_int))]
 
 
 class BatchConfig(C.Structure):
@@ -110,6 +112,7 @@ _SIGS = {
     "md_step_allpairs": (C.c_int, [_P, C.POINTER(LJ), C.c_double, C.POINTER(ForceResult)]),
     "md_bussi_apply": (C.c_int, [_P, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double]),
     "md_rdf_sample": (C.c_int, [_P, _P, C.c_double, C.c_int, _U64P]),
+    "md_rdf_sample_all": (C.c_int, [_P, C.c_double, C.c_int, _U64P]),
     "md_run": (C.c_int, [_P, _P, C.POINTER(RunConfig), _DP]),
     "md_bussi_rng_create": (C.c_int, [C.c_uint64, _PP]),
     "md_bussi_rng_destroy": (C.c_int, [_P]),
@@ -336,6 +339,11 @@ class State:
                                    _u64p(order)))
         return s, cell_of, start, order
 
+    def rdf_sample_all(self, rmax: float, bins: int, counts: np.ndarray):
+        """md_rdf_sample_all: adds every pair within rmax to counts (uint64[bins])."""
+        assert counts.dtype == np.uint64 and counts.shape == (bins,)
+        check(lib().md_rdf_sample_all(self.h, rmax, bins, _u64p(counts)))
+
     def kick_drift(self, dt):
         check(lib().md_kick_drift(self.h, dt))
 
@@ -425,9 +433,10 @@ class NeighbourList:
         check(lib().md_rdf_sample(state.h, self.h, rmax, bins, _u64p(counts)))
 
     def run(self, state: State, lj: LJ, dt: float, steps: int, precision=MD_MIXED, draws=None,
-            temperature=1.0, tau=0.1, use_graph=True, record=True, timing=None):
+            temperature=1.0, tau=0.1, use_graph=True, record=True, timing=None, rdf=None):
         """md_run. With `timing` (a list), eager launches are used and the summed force-kernel time in ms is
-        appended to it."""
+        appended to it. With rdf = (every, rmax, counts: uint64[bins]) the run adds the all-pairs histogram of
+        every `every`-th step to counts and returns (energies, samples)."""
         cfg = RunConfig()
         cfg.dt = dt
         cfg.precision = precision
@@ -443,10 +452,21 @@ class NeighbourList:
         ms = C.c_double(0.0)
         if timing is not None:
             cfg.force_kernel_ms = C.pointer(ms)
+        samples = C.c_int(0)
+        if rdf is not None:
+            every, rmax, counts = rdf
+            assert counts.dtype == np.uint64 and counts.ndim == 1
+            cfg.rdf_every = every
+            cfg.rdf_bins = counts.shape[0]
+            cfg.rdf_rmax = rmax
+            cfg.rdf_counts = counts.ctypes.data_as(_U64P)
+            cfg.rdf_samples = C.pointer(samples)
         energies = np.empty((steps, 4)) if record else None
         check(lib().md_run(state.h, self.h, C.byref(cfg), None if energies is None else energies.ctypes.data_as(_DP)))
         if timing is not None:
             timing.append(ms.value)
+        if rdf is not None:
+            return energies, samples.value
         return energies
 
     def close(self):

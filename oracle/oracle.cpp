@@ -849,12 +849,27 @@ int orc_pair_histogram(const orc_sim* s, double rmax, int bins, uint64_t* counts
 #pragma omp parallel
     {
         std::vector<std::uint64_t> local(static_cast<std::size_t>(bins), 0);
+        // min_image's arithmetic, with round(t) taken as the signed zero it returns for |t| < 1/2 (no library
+        // call for the near images), and pairs dropped as soon as a partial sum of squares reaches rmax^2 (the
+        // full sum can only be larger)
+        auto comp = [&](double dk) {
+            const double t = dk * p.inv_box;
+            const double images = std::fabs(t) < 0.5 ? std::copysign(0.0, t) : std::round(t);
+            return dk - p.box * images;
+        };
 #pragma omp for schedule(dynamic, 64)
         for (std::size_t i = 0; i < n; ++i)
             for (std::size_t j = i + 1; j < n; ++j) {
-                double d[3];
-                min_image(p, &p.pos[4 * i], &p.pos[4 * j], d);
-                const double r2 = norm2(d);
+                const double* ri = &p.pos[4 * i];
+                const double* rj = &p.pos[4 * j];
+                const double dx = comp(ri[0] - rj[0]);
+                const double xx = dx * dx;
+                if (!(xx < rmax2)) continue;
+                const double dy = comp(ri[1] - rj[1]);
+                const double xy = xx + dy * dy;
+                if (!(xy < rmax2)) continue;
+                const double dz = comp(ri[2] - rj[2]);
+                const double r2 = xy + dz * dz;
                 if (!(r2 < rmax2)) continue;
                 const std::size_t b = static_cast<std::size_t>(std::sqrt(r2) / width);
                 if (b < local.size()) ++local[b];
