@@ -207,6 +207,18 @@ typedef struct md_run_config {
    trajectory equals the one an ample padding gives. */
 int md_run(md_state* s, md_list* l, const md_run_config* cfg, double* energies);
 
+/* No counterpart in the reference: md_run for callers that cannot allocate device memory (code under graph
+   capture, serving loops, harnesses that forbid hipMalloc). The same steps and results as md_run, with the run's
+   counters, per-step records and Bussi draws in a static device array of the library (3 MiB per GPU, held until the
+   process ends) instead of buffers of its own, and no hipMalloc / hipFree / graph instantiation / event creation.
+   Limits: steps <= 65536; the list must already be built for the state (md_list_build); eager launches only
+   (use_graph and force_kernel_ms are ignored); no pair histogram (rdf_every must be 0); a mixed run folds the step
+   into the force kernel only when an earlier md_run made the spare position buffer (the result is the same). A row
+   overflow cannot be regrown without allocating: the call then fails with MD_ERR_OVERFLOW, the state frozen after
+   the overflowing step's opening kick and drift and the steps before it recorded. Every call on one GPU shares
+   that GPU's array: calls that overlap on one GPU (several contexts or threads) race. */
+int md_run_static(md_state* s, md_list* l, const md_run_config* cfg, double* energies);
+
 /* ---- Adjoint (no counterpart in the reference) ----------------------------------------------------------- */
 /* Gradients of L = mean_k U(r_k) + (mean_k P_k - pressure_target)^2 over k = 1..steps, P = (2 KE + W) / (3V)
    (the reference's virial pressure, main.cpp:209-211), with respect to epsilon, sigma and the initial positions and

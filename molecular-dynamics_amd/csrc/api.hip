@@ -1,4 +1,5 @@
 // Contexts, particle state, reductions and the System-level entry points of the mdgpu C ABI.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <exception>
@@ -256,6 +257,7 @@ int md_ctx_create(int device, md_ctx** out) {
         MD_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         MD_HIP(hipHostMalloc(reinterpret_cast<void**>(&c->host_scalars), 64 * sizeof(double), hipHostMallocDefault));
         MD_HIP(hipHostMalloc(reinterpret_cast<void**>(&c->host_flags), 16 * sizeof(int), hipHostMallocDefault));
+        c->run_scratch = run_scratch_address(device);
     } catch (...) {
         delete c;
         throw;
@@ -317,6 +319,9 @@ int md_state_create(md_ctx* ctx, size_t n, double box_length, md_state** out) {
         MD_HIP(hipMemsetAsync(s->scalars, 0, sizeof(double) * 16, st));
         if (n) k_init_state<<<blocks_for(n, 256), 256, 0, st>>>(n, s->mass, s->id);
         MD_HIP(hipGetLastError());
+        // the reduction rows every kernel of this library may need for n particles (six sums per 256-particle
+        // block or 128-particle tile at most), so that later calls (md_run_static among them) allocate none
+        state_reserve_partials(s, std::max<size_t>(blocks_for(n ? n : 1, 256), tile_count(n ? n : 1)) * 6 + 8);
         MD_HIP(hipStreamSynchronize(st));
     } catch (...) {
         md_state_destroy(s);

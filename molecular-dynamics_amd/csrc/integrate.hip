@@ -4,6 +4,9 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "device.hpp"
@@ -239,17 +242,31 @@ int md_step_allpairs(md_state* s, const md_lj* lj, double dt, md_force_result* o
     MD_API_END
 }
 
-int md_run(md_state* s, md_list* l, const md_run_config* cfg, double* energies) {
-    MD_API_BEGIN
-    if (!s || !l || !cfg) fail(MD_ERR_ARG, "state/list/config is NULL");
+}  // extern "C"
+
+namespace mdgpu {
+
+// Where a run keeps its per-step records, counters and Bussi draws: buffers of its own (md_run), or the static
+// scratch array (md_run_static), in which case the run may not allocate at all.
+struct RunMem {
+    double* energies;
+    int* counters;  // [0] records written, [1] Bussi draws consumed
+    double* draws;
+    bool no_alloc;
+};
+
+static void run_impl(md_state* s, md_list* l, const md_run_config* cfg, double* energies, const RunMem& rb) {
     if (cfg->steps < 0) fail(MD_ERR_ARG, "steps must be >= 0");
     if (cfg->precision != MD_FP64 && cfg->precision != MD_MIXED) fail(MD_ERR_ARG, "unknown precision");
     if (cfg->thermostat && !cfg->draws) fail(MD_ERR_ARG, "Bussi thermostat needs draws");
     check_cutoff(s, cfg->lj);
     // (a list last built without the tiled slot rows, e.g. by the adjoint's fixed-order build, is rebuilt first:
     // the captured step picks the tiled kernels)
-    if (!l->built || l->n != s->n || (tiling_enabled(s->n) && !l->tiled)) list_rebuild(l, s);
-    if (cfg->steps == 0 || s->n == 0) return MD_OK;
+    if (!l->built || l->n != s->n || (tiling_enabled(s->n) && !l->tiled)) {
+        if (rb.no_alloc) fail(MD_ERR_STATE, "md_run_static needs a list built for this state (md_list_build)");
+        list_rebuild(l, s);
+    }
+    if (cfg->steps == 0 || s->n == 0) return;
     // mixed runs on the tiled path need only the 16-bit slot rows: the device-gated rebuilds skip the int32 rows,
     // which are materialised again when a caller needs them
     if (cfg->precision == MD_FP64) ensure_rows32(l);
@@ -264,14 +281,9 @@ int md_run(md_state* s, md_list* l, const md_run_config* cfg, double* energies) 
     l->slots_only = cfg->precision == MD_MIXED && tiling_enabled(s->n);
     hipStream_t st = s->ctx->stream;
     const int steps = cfg->steps;
-    RunBuffers rb;
-    rb.energies = dev_alloc<double>(4 * static_cast<size_t>(steps));
-    rb.counters = dev_alloc<int>(4);
     MD_HIP(hipMemsetAsync(rb.counters, 0, 4 * sizeof(int), st));
-    if (cfg->thermostat) {
-        rb.draws = dev_alloc<double>(2 * static_cast<size_t>(steps));
+    if (cfg->thermostat)
         MD_HIP(hipMemcpyAsync(rb.draws, cfg->draws, 2 * sizeof(double) * steps, hipMemcpyHostToDevice, st));
-    }
     // clear the overflow word, the longest row (both accumulate over the run's builds) and the device rebuild
     // counter before the run
     MD_HIP(hipMemsetAsync(l->flags + FLAG_OVERFLOW, 0, 2 * sizeof(int), st));
@@ -284,11 +296,15 @@ int md_run(md_state* s, md_list* l, const md_run_config* cfg, double* energies) 
     const double c = cfg->thermostat ? std::exp(-cfg->dt / cfg->tau) : 0.0;
     const size_t nvel = 3 * s->n;
     // reserve the partials once, before any capture (no allocation may happen inside it): per-block rows of the
-    // 256-thread kernels, or per-tile rows of the tiled force kernel, whichever is more
-    state_reserve_partials(s, static_cast<size_t>(std::max<size_t>(blocks_for(s->n, 256), tile_count(s->n))) * 4 + 8);
+    // 256-thread kernels, or per-tile rows of the tiled force kernel, whichever is more (md_state_create has
+    // reserved as much already)
+    const size_t partials = static_cast<size_t>(std::max<size_t>(blocks_for(s->n, 256), tile_count(s->n))) * 4 + 8;
+    if (rb.no_alloc && s->partials_cap < partials) fail(MD_ERR_STATE, "reduction buffer too small for md_run_static");
+    state_reserve_partials(s, partials);
 
     // optional force-kernel timing: one event pair per force launch, eager launches only
-    const bool timed = cfg->force_kernel_ms != nullptr && !cfg->use_graph;
+    const bool use_graph = cfg->use_graph && !rb.no_alloc;  // instantiating a graph allocates
+    const bool timed = cfg->force_kernel_ms != nullptr && !use_graph && !rb.no_alloc;
     std::vector<hipEvent_t> events;
     struct EventGuard {
         std::vector<hipEvent_t>& ev;
@@ -314,6 +330,7 @@ int md_run(md_state* s, md_list* l, const md_run_config* cfg, double* energies) 
         }
     } rdf_guard{rdf};
     if (cfg->rdf_every > 0) {
+        if (rb.no_alloc) fail(MD_ERR_ARG, "md_run_static does not sample the pair histogram");
         if (!cfg->rdf_counts) fail(MD_ERR_ARG, "rdf_every needs rdf_counts");
         rdf = rdf_pass_create(s, cfg->rdf_rmax, cfg->rdf_bins);
     }
@@ -370,7 +387,7 @@ int md_run(md_state* s, md_list* l, const md_run_config* cfg, double* energies) 
     // `body` enqueued `replays` times: captured once and replayed as a hipGraph, or launched eagerly
     auto repeat = [&](int replays, auto&& body) {
         if (replays <= 0) return;
-        if (!cfg->use_graph) {
+        if (!use_graph) {
             for (int k = 0; k < replays; ++k) body();
             return;
         }
@@ -408,8 +425,10 @@ int md_run(md_state* s, md_list* l, const md_run_config* cfg, double* energies) 
     // is gated by the stale word of that parity, which its force kernel clears while setting the other one. The
     // head is the first kick-drift; the tail is the last step's forces, closing kick and record.
     const char* unfolded = std::getenv("MDGPU_UNFOLDED");  // 1: keep the separate kick-kick-drift kernel
+    // (without a spare position buffer, which an allocation-free run cannot make, the separate kernels do the
+    // same arithmetic)
     const bool can_fold = fused && cfg->precision == MD_MIXED && tiling_enabled(s->n) && l->tiled &&
-                          !(unfolded && unfolded[0] == '1');
+                          !(unfolded && unfolded[0] == '1') && !(rb.no_alloc && !s->pos_spare);
     if (can_fold && steps >= 2 && !s->pos_spare) s->pos_spare = dev_alloc<double4>(s->n);
     double4* bufs[2] = {nullptr, nullptr};
     struct SpareGuard {  // on an exception the spare is whichever buffer s->pos is not
@@ -442,7 +461,7 @@ int md_run(md_state* s, md_list* l, const md_run_config* cfg, double* energies) 
             spare_guard.on = true;
             if (!resume) kick_drift_enqueue(s, cfg->dt, l);
             const int bodies = n_steps - 1;
-            if (cfg->use_graph && bodies >= 2) {
+            if (use_graph && bodies >= 2) {
                 // one graph holds an even and an odd step (their buffers differ)
                 repeat(bodies / 2, [&]() {
                     enqueue_fold_body(0);
@@ -496,6 +515,19 @@ int md_run(md_state* s, md_list* l, const md_run_config* cfg, double* energies) 
         MD_HIP(hipMemcpyAsync(hf + FLAG_COUNT + 2, rb.counters, sizeof(int), hipMemcpyDeviceToHost, st));
         MD_HIP(hipStreamSynchronize(st));
         if (!hf[FLAG_OVERFLOW]) break;
+        if (rb.no_alloc) {
+            // growing the rows allocates: leave the state frozen where the overflow halted it
+            const int frozen = hf[FLAG_COUNT + 2] + 1;
+            l->built = false;
+            if (energies && frozen > 1) {
+                MD_HIP(hipMemcpyAsync(energies, rb.energies, 4 * sizeof(double) * (frozen - 1),
+                                      hipMemcpyDeviceToHost, st));
+                MD_HIP(hipStreamSynchronize(st));
+            }
+            fail(MD_ERR_OVERFLOW, "a neighbour-list row outgrew its padding at step " + std::to_string(frozen) +
+                                      " of md_run_static: the state stands after that step's opening kick and "
+                                      "drift, steps before it are recorded; rebuild the list with md_list_build");
+        }
         if (attempt >= 8) fail(MD_ERR_OVERFLOW, "neighbour-list rows keep overflowing during md_run");
         // the stretch froze at its step j (1-based) = the first one without a record, after that step's opening
         // kick and drift and its re-sort; folded, the positions are in the buffer step j reads
@@ -538,6 +570,63 @@ int md_run(md_state* s, md_list* l, const md_run_config* cfg, double* energies) 
         l->half_pairs = static_cast<size_t>(total);
         l->max_count = hf[FLAG_MAXCOUNT];
     }
+}
+
+// Scratch of md_run_static: the run's counters, then its per-step records (4 doubles per step), then its Bussi
+// draws (2 doubles per step). One copy per GPU, held until the process ends.
+constexpr int STATIC_MAX_STEPS = 65536;
+constexpr size_t STATIC_HEAD = 256;
+constexpr size_t STATIC_BYTES = STATIC_HEAD + sizeof(double) * 6 * STATIC_MAX_STEPS;
+static __device__ __align__(256) unsigned char scratch[STATIC_BYTES];
+
+// The scratch array's address on `device`, looked up once per device with that device current (host code must
+// use this address, not the array's name, which on the host is only a stand-in).
+void* run_scratch_address(int device) {
+    static std::mutex lock;
+    static std::map<int, void*> known;
+    std::lock_guard<std::mutex> guard(lock);
+    const auto it = known.find(device);
+    if (it != known.end()) return it->second;
+    int current = 0;
+    MD_HIP(hipGetDevice(&current));
+    MD_HIP(hipSetDevice(device));
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    const hipError_t back = hipSetDevice(current);
+    hip_check(e, "hipGetSymbolAddress(scratch)");
+    hip_check(back, "hipSetDevice");
+    known[device] = ptr;
+    return ptr;
+}
+
+}  // namespace mdgpu
+
+extern "C" {
+
+int md_run(md_state* s, md_list* l, const md_run_config* cfg, double* energies) {
+    MD_API_BEGIN
+    if (!s || !l || !cfg) fail(MD_ERR_ARG, "state/list/config is NULL");
+    RunBuffers buffers;
+    if (cfg->steps > 0 && s->n > 0) {
+        buffers.energies = dev_alloc<double>(4 * static_cast<size_t>(cfg->steps));
+        buffers.counters = dev_alloc<int>(4);
+        if (cfg->thermostat) buffers.draws = dev_alloc<double>(2 * static_cast<size_t>(cfg->steps));
+    }
+    run_impl(s, l, cfg, energies, RunMem{buffers.energies, buffers.counters, buffers.draws, false});
+    MD_API_END
+}
+
+int md_run_static(md_state* s, md_list* l, const md_run_config* cfg, double* energies) {
+    MD_API_BEGIN
+    if (!s || !l || !cfg) fail(MD_ERR_ARG, "state/list/config is NULL");
+    if (cfg->steps > STATIC_MAX_STEPS)
+        fail(MD_ERR_ARG, "md_run_static runs at most " + std::to_string(STATIC_MAX_STEPS) + " steps per call");
+    if (!s->ctx->run_scratch) fail(MD_ERR_STATE, "no scratch address for this context's device");
+    auto* base = static_cast<unsigned char*>(s->ctx->run_scratch);
+    const size_t steps = cfg->steps > 0 ? static_cast<size_t>(cfg->steps) : 0;
+    RunMem mem{reinterpret_cast<double*>(base + STATIC_HEAD), reinterpret_cast<int*>(base),
+               reinterpret_cast<double*>(base + STATIC_HEAD + 4 * sizeof(double) * steps), true};
+    run_impl(s, l, cfg, energies, mem);
     MD_API_END
 }
 

@@ -99,6 +99,7 @@ struct md_ctx {
     double* host_scalars = nullptr;  // 64 doubles
     int* host_flags = nullptr;       // 16 ints
     int live = 0;                    // states and lists created from this context and not yet destroyed
+    void* run_scratch = nullptr;     // this device's address of md_run_static's scratch array
 };
 
 struct md_state {
@@ -271,6 +272,7 @@ void tiled_forces_step_enqueue(md_state* s, md_list* l, const md_lj& p, double d
 
 // integrate.hip
 void kick_drift_enqueue(md_state* s, double dt, md_list* check_list);
+void* run_scratch_address(int device);  // md_run_static's scratch array on `device` (looked up once per device)
 void kick_enqueue(md_state* s, double dt, double* ke_dev);
 
 // rdf.hip: the cell pass over every pair within rmax (buffers for one state size, made before a run)

@@ -114,6 +114,7 @@ _SIGS = {
     "md_rdf_sample": (C.c_int, [_P, _P, C.c_double, C.c_int, _U64P]),
     "md_rdf_sample_all": (C.c_int, [_P, C.c_double, C.c_int, _U64P]),
     "md_run": (C.c_int, [_P, _P, C.POINTER(RunConfig), _DP]),
+    "md_run_static": (C.c_int, [_P, _P, C.POINTER(RunConfig), _DP]),
     "md_bussi_rng_create": (C.c_int, [C.c_uint64, _PP]),
     "md_bussi_rng_destroy": (C.c_int, [_P]),
     "md_bussi_rng_draws": (C.c_int, [_P, C.c_size_t, C.c_int, _DP]),
