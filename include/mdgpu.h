@@ -196,6 +196,10 @@ typedef struct md_run_config {
     double rdf_rmax;
     uint64_t* rdf_counts;
     int* rdf_samples;
+    /* 1: each record's kinetic energy is System::kineticEnergy of the step's final velocities (after the
+       thermostat's scale), in its summation order (System.cpp:67-76), as md_sim prints it (main.cpp:285); the
+       step's kicks then run as separate kernels (the same trajectory) */
+    int exact_kinetic;
 } md_run_config;
 
 /* Advances `steps` velocity-Verlet steps entirely on the device: the list update decision, re-sort and rebuild

@@ -368,6 +368,9 @@ static void run_impl(md_state* s, md_list* l, const md_run_config* cfg, double* 
         enqueue_forces();
         MD_HIP(hipGetLastError());
     };
+    // exact_kinetic: the record's kinetic energy is System::kineticEnergy of the step's final velocities, in its
+    // summation order (after the thermostat's scale with Bussi on)
+    const bool exact_ke = cfg->exact_kinetic != 0;
     auto enqueue_step = [&](bool kick_drift) {
         if (kick_drift) kick_drift_enqueue(s, cfg->dt, l);
         enqueue_forces();
@@ -376,7 +379,12 @@ static void run_impl(md_state* s, md_list* l, const md_run_config* cfg, double* 
             kinetic_exact_enqueue(s, s->scalars + 3);
             k_bussi<<<1, 1, 0, st>>>(s->scalars, rb.draws, rb.counters + 1, dof, kbar, c, s->halt);
             k_scale_dev<<<blocks_for(nvel, 256), 256, 0, st>>>(s->vel, nvel, s->scalars + 5, s->halt);
-            k_record<<<1, 1, 0, st>>>(s->scalars, 6, rb.energies, rb.counters, s->halt);
+            if (exact_ke) kinetic_exact_enqueue(s, s->scalars + 7);
+            k_record<<<1, 1, 0, st>>>(s->scalars, exact_ke ? 7 : 6, rb.energies, rb.counters, s->halt);
+        } else if (exact_ke) {
+            kick_enqueue(s, cfg->dt, nullptr);
+            kinetic_exact_enqueue(s, s->scalars + 3);
+            k_record<<<1, 1, 0, st>>>(s->scalars, 3, rb.energies, rb.counters, s->halt);
         } else {
             kick_enqueue(s, cfg->dt, s->scalars + 3);
             k_record<<<1, 1, 0, st>>>(s->scalars, 3, rb.energies, rb.counters, s->halt);
@@ -418,7 +426,7 @@ static void run_impl(md_state* s, md_list* l, const md_run_config* cfg, double* 
         hip_check(sync, "hipStreamSynchronize");
     };
 
-    const bool fused = !cfg->thermostat;
+    const bool fused = !cfg->thermostat && !exact_ke;
     // Mixed NVE on the tiled rows goes further: the force kernel itself carries each step (closing kick, kinetic
     // energy, opening kick and drift of the next step) for its own particles, reading the positions from one
     // buffer and writing the next step's into the other. Step k (k = 1 .. steps - 1) reads buffer (k - 1) % 2 and

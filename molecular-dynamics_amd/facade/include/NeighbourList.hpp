@@ -46,6 +46,10 @@ public:
     md_list* deviceList() const {
         return device_;
     }
+    // after a device-driven run (md_run) rebuilt the device list: the host views are materialised again
+    void deviceUpdated() {
+        hostCurrent_ = false;
+    }
 
 private:
     void materialise() const;

@@ -3,6 +3,7 @@
 #include "NeighbourList.hpp"
 #include "System.hpp"
 
+#include <cstdint>
 #include <vector>
 
 // g(r) from the neighbour-list pairs (reference: include/RadialDistribution.hpp, src/RadialDistribution.cpp). The
@@ -12,6 +13,8 @@ public:
     RadialDistribution(double rMax, int bins);
 
     void sample(const System& system, const NeighbourList& neighbourList);
+    // mdgpu extension: adds `samples` samples' pair counts taken on the device (md_run's in-run histogram)
+    void addCounts(const std::uint64_t* counts, int samples, const System& system);
     std::vector<double> radii() const;
     std::vector<double> values() const;
     bool write(const char* path) const;

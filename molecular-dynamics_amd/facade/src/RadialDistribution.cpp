@@ -21,6 +21,13 @@ void RadialDistribution::sample(const System& system, const NeighbourList& neigh
     ++samples_;
 }
 
+void RadialDistribution::addCounts(const std::uint64_t* counts, int samples, const System& system) {
+    for (std::size_t b = 0; b < histogram_.size(); ++b) histogram_[b] += static_cast<double>(counts[b]);
+    particles_ = static_cast<double>(system.size());
+    density_ = particles_ / std::pow(system.getBoxLength(), 3);
+    samples_ += samples;
+}
+
 std::vector<double> RadialDistribution::radii() const {
     std::vector<double> r(histogram_.size());
     for (std::size_t b = 0; b < r.size(); ++b) r[b] = (static_cast<double>(b) + 0.5) * binWidth_;

@@ -46,7 +46,7 @@ class RunConfig(C.Structure):
                 ("draws", C.POINTER(C.c_double)), ("use_graph", C.c_int),
                 ("force_kernel_ms", C.POINTER(C.c_double)), ("rdf_every", C.c_int), ("rdf_bins", C.c_int),
                 ("rdf_rmax", C.c_double), ("rdf_counts", C.POINTER(C.c_uint64)),
-                ("rdf_samples", C.POINTER(C.c_int))]
+                ("rdf_samples", C.POINTER(C.c_int)), ("exact_kinetic", C.c_int)]
 
 
 class BatchConfig(C.Structure):
@@ -434,7 +434,7 @@ class NeighbourList:
         check(lib().md_rdf_sample(state.h, self.h, rmax, bins, _u64p(counts)))
 
     def run(self, state: State, lj: LJ, dt: float, steps: int, precision=MD_MIXED, draws=None,
-            temperature=1.0, tau=0.1, use_graph=True, record=True, timing=None, rdf=None):
+            temperature=1.0, tau=0.1, use_graph=True, record=True, timing=None, rdf=None, exact_kinetic=False):
         """md_run. With `timing` (a list), eager launches are used and the summed force-kernel time in ms is
         appended to it. With rdf = (every, rmax, counts: uint64[bins]) the run adds the all-pairs histogram of
         every `every`-th step to counts and returns (energies, samples)."""
@@ -450,6 +450,7 @@ class NeighbourList:
             draws = np.ascontiguousarray(draws, dtype=np.float64).reshape(steps, 2)
             cfg.draws = draws.ctypes.data_as(_DP)
         cfg.use_graph = 1 if (use_graph and timing is None) else 0
+        cfg.exact_kinetic = 1 if exact_kinetic else 0
         ms = C.c_double(0.0)
         if timing is not None:
             cfg.force_kernel_ms = C.pointer(ms)
