@@ -11,6 +11,7 @@
 // d_ij = -d_ji exactly. Forces are therefore bit-equal to Forcefield.cpp:214-326 and to the O(N^2) path.
 // MIXED: double minimum image and r^2, float LJ factor (Forcefield.cpp:352-446), double accumulation.
 #include <cmath>
+#include <cstdlib>
 
 #include "device.hpp"
 #include "internal.hpp"
@@ -142,10 +143,18 @@ void forces_enqueue(md_state* s, md_list* l, const md_lj& p, int precision, doub
     const size_t n = s->n;
     const unsigned blocks = blocks_for(n ? n : 1, FORCE_THREADS);
     const LJConst lj = make_lj(p);
-    // mixed precision runs on the tiled rows; FP64 keeps the per-particle gather kernel, whose single lane per
-    // particle sums in the reference's order
+    // both precisions run on the tiled rows (MDGPU_FP64_GATHER=1 keeps FP64 on the per-particle gather kernel,
+    // whose sums the tiled FP64 kernel reproduces bit for bit)
     if (l->tiled && precision == MD_MIXED) {
         tiled_forces_enqueue(s, l, lj, p.cutoff, out_dev, clear_flag, ev_start, ev_stop);
+        return;
+    }
+    static const bool fp64_gather = [] {
+        const char* v = std::getenv("MDGPU_FP64_GATHER");
+        return v && v[0] == '1';
+    }();
+    if (l->tiled && precision == MD_FP64 && !fp64_gather) {
+        tiled_forces_fp64_enqueue(s, l, lj, out_dev, clear_flag, ev_start, ev_stop);
         return;
     }
     ensure_rows32(l);

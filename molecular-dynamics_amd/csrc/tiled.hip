@@ -1340,6 +1340,235 @@ __global__ void __launch_bounds__(TILE_THREADS)
     }
 }
 
+// ---- FP64 forces over the tiled rows ---------------------------------------------------------------------
+//
+// The halo holds the raw positions (x | y | z doubles, the same slots as the mixed kernel), so each separation is
+// the reference's minimum image of r_i - r_j (Forcefield.cpp:214-326), exactly; in flat tiles no image can enter
+// and the separation is r_i - r_j itself (the image term would be d - L * 0 = d). Particle i's four lanes each
+// evaluate one entry of every group of four consecutive row entries (lane l entry 4g + l, the slot rows' layout),
+// and lane 0 adds the group's four terms F/r d to the particle's sums in entry order, taking lanes 1..3's terms
+// over the quad (DPP): the sums are the single-lane gather kernel's, k_forces_fp64's, bit for bit. Entries past
+// the row's end add -0.0, which leaves every sum unchanged. U, W and the inside count are summed over the row's
+// j > i part per lane, then per particle and per tile.
+
+// Lane L's value for every lane of its group of four (DPP quad_perm, as two 32-bit halves).
+template <int L>
+__device__ __forceinline__ double quad_lane(double v) {
+    constexpr int ctrl = L | (L << 2) | (L << 4) | (L << 6);
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), ctrl, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), ctrl, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+
+struct Fp64Sums {
+    double fx = 0.0, fy = 0.0, fz = 0.0;  // valid on the particle's lane 0
+    double u = 0.0, w = 0.0;              // this lane's share of the j > i part
+    int c = 0;
+};
+
+// Lane 0: s += t(lane 0) + t(lane 1) + t(lane 2) + t(lane 3), one addition at a time in that order.
+__device__ __forceinline__ void add_group(Fp64Sums& a, double tx, double ty, double tz) {
+    a.fx += tx;
+    a.fy += ty;
+    a.fz += tz;
+    a.fx += quad_lane<1>(tx);
+    a.fy += quad_lane<1>(ty);
+    a.fz += quad_lane<1>(tz);
+    a.fx += quad_lane<2>(tx);
+    a.fy += quad_lane<2>(ty);
+    a.fz += quad_lane<2>(tz);
+    a.fx += quad_lane<3>(tx);
+    a.fy += quad_lane<3>(ty);
+    a.fz += quad_lane<3>(tz);
+}
+
+// One entry's terms (fp64_pair's arithmetic): t = F/r d, and the energy sums when the entry is in the j > i part.
+__device__ __forceinline__ void fp64_term(double dx, double dy, double dz, const LJConst& lj, bool upper, Fp64Sums& a,
+                                          double& tx, double& ty, double& tz) {
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    const double inv_r2 = 1.0 / r2;
+    const double s2 = lj.sig2 * inv_r2;
+    const double s6 = s2 * s2 * s2;
+    const double s12 = s6 * s6;
+    const bool inside = r2 < lj.rc2;
+    const double f = inside ? lj.c24 * (2.0 * s12 - s6) * inv_r2 : 0.0;
+    tx = f * dx;
+    ty = f * dy;
+    tz = f * dz;
+    if (upper) {
+        a.u += inside ? lj.c4 * (s12 - s6) - lj.shift : 0.0;
+        a.w += f * r2;
+        a.c += inside ? 1 : 0;
+    }
+}
+
+__global__ void __launch_bounds__(TILE_THREADS, 6)
+    k_forces_tiled_fp64(const double4* __restrict__ pos, size_t n, const uint16_t* __restrict__ rows16,
+                        const int32_t* __restrict__ rows, const int32_t* __restrict__ count,
+                        const int32_t* __restrict__ half, const int4* __restrict__ tile_ranges,
+                        const int* __restrict__ tile_nr, int row_len, int tiles, LJConst lj, double box,
+                        double inv_box, double* __restrict__ frc, double* __restrict__ partials,
+                        int* __restrict__ clear_flag, const int* halt) {
+    if (halted(halt)) return;
+    __shared__ double halo[3][TILE_HALO_MAX];
+    double* xs = halo[0];
+    double(*e_part)[TILE] = reinterpret_cast<double(*)[TILE]>(&halo[0][0]);
+    __shared__ int ra[RMAX], rgs[RMAX], rofs[RMAX + 1];
+    const HaloRanges hr{ra, rgs, rofs};
+    const int tile = xcd_item(static_cast<int>(blockIdx.x), tiles);
+    const int raw_nr = tile_nr[tile];
+    const int4 rr = static_cast<int>(threadIdx.x) < RMAX ? tile_ranges[static_cast<size_t>(tile) * RMAX + threadIdx.x]
+                                                         : make_int4(0, 0, 0, 0);
+    const int nr = tile_range_count(raw_nr);
+    const bool staged = nr > 0;
+    const bool flat = tile_is_flat(raw_nr);
+    const int lane = threadIdx.x & (LANES - 1);
+    const size_t i = static_cast<size_t>(tile) * TILE + (threadIdx.x >> 2);
+    const bool active = i < n;
+    const int m = active ? count[i] : 0;
+    const int h = active ? half[i] : 0;
+    const double4 ri = active ? pos[i] : make_double4(0.0, 0.0, 0.0, 0.0);
+    if (staged && static_cast<int>(threadIdx.x) < nr) {
+        ra[threadIdx.x] = rr.x;
+        rgs[threadIdx.x] = rr.y;
+        rofs[threadIdx.x] = rr.z;
+        if (static_cast<int>(threadIdx.x) == nr - 1) rofs[nr] = rr.z + rr.w;
+    }
+    __syncthreads();
+    if (staged) {
+        stage_halo(pos, hr, nr, rofs[nr], [&](int q, const double4& p) {
+            halo[0][q] = p.x;
+            halo[1][q] = p.y;
+            halo[2][q] = p.z;
+        });
+    }
+    const int chunks = (m + CHUNK - 1) / CHUNK;
+    const uint4* rp = reinterpret_cast<const uint4*>(rows16 + slot_row_base(i, row_len) + lane * 8);
+    constexpr int stride = CHUNK_STRIDE / 8;
+    // the row's first chunk word, loaded before the barrier
+    uint4 next = staged && chunks > 0 ? rp[0] : make_uint4(0u, 0u, 0u, 0u);
+    __syncthreads();
+    Fp64Sums a;
+    if (active) {
+        if (staged) {
+            auto run = [&](auto wrap) {
+                constexpr bool W = decltype(wrap)::value;
+                // one group: the lane's entry k = 4 g + lane at LDS byte offset `off`, if it exists
+                auto entry = [&](unsigned off, int k) {
+                    double tx = -0.0, ty = -0.0, tz = -0.0;
+                    if (k < m) {
+                        const double* xj = reinterpret_cast<const double*>(reinterpret_cast<const char*>(xs) + off);
+                        double dx = ri.x - xj[0], dy = ri.y - xj[TILE_HALO_MAX], dz = ri.z - xj[2 * TILE_HALO_MAX];
+                        if constexpr (W) {
+                            dx = min_image(dx, box, inv_box);
+                            dy = min_image(dy, box, inv_box);
+                            dz = min_image(dz, box, inv_box);
+                        }
+                        fp64_term(dx, dy, dz, lj, k < h, a, tx, ty, tz);
+                    }
+                    add_group(a, tx, ty, tz);
+                };
+#pragma unroll 1
+                for (int c = 0; c < chunks; ++c) {
+                    const uint4 cw = next;
+                    if (c + 1 < chunks) next = rp[(c + 1) * stride];  // the next chunk's word in flight
+                    const int k0 = c * CHUNK + lane;
+#pragma unroll 1
+                    for (int q = 0; q < 4; ++q) {  // word q holds the lane's entries 8 q / 2 .. : two per word
+                        const unsigned word = q == 0 ? cw.x : (q == 1 ? cw.y : (q == 2 ? cw.z : cw.w));
+                        entry(word & 0xFFFFu, k0 + (2 * q) * LANES);
+                        entry(word >> 16, k0 + (2 * q + 1) * LANES);
+                    }
+                }
+            };
+            if (flat)
+                run(std::false_type{});
+            else
+                run(std::true_type{});
+        } else {
+            // unstaged tile: entry 4 g + lane through the int32 rows, two groups of the lane in flight
+            const int groups = (m + LANES - 1) / LANES;
+            for (int g0 = 0; g0 < groups; g0 += UNSTAGED_BATCH) {
+                int j[UNSTAGED_BATCH];
+#pragma unroll
+                for (int b = 0; b < UNSTAGED_BATCH; ++b) {
+                    const int k = (g0 + b) * LANES + lane;
+                    j[b] = k < m ? rows[static_cast<size_t>(k) * n + i] : -1;
+                }
+                double xj[UNSTAGED_BATCH], yj[UNSTAGED_BATCH], zj[UNSTAGED_BATCH];
+#pragma unroll
+                for (int b = 0; b < UNSTAGED_BATCH; ++b) {
+                    if (j[b] < 0) continue;
+                    const double4 r = pos[j[b]];
+                    xj[b] = r.x;
+                    yj[b] = r.y;
+                    zj[b] = r.z;
+                }
+#pragma unroll
+                for (int b = 0; b < UNSTAGED_BATCH; ++b) {
+                    if (g0 + b >= groups) break;
+                    double tx = -0.0, ty = -0.0, tz = -0.0;
+                    if (j[b] >= 0)
+                        fp64_term(min_image(ri.x - xj[b], box, inv_box), min_image(ri.y - yj[b], box, inv_box),
+                                  min_image(ri.z - zj[b], box, inv_box), lj, (g0 + b) * LANES + lane < h, a, tx, ty,
+                                  tz);
+                    add_group(a, tx, ty, tz);
+                }
+            }
+        }
+        if (lane == 0) {
+            frc[i] = a.fx;
+            frc[n + i] = a.fy;
+            frc[2 * n + i] = a.fz;
+        }
+    }
+    // the particle's energy sums (its four lanes are adjacent)
+#pragma unroll
+    for (int off = 1; off < LANES; off <<= 1) {
+        a.u += __shfl_xor(a.u, off, WAVE);
+        a.w += __shfl_xor(a.w, off, WAVE);
+        a.c += __shfl_xor(a.c, off, WAVE);
+    }
+    const int pl = threadIdx.x >> 2;
+    __syncthreads();  // every wavefront is done with the halo
+    if (lane == 0) {
+        e_part[0][pl] = a.u;
+        e_part[1][pl] = a.w;
+        e_part[2][pl] = static_cast<double>(a.c);
+    }
+    if (clear_flag && blockIdx.x == 0 && threadIdx.x == 0) *clear_flag = 0;
+    __syncthreads();
+    if (threadIdx.x < WAVE) {
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            double v = e_part[q][threadIdx.x] + e_part[q][threadIdx.x + WAVE];
+            v = wave_sum(v);
+            if (threadIdx.x == 0) partials[static_cast<size_t>(tile) * 3 + q] = v;
+        }
+    }
+}
+
+static int slot_row_len(int padding);
+
+void tiled_forces_fp64_enqueue(md_state* s, md_list* l, const LJConst& lj, double* out_dev, int* clear_flag,
+                               hipEvent_t ev_start, hipEvent_t ev_stop) {
+    hipStream_t st = s->ctx->stream;
+    const size_t n = s->n;
+    const int tiles = tile_count(n ? n : 1);
+    state_reserve_partials(s, static_cast<size_t>(tiles) * 3);
+    // unstaged tiles gather through the int32 rows
+    ensure_rows32(l);
+    if (ev_start) MD_HIP(hipEventRecord(ev_start, st));
+    if (n > 0) {
+        hipLaunchKernelGGL(k_forces_tiled_fp64, dim3(tiles), dim3(TILE_THREADS), 0, st, s->pos, n, l->rows16, l->rows,
+                           l->count, l->half, l->tile_ranges, l->tile_nr, slot_row_len(l->padding), tiles, lj, s->box,
+                           s->inv_box, s->frc, s->partials, clear_flag, s->halt);
+        MD_HIP(hipGetLastError());
+    }
+    if (ev_stop) MD_HIP(hipEventRecord(ev_stop, st));
+    reduce_partials_enqueue(s, n > 0 ? tiles : 0, 3, out_dev);
+}
+
 bool tiling_enabled(size_t n) {
     // the slot rows index chunks with 32-bit strides: 32 n < 2^32
     if (n >= (size_t{1} << 27)) return false;

@@ -281,6 +281,33 @@ def test_mixed_run_folded_step_with_unstaged_tiles(ctx, monkeypatch):
     np.testing.assert_allclose(eb[:, 1], ea[:, 1], rtol=1e-12, atol=0)
 
 
+def test_fp64_tiled_forces_bit_exact_on_every_tile_kind(ctx):
+    """FP64 forces through the tiled rows at 108k atoms, skin 0.9, after 30 steps: flat and wrapping tiles that
+    stage their halo and tiles that gather through the int32 rows (about two thirds) are all bit-exact against the
+    oracle's deterministic mode."""
+    g = mdgpu.fcc_system(ctx, 30, 0.8442, 0.722, 1)
+    nl = mdgpu.NeighbourList(ctx, 2.5, 0.9)
+    nl.build(g)
+    nl.forces(g, LJ, mdgpu.MD_FP64)
+    nl.run(g, LJ, 0.005, 30, precision=mdgpu.MD_FP64, record=False)
+    st = g.state()
+    s = O.Sim.from_arrays(st["pos"][:, :3], g.box, vel=st["vel"].T)
+    s.forcefield(1.0, 1.0, 2.5, deterministic=True)
+    s.neighbour_list(2.5, 0.9, reverse=True)
+    s.list_build()
+    u = s.forces_list()
+    h = mdgpu.State.from_arrays(ctx, g.box, st["pos"], vel=st["vel"])
+    nh = mdgpu.NeighbourList(ctx, 2.5, 0.9)
+    nh.build(h)
+    info = nh.info()
+    assert info.tiled == 1 and 0 < info.unstaged_tiles < info.tiles
+    r = nh.forces(h, LJ, mdgpu.MD_FP64)
+    assert np.array_equal(h.download(mdgpu.FIELD_FORCE), s.state()["force"])
+    assert r.potential == pytest.approx(u, rel=1e-12, abs=0)
+    assert r.virial == pytest.approx(s.virial, rel=1e-12, abs=0)
+    assert r.inside_pairs == s.inside_pairs
+
+
 def test_float_halo_build_matches_double_build(ctx, monkeypatch):
     """A 256k-atom perturbed FCC state has 24 cells per side, so its tiled build stages the halo as float frame
     coordinates and re-tests in double the candidates within rounding of the list radius. Its lists equal those of the
