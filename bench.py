@@ -423,7 +423,7 @@ def main():
                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                     "kernel": ("k_forces_tiled_mixed<true>" if fold else "k_forces_tiled_mixed" if tiled
                                else "k_forces_mixed") if prec == mdgpu.MD_MIXED
-                    else "k_forces_tiled_fp64" if tiled else "k_forces_fp64",
+                    else "k_forces_tiled_fp64_pair" if tiled else "k_forces_fp64",
                     "avg_launch_ms": round(avg_ms, 5), "algorithmic_bytes_per_launch": round(bytes_per_atom * n)}
         if valu and fold:
             # the kernel's issue floor: its wavefront VALU instructions (rocprofv3 SQ_INSTS_VALU of the same

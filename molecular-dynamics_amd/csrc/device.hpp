@@ -13,9 +13,29 @@ namespace mdgpu {
 constexpr int WAVE = 64;
 
 // Minimum image of one separation component: d - L * round(d / L), round half away from zero (System.hpp:94).
+// round(x) is formed as trunc(x + copysign(0.5 - 2^-54, x)), two FP64 operations instead of the library's four:
+// the addition rounds to nearest even, so x + (0.5 - 2^-54) reaches the next integer exactly when x's fraction is
+// at least 1/2 (at a tie the exact sum lies half an ulp below the integer and rounds up to it; below a tie it stays
+// under it), which is round()'s result for every double, ties, infinities and signed zeros included.
 __device__ __forceinline__ double min_image(double d, double box, double inv_box) {
-    const double images = round(d * inv_box);
+    const double x = d * inv_box;
+    const double images = trunc(x + copysign(0.49999999999999994, x));
     return d - box * images;
+}
+
+// 1 / r2 as the compiler's IEEE double division computes it (v_div_scale, v_rcp_f64, two Newton steps, the
+// residual step through v_div_fmas, v_div_fixup), without its scaling and special-case steps, which change nothing
+// for 2^-767 <= r2 < 2^1022: the same correctly rounded quotient, in seven instructions instead of eleven. The LJ
+// kernels use the value only for r2 < rc^2; below 2^-767 (|r| < 1e-115) the LJ powers overflow and the force is
+// NaN either way.
+__device__ __forceinline__ double recip_rn(double r2) {
+    double y = __builtin_amdgcn_rcp(r2);
+    double e = fma(-r2, y, 1.0);
+    y = fma(y, e, y);
+    e = fma(-r2, y, 1.0);
+    y = fma(y, e, y);
+    e = fma(-r2, y, 1.0);
+    return fma(e, y, y);
 }
 
 // x - L * floor(x / L) (System.hpp:110-112).
@@ -57,6 +77,28 @@ __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
     for (int off = WAVE / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, WAVE);
     return v;
+}
+
+// Lanes' values moved by one DPP control (CTRL) within the rows of ROW_MASK; lanes not written read 0.
+template <int CTRL, int ROW_MASK = 0xF>
+__device__ __forceinline__ double dpp_f64(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROW_MASK, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROW_MASK, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+
+// Wavefront sum through DPP moves only (no LDS round trips): quads, half rows and rows by permutes and mirrors,
+// then row 15/31 broadcasts into lane 63, read back as a uniform value.
+__device__ __forceinline__ double wave_sum_dpp(double v) {
+    v += dpp_f64<0xB1>(v);         // quad_perm [1,0,3,2]
+    v += dpp_f64<0x4E>(v);         // quad_perm [2,3,0,1]
+    v += dpp_f64<0x141>(v);        // row_half_mirror
+    v += dpp_f64<0x140>(v);        // row_mirror: every lane holds its row's sum
+    v += dpp_f64<0x142, 0xA>(v);   // row_bcast:15 into rows 1 and 3
+    v += dpp_f64<0x143, 0xC>(v);   // row_bcast:31 into rows 2 and 3: lane 63 holds the total
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), WAVE - 1);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), WAVE - 1);
+    return __hiloint2double(hi, lo);
 }
 
 __device__ __forceinline__ float wave_sum(float v) {

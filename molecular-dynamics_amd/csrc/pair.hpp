@@ -31,7 +31,7 @@ __device__ __forceinline__ void fp64_pair(const double4& ri, const double4& rj, 
     const double dy = min_image(ri.y - rj.y, box, inv_box);
     const double dz = min_image(ri.z - rj.z, box, inv_box);
     const double r2 = dx * dx + dy * dy + dz * dz;
-    const double inv_r2 = 1.0 / r2;
+    const double inv_r2 = recip_rn(r2);
     const double s2 = lj.sig2 * inv_r2;
     const double s6 = s2 * s2 * s2;
     const double s12 = s6 * s6;

@@ -37,7 +37,6 @@ constexpr int TILE_THREADS = TILE * LANES;
 constexpr int RMAX = TILE_RANGES;
 constexpr int CR_CAP = 192;  // cell ranges before merging (9 neighbour columns x 2 wrap pieces x segments)
 constexpr int CMAX = 24;     // own cells of a tile covered by the stencil table
-constexpr int STAGE_ROUNDS = (TILE_HALO_MAX + TILE_THREADS - 1) / TILE_THREADS;
 constexpr int CHUNK = 32;    // slot-row entries per particle and chunk
 constexpr int CHUNK_STRIDE = TILE * CHUNK;  // entries between a tile's chunks
 constexpr int GHOST_SLOT = TILE_HALO_MAX - 1;  // LDS slot holding NaN coordinates: never a neighbour
@@ -62,8 +61,14 @@ constexpr int SETUP_TILES = 4;
 // has wrapped), and the tile and its halo span less than half the box. Then t - L rint(t / L) = t for every
 // staged coordinate t = x - ref, and staging skips the image term.
 constexpr int TILE_FLAT = 1 << 16;
+// The same per axis (TILE_FLAT_X << a): the tile's cells lie at least two cells from both faces normal to axis a
+// and span, with their stencil, less than half the box along it, so no periodic image enters that component of
+// any separation (the FP64 kernel skips the minimum image along it). TILE_FLAT is set when all three are.
+constexpr int TILE_FLAT_X = 1 << 17;
 __device__ __forceinline__ int tile_range_count(int raw) { return raw < 0 ? -1 : (raw & (TILE_FLAT - 1)); }
 __device__ __forceinline__ bool tile_is_flat(int raw) { return raw > 0 && (raw & TILE_FLAT) != 0; }
+// bit a set: the minimum image is needed along axis a
+__device__ __forceinline__ int tile_wrap_axes(int raw) { return raw > 0 ? (~raw >> 17) & 7 : 7; }
 static_assert(RMAX == WAVE, "one lane per halo range in k_tile_setup");
 static_assert(CHUNK_STRIDE == 4096, "slot_entry shifts the chunk index by 7 + 5 bits");
 static_assert(TILE == 2 * WAVE, "the force kernel's energy sum: one wavefront adds two particles per lane");
@@ -304,10 +309,19 @@ __global__ void __launch_bounds__(SETUP_TILES * WAVE)
             if (wave_path) nr = tile_cell_ranges_wave(c0, c1, side, cr, lane);
             if (lane == 0) {
                 if (!wave_path) nr = tile_cell_ranges(c0, c1, side, cr);
-                const int col = c0 / side, x = col / side, y = col % side, z0 = c0 % side, z1 = c1 % side;
+                // cells c = (x side + y) side + z; the tile's cells c0 .. c1 span columns col0 .. col1
+                const int col0 = c0 / side, col1 = c1 / side, z0 = c0 % side, z1 = c1 % side;
+                const int x0 = col0 / side, y0 = col0 % side, x1 = col1 / side, y1 = col1 % side;
                 nr_all[w] = nr;
-                flat_all[w] = col == c1 / side && x >= 2 && x <= side - 3 && y >= 2 && y <= side - 3 && z0 >= 2 &&
-                              z1 <= side - 3 && static_cast<double>(c1 - c0 + 3) < flat_cells;
+                auto axis = [&](int lo, int hi) {  // cells lo .. hi along an axis, stencil included
+                    return lo >= 2 && hi <= side - 3 && static_cast<double>(hi - lo + 3) < flat_cells;
+                };
+                // one column, or two adjacent ones (then the z range runs through both z faces)
+                const bool fx = col1 - col0 <= 1 && axis(min(x0, x1), max(x0, x1));
+                const bool fy = col1 - col0 <= 1 && axis(min(y0, y1), max(y0, y1));
+                const bool fz = col1 == col0 && axis(z0, z1);
+                flat_all[w] = (fx ? TILE_FLAT_X : 0) | (fy ? TILE_FLAT_X << 1 : 0) | (fz ? TILE_FLAT_X << 2 : 0) |
+                              (fx && fy && fz ? TILE_FLAT : 0);
             }
         } else if (lane == 0) {
             cr[0] = make_int2(0, side * side * side - 1);  // every particle
@@ -333,7 +347,7 @@ __global__ void __launch_bounds__(SETUP_TILES * WAVE)
     const int total = __shfl(incl, WAVE - 1, WAVE);
     if (lane < nr)
         tile_ranges[static_cast<size_t>(t) * RMAX + lane] = make_int4(first_cell, first, incl - cnt, cnt);
-    if (lane == 0) tile_nr[t] = (nr > 0 && total <= hmax) ? nr + (flat_all[w] ? TILE_FLAT : 0) : -1;
+    if (lane == 0) tile_nr[t] = (nr > 0 && total <= hmax) ? nr + flat_all[w] : -1;
     if (gather) store_gathered(pa, g, t, lane, n);
 }
 
@@ -385,13 +399,14 @@ __device__ __forceinline__ int slot_base_of_cell(int nc, HaloRanges h, const uin
     return slot_base_at(nc, h, static_cast<int>(start[nc]), nr);
 }
 
-// Stages the halo: thread q loads slots q, q + 512, ... (all loads issued before any is used), and hands each
-// loaded position with its slot to `store`.
-template <class Store>
+// Stages the halo: thread q of THREADS loads slots q, q + THREADS, ... (all loads issued before any is used), and
+// hands each loaded position with its slot to `store`.
+template <int THREADS = TILE_THREADS, class Store>
 __device__ __forceinline__ void stage_halo(const double4* __restrict__ pos, HaloRanges h, int nr, int total,
                                            Store store) {
-    int src[STAGE_ROUNDS];
-    // the thread's slots grow by TILE_THREADS per round: one binary search, then a short forward walk per round
+    constexpr int ROUNDS = (TILE_HALO_MAX + THREADS - 1) / THREADS;
+    int src[ROUNDS];
+    // the thread's slots grow by THREADS per round: one binary search, then a short forward walk per round
     int r = 0;
     {
         int lo = 0, hi = nr - 1;
@@ -406,8 +421,8 @@ __device__ __forceinline__ void stage_halo(const double4* __restrict__ pos, Halo
         r = lo;
     }
 #pragma unroll
-    for (int u = 0; u < STAGE_ROUNDS; ++u) {
-        const int q = static_cast<int>(threadIdx.x) + u * TILE_THREADS;
+    for (int u = 0; u < ROUNDS; ++u) {
+        const int q = static_cast<int>(threadIdx.x) + u * THREADS;
         if (q < total) {
             while (h.rofs[r + 1] <= q) ++r;  // rofs[nr] = total > q ends the walk
             src[u] = h.rgs[r] + (q - h.rofs[r]);
@@ -415,13 +430,54 @@ __device__ __forceinline__ void stage_halo(const double4* __restrict__ pos, Halo
             src[u] = -1;
         }
     }
-    double4 p[STAGE_ROUNDS];
+    double4 p[ROUNDS];
 #pragma unroll
-    for (int u = 0; u < STAGE_ROUNDS; ++u)
+    for (int u = 0; u < ROUNDS; ++u)
         if (src[u] >= 0) p[u] = pos[src[u]];
 #pragma unroll
-    for (int u = 0; u < STAGE_ROUNDS; ++u)
-        if (src[u] >= 0) store(static_cast<int>(threadIdx.x) + u * TILE_THREADS, p[u]);
+    for (int u = 0; u < ROUNDS; ++u)
+        if (src[u] >= 0) store(static_cast<int>(threadIdx.x) + u * THREADS, p[u]);
+}
+
+// Stages the halo with no LDS reads: every lane of each of the block's W wavefronts holds range r = lane of the
+// tile (rr: first cell, first particle, first slot, count; ranges r >= nr unused). A wavefront takes the 64-slot
+// windows wave, wave + W, ...; a window's first slot lies in the last range that starts at or before it (a ballot
+// of the ranges' first slots), and the ranges that start inside the window (another ballot, usually one or two)
+// take over the lanes at and past their first slot. All loads are issued before any is used.
+template <int W, class Store>
+__device__ __forceinline__ void stage_halo_lanes(const double4* __restrict__ pos, int4 rr, int nr, Store store) {
+    const int lane = static_cast<int>(threadIdx.x) % WAVE, wave = static_cast<int>(threadIdx.x) / WAVE;
+    const int first = lane < nr ? rr.z : INT_MAX;  // first slot of range `lane`
+    const int shift = rr.y - rr.z;                 // particle index minus slot within range `lane`
+    const int total = __builtin_amdgcn_readlane(rr.z + rr.w, nr - 1);
+    constexpr int ROUNDS = (TILE_HALO_MAX + W * WAVE - 1) / (W * WAVE);
+    int src[ROUNDS];
+#pragma unroll
+    for (int u = 0; u < ROUNDS; ++u) {
+        const int base = (wave + W * u) * WAVE;
+        const int q = base + lane;
+        int g = 0;
+        if (base < total) {
+            const uint64_t at_or_before = __builtin_amdgcn_ballot_w64(first <= base);  // bit 0 set: first[0] = 0
+            g = __builtin_amdgcn_readlane(shift, 63 - __builtin_clzll(at_or_before));
+            uint64_t inside = __builtin_amdgcn_ballot_w64(first > base && first < base + WAVE);
+            while (inside) {
+                const int b = __builtin_ctzll(inside);
+                inside &= inside - 1;
+                const int fb = __builtin_amdgcn_readlane(first, b);
+                const int gb = __builtin_amdgcn_readlane(shift, b);
+                g = q >= fb ? gb : g;
+            }
+        }
+        src[u] = q < total ? g + q : -1;
+    }
+    double4 p[ROUNDS];
+#pragma unroll
+    for (int u = 0; u < ROUNDS; ++u)
+        if (src[u] >= 0) p[u] = pos[src[u]];
+#pragma unroll
+    for (int u = 0; u < ROUNDS; ++u)
+        if (src[u] >= 0) store((wave + W * u) * WAVE + lane, p[u]);
 }
 
 struct TileArgs {
@@ -1343,208 +1399,219 @@ __global__ void __launch_bounds__(TILE_THREADS)
 // ---- FP64 forces over the tiled rows ---------------------------------------------------------------------
 //
 // The halo holds the raw positions (x | y | z doubles, the same slots as the mixed kernel), so each separation is
-// the reference's minimum image of r_i - r_j (Forcefield.cpp:214-326), exactly; in flat tiles no image can enter
-// and the separation is r_i - r_j itself (the image term would be d - L * 0 = d). Particle i's four lanes each
-// evaluate one entry of every group of four consecutive row entries (lane l entry 4g + l, the slot rows' layout),
-// and lane 0 adds the group's four terms F/r d to the particle's sums in entry order, taking lanes 1..3's terms
-// over the quad (DPP): the sums are the single-lane gather kernel's, k_forces_fp64's, bit for bit. Entries past
-// the row's end add -0.0, which leaves every sum unchanged. U, W and the inside count are summed over the row's
-// j > i part per lane, then per particle and per tile.
+// the reference's minimum image of r_i - r_j (Forcefield.cpp:214-326), exactly; along an axis where the tile is
+// flat no image can enter and the separation component is the difference itself (the image term would be
+// d - L * 0 = d).
+//
+// Two lanes per particle: a tile's 128 particles are worked by 256 lanes (four wavefronts, three blocks per CU as
+// the halo's LDS admits, so three wavefronts per SIMD). Of particle i's lane pair, the even lane evaluates the even
+// row entries and the odd lane the odd ones (the even lane's words 0 and 2 of each chunk, the odd lane's words 1
+// and 3); for every four entries 4g .. 4g + 3 the even lane adds t(4g), its partner's t(4g + 1) (one DPP move per
+// 32-bit half), t(4g + 2) and the partner's t(4g + 3) to the particle's sums, one addition at a time in entry
+// order: the sums are the gather kernel's, k_forces_fp64's, bit for bit. Against four lanes per particle this
+// halves the additions and transfers that only one lane of each group uses. An entry past the row's end reads
+// halo slot 0 (a finite position) with its force factor forced to zero, so its term is +-0, which leaves every sum
+// unchanged (a sum that starts at +0 never becomes -0 under round to nearest); the loop body is branch free.
+// U, W and the inside count are summed over the row's j > i part per lane, then per tile.
 
-// Lane L's value for every lane of its group of four (DPP quad_perm, as two 32-bit halves).
-template <int L>
-__device__ __forceinline__ double quad_lane(double v) {
-    constexpr int ctrl = L | (L << 2) | (L << 4) | (L << 6);
+constexpr int PAIR_THREADS = 2 * TILE;
+
+struct PairLaneSums {
+    double fx = 0.0, fy = 0.0, fz = 0.0;  // valid on the particle's even lane
+    double s12 = 0.0, s6 = 0.0;           // this lane's entries of the j > i part inside the cutoff
+    int c = 0;
+};
+
+// One entry's term t = F/r d (fp64_pair's arithmetic): zero unless `valid`; `upper` (entry in the j > i part) adds
+// the energy sums. Per pair U = 4 eps (s12 - s6) - shift and W = F r = 24 eps (2 s12 - s6), so the lanes sum s12 and
+// s6 and count the pairs, and the tile's U and W are formed from the three sums. 2 s12 - s6 is one FMA: the
+// doubling is exact, so it rounds once, as the reference's two operations do (for s12 below 2^1022, i.e. r above
+// 2^-85 sigma). With sigma = 1 (UNIT) s2 = 1 * inv_r2 = inv_r2 exactly.
+template <bool UNIT>
+__device__ __forceinline__ void pair_lane_term(double dx, double dy, double dz, bool valid, bool upper,
+                                               const LJConst& lj, PairLaneSums& a, double& tx, double& ty,
+                                               double& tz) {
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    const double inv_r2 = recip_rn(r2);
+    const double s2 = UNIT ? inv_r2 : lj.sig2 * inv_r2;
+    const double s6 = s2 * s2 * s2;
+    const double s12 = s6 * s6;
+    const double t2 = fma(2.0, s12, -s6);
+    double fr = lj.c24 * t2 * inv_r2;
+    asm("" : "+v"(fr));  // computed on every lane: the select stays a select, not a branch around the arithmetic
+    const bool inside = r2 < lj.rc2;
+    const double f = (inside && valid) ? fr : 0.0;
+    tx = f * dx;
+    ty = f * dy;
+    tz = f * dz;
+    const bool up = inside && upper;
+    a.s12 += up ? s12 : 0.0;
+    a.s6 += up ? s6 : 0.0;
+    a.c += up ? 1 : 0;
+}
+
+// Wavefront maximum of non-negative values through DPP moves (the sequence of wave_sum_dpp; unwritten lanes read 0).
+__device__ __forceinline__ int wave_max(int v) {
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, false));
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xF, 0xF, false));
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x141, 0xF, 0xF, false));
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x140, 0xF, 0xF, false));
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x142, 0xA, 0xF, false));
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x143, 0xC, 0xF, false));
+    return __builtin_amdgcn_readlane(v, WAVE - 1);
+}
+
+// The partner lane's value (DPP quad_perm [1, 1, 3, 3]: even lanes read the odd lane above, as two 32-bit halves).
+__device__ __forceinline__ double odd_partner(double v) {
+    constexpr int ctrl = 1 | (1 << 2) | (3 << 4) | (3 << 6);
     const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), ctrl, 0xF, 0xF, false);
     const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), ctrl, 0xF, 0xF, false);
     return __hiloint2double(hi, lo);
 }
 
-struct Fp64Sums {
-    double fx = 0.0, fy = 0.0, fz = 0.0;  // valid on the particle's lane 0
-    double u = 0.0, w = 0.0;              // this lane's share of the j > i part
-    int c = 0;
-};
-
-// Lane 0: s += t(lane 0) + t(lane 1) + t(lane 2) + t(lane 3), one addition at a time in that order.
-__device__ __forceinline__ void add_group(Fp64Sums& a, double tx, double ty, double tz) {
+// Even lane: s += t(this lane), then t(the odd partner), one addition at a time.
+__device__ __forceinline__ void add_pair(PairLaneSums& a, double tx, double ty, double tz) {
+    const double px = odd_partner(tx), py = odd_partner(ty), pz = odd_partner(tz);
     a.fx += tx;
     a.fy += ty;
     a.fz += tz;
-    a.fx += quad_lane<1>(tx);
-    a.fy += quad_lane<1>(ty);
-    a.fz += quad_lane<1>(tz);
-    a.fx += quad_lane<2>(tx);
-    a.fy += quad_lane<2>(ty);
-    a.fz += quad_lane<2>(tz);
-    a.fx += quad_lane<3>(tx);
-    a.fy += quad_lane<3>(ty);
-    a.fz += quad_lane<3>(tz);
+    a.fx += px;
+    a.fy += py;
+    a.fz += pz;
 }
 
-// One entry's terms (fp64_pair's arithmetic): t = F/r d, and the energy sums when the entry is in the j > i part.
-__device__ __forceinline__ void fp64_term(double dx, double dy, double dz, const LJConst& lj, bool upper, Fp64Sums& a,
-                                          double& tx, double& ty, double& tz) {
-    const double r2 = dx * dx + dy * dy + dz * dz;
-    const double inv_r2 = 1.0 / r2;
-    const double s2 = lj.sig2 * inv_r2;
-    const double s6 = s2 * s2 * s2;
-    const double s12 = s6 * s6;
-    const bool inside = r2 < lj.rc2;
-    const double f = inside ? lj.c24 * (2.0 * s12 - s6) * inv_r2 : 0.0;
-    tx = f * dx;
-    ty = f * dy;
-    tz = f * dz;
-    if (upper) {
-        a.u += inside ? lj.c4 * (s12 - s6) - lj.shift : 0.0;
-        a.w += f * r2;
-        a.c += inside ? 1 : 0;
-    }
-}
-
-__global__ void __launch_bounds__(TILE_THREADS, 6)
-    k_forces_tiled_fp64(const double4* __restrict__ pos, size_t n, const uint16_t* __restrict__ rows16,
-                        const int32_t* __restrict__ rows, const int32_t* __restrict__ count,
-                        const int32_t* __restrict__ half, const int4* __restrict__ tile_ranges,
-                        const int* __restrict__ tile_nr, int row_len, int tiles, LJConst lj, double box,
-                        double inv_box, double* __restrict__ frc, double* __restrict__ partials,
-                        int* __restrict__ clear_flag, const int* halt) {
+template <bool UNIT>
+__global__ void __launch_bounds__(PAIR_THREADS, 3)
+    k_forces_tiled_fp64_pair(const double4* __restrict__ pos, size_t n, const uint16_t* __restrict__ rows16,
+                             const int32_t* __restrict__ rows, const int32_t* __restrict__ count,
+                             const int32_t* __restrict__ half, const int4* __restrict__ tile_ranges,
+                             const int* __restrict__ tile_nr, int row_len, int tiles, LJConst lj, double box,
+                             double inv_box, double* __restrict__ frc, double* __restrict__ partials,
+                             int* __restrict__ clear_flag, const int* halt) {
     if (halted(halt)) return;
     __shared__ double halo[3][TILE_HALO_MAX];
-    double* xs = halo[0];
-    double(*e_part)[TILE] = reinterpret_cast<double(*)[TILE]>(&halo[0][0]);
-    __shared__ int ra[RMAX], rgs[RMAX], rofs[RMAX + 1];
-    const HaloRanges hr{ra, rgs, rofs};
+    __shared__ double e_part[PAIR_THREADS / WAVE][3];
     const int tile = xcd_item(static_cast<int>(blockIdx.x), tiles);
     const int raw_nr = tile_nr[tile];
-    const int4 rr = static_cast<int>(threadIdx.x) < RMAX ? tile_ranges[static_cast<size_t>(tile) * RMAX + threadIdx.x]
-                                                         : make_int4(0, 0, 0, 0);
+    // every wavefront holds the tile's ranges, one per lane
+    const int4 rr = tile_ranges[static_cast<size_t>(tile) * RMAX + threadIdx.x % WAVE];
     const int nr = tile_range_count(raw_nr);
     const bool staged = nr > 0;
-    const bool flat = tile_is_flat(raw_nr);
-    const int lane = threadIdx.x & (LANES - 1);
-    const size_t i = static_cast<size_t>(tile) * TILE + (threadIdx.x >> 2);
+    const int wrap_axes = tile_wrap_axes(raw_nr);
+    const int par = threadIdx.x & 1;  // 0: even entries, 1: odd entries
+    const size_t i = static_cast<size_t>(tile) * TILE + (threadIdx.x >> 1);
     const bool active = i < n;
     const int m = active ? count[i] : 0;
     const int h = active ? half[i] : 0;
     const double4 ri = active ? pos[i] : make_double4(0.0, 0.0, 0.0, 0.0);
-    if (staged && static_cast<int>(threadIdx.x) < nr) {
-        ra[threadIdx.x] = rr.x;
-        rgs[threadIdx.x] = rr.y;
-        rofs[threadIdx.x] = rr.z;
-        if (static_cast<int>(threadIdx.x) == nr - 1) rofs[nr] = rr.z + rr.w;
-    }
-    __syncthreads();
     if (staged) {
-        stage_halo(pos, hr, nr, rofs[nr], [&](int q, const double4& p) {
+        stage_halo_lanes<PAIR_THREADS / WAVE>(pos, rr, nr, [&](int q, const double4& p) {
             halo[0][q] = p.x;
             halo[1][q] = p.y;
             halo[2][q] = p.z;
         });
     }
+    // the lane's words of chunk c: w0 (entries 4 g + par) and w1 (entries 4 g + par + 2), g = 0..7
+    const uint4* rp = reinterpret_cast<const uint4*>(rows16 + slot_row_base(i, row_len)) + par;
+    constexpr int stride = CHUNK_STRIDE / 8;  // uint4 words between chunks
     const int chunks = (m + CHUNK - 1) / CHUNK;
-    const uint4* rp = reinterpret_cast<const uint4*>(rows16 + slot_row_base(i, row_len) + lane * 8);
-    constexpr int stride = CHUNK_STRIDE / 8;
-    // the row's first chunk word, loaded before the barrier
-    uint4 next = staged && chunks > 0 ? rp[0] : make_uint4(0u, 0u, 0u, 0u);
+    uint4 w0 = staged && chunks > 0 ? rp[0] : make_uint4(0u, 0u, 0u, 0u);
+    uint4 w1 = staged && chunks > 0 ? rp[2] : make_uint4(0u, 0u, 0u, 0u);
     __syncthreads();
-    Fp64Sums a;
-    if (active) {
-        if (staged) {
-            auto run = [&](auto wrap) {
-                constexpr bool W = decltype(wrap)::value;
-                // one group: the lane's entry k = 4 g + lane at LDS byte offset `off`, if it exists
-                auto entry = [&](unsigned off, int k) {
-                    double tx = -0.0, ty = -0.0, tz = -0.0;
-                    if (k < m) {
-                        const double* xj = reinterpret_cast<const double*>(reinterpret_cast<const char*>(xs) + off);
-                        double dx = ri.x - xj[0], dy = ri.y - xj[TILE_HALO_MAX], dz = ri.z - xj[2 * TILE_HALO_MAX];
-                        if constexpr (W) {
-                            dx = min_image(dx, box, inv_box);
-                            dy = min_image(dy, box, inv_box);
-                            dz = min_image(dz, box, inv_box);
-                        }
-                        fp64_term(dx, dy, dz, lj, k < h, a, tx, ty, tz);
-                    }
-                    add_group(a, tx, ty, tz);
-                };
-#pragma unroll 1
-                for (int c = 0; c < chunks; ++c) {
-                    const uint4 cw = next;
-                    if (c + 1 < chunks) next = rp[(c + 1) * stride];  // the next chunk's word in flight
-                    const int k0 = c * CHUNK + lane;
-#pragma unroll 1
-                    for (int q = 0; q < 4; ++q) {  // word q holds the lane's entries 8 q / 2 .. : two per word
-                        const unsigned word = q == 0 ? cw.x : (q == 1 ? cw.y : (q == 2 ? cw.z : cw.w));
-                        entry(word & 0xFFFFu, k0 + (2 * q) * LANES);
-                        entry(word >> 16, k0 + (2 * q + 1) * LANES);
-                    }
-                }
+    // the wavefront walks its chunks to the end of its longest row: groups of four entries past it are skipped
+    const int m_wave = wave_max(m);
+    PairLaneSums a;
+    if (staged) {
+        const char* hb = reinterpret_cast<const char*>(&halo[0][0]);
+        auto run = [&](auto wrap) {
+            constexpr int W = decltype(wrap)::value;  // axes that need the minimum image
+            auto term = [&](unsigned raw, int k, double& tx, double& ty, double& tz) {
+                const bool valid = k < m;
+                const double* xj = reinterpret_cast<const double*>(hb + (valid ? raw : 0u));
+                double dx = ri.x - xj[0], dy = ri.y - xj[TILE_HALO_MAX], dz = ri.z - xj[2 * TILE_HALO_MAX];
+                if constexpr ((W & 1) != 0) dx = min_image(dx, box, inv_box);
+                if constexpr ((W & 2) != 0) dy = min_image(dy, box, inv_box);
+                if constexpr ((W & 4) != 0) dz = min_image(dz, box, inv_box);
+                pair_lane_term<UNIT>(dx, dy, dz, valid, k < h, lj, a, tx, ty, tz);
             };
-            if (flat)
-                run(std::false_type{});
-            else
-                run(std::true_type{});
-        } else {
-            // unstaged tile: entry 4 g + lane through the int32 rows, two groups of the lane in flight
-            const int groups = (m + LANES - 1) / LANES;
-            for (int g0 = 0; g0 < groups; g0 += UNSTAGED_BATCH) {
-                int j[UNSTAGED_BATCH];
-#pragma unroll
-                for (int b = 0; b < UNSTAGED_BATCH; ++b) {
-                    const int k = (g0 + b) * LANES + lane;
-                    j[b] = k < m ? rows[static_cast<size_t>(k) * n + i] : -1;
+#pragma unroll 1
+            for (int c = 0; c < chunks; ++c) {
+                const uint4 c0 = w0, c1 = w1;
+                if (c + 1 < chunks) {  // the next chunk's words in flight
+                    w0 = rp[(c + 1) * stride];
+                    w1 = rp[(c + 1) * stride + 2];
                 }
-                double xj[UNSTAGED_BATCH], yj[UNSTAGED_BATCH], zj[UNSTAGED_BATCH];
+                const int kb = c * CHUNK + par;
 #pragma unroll
-                for (int b = 0; b < UNSTAGED_BATCH; ++b) {
-                    if (j[b] < 0) continue;
-                    const double4 r = pos[j[b]];
-                    xj[b] = r.x;
-                    yj[b] = r.y;
-                    zj[b] = r.z;
-                }
-#pragma unroll
-                for (int b = 0; b < UNSTAGED_BATCH; ++b) {
-                    if (g0 + b >= groups) break;
-                    double tx = -0.0, ty = -0.0, tz = -0.0;
-                    if (j[b] >= 0)
-                        fp64_term(min_image(ri.x - xj[b], box, inv_box), min_image(ri.y - yj[b], box, inv_box),
-                                  min_image(ri.z - zj[b], box, inv_box), lj, (g0 + b) * LANES + lane < h, a, tx, ty,
-                                  tz);
-                    add_group(a, tx, ty, tz);
+                for (int g = 0; g < 8; ++g) {
+                    if (c * CHUNK + 4 * g >= m_wave) break;
+                    const uint32_t u0 = g < 2 ? c0.x : (g < 4 ? c0.y : (g < 6 ? c0.z : c0.w));
+                    const uint32_t u1 = g < 2 ? c1.x : (g < 4 ? c1.y : (g < 6 ? c1.z : c1.w));
+                    const unsigned s0 = (g & 1) ? u0 >> 16 : u0 & 0xFFFFu;
+                    const unsigned s1 = (g & 1) ? u1 >> 16 : u1 & 0xFFFFu;
+                    double ax, ay, az, bx, by, bz;
+                    term(s0, kb + 4 * g, ax, ay, az);      // entry 4 g + par
+                    term(s1, kb + 4 * g + 2, bx, by, bz);  // entry 4 g + par + 2
+                    add_pair(a, ax, ay, az);
+                    add_pair(a, bx, by, bz);
                 }
             }
+        };
+        switch (wrap_axes) {
+            case 0: run(std::integral_constant<int, 0>{}); break;
+            case 1: run(std::integral_constant<int, 1>{}); break;
+            case 2: run(std::integral_constant<int, 2>{}); break;
+            case 3: run(std::integral_constant<int, 3>{}); break;
+            case 4: run(std::integral_constant<int, 4>{}); break;
+            case 5: run(std::integral_constant<int, 5>{}); break;
+            case 6: run(std::integral_constant<int, 6>{}); break;
+            default: run(std::integral_constant<int, 7>{}); break;
         }
-        if (lane == 0) {
-            frc[i] = a.fx;
-            frc[n + i] = a.fy;
-            frc[2 * n + i] = a.fz;
-        }
-    }
-    // the particle's energy sums (its four lanes are adjacent)
+    } else {
+        // unstaged tile: entries 2 g + par through the int32 rows, four of the lane's in flight
+        const int groups = (m + 1) / 2;
+#pragma unroll 1
+        for (int g0 = 0; g0 < groups; g0 += 4) {
+            int j[4];
 #pragma unroll
-    for (int off = 1; off < LANES; off <<= 1) {
-        a.u += __shfl_xor(a.u, off, WAVE);
-        a.w += __shfl_xor(a.w, off, WAVE);
-        a.c += __shfl_xor(a.c, off, WAVE);
+            for (int b = 0; b < 4; ++b) {
+                const int k = 2 * (g0 + b) + par;
+                j[b] = k < m ? rows[static_cast<size_t>(k) * n + i] : -1;
+            }
+            double4 rj[4];
+#pragma unroll
+            for (int b = 0; b < 4; ++b) rj[b] = j[b] >= 0 ? pos[j[b]] : ri;
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                if (g0 + b >= groups) break;
+                const int k = 2 * (g0 + b) + par;
+                double tx, ty, tz;
+                pair_lane_term<UNIT>(min_image(ri.x - rj[b].x, box, inv_box), min_image(ri.y - rj[b].y, box, inv_box),
+                                     min_image(ri.z - rj[b].z, box, inv_box), j[b] >= 0, k < h, lj, a, tx, ty, tz);
+                add_pair(a, tx, ty, tz);
+            }
+        }
     }
-    const int pl = threadIdx.x >> 2;
-    __syncthreads();  // every wavefront is done with the halo
-    if (lane == 0) {
-        e_part[0][pl] = a.u;
-        e_part[1][pl] = a.w;
-        e_part[2][pl] = static_cast<double>(a.c);
+    if (active && par == 0) {
+        frc[i] = a.fx;
+        frc[n + i] = a.fy;
+        frc[2 * n + i] = a.fz;
+    }
+    const double cnt = wave_sum_dpp(static_cast<double>(a.c));
+    const double s12 = wave_sum_dpp(a.s12), s6 = wave_sum_dpp(a.s6);
+    const int wave = threadIdx.x / WAVE;
+    if (threadIdx.x % WAVE == 0) {
+        e_part[wave][0] = lj.c4 * (s12 - s6) - lj.shift * cnt;
+        e_part[wave][1] = lj.c24 * (2.0 * s12 - s6);
+        e_part[wave][2] = cnt;
     }
     if (clear_flag && blockIdx.x == 0 && threadIdx.x == 0) *clear_flag = 0;
     __syncthreads();
-    if (threadIdx.x < WAVE) {
+    if (threadIdx.x < 3) {
+        double v = 0.0;
 #pragma unroll
-        for (int q = 0; q < 3; ++q) {
-            double v = e_part[q][threadIdx.x] + e_part[q][threadIdx.x + WAVE];
-            v = wave_sum(v);
-            if (threadIdx.x == 0) partials[static_cast<size_t>(tile) * 3 + q] = v;
-        }
+        for (int q = 0; q < PAIR_THREADS / WAVE; ++q) v += e_part[q][threadIdx.x];
+        partials[static_cast<size_t>(tile) * 3 + threadIdx.x] = v;
     }
 }
 
@@ -1560,9 +1627,16 @@ void tiled_forces_fp64_enqueue(md_state* s, md_list* l, const LJConst& lj, doubl
     ensure_rows32(l);
     if (ev_start) MD_HIP(hipEventRecord(ev_start, st));
     if (n > 0) {
-        hipLaunchKernelGGL(k_forces_tiled_fp64, dim3(tiles), dim3(TILE_THREADS), 0, st, s->pos, n, l->rows16, l->rows,
-                           l->count, l->half, l->tile_ranges, l->tile_nr, slot_row_len(l->padding), tiles, lj, s->box,
-                           s->inv_box, s->frc, s->partials, clear_flag, s->halt);
+        if (lj.sig2 == 1.0)
+            hipLaunchKernelGGL(k_forces_tiled_fp64_pair<true>, dim3(tiles), dim3(PAIR_THREADS), 0, st, s->pos, n,
+                               l->rows16, l->rows, l->count, l->half, l->tile_ranges, l->tile_nr,
+                               slot_row_len(l->padding), tiles, lj, s->box, s->inv_box, s->frc, s->partials,
+                               clear_flag, s->halt);
+        else
+            hipLaunchKernelGGL(k_forces_tiled_fp64_pair<false>, dim3(tiles), dim3(PAIR_THREADS), 0, st, s->pos, n,
+                               l->rows16, l->rows, l->count, l->half, l->tile_ranges, l->tile_nr,
+                               slot_row_len(l->padding), tiles, lj, s->box, s->inv_box, s->frc, s->partials,
+                               clear_flag, s->halt);
         MD_HIP(hipGetLastError());
     }
     if (ev_stop) MD_HIP(hipEventRecord(ev_stop, st));
