@@ -922,7 +922,10 @@ __device__ __forceinline__ void list_build_tile(
                 }
                 cnt += __popcll(mask);
             };
-            auto walk = [&](int off, int len, bool upper, auto interior_tag) {
+            // `upper_tag`: the U list (keep j > i) or the L list (keep j < i), a compile-time choice, so the order
+            // test is a single comparison whose lane mask the ballots use directly
+            auto walk = [&](int off, int len, auto upper_tag, auto interior_tag) {
+                constexpr bool upper = decltype(upper_tag)::value;
                 int q0 = 0;
                 if constexpr (FH) {
                     // A pair of batches takes the plain path unless a lane is within rounding of the cut-off
@@ -932,17 +935,17 @@ __device__ __forceinline__ void list_build_tile(
                         const float ra = dist_f(sa), rb2 = dist_f(sb);
                         const bool ua = unsure_f(ra), ub = unsure_f(rb2);
                         const bool oa = upper ? sa > slot_i : sa < slot_i, ob = upper ? sb > slot_i : sb < slot_i;
+                        const bool na = inside_f(ra), nb = inside_f(rb2);
+                        // every ballot next to its comparison, where it is that comparison's own lane mask
+                        const unsigned long long ma = __ballot(na) & __ballot(oa);
+                        const unsigned long long mb = __ballot(nb) & __ballot(ob);
                         if ((__ballot(ua) | __ballot(ub)) == 0) {
-                            const bool na = inside_f(ra), nb = inside_f(rb2);
-                            const unsigned long long ma = __ballot(na) & __ballot(oa);
-                            const unsigned long long mb = __ballot(nb) & __ballot(ob);
                             commit(sa, na && oa, ma);
                             commit(sb, nb && ob, mb);
                         } else {
-                            const bool na = resolve_f(sa, ra, ua), nb = resolve_f(sb, rb2, ub);
-                            const unsigned long long ma = __ballot(na && oa), mb = __ballot(nb && ob);
-                            commit(sa, na && oa, ma);
-                            commit(sb, nb && ob, mb);
+                            const bool xa = resolve_f(sa, ra, ua), xb = resolve_f(sb, rb2, ub);
+                            commit(sa, xa && oa, __ballot(xa && oa));
+                            commit(sb, xb && ob, __ballot(xb && ob));
                         }
                     }
                     if (q0 < len) {
@@ -970,11 +973,11 @@ __device__ __forceinline__ void list_build_tile(
                 }
             };
             auto both = [&](auto interior_tag) {
-                walk(__builtin_amdgcn_readfirstlane(lst[cl][0]), __builtin_amdgcn_readfirstlane(lst[cl][1]), true,
-                     interior_tag);
+                walk(__builtin_amdgcn_readfirstlane(lst[cl][0]), __builtin_amdgcn_readfirstlane(lst[cl][1]),
+                     std::true_type{}, interior_tag);
                 const int h = cnt;
-                walk(__builtin_amdgcn_readfirstlane(lst[cl][2]), __builtin_amdgcn_readfirstlane(lst[cl][3]), false,
-                     interior_tag);
+                walk(__builtin_amdgcn_readfirstlane(lst[cl][2]), __builtin_amdgcn_readfirstlane(lst[cl][3]),
+                     std::false_type{}, interior_tag);
                 return h;
             };
             const int hcnt = FH || interior ? both(std::true_type{}) : both(std::false_type{});
@@ -1450,17 +1453,6 @@ __device__ __forceinline__ void pair_lane_term(double dx, double dy, double dz, 
     a.c += up ? 1 : 0;
 }
 
-// Wavefront maximum of non-negative values through DPP moves (the sequence of wave_sum_dpp; unwritten lanes read 0).
-__device__ __forceinline__ int wave_max(int v) {
-    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, false));
-    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xF, 0xF, false));
-    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x141, 0xF, 0xF, false));
-    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x140, 0xF, 0xF, false));
-    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x142, 0xA, 0xF, false));
-    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x143, 0xC, 0xF, false));
-    return __builtin_amdgcn_readlane(v, WAVE - 1);
-}
-
 // The partner lane's value (DPP quad_perm [1, 1, 3, 3]: even lanes read the odd lane above, as two 32-bit halves).
 __device__ __forceinline__ double odd_partner(double v) {
     constexpr int ctrl = 1 | (1 << 2) | (3 << 4) | (3 << 6);
@@ -1519,7 +1511,7 @@ __global__ void __launch_bounds__(PAIR_THREADS, 3)
     uint4 w1 = staged && chunks > 0 ? rp[2] : make_uint4(0u, 0u, 0u, 0u);
     __syncthreads();
     // the wavefront walks its chunks to the end of its longest row: groups of four entries past it are skipped
-    const int m_wave = wave_max(m);
+    const int m_wave = wave_max_dpp(m);
     PairLaneSums a;
     if (staged) {
         const char* hb = reinterpret_cast<const char*>(&halo[0][0]);
