@@ -223,6 +223,26 @@ int md_run(md_state* s, md_list* l, const md_run_config* cfg, double* energies);
    that GPU's array: calls that overlap on one GPU (several contexts or threads) race. */
 int md_run_static(md_state* s, md_list* l, const md_run_config* cfg, double* energies);
 
+/* ---- Trajectory frames ------------------------------------------------------------------------------------ */
+/* Asynchronous download of what writeXyzFrame prints (Trajectory.cpp:4-25): positions and velocities in particle-id
+   order, the inverse of the cell re-sort permutation applied on the GPU. A frame is 6 doubles per particle, x y z
+   vx vy vz, row k holding the particle whose id is k. `slots` frames (1..64) can be in flight at once; each has
+   device staging and pinned host memory for `capacity` particles. */
+typedef struct md_frames md_frames;
+int md_frames_create(md_ctx* ctx, size_t capacity, int slots, md_frames** out);
+int md_frames_destroy(md_frames* f);
+/* Captures the state as it stands after the work already enqueued on the context's stream, and returns at once:
+   the id-order scatter runs on the context's stream, the copy to host on a stream of the frames' own, so the
+   steps enqueued afterwards overlap it. Slots are used in turn; the next one must have been released
+   (MD_ERR_STATE otherwise). *slot receives the slot used. */
+int md_frames_capture(md_frames* f, md_state* s, int* slot);
+/* Waits for the slot's copy; *frame points at its 6n doubles (pinned memory, valid until the slot is released),
+   *n and *box (may be NULL) receive the captured particle count and box length. MD_ERR_STATE if some id was not
+   below n (ids must be a permutation of 0..n-1, as the reference's writer assumes). md_frames_wait and
+   md_frames_release may be called from one thread other than the capturing one. */
+int md_frames_wait(md_frames* f, int slot, const double** frame, size_t* n, double* box);
+int md_frames_release(md_frames* f, int slot);
+
 /* ---- Adjoint (no counterpart in the reference) ----------------------------------------------------------- */
 /* Gradients of L = mean_k U(r_k) + (mean_k P_k - pressure_target)^2 over k = 1..steps, P = (2 KE + W) / (3V)
    (the reference's virial pressure, main.cpp:209-211), with respect to epsilon, sigma and the initial positions and

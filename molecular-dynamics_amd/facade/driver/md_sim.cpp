@@ -23,6 +23,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <numeric>
 #include <string_view>
 #include <vector>
@@ -315,6 +316,8 @@ int run(const Options& o) {
     const int stretchLength = xyz ? (o.xyzEvery % rdfEvery == 0 ? o.xyzEvery : std::gcd(o.xyzEvery, rdfEvery))
                                   : productionSteps;
     const bool rdfInRun = o.rdfPath && stretchLength % rdfEvery == 0;
+    // frames are captured on the GPU and printed by a writer thread while the next stretch runs
+    std::unique_ptr<XyzWriter> frames = xyz ? std::make_unique<XyzWriter>(xyz) : nullptr;
 
     const auto start = std::chrono::steady_clock::now();
     for (int done = 0; done < productionSteps;) {
@@ -337,11 +340,13 @@ int run(const Options& o) {
         }
         done += stretch;
         if (o.rdfPath && !rdfInRun && done % rdfEvery == 0) rdf.sample(system, neighbourList);
-        if (xyz && done % o.xyzEvery == 0) writeXyzFrame(xyz, system, (equilibrationSteps + done) * dt);
+        if (frames && done % o.xyzEvery == 0) frames->write(system, (equilibrationSteps + done) * dt);
     }
+    if (frames) frames->flush();
     const double seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
 
     if (csv) std::fclose(csv);
+    frames.reset();
     if (xyz) std::fclose(xyz);
     if (o.rdfPath && !rdf.write(o.rdfPath)) {
         std::fprintf(stderr, "Could not write %s: %s\n", o.rdfPath, std::strerror(errno));

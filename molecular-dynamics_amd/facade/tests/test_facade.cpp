@@ -636,6 +636,82 @@ TEST(Trajectory, FramesAreConsistentAcrossRebuilds) {
     EXPECT_LT(moved, 0.1);
 }
 
+// ---- mdgpu extension: asynchronous frames (XyzWriter) -----------------------------------------------------
+namespace {
+
+std::string contents(std::FILE* f) {
+    std::fflush(f);
+    std::string out;
+    if (std::fseek(f, 0, SEEK_SET) != 0) return out;
+    char buf[1 << 16];
+    for (std::size_t got; (got = std::fread(buf, 1, sizeof buf, f)) > 0;) out.append(buf, got);
+    return out;
+}
+
+using File = std::unique_ptr<std::FILE, int (*)(std::FILE*)>;
+
+// Trajectory.cpp:4-25 restated line by line: one fprintf per particle in id order
+void referenceFrame(std::FILE* f, const System& s, double time) {
+    std::vector<std::size_t> slot(s.size());
+    for (std::size_t i = 0; i < s.size(); ++i) slot[s.ids()[i]] = i;
+    const double l = s.getBoxLength();
+    std::fprintf(f, "%zu\n", s.size());
+    std::fprintf(f,
+                 "Lattice=\"%.10g 0 0 0 %.10g 0 0 0 %.10g\" Properties=species:S:1:pos:R:3:velo:R:3:id:I:1 "
+                 "Time=%.6f pbc=\"T T T\"\n",
+                 l, l, l, time);
+    for (std::size_t id = 0; id < s.size(); ++id) {
+        const auto r = s.positionOf(slot[id]);
+        const auto v = s.velocityOf(slot[id]);
+        std::fprintf(f, "Ar %.8f %.8f %.8f %.8f %.8f %.8f %zu\n", r[0], r[1], r[2], v[0], v[1], v[2], id);
+    }
+}
+
+}  // namespace
+
+TEST(TrajectoryAsync, FramesEqualTheSynchronousWriterAcrossStepsAndRebuilds) {
+    System s = liquid(6, 50);
+    LennardJonesForceField ff(1.0, 1.0, kCutOff);
+    VelocityVerletIntegrator vv(kDt);
+    NeighbourList list(kCutOff, kSkin);
+    list.build(s);
+    ff.computeForces(s, list);
+    File sync(std::tmpfile(), &std::fclose), async(std::tmpfile(), &std::fclose), ref(std::tmpfile(), &std::fclose);
+    ASSERT_TRUE(sync && async && ref);
+    {
+        XyzWriter writer(async.get(), 3);
+        for (int frame = 0; frame < 8; ++frame) {  // more frames than slots: write() waits for the oldest
+            writer.write(s, frame * 10 * kDt);
+            writeXyzFrame(sync.get(), s, frame * 10 * kDt);
+            referenceFrame(ref.get(), s, frame * 10 * kDt);
+            // the steps after a capture change the state, not the captured frame
+            for (int k = 0; k < 10; ++k) vv.step(s, ff, list);
+            if (frame == 4) list.build(s);  // a re-sort between frames
+        }
+        writer.flush();
+        const std::string a = contents(async.get());
+        EXPECT_TRUE(a == contents(sync.get()));
+        EXPECT_TRUE(a == contents(ref.get()));
+        EXPECT_GT(a.size(), std::size_t(8 * 864 * 60));  // eight frames of 864 lines of at least 60 characters
+    }
+}
+
+TEST(TrajectoryAsync, LargeFramesFormatInParallelLikeTheReference) {
+    System s = lattice(32);  // 131072 particles: the formatting splits into chunks
+    NeighbourList list(kCutOff, kSkin);
+    list.build(s);  // re-sorted: ids no longer follow storage order
+    File sync(std::tmpfile(), &std::fclose), async(std::tmpfile(), &std::fclose), ref(std::tmpfile(), &std::fclose);
+    ASSERT_TRUE(sync && async && ref);
+    XyzWriter writer(async.get());
+    writer.write(s, 1.25);
+    writeXyzFrame(sync.get(), s, 1.25);
+    referenceFrame(ref.get(), s, 1.25);
+    writer.flush();
+    const std::string r = contents(ref.get());
+    EXPECT_TRUE(contents(sync.get()) == r);
+    EXPECT_TRUE(contents(async.get()) == r);
+}
+
 int main(int argc, char** argv) {
     const std::string filter = argc > 1 ? argv[1] : "";
     int failed = 0, ran = 0;

@@ -127,6 +127,11 @@ _SIGS = {
     "md_batch_get_info": (C.c_int, [_P, C.POINTER(BatchInfo)]),
     "md_batch_replica": (C.c_int, [_P, C.c_int, _P, _IP, C.POINTER(C.c_size_t)]),
     "md_batch_tune": (C.c_int, [_P, C.POINTER(BatchConfig), C.c_int, _DP, _IP]),
+    "md_frames_create": (C.c_int, [_P, C.c_size_t, C.c_int, _PP]),
+    "md_frames_destroy": (C.c_int, [_P]),
+    "md_frames_capture": (C.c_int, [_P, _P, _IP]),
+    "md_frames_wait": (C.c_int, [_P, C.c_int, C.POINTER(_DP), C.POINTER(C.c_size_t), _DP]),
+    "md_frames_release": (C.c_int, [_P, C.c_int]),
 }
 
 
@@ -572,6 +577,44 @@ class Batch:
     def close(self):
         if self.h:
             lib().md_batch_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Frames:
+    """md_frames_*: trajectory frames captured on the GPU in id order and downloaded asynchronously."""
+
+    def __init__(self, ctx: Context, capacity: int, slots: int = 2):
+        self.ctx = ctx
+        self.h = C.c_void_p()
+        check(lib().md_frames_create(ctx.h, capacity, slots, C.byref(self.h)))
+
+    def capture(self, state: "State") -> int:
+        slot = C.c_int()
+        check(lib().md_frames_capture(self.h, state.h, C.byref(slot)))
+        return slot.value
+
+    def wait(self, slot: int):
+        """(n, 6) array (x y z vx vy vz per id, copied out of the pinned slot) and the box length."""
+        frame = _DP()
+        n = C.c_size_t()
+        box = C.c_double()
+        check(lib().md_frames_wait(self.h, slot, C.byref(frame), C.byref(n), C.byref(box)))
+        out = np.ctypeslib.as_array(frame, shape=(n.value * 6,)).reshape(n.value, 6).copy() if n.value else \
+            np.empty((0, 6))
+        return out, box.value
+
+    def release(self, slot: int):
+        check(lib().md_frames_release(self.h, slot))
+
+    def close(self):
+        if self.h:
+            lib().md_frames_destroy(self.h)
             self.h = C.c_void_p()
 
     def __del__(self):

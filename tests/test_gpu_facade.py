@@ -24,3 +24,13 @@ def test_facade_library_loads_without_a_device():
     lib = os.path.join(ROOT, "molecular-dynamics_amd", "facade", "libmdfacade.so")
     assert os.path.exists(lib)
     ctypes.CDLL(lib)
+
+
+def test_xyz_formatter_matches_printf():
+    """The XYZ line formatter of the trajectory writers against printf's "%.8f" / "%zu" on 18 M values (ties,
+    carries, signed zeros, subnormals, random bit patterns) and whole lines (facade/tests/test_format.cpp)."""
+    exe = os.path.join(ROOT, "molecular-dynamics_amd", "facade", "md_format_tests")
+    assert os.path.exists(exe), "md_format_tests not built: run __graft_entry__.build()"
+    p = subprocess.run([exe], capture_output=True, text=True, timeout=300)
+    assert p.returncode == 0, p.stdout[-3000:]
+    assert " 0 mismatches" in p.stdout
