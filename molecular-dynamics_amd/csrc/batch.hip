@@ -10,9 +10,14 @@
 //
 // A step is one fixed sequence of launches (and so one hipGraph): k_b_kick_drift raises a replica's stale flag
 // (NeighbourList::update's half-skin test) → the rebuild kernels, each exiting at once for a replica that is not
-// stale → k_b_forces → kick and kinetic energy (+ Bussi) → k_b_record, which also clears the stale flags. Each
-// kernel repeats the arithmetic of the single-system kernel for the same phase (cells.hip, forces.hip,
-// integrate.hip, api.hip), so an FP64 replica follows exactly the trajectory md_run gives its System.
+// stale → forces → kick and kinetic energy (+ Bussi) → k_b_record, which also clears the stale flags. Each kernel
+// repeats the arithmetic of the single-system kernel for the same phase (cells.hip, forces.hip, integrate.hip,
+// api.hip), so an FP64 replica follows exactly the trajectory md_run gives its System.
+//
+// Lists and forces use the tiled kernels of tiled.hip (one launch over every replica's tiles of 128 slots, each
+// tile reading its replica's box and cell grid from a table): the LDS-staged list build and the LDS-staged mixed
+// and FP64 force kernels, with the single system's rows and sums. MDGPU_UNTILED=1 keeps the per-particle kernels
+// k_b_list_build and k_b_forces below (the same rows; their forces take 1, 2 or 4 lanes per particle).
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -441,16 +446,16 @@ __global__ void __launch_bounds__(BT)
     block_build_stat<BT>(cnt, static_cast<unsigned>(hcnt), bstats + blockIdx.x);
 }
 
-// Per-replica totals of a build from its blocks' records (a replica's slots are whole blocks): half-pair count,
-// and the longest row and overflow over the stale replicas into flags[1] and flags[0].
-__global__ void k_b_build_totals(const Rep* __restrict__ reps, int count, const int2* __restrict__ bstats,
+// Per-replica totals of a build from its records (one per `unit` slots: a block of the per-particle build, or a
+// tile): half-pair count, and the longest row and overflow over the stale replicas into flags[1] and flags[0].
+__global__ void k_b_build_totals(const Rep* __restrict__ reps, int count, int unit, const int2* __restrict__ bstats,
                                  int padding, int* __restrict__ flags, unsigned long long* __restrict__ pairs,
                                  const int* __restrict__ stale, const int* halt) {
     if (halted(halt)) return;
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= count || !stale[r]) return;
     const Rep rp = reps[r];
-    const uint32_t b0 = rp.first / BT, b1 = (rp.first + rp.n + BT - 1) / BT;
+    const uint32_t b0 = rp.first / unit, b1 = (rp.first + rp.n + unit - 1) / unit;
     int mx = 0;
     unsigned long long hs = 0;
     for (uint32_t b = b0; b < b1; ++b) {
@@ -507,19 +512,23 @@ __global__ void __launch_bounds__(BT)
     block_sum<3, BT>(acc, partials + static_cast<size_t>(blockIdx.x) * 3);
 }
 
-// Per replica, in block order: potential, virial, inside pairs -> scalars[0..2].
-__global__ void k_b_reduce_forces(const Rep* __restrict__ reps, int count, int lanes,
-                                  const double* __restrict__ partials, double* __restrict__ scalars, const int* halt) {
+// Per replica, one wavefront: potential, virial, inside pairs -> scalars[0..2], from the force kernel's partial
+// rows, each covering `unit` particle slots (a tile, or a block of the per-particle kernel). Lane l adds rows
+// l, l + 64, ... of the replica, then a butterfly combines the lanes: a fixed order.
+constexpr int REDUCE_WAVES = 4;
+__global__ void __launch_bounds__(REDUCE_WAVES * WAVE)
+    k_b_reduce_forces(const Rep* __restrict__ reps, int count, int unit, const double* __restrict__ partials,
+                      double* __restrict__ scalars, const int* halt) {
     if (halted(halt)) return;
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = blockIdx.x * REDUCE_WAVES + threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
     if (r >= count) return;
     const Rep rp = reps[r];
-    const uint32_t pb = BT / lanes;
-    const uint32_t b0 = rp.first / pb, b1 = (rp.first + rp.n + pb - 1) / pb;
+    const uint32_t b0 = rp.first / unit, b1 = (rp.first + rp.n + unit - 1) / unit;
     double s[3] = {0.0, 0.0, 0.0};
-    for (uint32_t b = b0; b < b1; ++b)
+    for (uint32_t b = b0 + lane; b < b1; b += WAVE)
         for (int k = 0; k < 3; ++k) s[k] += partials[static_cast<size_t>(b) * 3 + k];
-    for (int k = 0; k < 3; ++k) scalars[r * SCALARS + k] = s[k];
+    for (int k = 0; k < 3; ++k) s[k] = wave_sum(s[k]);
+    if (lane < 3) scalars[r * SCALARS + lane] = lane == 0 ? s[0] : (lane == 1 ? s[1] : s[2]);
 }
 
 // After a host-driven build: count it and clear the stale flags.
@@ -534,6 +543,21 @@ __global__ void k_b_fill(int* __restrict__ p, int count, int value) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k < count) p[k] = value;
 }
+
+// Slot rows moved to a longer row length: tile t's chunks (CHUNK_STRIDE entries each) from t * TILE * old_len to
+// t * TILE * new_len; entries of the added chunks are left as they are (no row reaches them yet).
+constexpr int ROWS16_CHUNK = TILE_PARTICLES * 32;
+__global__ void k_b_regrow_rows16(const uint16_t* __restrict__ from, uint16_t* __restrict__ to, int tiles,
+                                  int old_len, int new_len) {
+    const size_t per_tile = static_cast<size_t>(TILE_PARTICLES) * old_len;
+    const size_t total = per_tile * tiles;
+    for (size_t e = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < total;
+         e += static_cast<size_t>(gridDim.x) * blockDim.x) {
+        const size_t t = e / per_tile, k = e % per_tile;
+        to[t * TILE_PARTICLES * new_len + k] = from[e];
+    }
+}
+static_assert(ROWS16_CHUNK == 4096, "a tile's slot rows are chunks of 128 particles x 32 entries");
 
 int grid_side_of(double box, double min_cell) {
     const int s = static_cast<int>(box / min_cell);
@@ -578,6 +602,18 @@ struct md_batch {
     int2* bstats = nullptr;     // per build block: longest row, half-row sum
     int* flags = nullptr;       // [0] overflow, [1] longest row, [2] step counter, [3] spare
     double* partials = nullptr; // force partials (up to blocks * 4 lanes * 3), then the kick's (blocks)
+    // tiled lists (the default): a table of every replica's system, the replica of each tile, per tile its halo
+    // ranges and build record, and the slot rows
+    bool tiled = false;
+    int tiles = 0;
+    double tsys_cutoff = 0.0;  // LJ cut-off the table's `reach` was computed for
+    std::vector<mdgpu::TileSystem> tsys;
+    mdgpu::TileSystem* d_tsys = nullptr;
+    uint16_t* tile_rep = nullptr;
+    int4* tile_ranges = nullptr;
+    int* tile_nr = nullptr;
+    uint16_t* rows16 = nullptr;
+    int2* tstats = nullptr;
     double* ke_part = nullptr;
     double* scalars = nullptr;
     // During md_batch_run: the overflow word. The run's kernels exit at once while it is set, so a replica's rebuild
@@ -621,6 +657,61 @@ void batch_free(md_batch* b) {
     dev_free(b->partials);
     dev_free(b->ke_part);
     dev_free(b->scalars);
+    dev_free(b->d_tsys);
+    dev_free(b->tile_rep);
+    dev_free(b->tile_ranges);
+    dev_free(b->tile_nr);
+    dev_free(b->rows16);
+    dev_free(b->tstats);
+}
+
+size_t rows16_entries(const md_batch* b) {
+    return static_cast<size_t>(b->tiles) * TILE_PARTICLES * tiled_row_len(b->padding);
+}
+
+// The replicas' table for the tiled kernels; `cutoff` is the LJ cut-off of the forces that follow.
+void upload_tile_systems(md_batch* b, double cutoff) {
+    const double radius = std::sqrt(b->radius2);
+    b->tsys.resize(b->count);
+    for (int r = 0; r < b->count; ++r) {
+        const Rep& rp = b->reps[r];
+        TileSystem& t = b->tsys[r];
+        t.first = rp.first;
+        t.end = rp.first + rp.n;
+        t.cell_base = rp.cell_first;
+        t.side = rp.side;
+        t.use_cells = rp.use_cells;
+        t.hmax = tile_halo_capacity(rp.n, rp.ncells);
+        t.interior_ok = rp.side >= 7 ? 1 : 0;
+        t.gate = r;
+        t.box = rp.box;
+        t.inv_box = rp.inv_box;
+        t.flat_cells = (0.5 * rp.box - b->skin) / (rp.box / rp.side);
+        t.reach = cutoff + radius + b->skin;
+    }
+    b->tsys_cutoff = cutoff;
+    MD_HIP(hipMemcpyAsync(b->d_tsys, b->tsys.data(), sizeof(TileSystem) * b->count, hipMemcpyHostToDevice,
+                          b->ctx->stream));
+    MD_HIP(hipStreamSynchronize(b->ctx->stream));  // the host table may change before the next upload
+}
+
+BatchTiles batch_tiles(md_batch* b) {
+    BatchTiles bt{};
+    bt.ts.one = b->tsys[0];
+    bt.ts.per = b->d_tsys;
+    bt.ts.rep_of_tile = b->tile_rep;
+    bt.ts.gates = b->stale;
+    bt.ts.stride = b->slots;
+    bt.ts.tiles = b->tiles;
+    bt.tile_ranges = b->tile_ranges;
+    bt.tile_nr = b->tile_nr;
+    bt.rows16 = b->rows16;
+    bt.stats = b->tstats;
+    bt.rows = b->rows;
+    bt.count = b->rcount;
+    bt.half = b->half;
+    bt.padding = b->padding;
+    return bt;
 }
 
 // Allocates every device buffer for the layout in b (sizes, reps) and uploads the tables.
@@ -659,6 +750,19 @@ void batch_alloc(md_batch* b) {
     b->partials = dev_alloc<double>(static_cast<size_t>(b->blocks) * (4 * 3 + 1));
     b->ke_part = dev_alloc<double>(b->ke_total);
     b->scalars = dev_alloc<double>(static_cast<size_t>(b->count) * SCALARS);
+    b->tiled = tiling_enabled(S);
+    b->tiles = static_cast<int>(S / TILE_PARTICLES);
+    if (b->tiled) {
+        b->d_tsys = dev_alloc<TileSystem>(b->count);
+        b->tile_rep = dev_alloc<uint16_t>(b->tiles);
+        b->tile_ranges = dev_alloc<int4>(static_cast<size_t>(b->tiles) * TILE_RANGES);
+        b->tile_nr = dev_alloc<int>(b->tiles);
+        b->rows16 = dev_alloc<uint16_t>(rows16_entries(b));
+        b->tstats = dev_alloc<int2>(b->tiles);
+        // empty tiles (past a replica's end) stay unstaged with no particles
+        k_b_fill<<<blocks_for(b->tiles, 256), 256, 0, st>>>(b->tile_nr, b->tiles, -1);
+        MD_HIP(hipGetLastError());
+    }
     MD_HIP(hipMemsetAsync(b->counts, 0, sizeof(uint32_t) * b->cells_total, st));
     MD_HIP(hipMemsetAsync(b->fill, 0, sizeof(uint32_t) * b->cells_total, st));
     MD_HIP(hipMemsetAsync(b->rebuilds, 0, sizeof(int) * b->count, st));
@@ -671,12 +775,14 @@ void batch_alloc(md_batch* b) {
     MD_HIP(hipMemsetAsync(b->mass, 0, sizeof(double) * S, st));
     MD_HIP(hipMemsetAsync(b->id, 0, sizeof(uint64_t) * S, st));
 
-    std::vector<uint16_t> brep(b->blocks), crep(b->cells_total);
+    std::vector<uint16_t> brep(b->blocks), crep(b->cells_total), trep(b->tiles);
     std::vector<uint32_t> kstart(b->ke_total);
     for (int r = 0; r < b->count; ++r) {
         const Rep& rp = b->reps[r];
         const uint32_t nb = (rp.n + BT - 1) / BT;
         for (uint32_t k = 0; k < (nb ? nb : 1); ++k) brep[rp.first / BT + k] = static_cast<uint16_t>(r);
+        for (uint32_t k = 0; k < (nb ? nb : 1) * (BT / TILE_PARTICLES); ++k)
+            trep[rp.first / TILE_PARTICLES + k] = static_cast<uint16_t>(r);
         for (uint32_t c = 0; c <= rp.ncells; ++c) crep[rp.cell_first + c] = static_cast<uint16_t>(r);
         for (uint32_t k = 0; k < rp.ke_blocks; ++k) kstart[rp.ke_first + k] = rp.first + k * KE_TERMS;
     }
@@ -686,7 +792,45 @@ void batch_alloc(md_batch* b) {
     if (b->ke_total)
         MD_HIP(hipMemcpyAsync(b->ke_start, kstart.data(), sizeof(uint32_t) * kstart.size(), hipMemcpyHostToDevice,
                               st));
+    if (b->tiled)
+        MD_HIP(hipMemcpyAsync(b->tile_rep, trep.data(), sizeof(uint16_t) * trep.size(), hipMemcpyHostToDevice, st));
     MD_HIP(hipStreamSynchronize(st));  // the host vectors above go out of scope
+    if (b->tiled) upload_tile_systems(b, b->cutoff);
+}
+
+// Longer rows for every replica, keeping the current ones (column-major int32 rows: the same entries at the same
+// places; slot rows: each tile's chunks moved to the new tile stride).
+void grow_rows(md_batch* b, int grown, bool keep) {
+    hipStream_t st = b->ctx->stream;
+    int32_t* rows = dev_alloc<int32_t>(static_cast<size_t>(grown) * b->slots);
+    uint16_t* rows16 = nullptr;
+    hipError_t ce = hipSuccess;
+    if (b->tiled) {
+        const int old_len = tiled_row_len(b->padding), new_len = tiled_row_len(grown);
+        const size_t entries = static_cast<size_t>(b->tiles) * TILE_PARTICLES * new_len;
+        ce = hipMalloc(reinterpret_cast<void**>(&rows16), entries * sizeof(uint16_t));
+        if (ce == hipSuccess && keep && new_len != old_len) {
+            k_b_regrow_rows16<<<2048, 256, 0, st>>>(b->rows16, rows16, b->tiles, old_len, new_len);
+            ce = hipGetLastError();
+        } else if (ce == hipSuccess && keep) {
+            ce = hipMemcpyAsync(rows16, b->rows16, entries * sizeof(uint16_t), hipMemcpyDeviceToDevice, st);
+        }
+    }
+    if (ce == hipSuccess && keep)
+        ce = hipMemcpyAsync(rows, b->rows, sizeof(int32_t) * b->padding * b->slots, hipMemcpyDeviceToDevice, st);
+    if (ce == hipSuccess) ce = hipStreamSynchronize(st);
+    if (ce != hipSuccess) {
+        dev_free(rows);
+        if (rows16) (void)hipFree(rows16);
+        hip_check(ce, "growing the neighbour rows");
+    }
+    dev_free(b->rows);
+    b->rows = rows;
+    if (b->tiled) {
+        dev_free(b->rows16);
+        b->rows16 = rows16;
+    }
+    b->padding = grown;
 }
 
 void enqueue_rebuild(md_batch* b) {
@@ -701,18 +845,48 @@ void enqueue_rebuild(md_batch* b) {
     k_b_sort<<<blocks_for(b->cells_total * WAVE, BT), BT, 0, st>>>(b->d_reps, b->cell_rep, b->cells_total, S,
                                                                    b->start, b->order_raw, b->fill, b->counts, pa,
                                                                    b->stale, b->halt);
-    BatchCopyBack cb{b->vel_alt, b->frc_alt, b->mass_alt, b->id_alt, b->pos, b->vel, b->frc, b->mass, b->id};
-    k_b_list_build<<<b->blocks, BT, 0, st>>>(b->d_reps, b->block_rep, b->pos_alt, S, b->cell_of_alt, b->start,
-                                             b->radius2, b->padding, b->rows, b->rcount, b->half, b->bstats, cb,
-                                             b->stale, b->halt);
-    k_b_build_totals<<<blocks_for(b->count, 256), 256, 0, st>>>(b->d_reps, b->count, b->bstats, b->padding,
-                                                                 b->flags, b->pairs, b->stale, b->halt);
+    if (b->tiled) {
+        // the re-sorted arrays go back into place from the build, which reads the positions from pos_alt
+        TiledCopyBack cb{};
+        cb.enabled = 1;
+        cb.vel2 = b->vel_alt;
+        cb.frc2 = b->frc_alt;
+        cb.mass2 = b->mass_alt;
+        cb.id2 = b->id_alt;
+        cb.cell_of2 = b->cell_of_alt;
+        cb.pos = b->pos;
+        cb.vel = b->vel;
+        cb.frc = b->frc;
+        cb.mass = b->mass;
+        cb.id = b->id;
+        cb.cell_of = b->cell_of;
+        tiled_batch_build_enqueue(st, batch_tiles(b), b->pos_alt, b->cell_of_alt, b->start, b->radius2, cb, b->halt);
+        k_b_build_totals<<<blocks_for(b->count, 256), 256, 0, st>>>(b->d_reps, b->count, TILE_PARTICLES, b->tstats,
+                                                                     b->padding, b->flags, b->pairs, b->stale,
+                                                                     b->halt);
+    } else {
+        BatchCopyBack cb{b->vel_alt, b->frc_alt, b->mass_alt, b->id_alt, b->pos, b->vel, b->frc, b->mass, b->id};
+        k_b_list_build<<<b->blocks, BT, 0, st>>>(b->d_reps, b->block_rep, b->pos_alt, S, b->cell_of_alt, b->start,
+                                                 b->radius2, b->padding, b->rows, b->rcount, b->half, b->bstats, cb,
+                                                 b->stale, b->halt);
+        k_b_build_totals<<<blocks_for(b->count, 256), 256, 0, st>>>(b->d_reps, b->count, BT, b->bstats, b->padding,
+                                                                     b->flags, b->pairs, b->stale, b->halt);
+    }
     MD_HIP(hipGetLastError());
 }
 
 void enqueue_forces(md_batch* b, const LJConst& lj, int precision, int lanes, hipEvent_t e0, hipEvent_t e1) {
     hipStream_t st = b->ctx->stream;
     const size_t S = b->slots;
+    if (b->tiled) {
+        if (e0) MD_HIP(hipEventRecord(e0, st));
+        tiled_batch_forces_enqueue(st, batch_tiles(b), b->pos, lj, precision, b->frc, b->partials, b->halt);
+        if (e1) MD_HIP(hipEventRecord(e1, st));
+        k_b_reduce_forces<<<blocks_for(b->count, REDUCE_WAVES), REDUCE_WAVES * WAVE, 0, st>>>(
+            b->d_reps, b->count, TILE_PARTICLES, b->partials, b->scalars, b->halt);
+        MD_HIP(hipGetLastError());
+        return;
+    }
     if (precision != MD_MIXED) lanes = 1;
     const unsigned grid = static_cast<unsigned>(b->blocks) * lanes;
     if (e0) MD_HIP(hipEventRecord(e0, st));
@@ -732,8 +906,8 @@ void enqueue_forces(md_batch* b, const LJConst& lj, int precision, int lanes, hi
 #undef MD_B_FORCES
     MD_HIP(hipGetLastError());
     if (e1) MD_HIP(hipEventRecord(e1, st));
-    k_b_reduce_forces<<<blocks_for(b->count, 256), 256, 0, st>>>(b->d_reps, b->count, lanes, b->partials,
-                                                                 b->scalars, b->halt);
+    k_b_reduce_forces<<<blocks_for(b->count, REDUCE_WAVES), REDUCE_WAVES * WAVE, 0, st>>>(
+        b->d_reps, b->count, BT / lanes, b->partials, b->scalars, b->halt);
     MD_HIP(hipGetLastError());
 }
 
@@ -755,9 +929,7 @@ void batch_build(md_batch* b) {
         if (!hf[0]) break;
         if (attempt > 8) fail(MD_ERR_OVERFLOW, "neighbour-list rows keep overflowing");
         // the replicas are sorted now (a re-sort is the identity); rebuild all with longer rows
-        b->padding = (hf[1] + hf[1] / 4 + 16 + 7) / 8 * 8;
-        dev_free(b->rows);
-        b->rows = dev_alloc<int32_t>(static_cast<size_t>(b->padding) * b->slots);
+        grow_rows(b, (hf[1] + hf[1] / 4 + 16 + 7) / 8 * 8, false);
         k_b_fill<<<blocks_for(b->count, 256), 256, 0, st>>>(b->stale, b->count, 1);
     }
     k_b_finish_build<<<blocks_for(b->count, 256), 256, 0, st>>>(b->count, b->stale, b->rebuilds);
@@ -935,6 +1107,7 @@ int md_batch_run(md_batch* b, const md_batch_config* cfg, double* energies) {
     check_boxes(b, cfg->lj.cutoff);
     hipStream_t st = b->ctx->stream;
     const LJConst lj = make_lj(cfg->lj);
+    if (b->tiled && cfg->lj.cutoff != b->tsys_cutoff) upload_tile_systems(b, cfg->lj.cutoff);
     const size_t S = b->slots;
     const int count = b->count;
     if (!b->forces_valid) {
@@ -1077,18 +1250,7 @@ int md_batch_run(md_batch* b, const md_batch_config* cfg, double* energies) {
         // nothing (they are sorted already).
         done = hf[2];
         const int longest = hf[1];
-        const int grown = (longest + longest / 4 + 16 + 7) / 8 * 8;
-        int32_t* rows = dev_alloc<int32_t>(static_cast<size_t>(grown) * b->slots);
-        const hipError_t ce = hipMemcpyAsync(rows, b->rows, sizeof(int32_t) * b->padding * b->slots,
-                                             hipMemcpyDeviceToDevice, st);
-        if (ce != hipSuccess) {
-            dev_free(rows);
-            hip_check(ce, "hipMemcpyAsync");
-        }
-        MD_HIP(hipStreamSynchronize(st));
-        dev_free(b->rows);
-        b->rows = rows;
-        b->padding = grown;
+        grow_rows(b, (longest + longest / 4 + 16 + 7) / 8 * 8, true);
         MD_HIP(hipMemsetAsync(b->flags, 0, sizeof(int), st));
     }
     if (energies)
@@ -1150,6 +1312,11 @@ int md_batch_tune(md_batch* b, const md_batch_config* cfg, int trial_steps, doub
     copy(tmp.rcount, b->rcount, sizeof(int32_t) * S);
     copy(tmp.half, b->half, sizeof(int32_t) * S);
     copy(tmp.pairs, b->pairs, sizeof(unsigned long long) * b->count);
+    if (b->tiled) {
+        copy(tmp.tile_ranges, b->tile_ranges, sizeof(int4) * b->tiles * TILE_RANGES);
+        copy(tmp.tile_nr, b->tile_nr, sizeof(int) * b->tiles);
+        copy(tmp.rows16, b->rows16, sizeof(uint16_t) * rows16_entries(b));
+    }
     MD_HIP(hipStreamSynchronize(st));
     const int options[3] = {1, 2, 4};
     double best = 0.0;

@@ -439,18 +439,20 @@ void list_prepare(md_list* l, md_state* s) {
         l->tile_nr = dev_alloc<int>(tiles);
         l->tiles_cap = tiles;
     }
-    // LDS room for a tile's halo: 1.5x the expected union of the 27-cell stencils of the cells a tile spans
-    // (x | y | z doubles, at most TILE_HALO_MAX particles), or the whole system when that is smaller
-    {
-        const double per_cell = static_cast<double>(s->n) / static_cast<double>(ncells);
-        const double est = 1.5 * 9.0 * (TILE_PARTICLES / (per_cell > 0.5 ? per_cell : 0.5) + 3.0) * per_cell;
-        long h = static_cast<long>(std::ceil(est / 8.0)) * 8;
-        h = std::max<long>(256, std::min<long>(TILE_HALO_MAX - 8, h));  // the last slot is the build's ghost
-        const long all = static_cast<long>((s->n + 7) / 8 * 8);
-        l->hmax = static_cast<int>(std::min<long>(h, all > 0 ? all : 8));
-        if (static_cast<long>(s->n) <= TILE_HALO_MAX - 8) l->hmax = static_cast<int>(all > 0 ? all : 8);
-    }
+    l->hmax = tile_halo_capacity(s->n, ncells);
     l->n = s->n;
+}
+
+// LDS room for a tile's halo: 1.5x the expected union of the 27-cell stencils of the cells a tile spans
+// (x | y | z doubles, at most TILE_HALO_MAX particles), or the whole system when that is smaller
+int tile_halo_capacity(size_t n, size_t ncells) {
+    const double per_cell = static_cast<double>(n) / static_cast<double>(ncells);
+    const double est = 1.5 * 9.0 * (TILE_PARTICLES / (per_cell > 0.5 ? per_cell : 0.5) + 3.0) * per_cell;
+    long h = static_cast<long>(std::ceil(est / 8.0)) * 8;
+    h = std::max<long>(256, std::min<long>(TILE_HALO_MAX - 8, h));  // the last slot is the build's ghost
+    const long all = static_cast<long>((n + 7) / 8 * 8);
+    if (static_cast<long>(n) <= TILE_HALO_MAX - 8) return static_cast<int>(all > 0 ? all : 8);
+    return static_cast<int>(std::min<long>(h, all > 0 ? all : 8));
 }
 
 PermuteArgs permute_args(md_list* l, md_state* s, bool carry_forces) {

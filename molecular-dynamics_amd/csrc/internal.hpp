@@ -256,6 +256,70 @@ constexpr int TILE_RANGES = 64;
 // 53248 B (13 x 4 KB) and so at three blocks per CU (160 KB of LDS); the few tiles whose halo is larger take the
 // unstaged path
 constexpr int TILE_HALO_MAX = 2184;
+// The system a tile belongs to: the one System of the launch, or one replica of a batch (batch.hip), whose
+// particles are the batch slots [first, end) and whose cell offsets are start[cell_base .. cell_base + side^3].
+// Tiles never straddle systems (a replica starts on a tile boundary); tiles past a replica's end are empty.
+struct TileSystem {
+    uint32_t first, end;
+    uint32_t cell_base;
+    int side, use_cells;
+    int hmax;         // LDS capacity of a tile's halo, in particles
+    int interior_ok;  // side >= 7: an interior cell's 27 neighbours are nearer than L/2 along every axis
+    int gate;         // batch: index of the replica's rebuild flag in TileSystems::gates
+    double box, inv_box;
+    double flat_cells;  // cells a tile and its halo may span with every coordinate inside half a box
+    double reach;       // rc + list radius + skin, the mixed force kernel's compactness test
+};
+
+// What a launch sees: every tile in `one`, or tile t in per[rep_of_tile[t]]; `stride` is the column stride of the
+// int32 rows and of the component-major velocity and force arrays (n, or the batch's slot count).
+struct TileSystems {
+    TileSystem one;
+    const TileSystem* per;
+    const uint16_t* rep_of_tile;
+    const int* gates;  // batch: per replica, nonzero when its list is rebuilt (the rebuild kernels skip the others)
+    size_t stride;
+    int tiles;
+};
+
+// The re-sorted arrays a list build moves back into place (enabled: the build reads the re-sorted positions).
+struct TiledCopyBack {
+    int enabled;
+    const double* vel2;
+    const double* frc2;
+    const double* mass2;
+    const uint64_t* id2;
+    const int* cell_of2;
+    double4* pos;
+    double* vel;
+    double* frc;
+    double* mass;
+    uint64_t* id;
+    int* cell_of;
+};
+
+// A batch's tiled list (buffers owned by batch.hip): slot rows, int32 rows of the unstaged tiles, per tile its halo
+// ranges and build record.
+struct BatchTiles {
+    TileSystems ts;
+    int4* tile_ranges;
+    int* tile_nr;
+    uint16_t* rows16;
+    int2* stats;
+    int32_t* rows;
+    int32_t* count;
+    int32_t* half;
+    int padding;
+};
+int tile_halo_capacity(size_t n, size_t ncells);  // (cells.hip) LDS slots for a tile's halo of a system
+int tiled_row_len(int padding);                   // slot-row entries per particle
+// rebuild of the gated replicas: halo ranges from the re-sorted cells, then rows from build_pos (copied back by cb)
+void tiled_batch_build_enqueue(hipStream_t st, const BatchTiles& bt, const double4* build_pos, const int* cell_of,
+                               const uint32_t* start, double radius2, const TiledCopyBack& cb, const int* halt);
+// forces of every replica: per tile U, W, inside count into partials[3 tile ..]
+struct LJConst;
+void tiled_batch_forces_enqueue(hipStream_t st, const BatchTiles& bt, const double4* pos, const LJConst& lj,
+                                int precision, double* frc, double* partials, const int* halt);
 size_t tiled_rows16_entries(size_t n, int padding);
 void tiled_list_enqueue(md_list* l, md_state* s, bool from_alt, const int* gate, bool write32, bool carry_forces);
 void tiled_rows32_enqueue(md_list* l);                          // int32 rows from the slot rows

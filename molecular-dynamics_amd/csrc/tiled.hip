@@ -69,6 +69,44 @@ __device__ __forceinline__ int tile_range_count(int raw) { return raw < 0 ? -1 :
 __device__ __forceinline__ bool tile_is_flat(int raw) { return raw > 0 && (raw & TILE_FLAT) != 0; }
 // bit a set: the minimum image is needed along axis a
 __device__ __forceinline__ int tile_wrap_axes(int raw) { return raw > 0 ? (~raw >> 17) & 7 : 7; }
+__device__ __forceinline__ uint32_t uniform_u32(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ int uniform_i32(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ double uniform_f64(double v) {
+    return __hiloint2double(__builtin_amdgcn_readfirstlane(__double2hiint(v)),
+                            __builtin_amdgcn_readfirstlane(__double2loint(v)));
+}
+
+// BATCH: the tile's replica from the tables (the same for every lane, copied into scalar registers); otherwise the
+// launch's one system, straight from the kernel arguments.
+template <bool BATCH>
+__device__ __forceinline__ TileSystem tile_system(const TileSystems& ts, int tile) {
+    if constexpr (!BATCH) {
+        return ts.one;
+    } else {
+        const TileSystem& p = ts.per[uniform_u32(ts.rep_of_tile[tile])];
+        TileSystem r;
+        r.first = uniform_u32(p.first);
+        r.end = uniform_u32(p.end);
+        r.cell_base = uniform_u32(p.cell_base);
+        r.side = uniform_i32(p.side);
+        r.use_cells = uniform_i32(p.use_cells);
+        r.hmax = uniform_i32(p.hmax);
+        r.interior_ok = uniform_i32(p.interior_ok);
+        r.gate = uniform_i32(p.gate);
+        r.box = uniform_f64(p.box);
+        r.inv_box = uniform_f64(p.inv_box);
+        r.flat_cells = uniform_f64(p.flat_cells);
+        r.reach = uniform_f64(p.reach);
+        return r;
+    }
+}
+// A rebuild kernel's tile whose replica is not being rebuilt (never for the single system).
+template <bool BATCH>
+__device__ __forceinline__ bool tile_gated_off(const TileSystems& ts, const TileSystem& sys) {
+    if constexpr (!BATCH) return false;
+    else return ts.gates[sys.gate] == 0;
+}
+
 static_assert(RMAX == WAVE, "one lane per halo range in k_tile_setup");
 static_assert(CHUNK_STRIDE == 4096, "slot_entry shifts the chunk index by 7 + 5 bits");
 static_assert(TILE == 2 * WAVE, "the force kernel's energy sum: one wavefront adds two particles per lane");
@@ -242,7 +280,7 @@ __device__ __forceinline__ void load_gathered(const PermuteArgs& pa, const uint3
 #pragma unroll
     for (int u = 0; u < GATHER_PER_LANE; ++u) {
         const size_t k = static_cast<size_t>(t) * TILE + u * WAVE + lane;
-        if (k >= n) continue;
+        if (k >= n) continue;  // single system only: n is both the bound and the component stride
         const size_t v = order[k];
         const double4 p = pa.pos[v];
         g.r[u][0] = p.x;
@@ -281,18 +319,24 @@ __device__ __forceinline__ void store_gathered(const PermuteArgs& pa, const Gath
 // Halo ranges of every tile: one wavefront per tile. The wavefront merges the cell ranges (tiles spanning more than
 // three cell columns, on small grids: lane 0 alone), then lane r reads the particle span of range r and a
 // wavefront scan assigns the LDS slots. tile_nr = -1 marks a tile whose halo does not fit in `hmax` slots.
+template <bool BATCH>
 __global__ void __launch_bounds__(SETUP_TILES * WAVE)
-    k_tile_setup(size_t n, const int* __restrict__ cell_of, const uint32_t* __restrict__ start, int side,
-                 int use_cells, int hmax, int tiles, double flat_cells, int4* __restrict__ tile_ranges,
-                 int* __restrict__ tile_nr, const uint32_t* __restrict__ order, PermuteArgs pa, const int* gate,
-                 const int* halt) {
+    k_tile_setup(const int* __restrict__ cell_of, const uint32_t* __restrict__ start_all, TileSystems ts,
+                 int4* __restrict__ tile_ranges, int* __restrict__ tile_nr, const uint32_t* __restrict__ order,
+                 PermuteArgs pa, const int* gate, const int* halt) {
     if (gated_off(gate, halt)) return;
     __shared__ int2 cr_all[SETUP_TILES][CR_CAP];
     __shared__ int nr_all[SETUP_TILES];
     __shared__ int flat_all[SETUP_TILES];
     const int w = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
     const int t = blockIdx.x * SETUP_TILES + w;
-    const bool live = t < tiles;
+    const TileSystem sys = tile_system<BATCH>(ts, t < ts.tiles ? t : 0);
+    const size_t n = sys.end;  // one past the system's last particle
+    // empty tiles (past a replica's end) keep tile_nr = -1 from the batch's set-up
+    const bool live = t < ts.tiles && static_cast<size_t>(t) * TILE < n && !tile_gated_off<BATCH>(ts, sys);
+    const int side = sys.side, use_cells = sys.use_cells, hmax = sys.hmax;
+    const double flat_cells = sys.flat_cells;
+    const uint32_t* const start = start_all + sys.cell_base;
     int2* cr = cr_all[w];
     // after a re-sort: the state of the tile's particles, loaded in the new order while the ranges are merged and
     // stored at the end (positions into ref, the rest into the alt arrays)
@@ -480,29 +524,6 @@ __device__ __forceinline__ void stage_halo_lanes(const double4* __restrict__ pos
         if (src[u] >= 0) store((wave + W * u) * WAVE + lane, p[u]);
 }
 
-struct TileArgs {
-    int side;
-    int use_cells;
-    int hmax;   // LDS capacity in particles
-    int tiles;
-    int interior_ok;  // side >= 7: an interior cell's 27 neighbours are nearer than L/2 along every axis
-};
-
-struct TiledCopyBack {
-    int enabled;
-    const double* vel2;
-    const double* frc2;
-    const double* mass2;
-    const uint64_t* id2;
-    const int* cell_of2;
-    double4* pos;
-    double* vel;
-    double* frc;
-    double* mass;
-    uint64_t* id;
-    int* cell_of;
-};
-
 // One stencil segment of an own cell: neighbour cell id, LDS slot of its first particle, its particle count.
 struct Segment {
     int nc;
@@ -524,7 +545,7 @@ struct LaneScan {
     const double4* pos;
     double4 ri;
     double box, inv_box, radius2;
-    size_t n;
+    size_t stride;  // column stride of the int32 rows
     size_t i;
     int lane, group_shift;
     unsigned lower_mask;
@@ -566,7 +587,7 @@ struct LaneScan {
             if (accept) {
                 const unsigned at = static_cast<unsigned>(cnt) + __popc(group & lower_mask);
                 if (at < static_cast<unsigned>(padding)) {
-                    row[static_cast<uint64_t>(at) * n] = static_cast<int32_t>(j);
+                    row[static_cast<uint64_t>(at) * stride] = static_cast<int32_t>(j);
                     row16[slot_entry(at)] = static_cast<uint16_t>(slot * SLOT_BYTES);
                 }
             }
@@ -585,15 +606,19 @@ struct LaneScan {
 // FH (float halo): the halo is staged as float coordinates in the frame of the tile's first particle (half the LDS
 // of doubles, three blocks per CU). A candidate is accepted when its float r^2 lies below the list radius by more
 // than the rounding bound `band` of the frame, rejected when above it by more, and otherwise tested again exactly in
-// double from global memory with the reference's minimum image; so the rows are those of the double walk. Tiles
-// whose frame is not compact (2 e + 4 cells + 1 >= L, e the largest own frame coordinate) take the unstaged scan.
+// double from global memory with the reference's minimum image; so the rows are those of the double walk. In a
+// compact frame (2 e + 4 cells + 1 < L, e the largest own frame coordinate) a float separation is already the
+// minimum image; otherwise (small boxes) the walk takes the float minimum image of each separation, with a wider
+// band.
 template <bool FH>
 __device__ __forceinline__ void list_build_tile(
-    int tile, double* lds, const double4* __restrict__ pos, size_t n, const int* __restrict__ cell_of,
-    const uint32_t* __restrict__ start, const TileArgs& ta, double box, double inv_box, double radius2, int padding,
+    int tile, double* lds, const double4* __restrict__ pos, const int* __restrict__ cell_of,
+    const uint32_t* __restrict__ start, const TileSystem& sys, size_t stride, double radius2, int padding,
     int write32, int row_len, int32_t* __restrict__ rows, uint16_t* __restrict__ rows16, int32_t* __restrict__ count,
     int32_t* __restrict__ half, int2* __restrict__ stats, const int4* __restrict__ tile_ranges,
     const int* __restrict__ tile_nr, const TiledCopyBack& cb) {
+    const size_t n = sys.end;  // one past the system's last particle
+    const double box = sys.box, inv_box = sys.inv_box;
     double* xs = lds;  // fixed strides: the three reads of a slot share one address plus immediate offsets
     double* ys = lds + TILE_HALO_MAX;
     double* zs = lds + 2 * TILE_HALO_MAX;
@@ -623,18 +648,18 @@ __device__ __forceinline__ void list_build_tile(
     const HaloRanges hr{ra, rgs, rofs};
     const size_t i0 = static_cast<size_t>(tile) * TILE;
     const size_t i1 = (i0 + TILE) < n ? (i0 + TILE) : n;
-    const int s = ta.side;
+    const int s = sys.side;
     const int nr = tile_range_count(tile_nr[tile]);
     const bool staged = nr > 0;
-    const int c_first = ta.use_cells ? cell_of[i0] : 0;
-    const int ncl = ta.use_cells ? cell_of[i1 - 1] - c_first + 1 : 0;
-    const bool use_lists = staged && ta.use_cells && ncl <= CMAX;
+    const int c_first = sys.use_cells ? cell_of[i0] : 0;
+    const int ncl = sys.use_cells ? cell_of[i1 - 1] - c_first + 1 : 0;
+    const bool use_lists = staged && sys.use_cells && ncl <= CMAX;
     const uint32_t n32 = static_cast<uint32_t>(n);
     // the halo ranges, loaded together with the header (every tile owns RMAX entries; those past nr are unused)
     const int4 rr = static_cast<int>(threadIdx.x) < RMAX ? tile_ranges[static_cast<size_t>(tile) * RMAX + threadIdx.x]
                                                          : make_int4(0, 0, 0, 0);
     // the cells of the tile's own particles, loaded together with the header so the walk reads them from LDS
-    const int my_cell = ta.use_cells && static_cast<size_t>(threadIdx.x) < i1 - i0 ? cell_of[i0 + threadIdx.x] : 0;
+    const int my_cell = sys.use_cells && static_cast<size_t>(threadIdx.x) < i1 - i0 ? cell_of[i0 + threadIdx.x] : 0;
     if (threadIdx.x == 0) {
         s_outside = 0;
         s_lists_ok = 0;
@@ -661,9 +686,9 @@ __device__ __forceinline__ void list_build_tile(
             cbv[2] = __double_as_longlong(q.z);
             cbv[3] = __double_as_longlong(q.w);
         } else if (cb_group == 1) {
-            for (int c = 0; c < 3; ++c) cbv[c] = __double_as_longlong(cb.vel2[c * n + cb_i]);
+            for (int c = 0; c < 3; ++c) cbv[c] = __double_as_longlong(cb.vel2[c * stride + cb_i]);
         } else if (cb_group == 2) {
-            for (int c = 0; c < 3; ++c) cbv[c] = __double_as_longlong(cb.frc2[c * n + cb_i]);
+            for (int c = 0; c < 3; ++c) cbv[c] = __double_as_longlong(cb.frc2[c * stride + cb_i]);
         } else {
             cbv[0] = __double_as_longlong(cb.mass2[cb_i]);
             cbv[1] = cb.id2[cb_i];
@@ -798,7 +823,7 @@ __device__ __forceinline__ void list_build_tile(
     if (use_lists && static_cast<int>(threadIdx.x) < ncl) {
         const int cl = threadIdx.x, c = c_first + cl;
         const int cx = c / (s * s), cy = (c / s) % s, cz = c % s;
-        cl_interior[cl] = ta.interior_ok && cx >= 1 && cx <= s - 2 && cy >= 1 && cy <= s - 2 && cz >= 1 &&
+        cl_interior[cl] = sys.interior_ok && cx >= 1 && cx <= s - 2 && cy >= 1 && cy <= s - 2 && cz >= 1 &&
                           cz <= s - 2;
     }
     __syncthreads();
@@ -819,14 +844,21 @@ __device__ __forceinline__ void list_build_tile(
     // FH: the frame holds minimum images when every own particle lies within e of the origin and 2 e + 4 cells + 1
     // < L (a candidate is within 2 cells of its own particle); then a float r^2 is within `band` of the exact one,
     // band bounding the rounding of frame coordinates up to M = e + 2 cells + 1 (ulp(M) / 2 <= M 2^-24 each)
+    // Tiles whose frame is not compact (small boxes) take the minimum image of each float separation,
+    // d - L_f rint(d / L_f): a component then differs from the exact minimum-image one by at most
+    // 3 x 3.5 L 2^-24 (the two frame coordinates, the difference, L_f and the fma each round once; where the
+    // rounded quotient picks the other image, both images lie within that of L/2), and the band below bounds the
+    // resulting r^2 error near the list radius with a factor 2 to spare.
     const double cell = box / s;
     const double fe = FH ? static_cast<double>(__int_as_float(s_extent)) : 0.0;
     const bool compact = !FH || 2.0 * fe + 4.0 * cell + 1.0 < box;
     const double big_m = fe + 2.0 * cell + 1.0, r_l = sqrt(radius2);
-    const double band = 4.0 * (2.0 * 1.7320508075688772 * r_l * (2.0 * big_m + 3.0 + r_l) + 3.0 * radius2) *
-                        5.9604644775390625e-08;  // 2^-24
+    constexpr double EPS32 = 5.9604644775390625e-08;  // 2^-24
+    const double band = compact ? 4.0 * (2.0 * 1.7320508075688772 * r_l * (2.0 * big_m + 3.0 + r_l) + 3.0 * radius2) * EPS32
+                                : (72.75 * r_l * box + 8.0 * radius2) * EPS32;
     const float r2c = static_cast<float>(radius2), bandf = static_cast<float>(band);
-    const bool lists_ok = use_lists && s_lists_ok && padding <= ROWBUF_CAP && compact;
+    const float box_f = static_cast<float>(box), inv_box_f = static_cast<float>(inv_box);
+    const bool lists_ok = use_lists && s_lists_ok && padding <= ROWBUF_CAP;
     // fill: each lane copies its segment's slots; the list's lanes then pad it to a multiple of 64 with the ghost
 #pragma unroll
     for (int u = 0; u < LIST_ROUNDS; ++u) {
@@ -882,9 +914,15 @@ __device__ __forceinline__ void list_build_tile(
             // FH: d = float r^2 minus the list radius squared (infinite for the ghost); a candidate is inside when
             // d < -band, unsure when |d| <= band (both from the same d, so every candidate is one of inside, unsure,
             // outside), and an unsure one is decided by the exact test in double
-            auto dist_f = [&](int sl) {
+            // `flat_tag`: a compact frame (no image term); otherwise the float minimum image per component
+            auto dist_f = [&](int sl, auto flat_tag) {
                 const float* const cj = coords_f(sl);
-                const float dx = xif - cj[0], dy = yif - cj[TILE_HALO_MAX], dz = zif - cj[2 * TILE_HALO_MAX];
+                float dx = xif - cj[0], dy = yif - cj[TILE_HALO_MAX], dz = zif - cj[2 * TILE_HALO_MAX];
+                if constexpr (!decltype(flat_tag)::value) {
+                    dx = fmaf(-box_f, __builtin_rintf(dx * inv_box_f), dx);
+                    dy = fmaf(-box_f, __builtin_rintf(dy * inv_box_f), dy);
+                    dz = fmaf(-box_f, __builtin_rintf(dz * inv_box_f), dz);
+                }
                 return fmaf(dx, dx, fmaf(dy, dy, dz * dz)) - r2c;
             };
             auto unsure_f = [&](float d) { return fabsf(d) <= bandf; };
@@ -917,7 +955,7 @@ __device__ __forceinline__ void list_build_tile(
                         __builtin_amdgcn_mbcnt_lo(static_cast<unsigned>(mask), static_cast<unsigned>(cnt)));
                     if (at < static_cast<unsigned>(padding)) {
                         rb[at] = static_cast<uint16_t>(sl);
-                        if (write32) row[static_cast<uint64_t>(at) * n] = halo_particle(sl / LIST_BYTES, hr, nr);
+                        if (write32) row[static_cast<uint64_t>(at) * stride] = halo_particle(sl / LIST_BYTES, hr, nr);
                     }
                 }
                 cnt += __popcll(mask);
@@ -932,7 +970,7 @@ __device__ __forceinline__ void list_build_tile(
                     // (a few in a thousand batches), which re-tests those lanes exactly.
                     for (; q0 + 2 * WAVE <= len; q0 += 2 * WAVE) {
                         const int sa = lists[off + q0 + lane], sb = lists[off + q0 + WAVE + lane];
-                        const float ra = dist_f(sa), rb2 = dist_f(sb);
+                        const float ra = dist_f(sa, interior_tag), rb2 = dist_f(sb, interior_tag);
                         const bool ua = unsure_f(ra), ub = unsure_f(rb2);
                         const bool oa = upper ? sa > slot_i : sa < slot_i, ob = upper ? sb > slot_i : sb < slot_i;
                         const bool na = inside_f(ra), nb = inside_f(rb2);
@@ -950,7 +988,7 @@ __device__ __forceinline__ void list_build_tile(
                     }
                     if (q0 < len) {
                         const int sa = lists[off + q0 + lane];
-                        const float ra = dist_f(sa);
+                        const float ra = dist_f(sa, interior_tag);
                         const bool ua = unsure_f(ra), oa = upper ? sa > slot_i : sa < slot_i;
                         const bool na = __ballot(ua) == 0 ? inside_f(ra) : resolve_f(sa, ra, ua);
                         commit(sa, na && oa, __ballot(na && oa));
@@ -980,7 +1018,8 @@ __device__ __forceinline__ void list_build_tile(
                      std::false_type{}, interior_tag);
                 return h;
             };
-            const int hcnt = FH || interior ? both(std::true_type{}) : both(std::false_type{});
+            // FH: the interior tag marks a compact frame
+            const int hcnt = (FH ? compact : interior) ? both(std::true_type{}) : both(std::false_type{});
             {
                 // the row leaves LDS as 16-byte chunk words: lane 4 c + w packs entries 32 c + w + 4 t, t = 0..7
                 // (the wavefront's LDS accesses complete in issue order, so every lane's entries are in place)
@@ -1025,7 +1064,7 @@ __device__ __forceinline__ void list_build_tile(
             sc.box = box;
             sc.inv_box = inv_box;
             sc.radius2 = radius2;
-            sc.n = n;
+            sc.stride = stride;
             sc.i = i;
             sc.lane = lane;
             sc.group_shift = (threadIdx.x & (WAVE - 1)) & ~(LANES - 1);
@@ -1046,7 +1085,7 @@ __device__ __forceinline__ void list_build_tile(
                     else
                         return 0;
                 };
-                if (ta.use_cells) {
+                if (sys.use_cells) {
                     const int c = cell_of[i];
                     const int cx = c / (s * s), cy = (c / s) % s, cz = c % s;
 #pragma unroll 1
@@ -1072,10 +1111,10 @@ __device__ __forceinline__ void list_build_tile(
                         sc.scan<STAGED, false, SLOTS>(start[nc], nc == c ? self : start[nc + 1], base_of(nc));
                     }
                 } else {
-                    // every particle is in the single staged range, at slot = its index
-                    sc.scan<STAGED, false, SLOTS>(self + 1, n32, static_cast<int>(self + 1));
+                    // every particle of the system is in the single staged range, at slot = its index - first
+                    sc.scan<STAGED, false, SLOTS>(self + 1, n32, static_cast<int>(self + 1 - sys.first));
                     hcnt = sc.cnt;
-                    sc.scan<STAGED, false, SLOTS>(0, self, 0);
+                    sc.scan<STAGED, false, SLOTS>(sys.first, self, 0);
                 }
             };
             if (staged && !FH)
@@ -1097,9 +1136,9 @@ __device__ __forceinline__ void list_build_tile(
             cb.pos[cb_i] = make_double4(__longlong_as_double(cbv[0]), __longlong_as_double(cbv[1]),
                                         __longlong_as_double(cbv[2]), __longlong_as_double(cbv[3]));
         } else if (cb_group == 1) {
-            for (int c = 0; c < 3; ++c) cb.vel[c * n + cb_i] = __longlong_as_double(cbv[c]);
+            for (int c = 0; c < 3; ++c) cb.vel[c * stride + cb_i] = __longlong_as_double(cbv[c]);
         } else if (cb_group == 2) {
-            for (int c = 0; c < 3; ++c) cb.frc[c * n + cb_i] = __longlong_as_double(cbv[c]);
+            for (int c = 0; c < 3; ++c) cb.frc[c * stride + cb_i] = __longlong_as_double(cbv[c]);
         } else {
             cb.mass[cb_i] = __longlong_as_double(cbv[0]);
             cb.id[cb_i] = cbv[1];
@@ -1110,30 +1149,34 @@ __device__ __forceinline__ void list_build_tile(
 }
 
 // six wavefronts per SIMD (three blocks per CU, which the float halo's LDS allows) cap the registers at 80
-template <bool FH>
+template <bool FH, bool BATCH>
 __global__ void __launch_bounds__(TILE_THREADS, 6)
-    k_list_build_tiled(const double4* __restrict__ pos, size_t n, const int* __restrict__ cell_of,
-                       const uint32_t* __restrict__ start, TileArgs ta, double box, double inv_box, double radius2,
-                       int padding, int write32, int row_len, int32_t* __restrict__ rows,
-                       uint16_t* __restrict__ rows16, int32_t* __restrict__ count, int32_t* __restrict__ half,
-                       int2* __restrict__ stats, const int4* __restrict__ tile_ranges,
-                       const int* __restrict__ tile_nr, TiledCopyBack cb, const int* gate, const int* halt) {
+    k_list_build_tiled(const double4* __restrict__ pos, const int* __restrict__ cell_of,
+                       const uint32_t* __restrict__ start, TileSystems ts, double radius2, int padding, int write32,
+                       int row_len, int32_t* __restrict__ rows, uint16_t* __restrict__ rows16,
+                       int32_t* __restrict__ count, int32_t* __restrict__ half, int2* __restrict__ stats,
+                       const int4* __restrict__ tile_ranges, const int* __restrict__ tile_nr, TiledCopyBack cb,
+                       const int* gate, const int* halt) {
     if (gated_off(gate, halt)) return;
+    const int tile = xcd_item(static_cast<int>(blockIdx.x), ts.tiles);
+    const TileSystem sys = tile_system<BATCH>(ts, tile);
+    if (tile_gated_off<BATCH>(ts, sys) || static_cast<size_t>(tile) * TILE >= sys.end) return;  // not rebuilt, or empty
     extern __shared__ double lds[];
-    list_build_tile<FH>(xcd_item(static_cast<int>(blockIdx.x), ta.tiles), lds, pos, n, cell_of, start, ta, box,
-                        inv_box, radius2, padding, write32, row_len, rows, rows16, count, half, stats, tile_ranges,
-                        tile_nr, cb);
+    list_build_tile<FH>(tile, lds, pos, cell_of, start + sys.cell_base, sys, ts.stride, radius2, padding, write32,
+                        row_len, rows, rows16, count, half, stats, tile_ranges, tile_nr, cb);
 }
 
 // int32 rows from the 16-bit slot rows (builds that wrote only the slots): slot -> particle through the tile's
 // halo ranges. Unstaged tiles already hold int32 rows.
+template <bool BATCH>
 __global__ void __launch_bounds__(TILE_THREADS)
-    k_rows_from_slots(size_t n, const uint16_t* __restrict__ rows16, int row_len, const int32_t* __restrict__ count,
-                      const int4* __restrict__ tile_ranges, const int* __restrict__ tile_nr,
-                      int32_t* __restrict__ rows) {
+    k_rows_from_slots(TileSystems ts, const uint16_t* __restrict__ rows16, int row_len,
+                      const int32_t* __restrict__ count, const int4* __restrict__ tile_ranges,
+                      const int* __restrict__ tile_nr, int32_t* __restrict__ rows) {
     __shared__ int ra[RMAX], rgs[RMAX], rofs[RMAX + 1];
     const HaloRanges hr{ra, rgs, rofs};
     const int tile = blockIdx.x;
+    const size_t n = tile_system<BATCH>(ts, tile).end, stride = ts.stride;
     const int nr = tile_range_count(tile_nr[tile]);
     if (nr <= 0) return;
     load_ranges(tile_ranges, tile, nr, hr);
@@ -1144,7 +1187,8 @@ __global__ void __launch_bounds__(TILE_THREADS)
     const int m = count[i];
     const uint16_t* row16 = rows16 + slot_row_base(i, row_len);
     for (int k = lane; k < m; k += LANES)
-        rows[static_cast<size_t>(k) * n + i] = halo_particle(row16[slot_entry(static_cast<uint32_t>(k))] / SLOT_BYTES, hr, nr);
+        rows[static_cast<size_t>(k) * stride + i] =
+            halo_particle(row16[slot_entry(static_cast<uint32_t>(k))] / SLOT_BYTES, hr, nr);
 }
 
 constexpr int UNSTAGED_BATCH = 2;  // row entries per lane in flight on the force kernel's unstaged path
@@ -1154,13 +1198,12 @@ constexpr int UNSTAGED_BATCH = 2;  // row entries per lane in flight on the forc
 struct TileFrame {
     double box, inv_box;
     double reach;  // rc + list radius + skin, for the compactness test
-    int tiles;
 };
 
 // Arguments of the step that the force kernel can carry out for its own particles (see k_forces_tiled_mixed).
 struct ForceStep {
     double4* pos_next;     // positions after the drift (a buffer other than the one being read)
-    double* vel;           // velocities, x | y | z blocks of n
+    double* vel;           // velocities, x | y | z blocks of the stride
     const double* mass;
     const double4* ref;    // positions at the last list build
     double half_dt, dt, max_disp2;
@@ -1183,12 +1226,12 @@ __device__ __forceinline__ double frame_coord(double x, double ref, const TileFr
 // drift of the next step, and the displacement test of the next list check (the arithmetic of k_kick_kick_drift).
 // The new positions go to a second buffer, since other tiles' halos still read this step's; the forces themselves
 // are not stored.
-template <bool STEP>
+template <bool STEP, bool BATCH>
 __global__ void __launch_bounds__(TILE_THREADS)
-    k_forces_tiled_mixed(const double4* __restrict__ pos, size_t n, const uint16_t* __restrict__ rows16,
+    k_forces_tiled_mixed(const double4* __restrict__ pos, TileSystems ts, const uint16_t* __restrict__ rows16,
                          const int32_t* __restrict__ rows, const int32_t* __restrict__ count,
                          const int4* __restrict__ tile_ranges, const int* __restrict__ tile_nr, int row_len,
-                         LJConst lj, TileFrame fr, double* __restrict__ frc, double* __restrict__ partials,
+                         LJConst lj, double* __restrict__ frc, double* __restrict__ partials,
                          int* __restrict__ clear_flag, ForceStep fs, const int* halt) {
     if (halted(halt)) return;
     // The halo is a static array: its address is known when the kernel is compiled, so a slot's byte offset, as the
@@ -1205,7 +1248,10 @@ __global__ void __launch_bounds__(TILE_THREADS)
     __shared__ int ra[RMAX], rgs[RMAX], rofs[RMAX + 1];
     __shared__ int extent_bits;
     const HaloRanges hr{ra, rgs, rofs};
-    const int tile = xcd_item(static_cast<int>(blockIdx.x), fr.tiles);
+    const int tile = xcd_item(static_cast<int>(blockIdx.x), ts.tiles);
+    const TileSystem sys = tile_system<BATCH>(ts, tile);
+    const TileFrame fr{sys.box, sys.inv_box, sys.reach};
+    const size_t n = sys.end, pstride = ts.stride;  // pstride: column / component stride
     const int raw_nr = tile_nr[tile];
     // the halo ranges, loaded together with the header (every tile owns RMAX entries; those past nr are unused)
     const int4 rr = static_cast<int>(threadIdx.x) < RMAX ? tile_ranges[static_cast<size_t>(tile) * RMAX + threadIdx.x]
@@ -1311,7 +1357,7 @@ __global__ void __launch_bounds__(TILE_THREADS)
 #pragma unroll
                 for (int b = 0; b < UNSTAGED_BATCH; ++b) {
                     const int k = k0 + b * LANES;
-                    j[b] = k < m ? rows[static_cast<size_t>(k) * n + i] : -1;
+                    j[b] = k < m ? rows[static_cast<size_t>(k) * pstride + i] : -1;
                 }
                 double xj[UNSTAGED_BATCH], yj[UNSTAGED_BATCH], zj[UNSTAGED_BATCH];  // plain doubles stay in registers
 #pragma unroll
@@ -1353,7 +1399,7 @@ __global__ void __launch_bounds__(TILE_THREADS)
         if (active) {
             m = fs.mass[i];
             kick = fs.half_dt * fc / m;
-            v = fs.vel[comp * n + i] + kick;  // closing kick of this step
+            v = fs.vel[comp * pstride + i] + kick;  // closing kick of this step
         }
         const double vy = __shfl_xor(v, 1, WAVE), vz = __shfl_xor(v, 2, WAVE);
         if (active) {
@@ -1366,7 +1412,7 @@ __global__ void __launch_bounds__(TILE_THREADS)
         if (active) {
             double* next = reinterpret_cast<double*>(fs.pos_next + i);
             if (lane < 3) {
-                fs.vel[comp * n + i] = v;
+                fs.vel[comp * pstride + i] = v;
                 next[lane] = r;
             } else {
                 next[3] = pi[3];
@@ -1374,7 +1420,7 @@ __global__ void __launch_bounds__(TILE_THREADS)
             if (lane == 0 && d * d + dy * dy + dz * dz > fs.max_disp2) *fs.stale_next = 1;
         }
     }
-    if (active && !STEP && lane < 3) frc[lane * n + i] = fc;
+    if (active && !STEP && lane < 3) frc[lane * pstride + i] = fc;
     __syncthreads();  // every wavefront is done with the halo
     if (lane == 0) {
         // every pair was met from both ends: half of each full-row sum (zero for the slots past n); the potential's
@@ -1472,18 +1518,20 @@ __device__ __forceinline__ void add_pair(PairLaneSums& a, double tx, double ty, 
     a.fz += pz;
 }
 
-template <bool UNIT>
+template <bool UNIT, bool BATCH>
 __global__ void __launch_bounds__(PAIR_THREADS, 3)
-    k_forces_tiled_fp64_pair(const double4* __restrict__ pos, size_t n, const uint16_t* __restrict__ rows16,
+    k_forces_tiled_fp64_pair(const double4* __restrict__ pos, TileSystems ts, const uint16_t* __restrict__ rows16,
                              const int32_t* __restrict__ rows, const int32_t* __restrict__ count,
                              const int32_t* __restrict__ half, const int4* __restrict__ tile_ranges,
-                             const int* __restrict__ tile_nr, int row_len, int tiles, LJConst lj, double box,
-                             double inv_box, double* __restrict__ frc, double* __restrict__ partials,
-                             int* __restrict__ clear_flag, const int* halt) {
+                             const int* __restrict__ tile_nr, int row_len, LJConst lj, double* __restrict__ frc,
+                             double* __restrict__ partials, int* __restrict__ clear_flag, const int* halt) {
     if (halted(halt)) return;
     __shared__ double halo[3][TILE_HALO_MAX];
     __shared__ double e_part[PAIR_THREADS / WAVE][3];
-    const int tile = xcd_item(static_cast<int>(blockIdx.x), tiles);
+    const int tile = xcd_item(static_cast<int>(blockIdx.x), ts.tiles);
+    const TileSystem sys = tile_system<BATCH>(ts, tile);
+    const size_t n = sys.end, pstride = ts.stride;  // pstride: column / component stride
+    const double box = sys.box, inv_box = sys.inv_box;
     const int raw_nr = tile_nr[tile];
     // every wavefront holds the tile's ranges, one per lane
     const int4 rr = tile_ranges[static_cast<size_t>(tile) * RMAX + threadIdx.x % WAVE];
@@ -1568,7 +1616,7 @@ __global__ void __launch_bounds__(PAIR_THREADS, 3)
 #pragma unroll
             for (int b = 0; b < 4; ++b) {
                 const int k = 2 * (g0 + b) + par;
-                j[b] = k < m ? rows[static_cast<size_t>(k) * n + i] : -1;
+                j[b] = k < m ? rows[static_cast<size_t>(k) * pstride + i] : -1;
             }
             double4 rj[4];
 #pragma unroll
@@ -1586,8 +1634,8 @@ __global__ void __launch_bounds__(PAIR_THREADS, 3)
     }
     if (active && par == 0) {
         frc[i] = a.fx;
-        frc[n + i] = a.fy;
-        frc[2 * n + i] = a.fz;
+        frc[pstride + i] = a.fy;
+        frc[2 * pstride + i] = a.fz;
     }
     const double cnt = wave_sum_dpp(static_cast<double>(a.c));
     const double s12 = wave_sum_dpp(a.s12), s6 = wave_sum_dpp(a.s6);
@@ -1609,6 +1657,29 @@ __global__ void __launch_bounds__(PAIR_THREADS, 3)
 
 static int slot_row_len(int padding);
 
+// The launch view of one System: every tile in ts.one.
+static TileSystems single_system(const md_state* s, const md_list* l, double reach) {
+    TileSystems ts{};
+    TileSystem& o = ts.one;
+    const size_t n = s->n;
+    o.first = 0;
+    o.end = static_cast<uint32_t>(n);
+    o.cell_base = 0;
+    o.side = l->side;
+    o.use_cells = l->side >= 3 ? 1 : 0;
+    o.hmax = l->hmax;
+    o.interior_ok = l->side >= 7 ? 1 : 0;
+    o.gate = 0;
+    o.box = s->box;
+    o.inv_box = s->inv_box;
+    // cells that a tile and its halo may span with every coordinate inside half a box (skin: the largest drift)
+    o.flat_cells = l->side > 0 ? (0.5 * s->box - l->skin) / (s->box / l->side) : 0.0;
+    o.reach = reach;
+    ts.stride = n;
+    ts.tiles = tile_count(n ? n : 1);
+    return ts;
+}
+
 void tiled_forces_fp64_enqueue(md_state* s, md_list* l, const LJConst& lj, double* out_dev, int* clear_flag,
                                hipEvent_t ev_start, hipEvent_t ev_stop) {
     hipStream_t st = s->ctx->stream;
@@ -1619,16 +1690,11 @@ void tiled_forces_fp64_enqueue(md_state* s, md_list* l, const LJConst& lj, doubl
     ensure_rows32(l);
     if (ev_start) MD_HIP(hipEventRecord(ev_start, st));
     if (n > 0) {
-        if (lj.sig2 == 1.0)
-            hipLaunchKernelGGL(k_forces_tiled_fp64_pair<true>, dim3(tiles), dim3(PAIR_THREADS), 0, st, s->pos, n,
-                               l->rows16, l->rows, l->count, l->half, l->tile_ranges, l->tile_nr,
-                               slot_row_len(l->padding), tiles, lj, s->box, s->inv_box, s->frc, s->partials,
-                               clear_flag, s->halt);
-        else
-            hipLaunchKernelGGL(k_forces_tiled_fp64_pair<false>, dim3(tiles), dim3(PAIR_THREADS), 0, st, s->pos, n,
-                               l->rows16, l->rows, l->count, l->half, l->tile_ranges, l->tile_nr,
-                               slot_row_len(l->padding), tiles, lj, s->box, s->inv_box, s->frc, s->partials,
-                               clear_flag, s->halt);
+        const TileSystems ts = single_system(s, l, 0.0);
+        auto kernel = lj.sig2 == 1.0 ? k_forces_tiled_fp64_pair<true, false> : k_forces_tiled_fp64_pair<false, false>;
+        hipLaunchKernelGGL(kernel, dim3(tiles), dim3(PAIR_THREADS), 0, st, s->pos, ts, l->rows16, l->rows, l->count,
+                           l->half, l->tile_ranges, l->tile_nr, slot_row_len(l->padding), lj, s->frc, s->partials,
+                           clear_flag, s->halt);
         MD_HIP(hipGetLastError());
     }
     if (ev_stop) MD_HIP(hipEventRecord(ev_stop, st));
@@ -1653,7 +1719,11 @@ void tiled_rows32_enqueue(md_list* l) {
     const size_t n = l->n;
     if (n == 0) return;
     const int tiles = tile_count(n);
-    hipLaunchKernelGGL(k_rows_from_slots, dim3(tiles), dim3(TILE_THREADS), 0, l->ctx->stream, n, l->rows16,
+    TileSystems ts{};
+    ts.one.end = static_cast<uint32_t>(n);
+    ts.stride = n;
+    ts.tiles = tiles;
+    hipLaunchKernelGGL(k_rows_from_slots<false>, dim3(tiles), dim3(TILE_THREADS), 0, l->ctx->stream, ts, l->rows16,
                        slot_row_len(l->padding), l->count, l->tile_ranges, l->tile_nr, l->rows);
     MD_HIP(hipGetLastError());
 }
@@ -1686,7 +1756,7 @@ void tiled_list_enqueue(md_list* l, md_state* s, bool from_alt, const int* gate,
         cb.cell_of = l->cell_of;
     }
     const int tiles = tile_count(n);
-    TileArgs ta{l->side, l->side >= 3 ? 1 : 0, l->hmax, tiles, l->side >= 7 ? 1 : 0};
+    const TileSystems ts = single_system(s, l, 0.0);
     // after a re-sort the tile setup gathers the state into the alt arrays, the positions only into `ref` (the
     // build-time positions, which they are): the build reads them there and copies them back into place
     PermuteArgs gather{};
@@ -1696,22 +1766,18 @@ void tiled_list_enqueue(md_list* l, md_state* s, bool from_alt, const int* gate,
     }
     const double4* build_pos = from_alt ? l->ref : s->pos;
     const int* cells = from_alt ? l->cell_of_alt : l->cell_of;
-    // cells that a tile and its halo may span with every coordinate inside half a box (skin: the largest drift)
-    const double flat_cells = (0.5 * s->box - l->skin) / (s->box / l->side);
-    hipLaunchKernelGGL(k_tile_setup, dim3((tiles + SETUP_TILES - 1) / SETUP_TILES), dim3(SETUP_TILES * WAVE), 0, st,
-                       n, cells, l->cell_start, ta.side, ta.use_cells, ta.hmax, tiles, flat_cells, l->tile_ranges,
-                       l->tile_nr, l->order, gather, gate, s->halt);
+    hipLaunchKernelGGL(k_tile_setup<false>, dim3((tiles + SETUP_TILES - 1) / SETUP_TILES), dim3(SETUP_TILES * WAVE), 0, st,
+                       cells, l->cell_start, ts, l->tile_ranges, l->tile_nr, l->order, gather, gate, s->halt);
     // the float halo where the grid is wide enough for compact frames and the cells dense enough for the
     // candidate lists (MDGPU_BUILD_F64=1 keeps the double halo)
     const char* f64 = std::getenv("MDGPU_BUILD_F64");
     const bool fh = l->side >= 20 && static_cast<double>(n) >= 6.0 * static_cast<double>(l->ncells) &&
                     !(f64 && f64[0] == '1');
-    auto kernel = fh ? k_list_build_tiled<true> : k_list_build_tiled<false>;
+    auto kernel = fh ? k_list_build_tiled<true, false> : k_list_build_tiled<false, false>;
     const size_t lds = static_cast<size_t>(TILE_HALO_MAX) * 3 * (fh ? sizeof(float) : sizeof(double));
-    hipLaunchKernelGGL(kernel, dim3(tiles), dim3(TILE_THREADS), lds, st, build_pos, n, cells,
-                       l->cell_start, ta, s->box, s->inv_box, l->radius2, l->padding, write32 ? 1 : 0,
-                       slot_row_len(l->padding), l->rows, l->rows16, l->count, l->half, l->build_stats,
-                       l->tile_ranges, l->tile_nr, cb, gate, s->halt);
+    hipLaunchKernelGGL(kernel, dim3(tiles), dim3(TILE_THREADS), lds, st, build_pos, cells, l->cell_start, ts,
+                       l->radius2, l->padding, write32 ? 1 : 0, slot_row_len(l->padding), l->rows, l->rows16,
+                       l->count, l->half, l->build_stats, l->tile_ranges, l->tile_nr, cb, gate, s->halt);
     MD_HIP(hipGetLastError());
     finish_build_enqueue(l, tiles, gate, s->halt);
     MD_HIP(hipGetLastError());
@@ -1725,25 +1791,53 @@ static void tiled_forces_launch(md_state* s, md_list* l, const LJConst& lj, doub
     const int tiles = tile_count(n ? n : 1);
     const int sums = step ? 4 : 3;
     state_reserve_partials(s, static_cast<size_t>(tiles) * sums);
-    TileFrame fr;
-    fr.box = s->box;
-    fr.inv_box = s->inv_box;
-    fr.reach = cutoff + l->radius + l->skin;
-    fr.tiles = tiles;
+    const TileSystems ts = single_system(s, l, cutoff + l->radius + l->skin);
     if (ev_start) MD_HIP(hipEventRecord(ev_start, st));
     if (n > 0) {
         if (step)
-            hipLaunchKernelGGL(k_forces_tiled_mixed<true>, dim3(tiles), dim3(TILE_THREADS), 0, st, s->pos, n,
+            hipLaunchKernelGGL((k_forces_tiled_mixed<true, false>), dim3(tiles), dim3(TILE_THREADS), 0, st, s->pos, ts,
                                l->rows16, l->rows, l->count, l->tile_ranges, l->tile_nr, slot_row_len(l->padding),
-                               lj, fr, s->frc, s->partials, clear_flag, *step, s->halt);
+                               lj, s->frc, s->partials, clear_flag, *step, s->halt);
         else
-            hipLaunchKernelGGL(k_forces_tiled_mixed<false>, dim3(tiles), dim3(TILE_THREADS), 0, st, s->pos, n,
+            hipLaunchKernelGGL((k_forces_tiled_mixed<false, false>), dim3(tiles), dim3(TILE_THREADS), 0, st, s->pos, ts,
                                l->rows16, l->rows, l->count, l->tile_ranges, l->tile_nr, slot_row_len(l->padding),
-                               lj, fr, s->frc, s->partials, clear_flag, ForceStep{}, s->halt);
+                               lj, s->frc, s->partials, clear_flag, ForceStep{}, s->halt);
         MD_HIP(hipGetLastError());
     }
     if (ev_stop) MD_HIP(hipEventRecord(ev_stop, st));
     reduce_partials_enqueue(s, n > 0 ? tiles : 0, sums, out_dev, rec);
+}
+
+int tiled_row_len(int padding) { return slot_row_len(padding); }
+
+void tiled_batch_build_enqueue(hipStream_t st, const BatchTiles& bt, const double4* build_pos, const int* cell_of,
+                               const uint32_t* start, double radius2, const TiledCopyBack& cb, const int* halt) {
+    const int tiles = bt.ts.tiles;
+    hipLaunchKernelGGL(k_tile_setup<true>, dim3((tiles + SETUP_TILES - 1) / SETUP_TILES), dim3(SETUP_TILES * WAVE), 0,
+                       st, cell_of, start, bt.ts, bt.tile_ranges, bt.tile_nr, nullptr, PermuteArgs{}, nullptr, halt);
+    MD_HIP(hipGetLastError());
+    // the float halo: three blocks per CU, and float distance tests (with the minimum image in small boxes)
+    hipLaunchKernelGGL((k_list_build_tiled<true, true>), dim3(tiles), dim3(TILE_THREADS),
+                       static_cast<size_t>(TILE_HALO_MAX) * 3 * sizeof(float), st, build_pos, cell_of, start, bt.ts,
+                       radius2, bt.padding, 0, slot_row_len(bt.padding), bt.rows, bt.rows16, bt.count, bt.half,
+                       bt.stats, bt.tile_ranges, bt.tile_nr, cb, nullptr, halt);
+    MD_HIP(hipGetLastError());
+}
+
+void tiled_batch_forces_enqueue(hipStream_t st, const BatchTiles& bt, const double4* pos, const LJConst& lj,
+                                int precision, double* frc, double* partials, const int* halt) {
+    const int tiles = bt.ts.tiles;
+    const int row_len = slot_row_len(bt.padding);
+    if (precision == MD_MIXED) {
+        hipLaunchKernelGGL((k_forces_tiled_mixed<false, true>), dim3(tiles), dim3(TILE_THREADS), 0, st, pos, bt.ts,
+                           bt.rows16, bt.rows, bt.count, bt.tile_ranges, bt.tile_nr, row_len, lj, frc, partials,
+                           nullptr, ForceStep{}, halt);
+    } else {
+        auto kernel = lj.sig2 == 1.0 ? k_forces_tiled_fp64_pair<true, true> : k_forces_tiled_fp64_pair<false, true>;
+        hipLaunchKernelGGL(kernel, dim3(tiles), dim3(PAIR_THREADS), 0, st, pos, bt.ts, bt.rows16, bt.rows, bt.count,
+                           bt.half, bt.tile_ranges, bt.tile_nr, row_len, lj, frc, partials, nullptr, halt);
+    }
+    MD_HIP(hipGetLastError());
 }
 
 void tiled_forces_enqueue(md_state* s, md_list* l, const LJConst& lj, double cutoff, double* out_dev,
