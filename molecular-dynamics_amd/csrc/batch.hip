@@ -34,6 +34,7 @@ namespace mdgpu {
 constexpr int BT = 256;          // particle slots per block; replicas start on block boundaries
 constexpr int KE_TERMS = 1024;   // the reference's deterministicSum block (Parallel.hpp:32-57)
 constexpr int SCALARS = 8;       // per replica: U, W, inside, KE, -, alpha, KE', -
+constexpr int REDUCE_WAVES = 4;  // replicas per block of the per-replica wavefront kernels
 
 struct Rep {
     uint32_t first, n;            // particle slots [first, first + n)
@@ -73,11 +74,13 @@ struct BatchCopyBack {
 // ---- integration -----------------------------------------------------------------------------------------
 
 // k_kick_drift per replica box; the staleness test raises stale[r].
+// With `scalars` (Bussi runs), the velocities are first scaled by the replica's alpha from the previous step's
+// thermostat (k_scale of that step, moved here: the same products, one pass over the velocities fewer).
 __global__ void __launch_bounds__(BT)
     k_b_kick_drift(const Rep* __restrict__ reps, const uint16_t* __restrict__ block_rep, double4* __restrict__ pos,
                    double* __restrict__ vel, const double* __restrict__ frc, const double* __restrict__ mass,
                    size_t slots, double half_dt, double dt, const double4* __restrict__ ref, double max_disp2,
-                   int* __restrict__ stale, const int* halt) {
+                   int* __restrict__ stale, const double* __restrict__ scalars, const int* halt) {
     if (halted(halt)) return;
     const int r = block_rep[blockIdx.x];
     const Rep rp = reps[r];
@@ -89,7 +92,9 @@ __global__ void __launch_bounds__(BT)
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         const double kick = half_dt * frc[c * slots + i];
-        v[c] = vel[c * slots + i] + kick / m;
+        double vc = vel[c * slots + i];
+        if (scalars) vc = vc * scalars[r * SCALARS + 5];
+        v[c] = vc + kick / m;
         vel[c * slots + i] = v[c];
     }
     p.x = wrap_coord(p.x + dt * v[0], rp.box, rp.inv_box);
@@ -140,24 +145,34 @@ __global__ void k_b_reduce_ke(const Rep* __restrict__ reps, int count, const dou
     scalars[r * SCALARS + 3] = s;
 }
 
-// Reference-order kinetic energy (k_ke_blocks / k_ke_final per replica): 1024-particle blocks summed
-// sequentially, then the block sums sequentially.
+// The closing kick (k_b_kick) of one reference-order kinetic-energy block of 1024 particles, then the block's
+// sequential sum of (0.5 m) v^2 from the kicked velocities (k_ke_blocks per replica; k_b_ke_final adds the block
+// sums in order).
 __global__ void __launch_bounds__(BT)
-    k_b_ke_blocks(const Rep* __restrict__ reps, const uint16_t* __restrict__ block_rep,
-                  const uint32_t* __restrict__ ke_start, const double* __restrict__ vel,
-                  const double* __restrict__ mass, size_t slots, double* __restrict__ ke_part, const int* halt) {
+    k_b_kick_ke(const Rep* __restrict__ reps, const uint16_t* __restrict__ block_rep,
+                const uint32_t* __restrict__ ke_start, double* __restrict__ vel, const double* __restrict__ frc,
+                const double* __restrict__ mass, size_t slots, double half_dt, double* __restrict__ ke_part,
+                const int* halt) {
     if (halted(halt)) return;
     __shared__ double terms[KE_TERMS];
     const uint32_t base = ke_start[blockIdx.x];
     const Rep rp = reps[block_rep[base / BT]];
     const uint32_t end = rp.first + rp.n;
+#pragma unroll
     for (int t = threadIdx.x; t < KE_TERMS; t += BT) {
         const size_t i = static_cast<size_t>(base) + t;
         double term = 0.0;
         if (i < end) {
-            const double vx = vel[i], vy = vel[slots + i], vz = vel[2 * slots + i];
-            const double half_m = 0.5 * mass[i];
-            term = half_m * (vx * vx + vy * vy + vz * vz);
+            const double m = mass[i];
+            double v[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const double kick = half_dt * frc[c * slots + i];
+                v[c] = vel[c * slots + i] + kick / m;
+                vel[c * slots + i] = v[c];
+            }
+            const double half_m = 0.5 * m;
+            term = half_m * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
         }
         terms[t] = term;
     }
@@ -168,6 +183,12 @@ __global__ void __launch_bounds__(BT)
         for (uint32_t t = 0; t < cnt; ++t) s += terms[t];
         ke_part[blockIdx.x] = s;
     }
+}
+
+// alpha = 1 for every replica: the velocity scale the first opening kick of a Bussi run applies.
+__global__ void k_b_unit_alpha(int count, double* __restrict__ scalars) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < count) scalars[r * SCALARS + 5] = 1.0;
 }
 
 __global__ void k_b_ke_final(const Rep* __restrict__ reps, int count, const double* __restrict__ ke_part,
@@ -245,12 +266,13 @@ __global__ void __launch_bounds__(1024)
 __global__ void __launch_bounds__(BT)
     k_b_cell_count(const Rep* __restrict__ reps, const uint16_t* __restrict__ block_rep,
                    const double4* __restrict__ pos, int* __restrict__ cell_of, uint32_t* __restrict__ counts,
-                   const int* __restrict__ stale, const int* halt) {
+                   const int* __restrict__ act_blocks, const int* halt) {
     if (halted(halt)) return;
-    const int r = block_rep[blockIdx.x];
-    if (!stale[r]) return;
+    if (static_cast<int>(blockIdx.x) >= act_blocks[0]) return;
+    const int blk = act_blocks[1 + blockIdx.x];
+    const int r = block_rep[blk];
     const Rep rp = reps[r];
-    const size_t i = static_cast<size_t>(blockIdx.x) * BT + threadIdx.x;
+    const size_t i = static_cast<size_t>(blk) * BT + threadIdx.x;
     const bool valid = i < static_cast<size_t>(rp.first) + rp.n;
     int c = -1;
     if (valid) {
@@ -302,12 +324,13 @@ __global__ void __launch_bounds__(BT)
 __global__ void __launch_bounds__(BT)
     k_b_scatter(const Rep* __restrict__ reps, const uint16_t* __restrict__ block_rep,
                 const int* __restrict__ cell_of, const uint32_t* __restrict__ start, uint32_t* __restrict__ fill,
-                uint32_t* __restrict__ order_raw, const int* __restrict__ stale, const int* halt) {
+                uint32_t* __restrict__ order_raw, const int* __restrict__ act_blocks, const int* halt) {
     if (halted(halt)) return;
-    const int r = block_rep[blockIdx.x];
-    if (!stale[r]) return;
+    if (static_cast<int>(blockIdx.x) >= act_blocks[0]) return;
+    const int blk = act_blocks[1 + blockIdx.x];
+    const int r = block_rep[blk];
     const Rep rp = reps[r];
-    const size_t i = static_cast<size_t>(blockIdx.x) * BT + threadIdx.x;
+    const size_t i = static_cast<size_t>(blk) * BT + threadIdx.x;
     const bool valid = i < static_cast<size_t>(rp.first) + rp.n;
     const int c = valid ? cell_of[i] : -1;
     const uint32_t cw = rp.cell_first + static_cast<uint32_t>(c);
@@ -318,20 +341,20 @@ __global__ void __launch_bounds__(BT)
     if (valid) order_raw[start[cw] + base + static_cast<uint32_t>((threadIdx.x % WAVE) - run.first)] = static_cast<uint32_t>(i);
 }
 
-// k_cell_sort over every replica's cells: one wavefront per cell word.
+// k_cell_sort over the cells of the replicas being rebuilt: one wavefront per cell word of act_cells.
 __global__ void __launch_bounds__(BT)
-    k_b_sort(const Rep* __restrict__ reps, const uint16_t* __restrict__ cell_rep, size_t cells_total, size_t slots,
+    k_b_sort(const Rep* __restrict__ reps, const uint16_t* __restrict__ cell_rep, size_t slots,
              const uint32_t* __restrict__ start, const uint32_t* __restrict__ order_raw, uint32_t* __restrict__ fill,
-             uint32_t* __restrict__ counts, BatchPermute pa, const int* __restrict__ stale, const int* halt) {
+             uint32_t* __restrict__ counts, BatchPermute pa, const uint32_t* __restrict__ act_cells,
+             const int* halt) {
     if (halted(halt)) return;
-    const size_t g = (static_cast<size_t>(blockIdx.x) * BT + threadIdx.x) / WAVE;
+    const size_t w = (static_cast<size_t>(blockIdx.x) * BT + threadIdx.x) / WAVE;
     const int lane = threadIdx.x % WAVE;
-    if (g >= cells_total) return;
+    if (w >= act_cells[0]) return;
+    const uint32_t g = act_cells[1 + w];
     const int r = cell_rep[g];
-    if (!stale[r]) return;
     const Rep rp = reps[r];
-    const uint32_t cell = static_cast<uint32_t>(g) - rp.cell_first;
-    if (cell >= rp.ncells) return;  // the replica's end word
+    const uint32_t cell = g - rp.cell_first;
     const uint32_t b = start[g];
     const uint32_t m = start[g + 1] - b;
     if (lane == 0) {
@@ -448,24 +471,115 @@ __global__ void __launch_bounds__(BT)
 
 // Per-replica totals of a build from its records (one per `unit` slots: a block of the per-particle build, or a
 // tile): half-pair count, and the longest row and overflow over the stale replicas into flags[1] and flags[0].
-__global__ void k_b_build_totals(const Rep* __restrict__ reps, int count, int unit, const int2* __restrict__ bstats,
-                                 int padding, int* __restrict__ flags, unsigned long long* __restrict__ pairs,
-                                 const int* __restrict__ stale, const int* halt) {
+// One wavefront per replica (integer maximum and sum: any order gives the same totals).
+__global__ void __launch_bounds__(REDUCE_WAVES * WAVE)
+    k_b_build_totals(const Rep* __restrict__ reps, int count, int unit, const int2* __restrict__ bstats,
+                     int padding, int* __restrict__ flags, unsigned long long* __restrict__ pairs,
+                     const int* __restrict__ stale, const int* halt) {
     if (halted(halt)) return;
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = blockIdx.x * REDUCE_WAVES + threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
     if (r >= count || !stale[r]) return;
     const Rep rp = reps[r];
     const uint32_t b0 = rp.first / unit, b1 = (rp.first + rp.n + unit - 1) / unit;
     int mx = 0;
     unsigned long long hs = 0;
-    for (uint32_t b = b0; b < b1; ++b) {
+    for (uint32_t b = b0 + lane; b < b1; b += WAVE) {
         const int2 v = bstats[b];
         mx = v.x > mx ? v.x : mx;
         hs += static_cast<unsigned>(v.y);
     }
-    pairs[r] = hs;
-    atomicMax(&flags[1], mx);
-    if (mx > padding) atomicOr(&flags[0], 1);
+#pragma unroll
+    for (int off = WAVE / 2; off > 0; off >>= 1) {
+        const int o = __shfl_xor(mx, off, WAVE);
+        mx = o > mx ? o : mx;
+        hs += __shfl_xor(hs, off, WAVE);
+    }
+    if (lane == 0) {
+        pairs[r] = hs;
+        atomicMax(&flags[1], mx);
+        if (mx > padding) atomicOr(&flags[0], 1);
+    }
+}
+
+// The work lists of a rebuild, over the stale replicas in replica order: their non-empty tiles (act_tiles), their
+// 256-slot blocks (act_blocks) and their cell words (act_cells, end words excluded); word 0 of each holds its
+// length. The rebuild kernels take their k-th work item from these lists, so that a launch over the whole batch
+// spends its blocks on the replicas being rebuilt only. One block.
+constexpr int ACT_THREADS = 1024;
+__global__ void __launch_bounds__(ACT_THREADS)
+    k_b_activate(const Rep* __restrict__ reps, int count, const int* __restrict__ stale, int* __restrict__ act_tiles,
+                 int* __restrict__ act_blocks, uint32_t* __restrict__ act_cells, const int* halt) {
+    if (halted(halt)) return;
+    // per stale replica of the chunk (compacted): first slot, particles, first cell word, cells, output offsets
+    __shared__ uint32_t q_first[ACT_THREADS], q_n[ACT_THREADS], q_cell[ACT_THREADS], q_ncells[ACT_THREADS];
+    __shared__ uint32_t q_ot[ACT_THREADS], q_ob[ACT_THREADS], q_oc[ACT_THREADS];
+    __shared__ uint32_t wsum[4][ACT_THREADS / WAVE];
+    __shared__ uint32_t base[4];  // [0] stale replicas in the chunk, [1..3] tiles, blocks, cells so far
+    const int tid = threadIdx.x, lane = tid % WAVE, wave = tid / WAVE;
+    if (tid < 4) base[tid] = 0;
+    __syncthreads();
+    for (int r0 = 0; r0 < count; r0 += ACT_THREADS) {
+        const int r = r0 + tid;
+        uint32_t v[4] = {0u, 0u, 0u, 0u};
+        Rep rp{};
+        if (r < count && stale[r]) {
+            rp = reps[r];
+            v[0] = 1u;
+            v[1] = (rp.n + TILE_PARTICLES - 1) / TILE_PARTICLES;
+            v[2] = (rp.n + BT - 1) / BT;
+            v[3] = rp.ncells;
+        }
+        uint32_t incl[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            incl[k] = v[k];
+#pragma unroll
+            for (int d = 1; d < WAVE; d <<= 1) {
+                const uint32_t o = __shfl_up(incl[k], d, WAVE);
+                if (lane >= d) incl[k] += o;
+            }
+            if (lane == WAVE - 1) wsum[k][wave] = incl[k];
+        }
+        __syncthreads();
+        uint32_t excl[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            uint32_t before = k == 0 ? 0u : base[k];
+            for (int w = 0; w < wave; ++w) before += wsum[k][w];
+            excl[k] = before + incl[k] - v[k];
+        }
+        if (v[0]) {
+            const uint32_t q = excl[0];
+            q_first[q] = rp.first;
+            q_n[q] = rp.n;
+            q_cell[q] = rp.cell_first;
+            q_ncells[q] = rp.ncells;
+            q_ot[q] = excl[1];
+            q_ob[q] = excl[2];
+            q_oc[q] = excl[3];
+        }
+        __syncthreads();
+        if (tid == ACT_THREADS - 1) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) base[k] = excl[k] + v[k];
+        }
+        __syncthreads();
+        const uint32_t stale_here = base[0];
+        for (uint32_t q = 0; q < stale_here; ++q) {  // every thread walks the same list: uniform bounds
+            const uint32_t first = q_first[q], n = q_n[q];
+            const uint32_t nt = (n + TILE_PARTICLES - 1) / TILE_PARTICLES, nb = (n + BT - 1) / BT;
+            for (uint32_t k = tid; k < nt; k += ACT_THREADS)
+                act_tiles[1 + q_ot[q] + k] = static_cast<int>(first / TILE_PARTICLES + k);
+            for (uint32_t k = tid; k < nb; k += ACT_THREADS) act_blocks[1 + q_ob[q] + k] = static_cast<int>(first / BT + k);
+            for (uint32_t k = tid; k < q_ncells[q]; k += ACT_THREADS) act_cells[1 + q_oc[q] + k] = q_cell[q] + k;
+        }
+        __syncthreads();  // the q_* arrays and base[0] are rewritten by the next chunk
+    }
+    if (tid == 0) {
+        act_tiles[0] = static_cast<int>(base[1]);
+        act_blocks[0] = static_cast<int>(base[2]);
+        act_cells[0] = base[3];
+    }
 }
 
 // ---- forces --------------------------------------------------------------------------------------------
@@ -515,7 +629,6 @@ __global__ void __launch_bounds__(BT)
 // Per replica, one wavefront: potential, virial, inside pairs -> scalars[0..2], from the force kernel's partial
 // rows, each covering `unit` particle slots (a tile, or a block of the per-particle kernel). Lane l adds rows
 // l, l + 64, ... of the replica, then a butterfly combines the lanes: a fixed order.
-constexpr int REDUCE_WAVES = 4;
 __global__ void __launch_bounds__(REDUCE_WAVES * WAVE)
     k_b_reduce_forces(const Rep* __restrict__ reps, int count, int unit, const double* __restrict__ partials,
                       double* __restrict__ scalars, const int* halt) {
@@ -614,6 +727,10 @@ struct md_batch {
     int* tile_nr = nullptr;
     uint16_t* rows16 = nullptr;
     int2* tstats = nullptr;
+    // a rebuild's work lists (k_b_activate): the stale replicas' tiles, 256-slot blocks and cell words
+    int* act_tiles = nullptr;
+    int* act_blocks = nullptr;
+    uint32_t* act_cells = nullptr;
     double* ke_part = nullptr;
     double* scalars = nullptr;
     // During md_batch_run: the overflow word. The run's kernels exit at once while it is set, so a replica's rebuild
@@ -663,6 +780,9 @@ void batch_free(md_batch* b) {
     dev_free(b->tile_nr);
     dev_free(b->rows16);
     dev_free(b->tstats);
+    dev_free(b->act_tiles);
+    dev_free(b->act_blocks);
+    dev_free(b->act_cells);
 }
 
 size_t rows16_entries(const md_batch* b) {
@@ -701,6 +821,7 @@ BatchTiles batch_tiles(md_batch* b) {
     bt.ts.per = b->d_tsys;
     bt.ts.rep_of_tile = b->tile_rep;
     bt.ts.gates = b->stale;
+    bt.ts.active = b->act_tiles;
     bt.ts.stride = b->slots;
     bt.ts.tiles = b->tiles;
     bt.tile_ranges = b->tile_ranges;
@@ -752,6 +873,9 @@ void batch_alloc(md_batch* b) {
     b->scalars = dev_alloc<double>(static_cast<size_t>(b->count) * SCALARS);
     b->tiled = tiling_enabled(S);
     b->tiles = static_cast<int>(S / TILE_PARTICLES);
+    b->act_tiles = dev_alloc<int>(1 + static_cast<size_t>(b->tiles));
+    b->act_blocks = dev_alloc<int>(1 + static_cast<size_t>(b->blocks));
+    b->act_cells = dev_alloc<uint32_t>(1 + b->cells_total);
     if (b->tiled) {
         b->d_tsys = dev_alloc<TileSystem>(b->count);
         b->tile_rep = dev_alloc<uint16_t>(b->tiles);
@@ -836,15 +960,18 @@ void grow_rows(md_batch* b, int grown, bool keep) {
 void enqueue_rebuild(md_batch* b) {
     hipStream_t st = b->ctx->stream;
     const size_t S = b->slots;
-    k_b_cell_count<<<b->blocks, BT, 0, st>>>(b->d_reps, b->block_rep, b->pos, b->cell_of, b->counts, b->stale, b->halt);
+    k_b_activate<<<1, ACT_THREADS, 0, st>>>(b->d_reps, b->count, b->stale, b->act_tiles, b->act_blocks, b->act_cells,
+                                            b->halt);
+    k_b_cell_count<<<b->blocks, BT, 0, st>>>(b->d_reps, b->block_rep, b->pos, b->cell_of, b->counts, b->act_blocks,
+                                             b->halt);
     k_b_scan<<<b->count, BT, 0, st>>>(b->d_reps, b->counts, b->start, b->stale, b->pairs, b->halt);
     k_b_scatter<<<b->blocks, BT, 0, st>>>(b->d_reps, b->block_rep, b->cell_of, b->start, b->fill, b->order_raw,
-                                          b->stale, b->halt);
+                                          b->act_blocks, b->halt);
     BatchPermute pa{b->pos, b->vel, b->frc, b->mass, b->id, b->pos_alt, b->vel_alt,
                     b->frc_alt, b->mass_alt, b->id_alt, b->ref, b->cell_of_alt};
-    k_b_sort<<<blocks_for(b->cells_total * WAVE, BT), BT, 0, st>>>(b->d_reps, b->cell_rep, b->cells_total, S,
-                                                                   b->start, b->order_raw, b->fill, b->counts, pa,
-                                                                   b->stale, b->halt);
+    k_b_sort<<<blocks_for(b->cells_total * WAVE, BT), BT, 0, st>>>(b->d_reps, b->cell_rep, S, b->start,
+                                                                   b->order_raw, b->fill, b->counts, pa,
+                                                                   b->act_cells, b->halt);
     if (b->tiled) {
         // the re-sorted arrays go back into place from the build, which reads the positions from pos_alt
         TiledCopyBack cb{};
@@ -861,16 +988,15 @@ void enqueue_rebuild(md_batch* b) {
         cb.id = b->id;
         cb.cell_of = b->cell_of;
         tiled_batch_build_enqueue(st, batch_tiles(b), b->pos_alt, b->cell_of_alt, b->start, b->radius2, cb, b->halt);
-        k_b_build_totals<<<blocks_for(b->count, 256), 256, 0, st>>>(b->d_reps, b->count, TILE_PARTICLES, b->tstats,
-                                                                     b->padding, b->flags, b->pairs, b->stale,
-                                                                     b->halt);
+        k_b_build_totals<<<blocks_for(b->count, REDUCE_WAVES), REDUCE_WAVES * WAVE, 0, st>>>(
+            b->d_reps, b->count, TILE_PARTICLES, b->tstats, b->padding, b->flags, b->pairs, b->stale, b->halt);
     } else {
         BatchCopyBack cb{b->vel_alt, b->frc_alt, b->mass_alt, b->id_alt, b->pos, b->vel, b->frc, b->mass, b->id};
         k_b_list_build<<<b->blocks, BT, 0, st>>>(b->d_reps, b->block_rep, b->pos_alt, S, b->cell_of_alt, b->start,
                                                  b->radius2, b->padding, b->rows, b->rcount, b->half, b->bstats, cb,
                                                  b->stale, b->halt);
-        k_b_build_totals<<<blocks_for(b->count, 256), 256, 0, st>>>(b->d_reps, b->count, BT, b->bstats, b->padding,
-                                                                     b->flags, b->pairs, b->stale, b->halt);
+        k_b_build_totals<<<blocks_for(b->count, REDUCE_WAVES), REDUCE_WAVES * WAVE, 0, st>>>(
+            b->d_reps, b->count, BT, b->bstats, b->padding, b->flags, b->pairs, b->stale, b->halt);
     }
     MD_HIP(hipGetLastError());
 }
@@ -1164,10 +1290,18 @@ int md_batch_run(md_batch* b, const md_batch_config* cfg, double* energies) {
         ~HaltGuard() { b->halt = nullptr; }
     } halt_guard{b};
     b->halt = b->flags;
+    // Bussi runs: each opening kick first applies the previous step's velocity scale (1 before the first step);
+    // the last step's scale is applied after the run.
+    const double* alpha = cfg->thermostat ? b->scalars : nullptr;
+    if (cfg->thermostat) {
+        k_b_unit_alpha<<<blocks_for(count, 256), 256, 0, st>>>(count, b->scalars);
+        MD_HIP(hipGetLastError());
+    }
     auto enqueue_step = [&](bool kick_drift) {
         if (kick_drift)
             k_b_kick_drift<<<b->blocks, BT, 0, st>>>(b->d_reps, b->block_rep, b->pos, b->vel, b->frc, b->mass, S,
-                                                     half_dt, cfg->dt, b->ref, b->max_disp2, b->stale, b->halt);
+                                                     half_dt, cfg->dt, b->ref, b->max_disp2, b->stale, alpha,
+                                                     b->halt);
         enqueue_rebuild(b);
         hipEvent_t e0 = nullptr, e1 = nullptr;
         if (timed) {  // one pair per force launch (a resumed step launches twice)
@@ -1178,14 +1312,11 @@ int md_batch_run(md_batch* b, const md_batch_config* cfg, double* energies) {
         }
         enqueue_forces(b, lj, cfg->precision, lanes, e0, e1);
         if (cfg->thermostat) {
-            k_b_kick<<<b->blocks, BT, 0, st>>>(b->d_reps, b->block_rep, b->vel, b->frc, b->mass, S, half_dt, nullptr,
-                                               b->halt);
-            k_b_ke_blocks<<<b->ke_total, BT, 0, st>>>(b->d_reps, b->block_rep, b->ke_start, b->vel, b->mass, S,
-                                                      b->ke_part, b->halt);
+            k_b_kick_ke<<<b->ke_total, BT, 0, st>>>(b->d_reps, b->block_rep, b->ke_start, b->vel, b->frc, b->mass, S,
+                                                    half_dt, b->ke_part, b->halt);
             k_b_ke_final<<<blocks_for(count, 256), 256, 0, st>>>(b->d_reps, count, b->ke_part, b->scalars, b->halt);
             k_b_bussi<<<blocks_for(count, 256), 256, 0, st>>>(count, b->scalars, rb.draws, steps, rb.draw_index,
                                                               rb.dof_kbar, c, b->halt);
-            k_b_scale<<<b->blocks, BT, 0, st>>>(b->d_reps, b->block_rep, b->vel, S, b->scalars, b->halt);
             k_b_record<<<1, 1024, 0, st>>>(count, b->scalars, 6, rb.energies, b->stale, b->rebuilds, b->flags,
                                            b->halt);
         } else {
@@ -1252,6 +1383,10 @@ int md_batch_run(md_batch* b, const md_batch_config* cfg, double* energies) {
         const int longest = hf[1];
         grow_rows(b, (longest + longest / 4 + 16 + 7) / 8 * 8, true);
         MD_HIP(hipMemsetAsync(b->flags, 0, sizeof(int), st));
+    }
+    if (cfg->thermostat) {  // the last step's velocity scale
+        k_b_scale<<<b->blocks, BT, 0, st>>>(b->d_reps, b->block_rep, b->vel, S, b->scalars, b->halt);
+        MD_HIP(hipGetLastError());
     }
     if (energies)
         MD_HIP(hipMemcpyAsync(energies, rb.energies, sizeof(double) * steps * count * 4, hipMemcpyDeviceToHost, st));

@@ -278,6 +278,8 @@ struct TileSystems {
     const TileSystem* per;
     const uint16_t* rep_of_tile;
     const int* gates;  // batch: per replica, nonzero when its list is rebuilt (the rebuild kernels skip the others)
+    // batch rebuild kernels: active[0] tiles are rebuilt this step, active[1 + k] the k-th of them (replica order)
+    const int* active;
     size_t stride;
     int tiles;
 };

@@ -329,7 +329,8 @@ __global__ void __launch_bounds__(SETUP_TILES * WAVE)
     __shared__ int nr_all[SETUP_TILES];
     __shared__ int flat_all[SETUP_TILES];
     const int w = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
-    const int t = blockIdx.x * SETUP_TILES + w;
+    int t = blockIdx.x * SETUP_TILES + w;
+    if constexpr (BATCH) t = t < uniform_i32(ts.active[0]) ? uniform_i32(ts.active[1 + t]) : ts.tiles;
     const TileSystem sys = tile_system<BATCH>(ts, t < ts.tiles ? t : 0);
     const size_t n = sys.end;  // one past the system's last particle
     // empty tiles (past a replica's end) keep tile_nr = -1 from the batch's set-up
@@ -1158,7 +1159,14 @@ __global__ void __launch_bounds__(TILE_THREADS, 6)
                        const int4* __restrict__ tile_ranges, const int* __restrict__ tile_nr, TiledCopyBack cb,
                        const int* gate, const int* halt) {
     if (gated_off(gate, halt)) return;
-    const int tile = xcd_item(static_cast<int>(blockIdx.x), ts.tiles);
+    int tile;
+    if constexpr (BATCH) {  // the k-th rebuilt tile, k spread over the XCDs in contiguous runs
+        const int rebuilt = uniform_i32(ts.active[0]);
+        if (static_cast<int>(blockIdx.x) >= rebuilt) return;
+        tile = uniform_i32(ts.active[1 + xcd_item(static_cast<int>(blockIdx.x), rebuilt)]);
+    } else {
+        tile = xcd_item(static_cast<int>(blockIdx.x), ts.tiles);
+    }
     const TileSystem sys = tile_system<BATCH>(ts, tile);
     if (tile_gated_off<BATCH>(ts, sys) || static_cast<size_t>(tile) * TILE >= sys.end) return;  // not rebuilt, or empty
     extern __shared__ double lds[];
