@@ -45,6 +45,10 @@ WORKLOADS = {
                "LJ NVE N=2048 (FCC 8^3x4) rho*=0.8442 T*=0.722 seed 1 rc=2.5 skin=0.3 dt=0.005 fp64"),
 }
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "force_kernel_traffic.json")
+# batched workload: the autotuner's candidates (skin sets the rebuild cadence; headroom the list padding as a
+# multiple of the longest row the best skin built)
+TUNE_SKINS = (0.2, 0.3, 0.4, 0.5, 0.6)
+TUNE_HEADROOM = (1.1, 1.3)
 
 
 def log(msg):
@@ -134,9 +138,10 @@ def batched_replicas(count, base_seed=1):
     return mdgpu.batched_replica_specs(count, base_seed)
 
 
-def measure_batched(ctx, replicas, steps, warm, plain_replicas, plain_steps, with_plain=True):
-    """Batched replicas (Bussi tau 0.1, mixed, skin 0.3) in one md_batch (after md_batch_tune), against the same
-    replicas advanced one after another by md_run (each captured as a hipGraph, the fastest plain path)."""
+def measure_batched(ctx, replicas, steps, warm, plain_replicas, plain_steps, with_plain=True, use_graph=True):
+    """Batched replicas (Bussi tau 0.1, mixed) in one md_batch, its skin and padding chosen by md_batch_autotune,
+    against the same replicas advanced one after another by md_run with that skin (each captured as a hipGraph, the
+    fastest plain path)."""
     lj = mdgpu.LJ(1.0, 1.0, 2.5)
     specs = batched_replicas(replicas)
     t_setup = time.perf_counter()
@@ -150,16 +155,24 @@ def measure_batched(ctx, replicas, steps, warm, plain_replicas, plain_steps, wit
     batch = mdgpu.Batch(ctx, states, 2.5, 0.3)
     del states
     log(f"batched setup: {len(specs)} replicas, N_total={n_total} in {time.perf_counter() - t_setup:.1f} s")
-    best, tune_ms = batch.tune(lj, 0.005, max(warm, 1), min(20, max(warm, 1)), precision=mdgpu.MD_MIXED,
-                               temperatures=temps, draws=draws_w)
-    log(f"autotuner: lanes {best}, ms/step for 1/2/4 lanes: {np.round(tune_ms, 3).tolist()}")
     if warm:
-        batch.run(lj, 0.005, warm, precision=mdgpu.MD_MIXED, temperatures=temps, draws=draws_w, use_graph=True,
-                  record=False)
+        batch.run(lj, 0.005, warm, precision=mdgpu.MD_MIXED, temperatures=temps, draws=draws_w,
+                  use_graph=use_graph, record=False)
+    # the autotuner, from the warmed-up state: rebuild cadence (skin), then list padding for the best skin (trials
+    # of 40 steps, each from that same state on a scratch copy); the batch then takes the fastest setting
+    tune_steps = 40
+    draws_t = np.stack([rngs[r].draws(n_list[r], tune_steps) for r in range(len(specs))])
+    tuned = batch.autotune(lj, 0.005, tune_steps, tune_steps, skins=TUNE_SKINS, headroom=TUNE_HEADROOM,
+                           tune_lanes=True, precision=mdgpu.MD_MIXED, temperatures=temps, draws=draws_t,
+                           use_graph=use_graph)
+    log("autotuner: " + ", ".join(f"skin {t['skin']:.2f} pad {t['padding']} lanes {t['lanes']}: "
+                                  f"{t['ms_per_step']:.3f} ms" for t in tuned["trials"]) +
+        f" -> skin {tuned['skin']:.2f}, padding {tuned['padding']}")
     ctx.synchronize()
     reb0 = batch.info().rebuilds
     t0 = time.perf_counter()
-    en = batch.run(lj, 0.005, steps, precision=mdgpu.MD_MIXED, temperatures=temps, draws=draws, use_graph=True)
+    en = batch.run(lj, 0.005, steps, precision=mdgpu.MD_MIXED, temperatures=temps, draws=draws,
+                   use_graph=use_graph)
     ctx.synchronize()
     elapsed = time.perf_counter() - t0
     info = batch.info()
@@ -172,11 +185,14 @@ def measure_batched(ctx, replicas, steps, warm, plain_replicas, plain_steps, wit
     batch.run(lj, 0.005, k_steps, precision=mdgpu.MD_MIXED, temperatures=temps, draws=draws_k, timing=timing,
               record=False)
     k_full = 2.0 * info.half_pairs / n_total
-    bytes_per_atom = 32.0 + 8.0 + 4.0 * k_full + 24.0
+    if info.tiled:  # own position, row length, slot row (2 B per entry), force
+        bytes_per_atom, kernel = 32.0 + 4.0 + 2.0 * k_full + 24.0, "k_forces_tiled_mixed<false, true>"
+    else:  # own position, row and half lengths, int32 row, force
+        bytes_per_atom, kernel = 32.0 + 8.0 + 4.0 * k_full + 24.0, f"k_b_forces<true, {info.lanes}>"
     avg_ms = timing[0] / k_steps
     achieved = bytes_per_atom * n_total / (avg_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_b_forces",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": kernel,
                 "avg_launch_ms": round(avg_ms, 5), "algorithmic_bytes_per_launch": round(bytes_per_atom * n_total)}
     mean_t = float(np.mean(2.0 * en[-1, :, 1] / (3.0 * np.array(n_list) - 3.0) / temps))
     del batch
@@ -188,7 +204,7 @@ def measure_batched(ctx, replicas, steps, warm, plain_replicas, plain_steps, wit
         plain_states = [mdgpu.fcc_system(ctx, c, rho, t, seed) for c, rho, t, seed in sample]
         lists = []
         for st in plain_states:
-            nl = mdgpu.NeighbourList(ctx, 2.5, 0.3)
+            nl = mdgpu.NeighbourList(ctx, 2.5, tuned["skin"])  # the skin the batch runs with
             nl.build(st)
             nl.forces(st, lj, mdgpu.MD_MIXED)
             lists.append(nl)
@@ -205,11 +221,13 @@ def measure_batched(ctx, replicas, steps, warm, plain_replicas, plain_steps, wit
         log(f"plain replica-by-replica: {len(sample)} replicas x {p_steps} steps in {p_el:.3f} s: {plain:.4g}")
     return {
         "value": value, "steps": steps, "warmup": warm, "ms_per_step": 1000.0 * elapsed / steps,
-        "replicas": len(specs), "N_total": n_total, "lanes": best, "roofline": roofline,
+        "replicas": len(specs), "N_total": n_total, "skin": tuned["skin"], "padding": tuned["padding"],
+        "particles_per_wavefront": 16 if info.tiled else 64 // info.lanes, "roofline": roofline,
         "plain_replica_by_replica": {"value": plain, "unit": "atom-steps/s", "replicas": len(sample),
                                      "steps": p_steps, "path": "md_run per replica, hipGraph"},
         "speedup_vs_plain": value / plain if plain else None,
-        "autotuner_ms_per_step": {"lanes1": tune_ms[0], "lanes2": tune_ms[1], "lanes4": tune_ms[2]},
+        "autotuner": {"trial_steps": tune_steps, "trials": [{k: (round(v, 4) if isinstance(v, float) else v)
+                                                            for k, v in t.items()} for t in tuned["trials"]]},
         "rebuilds": int(info.rebuilds - reb0), "k_full_mean": k_full,
         "mean_final_temperature_over_target": mean_t,
     }
@@ -218,7 +236,7 @@ def measure_batched(ctx, replicas, steps, warm, plain_replicas, plain_steps, wit
 def run_batched(args):
     ctx = mdgpu.Context(0)
     m = measure_batched(ctx, args.replicas, args.steps, args.warmup, args.plain_replicas, args.plain_steps,
-                        not args.no_plain)
+                        not args.no_plain, not args.eager)
     out = {
         "metric": "batched atom-steps/s (512 Bussi replicas, 1 GPU)" if args.replicas == 512
         else "batched atom-steps/s (1 GPU)",
@@ -227,11 +245,12 @@ def run_batched(args):
         "dtype": "f32 pair maths, f64 positions/accumulation",
         "data": "synthetic: per-replica FCC lattice + Maxwell-Boltzmann velocities generated by the library",
         "config": {"workload": f"{m['replicas']} LJ replicas, FCC n~U{{5..20}}, rho~U[0.70,0.90], T~U[0.7,1.5], "
-                               f"Bussi tau=0.1, seed 1+r, rc=2.5 skin=0.3 dt=0.005 mixed",
-                   "replicas": m["replicas"], "N_total": m["N_total"], "lanes": m["lanes"], "graph": True},
+                               f"Bussi tau=0.1, seed 1+r, rc=2.5 skin={m['skin']:g} (autotuned) dt=0.005 mixed",
+                   "replicas": m["replicas"], "N_total": m["N_total"], "skin": m["skin"], "graph": not args.eager},
         "roofline": m["roofline"], "cpu_baseline": None,
     }
-    out.update({k: m[k] for k in ("plain_replica_by_replica", "speedup_vs_plain", "autotuner_ms_per_step",
+    out.update({k: m[k] for k in ("plain_replica_by_replica", "speedup_vs_plain", "autotuner", "padding",
+                                  "particles_per_wavefront",
                                   "rebuilds", "k_full_mean", "mean_final_temperature_over_target")})
     print(json.dumps(out), flush=True)
 
@@ -346,6 +365,8 @@ def main():
     ap.add_argument("--replicas", type=int, default=512, help="batched workload: replica count")
     ap.add_argument("--plain-replicas", type=int, default=32, help="batched workload: replicas in the plain sample")
     ap.add_argument("--plain-steps", type=int, default=200, help="batched workload: steps of the plain sample")
+    ap.add_argument("--eager", action="store_true",
+                    help="batched workload: launch each step's kernels eagerly instead of replaying a hipGraph")
     ap.add_argument("--no-plain", action="store_true",
                     help="batched / adjoint workloads: skip the plain md_run comparison")
     ap.add_argument("--cpu-seconds", type=float, default=8.0, help="CPU baseline: seconds per thread count")
@@ -462,8 +483,9 @@ def main():
         d, b, a = extras["drift"], extras["batched"], extras["adjoint"]
         out["drift"] = d
         out["batched_vs_plain"] = b["speedup_vs_plain"]
-        out["batched"] = {k: b[k] for k in ("value", "steps", "ms_per_step", "replicas", "N_total", "lanes",
-                                            "plain_replica_by_replica", "autotuner_ms_per_step", "roofline")}
+        out["batched"] = {k: b[k] for k in ("value", "steps", "ms_per_step", "replicas", "N_total", "skin",
+                                            "padding", "particles_per_wavefront", "plain_replica_by_replica",
+                                            "autotuner", "roofline")}
         out["adjoint_over_forward"] = a["adjoint_over_forward"]
         out["adjoint"] = {k: a[k] for k in ("N", "steps", "checkpoint_every", "forward_ms", "reverse_ms",
                                             "plain_md_run_ms", "reverse_over_plain_forward")}

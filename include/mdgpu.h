@@ -303,6 +303,8 @@ typedef struct md_batch_info {
     int lanes;                   /* tuned lanes per particle of the mixed force kernel */
     long long rebuilds;          /* list rebuilds summed over replicas, including the initial ones */
     size_t half_pairs;           /* half-list pairs summed over replicas, as of each replica's last build */
+    double skin;                 /* list radius = cutoff + skin (md_batch_set_skin, md_batch_autotune) */
+    int tiled;                   /* 1: the LDS-tiled list build and force kernels (16 particles per wavefront) */
 } md_batch_info;
 
 /* Copies the current state of each states[r] (which stays owned by the caller) into one batch, builds every
@@ -322,6 +324,46 @@ int md_batch_replica(md_batch* b, int r, md_state* dst, int* rebuilds, size_t* h
    batch itself does not move), keeps the fastest as the batch's lanes, and writes the three timings (ms per
    step) to ms_per_step[3] (may be NULL). MIXED only. */
 int md_batch_tune(md_batch* b, const md_batch_config* cfg, int trial_steps, double* ms_per_step, int* best_lanes);
+
+/* A new list radius cutoff + skin for every replica (the rebuild cadence: a replica rebuilds once a particle has
+   moved skin / 2): new cell grids and rows of `padding` entries (0 = the density estimate for that radius), then
+   every replica re-sorted and rebuilt at its current state; the next md_batch_run starts with a force evaluation
+   over the new rows. Fails with the reference's message if some box is shorter than twice cutoff + skin. No counterpart in the reference (its skin is fixed at construction,
+   NeighbourList.hpp:15). */
+int md_batch_set_skin(md_batch* b, double skin, int padding);
+
+#define MD_TUNE_MAX_TRIALS 32
+typedef struct md_batch_tune_space {
+    const double* skins;     /* candidate skins (rebuild cadence); NULL = the batch's own */
+    int n_skins;
+    const double* headroom;  /* candidate paddings as multiples (>= 1) of the longest row the best skin built;
+                                NULL = the density estimate */
+    int n_headroom;
+    int tune_lanes;          /* also try 1, 2 and 4 lanes per particle (the per-particle kernel of untiled
+                                batches; the tiled kernels work 16 particles per wavefront, 4 lanes each) */
+} md_batch_tune_space;
+
+typedef struct md_batch_tune_result {
+    double skin;             /* chosen, and now the batch's */
+    int padding;             /* the batch's padding after tuning */
+    int lanes;               /* lanes per particle of the force kernel the batch runs */
+    double ms_per_step;      /* the chosen setting's trial */
+    int trials;
+    double trial_skin[MD_TUNE_MAX_TRIALS];
+    int trial_padding[MD_TUNE_MAX_TRIALS];
+    int trial_lanes[MD_TUNE_MAX_TRIALS];
+    double trial_ms[MD_TUNE_MAX_TRIALS];  /* ms per step, wall clock over trial_steps steps of cfg */
+} md_batch_tune_result;
+
+/* Autotuner over rebuild cadence, list padding and (untiled) lanes per particle: each trial copies the batch's
+   state into a scratch batch, lays it out for the candidate (md_batch_set_skin), and times `trial_steps` steps of
+   cfg (any precision, NVE or Bussi); skins first, then paddings for the best skin, then lanes. The batch itself
+   then takes the fastest setting (one re-sort and rebuild at its current state). FP64 forces over any list radius
+   are the same sums (extra list entries beyond the cut-off add nothing), so a tuned FP64 run differs from an
+   untuned one only through the order the re-sorts leave the particles in (rounding level). No counterpart in
+   the reference. */
+int md_batch_autotune(md_batch* b, const md_batch_config* cfg, int trial_steps, const md_batch_tune_space* space,
+                      md_batch_tune_result* out);
 
 #ifdef __cplusplus
 }

@@ -1062,6 +1062,141 @@ void batch_build(md_batch* b) {
     MD_HIP(hipGetLastError());
 }
 
+// A new list radius (cutoff + skin) for every replica: new cell grids, cell words and rows (padding rows per
+// particle; <= 0: the density estimate for the new radius), then every replica re-sorted and rebuilt from its
+// current state. The particle slots stay where they are, so positions, velocities, masses and ids carry over; only
+// the order within each replica changes, as at any rebuild, and the next run starts with a force evaluation.
+void relayout(md_batch* b, double skin, int padding) {
+    hipStream_t st = b->ctx->stream;
+    if (!(skin >= 0.0)) fail(MD_ERR_ARG, "skin must be >= 0");
+    const double radius = b->cutoff + skin;
+    for (const Rep& rp : b->reps)
+        if (2.0 * radius > rp.box) fail(MD_ERR_RUNTIME, "Cutoff + skin must be less than half the box length");
+    MD_HIP(hipStreamSynchronize(st));
+    size_t cell = 0;
+    int pad = 8;
+    for (Rep& rp : b->reps) {
+        rp.side = grid_side_of(rp.box, radius);
+        rp.use_cells = rp.side >= 3 ? 1 : 0;
+        rp.ncells = static_cast<uint32_t>(rp.side) * rp.side * rp.side;
+        rp.cell_first = static_cast<uint32_t>(cell);
+        rp.per_length = rp.side / rp.box;
+        cell += rp.ncells + 1;
+        pad = std::max(pad, padding_for(rp.n, rp.box, radius));
+        if (cell > 0xffffffffu) fail(MD_ERR_ARG, "batch too large for 32-bit cell words");
+    }
+    b->skin = skin;
+    b->radius2 = radius * radius;
+    b->max_disp2 = 0.25 * skin * skin;
+    b->cells_total = cell;
+    b->padding = padding > 0 ? (padding + 7) / 8 * 8 : pad;
+    dev_free(b->counts);
+    dev_free(b->start);
+    dev_free(b->fill);
+    dev_free(b->cell_rep);
+    dev_free(b->act_cells);
+    dev_free(b->rows);
+    dev_free(b->rows16);
+    b->counts = dev_alloc<uint32_t>(b->cells_total);
+    b->start = dev_alloc<uint32_t>(b->cells_total);
+    b->fill = dev_alloc<uint32_t>(b->cells_total);
+    b->cell_rep = dev_alloc<uint16_t>(b->cells_total);
+    b->act_cells = dev_alloc<uint32_t>(1 + b->cells_total);
+    b->rows = dev_alloc<int32_t>(static_cast<size_t>(b->padding) * b->slots);
+    if (b->tiled) b->rows16 = dev_alloc<uint16_t>(rows16_entries(b));
+    MD_HIP(hipMemsetAsync(b->counts, 0, sizeof(uint32_t) * b->cells_total, st));
+    MD_HIP(hipMemsetAsync(b->fill, 0, sizeof(uint32_t) * b->cells_total, st));
+    std::vector<uint16_t> crep(b->cells_total);
+    for (int r = 0; r < b->count; ++r) {
+        const Rep& rp = b->reps[r];
+        for (uint32_t c = 0; c <= rp.ncells; ++c) crep[rp.cell_first + c] = static_cast<uint16_t>(r);
+    }
+    MD_HIP(hipMemcpyAsync(b->d_reps, b->reps.data(), sizeof(Rep) * b->count, hipMemcpyHostToDevice, st));
+    MD_HIP(hipMemcpyAsync(b->cell_rep, crep.data(), sizeof(uint16_t) * crep.size(), hipMemcpyHostToDevice, st));
+    MD_HIP(hipStreamSynchronize(st));  // crep goes out of scope
+    if (b->tiled) upload_tile_systems(b, b->tsys_cutoff > 0.0 ? b->tsys_cutoff : b->cutoff);
+    k_b_fill<<<blocks_for(b->count, 256), 256, 0, st>>>(b->stale, b->count, 1);
+    MD_HIP(hipGetLastError());
+    batch_build(b);
+    MD_HIP(hipStreamSynchronize(st));
+    // the next run evaluates the forces over the new rows first (their summation order follows the new sort)
+    b->forces_valid = false;
+}
+
+// A scratch batch with b's layout and contents (the autotuner's trials run on it, so b does not move).
+void scratch_copy(const md_batch* b, md_batch& tmp) {
+    hipStream_t st = b->ctx->stream;
+    tmp.ctx = b->ctx;
+    tmp.count = b->count;
+    tmp.slots = b->slots;
+    tmp.particles = b->particles;
+    tmp.cells_total = b->cells_total;
+    tmp.blocks = b->blocks;
+    tmp.ke_total = b->ke_total;
+    tmp.cutoff = b->cutoff;
+    tmp.skin = b->skin;
+    tmp.radius2 = b->radius2;
+    tmp.max_disp2 = b->max_disp2;
+    tmp.padding = b->padding;
+    tmp.lanes = b->lanes;
+    tmp.forces_valid = b->forces_valid;
+    tmp.reps = b->reps;
+    batch_alloc(&tmp);
+    const size_t S = b->slots;
+    auto copy = [&](void* dst, const void* src, size_t bytes) {
+        MD_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, st));
+    };
+    copy(tmp.pos, b->pos, sizeof(double4) * S);
+    copy(tmp.ref, b->ref, sizeof(double4) * S);
+    copy(tmp.vel, b->vel, sizeof(double) * 3 * S);
+    copy(tmp.frc, b->frc, sizeof(double) * 3 * S);
+    copy(tmp.mass, b->mass, sizeof(double) * S);
+    copy(tmp.id, b->id, sizeof(uint64_t) * S);
+    copy(tmp.cell_of, b->cell_of, sizeof(int) * S);
+    copy(tmp.start, b->start, sizeof(uint32_t) * b->cells_total);
+    copy(tmp.rows, b->rows, sizeof(int32_t) * b->padding * S);
+    copy(tmp.rcount, b->rcount, sizeof(int32_t) * S);
+    copy(tmp.half, b->half, sizeof(int32_t) * S);
+    copy(tmp.pairs, b->pairs, sizeof(unsigned long long) * b->count);
+    if (b->tiled) {
+        copy(tmp.tile_ranges, b->tile_ranges, sizeof(int4) * b->tiles * TILE_RANGES);
+        copy(tmp.tile_nr, b->tile_nr, sizeof(int) * b->tiles);
+        copy(tmp.rows16, b->rows16, sizeof(uint16_t) * rows16_entries(b));
+    }
+    MD_HIP(hipStreamSynchronize(st));
+}
+
+// Puts b's particle state (and force validity) back into the scratch batch before a trial, so every trial starts
+// from the same state; the trial's relayout then re-sorts and rebuilds it.
+void reset_particles(const md_batch* b, md_batch& tmp) {
+    hipStream_t st = b->ctx->stream;
+    const size_t S = b->slots;
+    MD_HIP(hipMemcpyAsync(tmp.pos, b->pos, sizeof(double4) * S, hipMemcpyDeviceToDevice, st));
+    MD_HIP(hipMemcpyAsync(tmp.vel, b->vel, sizeof(double) * 3 * S, hipMemcpyDeviceToDevice, st));
+    MD_HIP(hipMemcpyAsync(tmp.frc, b->frc, sizeof(double) * 3 * S, hipMemcpyDeviceToDevice, st));
+    MD_HIP(hipMemcpyAsync(tmp.mass, b->mass, sizeof(double) * S, hipMemcpyDeviceToDevice, st));
+    MD_HIP(hipMemcpyAsync(tmp.id, b->id, sizeof(uint64_t) * S, hipMemcpyDeviceToDevice, st));
+    MD_HIP(hipStreamSynchronize(st));
+    tmp.forces_valid = b->forces_valid;
+}
+
+// A trial's configuration: cfg over its first `steps` steps (each replica's Bussi draws taken from the start of its
+// own [cfg->steps][2] block, so the trial's draws are laid out [count][steps][2] as md_batch_run reads them).
+md_batch_config trial_config(const md_batch_config* cfg, int count, int steps, int lanes, std::vector<double>& draws) {
+    md_batch_config c = *cfg;
+    c.steps = steps;
+    c.lanes = lanes;
+    c.force_kernel_ms = nullptr;
+    if (cfg->thermostat && cfg->draws) {
+        draws.resize(static_cast<size_t>(count) * steps * 2);
+        for (int r = 0; r < count; ++r)
+            std::memcpy(draws.data() + static_cast<size_t>(r) * steps * 2,
+                        cfg->draws + static_cast<size_t>(r) * cfg->steps * 2, sizeof(double) * steps * 2);
+        c.draws = draws.data();
+    }
+    return c;
+}
+
 }  // namespace
 }  // namespace mdgpu
 
@@ -1183,6 +1318,8 @@ int md_batch_get_info(const md_batch* b, md_batch_info* info) {
     info->lanes = b->lanes;
     info->rebuilds = 0;
     info->half_pairs = 0;
+    info->skin = b->skin;
+    info->tiled = b->tiled ? 1 : 0;
     for (int r = 0; r < b->count; ++r) {
         info->rebuilds += reb[r];
         info->half_pairs += pairs[r];
@@ -1408,59 +1545,19 @@ int md_batch_tune(md_batch* b, const md_batch_config* cfg, int trial_steps, doub
     if (!b || !cfg) fail(MD_ERR_ARG, "batch/config is NULL");
     if (cfg->precision != MD_MIXED) fail(MD_ERR_ARG, "the autotuner picks lanes for the mixed-precision kernel");
     if (trial_steps <= 0 || trial_steps > cfg->steps) fail(MD_ERR_ARG, "trial_steps must be in 1..cfg->steps");
-    hipStream_t st = b->ctx->stream;
     // a scratch copy of the batch, so that the trials leave the batch itself where it is
     md_batch tmp;
-    tmp.ctx = b->ctx;
-    tmp.count = b->count;
-    tmp.slots = b->slots;
-    tmp.particles = b->particles;
-    tmp.cells_total = b->cells_total;
-    tmp.blocks = b->blocks;
-    tmp.ke_total = b->ke_total;
-    tmp.cutoff = b->cutoff;
-    tmp.skin = b->skin;
-    tmp.radius2 = b->radius2;
-    tmp.max_disp2 = b->max_disp2;
-    tmp.padding = b->padding;
-    tmp.lanes = b->lanes;
-    tmp.forces_valid = b->forces_valid;
-    tmp.reps = b->reps;
     struct Guard {
         md_batch* t;
         ~Guard() { batch_free(t); }
     } guard{&tmp};
-    batch_alloc(&tmp);
-    const size_t S = b->slots;
-    auto copy = [&](void* dst, const void* src, size_t bytes) {
-        MD_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, st));
-    };
-    copy(tmp.pos, b->pos, sizeof(double4) * S);
-    copy(tmp.ref, b->ref, sizeof(double4) * S);
-    copy(tmp.vel, b->vel, sizeof(double) * 3 * S);
-    copy(tmp.frc, b->frc, sizeof(double) * 3 * S);
-    copy(tmp.mass, b->mass, sizeof(double) * S);
-    copy(tmp.id, b->id, sizeof(uint64_t) * S);
-    copy(tmp.cell_of, b->cell_of, sizeof(int) * S);
-    copy(tmp.start, b->start, sizeof(uint32_t) * b->cells_total);
-    copy(tmp.rows, b->rows, sizeof(int32_t) * b->padding * S);
-    copy(tmp.rcount, b->rcount, sizeof(int32_t) * S);
-    copy(tmp.half, b->half, sizeof(int32_t) * S);
-    copy(tmp.pairs, b->pairs, sizeof(unsigned long long) * b->count);
-    if (b->tiled) {
-        copy(tmp.tile_ranges, b->tile_ranges, sizeof(int4) * b->tiles * TILE_RANGES);
-        copy(tmp.tile_nr, b->tile_nr, sizeof(int) * b->tiles);
-        copy(tmp.rows16, b->rows16, sizeof(uint16_t) * rows16_entries(b));
-    }
-    MD_HIP(hipStreamSynchronize(st));
+    scratch_copy(b, tmp);
     const int options[3] = {1, 2, 4};
     double best = 0.0;
     int best_l = b->lanes;
+    std::vector<double> draws;
     for (int k = 0; k < 3; ++k) {
-        md_batch_config c = *cfg;
-        c.steps = trial_steps;
-        c.lanes = options[k];
-        c.force_kernel_ms = nullptr;
+        const md_batch_config c = trial_config(cfg, b->count, trial_steps, options[k], draws);
         const auto t0 = std::chrono::steady_clock::now();
         const int rc = md_batch_run(&tmp, &c, nullptr);
         if (rc != MD_OK) fail(rc, std::string("autotuner trial failed: ") + md_last_error());
@@ -1474,6 +1571,120 @@ int md_batch_tune(md_batch* b, const md_batch_config* cfg, int trial_steps, doub
     }
     b->lanes = best_l;
     if (best_lanes) *best_lanes = best_l;
+    MD_API_END
+}
+
+int md_batch_autotune(md_batch* b, const md_batch_config* cfg, int trial_steps, const md_batch_tune_space* space,
+                      md_batch_tune_result* out) {
+    MD_API_BEGIN
+    if (!b || !cfg || !out) fail(MD_ERR_ARG, "batch/config/result is NULL");
+    if (trial_steps <= 0 || trial_steps > cfg->steps) fail(MD_ERR_ARG, "trial_steps must be in 1..cfg->steps");
+    const int n_skins = space && space->skins ? space->n_skins : 0;
+    const int n_head = space && space->headroom ? space->n_headroom : 0;
+    const bool try_lanes = space && space->tune_lanes && !b->tiled && cfg->precision == MD_MIXED;
+    if (n_skins < 0 || n_head < 0) fail(MD_ERR_ARG, "negative candidate count");
+    if (std::max(n_skins, 1) + n_head + (try_lanes ? 3 : 0) > MD_TUNE_MAX_TRIALS)
+        fail(MD_ERR_ARG, "too many autotuner trials");
+    for (int k = 0; k < n_skins; ++k) {
+        const double s = space->skins[k];
+        if (!(s >= 0.0)) fail(MD_ERR_ARG, "candidate skins must be >= 0");
+        for (const Rep& rp : b->reps)
+            if (2.0 * (b->cutoff + s) > rp.box)
+                fail(MD_ERR_RUNTIME, "Cutoff + skin must be less than half the box length");
+    }
+    for (int k = 0; k < n_head; ++k)
+        if (!(space->headroom[k] >= 1.0)) fail(MD_ERR_ARG, "padding headroom must be >= 1");
+    std::memset(out, 0, sizeof(*out));
+    md_batch tmp;
+    struct Guard {
+        md_batch* t;
+        ~Guard() { batch_free(t); }
+    } guard{&tmp};
+    scratch_copy(b, tmp);
+    hipStream_t st = b->ctx->stream;
+    std::vector<double> draws;
+    // one trial: b's state into the scratch batch, laid out for (skin, padding), then trial_steps timed steps
+    // (the relayout's re-sort and rebuild are not timed); longest: the longest row built during the trial
+    auto trial = [&](double skin, int padding, int lanes, int* longest) {
+        reset_particles(b, tmp);
+        relayout(&tmp, skin, padding);
+        const md_batch_config c = trial_config(cfg, b->count, trial_steps, lanes, draws);
+        const auto t0 = std::chrono::steady_clock::now();
+        const int rc = md_batch_run(&tmp, &c, nullptr);
+        if (rc != MD_OK) fail(rc, std::string("autotuner trial failed: ") + md_last_error());
+        const double ms =
+            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / trial_steps;
+        if (longest) {
+            int f[2] = {0, 0};
+            MD_HIP(hipMemcpy(f, tmp.flags, sizeof(f), hipMemcpyDeviceToHost));
+            *longest = std::max(f[1], 1);
+        }
+        if (out->trials < MD_TUNE_MAX_TRIALS) {
+            out->trial_skin[out->trials] = skin;
+            out->trial_padding[out->trials] = tmp.padding;
+            out->trial_lanes[out->trials] = lanes;
+            out->trial_ms[out->trials] = ms;
+            ++out->trials;
+        }
+        return ms;
+    };
+    const int lanes0 = b->lanes;
+    // rebuild cadence: the list radius cutoff + skin, each skin with the density estimate of its padding
+    double best_skin = b->skin, best_ms = 0.0;
+    int longest = 0;
+    if (n_skins == 0) {
+        best_ms = trial(b->skin, b->padding, lanes0, &longest);
+    } else {
+        for (int k = 0; k < n_skins; ++k) {
+            int lk = 0;
+            const double ms = trial(space->skins[k], 0, lanes0, &lk);
+            if (k == 0 || ms < best_ms) {
+                best_ms = ms;
+                best_skin = space->skins[k];
+                longest = lk;
+            }
+        }
+    }
+    // list padding: multiples of the longest row the chosen skin built (a row that outgrows it costs a halt,
+    // regrowth and resume inside the run, which the trial's time includes)
+    int best_pad = 0;  // 0: the density estimate the skin trials used
+    for (int k = 0; k < n_head; ++k) {
+        const int pad = static_cast<int>(std::ceil(longest * space->headroom[k]));
+        const double ms = trial(best_skin, pad, lanes0, nullptr);
+        if (ms < best_ms) {
+            best_ms = ms;
+            best_pad = pad;
+        }
+    }
+    // particles per wavefront of the per-particle force kernel (1, 2 or 4 lanes each)
+    int best_lanes = lanes0;
+    if (try_lanes) {
+        const int options[3] = {1, 2, 4};
+        for (int k = 0; k < 3; ++k) {
+            const double ms = trial(best_skin, best_pad, options[k], nullptr);
+            if (ms < best_ms) {
+                best_ms = ms;
+                best_lanes = options[k];
+            }
+        }
+    }
+    // the batch itself takes the chosen setting (re-sorted and rebuilt at its current state)
+    const int want_pad = best_pad > 0 ? (best_pad + 7) / 8 * 8 : 0;
+    if (best_skin != b->skin || (want_pad > 0 && want_pad != b->padding)) relayout(b, best_skin, want_pad);
+    b->lanes = best_lanes;
+    MD_HIP(hipStreamSynchronize(st));
+    out->skin = best_skin;
+    out->padding = b->padding;
+    out->lanes = b->tiled ? 4 : best_lanes;
+    out->ms_per_step = best_ms;
+    MD_API_END
+}
+
+int md_batch_set_skin(md_batch* b, double skin, int padding) {
+    MD_API_BEGIN
+    if (!b) fail(MD_ERR_ARG, "batch is NULL");
+    if (padding < 0) fail(MD_ERR_ARG, "padding must be >= 0");
+    relayout(b, skin, padding);
     MD_API_END
 }
 

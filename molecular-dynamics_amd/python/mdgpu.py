@@ -58,7 +58,23 @@ class BatchConfig(C.Structure):
 
 class BatchInfo(C.Structure):
     _fields_ = [("count", C.c_int), ("particles", C.c_size_t), ("slots", C.c_size_t), ("padding", C.c_int),
-                ("lanes", C.c_int), ("rebuilds", C.c_longlong), ("half_pairs", C.c_size_t)]
+                ("lanes", C.c_int), ("rebuilds", C.c_longlong), ("half_pairs", C.c_size_t), ("skin", C.c_double),
+                ("tiled", C.c_int)]
+
+
+TUNE_MAX_TRIALS = 32
+
+
+class TuneSpace(C.Structure):
+    _fields_ = [("skins", C.POINTER(C.c_double)), ("n_skins", C.c_int), ("headroom", C.POINTER(C.c_double)),
+                ("n_headroom", C.c_int), ("tune_lanes", C.c_int)]
+
+
+class TuneResult(C.Structure):
+    _fields_ = [("skin", C.c_double), ("padding", C.c_int), ("lanes", C.c_int), ("ms_per_step", C.c_double),
+                ("trials", C.c_int), ("trial_skin", C.c_double * TUNE_MAX_TRIALS),
+                ("trial_padding", C.c_int * TUNE_MAX_TRIALS), ("trial_lanes", C.c_int * TUNE_MAX_TRIALS),
+                ("trial_ms", C.c_double * TUNE_MAX_TRIALS)]
 
 
 class AdjointConfig(C.Structure):
@@ -127,6 +143,9 @@ _SIGS = {
     "md_batch_get_info": (C.c_int, [_P, C.POINTER(BatchInfo)]),
     "md_batch_replica": (C.c_int, [_P, C.c_int, _P, _IP, C.POINTER(C.c_size_t)]),
     "md_batch_tune": (C.c_int, [_P, C.POINTER(BatchConfig), C.c_int, _DP, _IP]),
+    "md_batch_autotune": (C.c_int, [_P, C.POINTER(BatchConfig), C.c_int, C.POINTER(TuneSpace),
+                                    C.POINTER(TuneResult)]),
+    "md_batch_set_skin": (C.c_int, [_P, C.c_double, C.c_int]),
     "md_frames_create": (C.c_int, [_P, C.c_size_t, C.c_int, _PP]),
     "md_frames_destroy": (C.c_int, [_P]),
     "md_frames_capture": (C.c_int, [_P, _P, _IP]),
@@ -561,6 +580,32 @@ class Batch:
         check(lib().md_batch_tune(self.h, C.byref(cfg), trial_steps, ms.ctypes.data_as(_DP), C.byref(best)))
         del keep
         return best.value, ms
+
+    def autotune(self, lj: LJ, dt: float, steps: int, trial_steps: int, skins=None, headroom=None,
+                 tune_lanes=False, precision=MD_MIXED, temperatures=None, draws=None, tau=0.1, use_graph=True):
+        """md_batch_autotune: the batch takes the fastest (skin, padding, lanes); returns a dict with the choice
+        and every trial (skin, padding, lanes, ms per step)."""
+        cfg, keep = self._config(lj, dt, steps, precision, temperatures, draws, tau, use_graph, 0)
+        sp = TuneSpace()
+        sk = hr = None
+        if skins is not None:
+            sk = np.ascontiguousarray(skins, dtype=np.float64)
+            sp.skins, sp.n_skins = sk.ctypes.data_as(_DP), len(sk)
+        if headroom is not None:
+            hr = np.ascontiguousarray(headroom, dtype=np.float64)
+            sp.headroom, sp.n_headroom = hr.ctypes.data_as(_DP), len(hr)
+        sp.tune_lanes = 1 if tune_lanes else 0
+        out = TuneResult()
+        check(lib().md_batch_autotune(self.h, C.byref(cfg), trial_steps, C.byref(sp), C.byref(out)))
+        del keep, sk, hr
+        trials = [{"skin": out.trial_skin[k], "padding": out.trial_padding[k], "lanes": out.trial_lanes[k],
+                   "ms_per_step": out.trial_ms[k]} for k in range(out.trials)]
+        return {"skin": out.skin, "padding": out.padding, "lanes": out.lanes, "ms_per_step": out.ms_per_step,
+                "trials": trials}
+
+    def set_skin(self, skin: float, padding: int = 0):
+        """md_batch_set_skin: new list radius cutoff + skin for every replica, re-sorted and rebuilt."""
+        check(lib().md_batch_set_skin(self.h, skin, padding))
 
     def info(self) -> BatchInfo:
         out = BatchInfo()
