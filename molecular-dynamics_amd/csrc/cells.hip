@@ -435,8 +435,10 @@ void list_prepare(md_list* l, md_state* s) {
     if (tiles > l->tiles_cap || !l->tile_nr) {
         dev_free(l->tile_ranges);
         dev_free(l->tile_nr);
+        dev_free(l->tile_order);
         l->tile_ranges = dev_alloc<int4>(tiles * TILE_RANGES);
         l->tile_nr = dev_alloc<int>(tiles);
+        l->tile_order = dev_alloc<int>(tiles);
         l->tiles_cap = tiles;
     }
     l->hmax = tile_halo_capacity(s->n, ncells);
@@ -754,26 +756,34 @@ void list_enqueue_conditional_rebuild_fixed(md_list* l, md_state* s) {
     list_fixed_enqueue(l, s, l->flags + FLAG_STALE);
 }
 
+// The device buffers of a list, each applied to f.
+template <class F>
+static void for_list_buffers(md_list* l, F f) {
+    f(l->cell_count);
+    f(l->cell_start);
+    f(l->scan_tmp);
+    f(l->cell_of);
+    f(l->cell_of_alt);
+    f(l->order_raw);
+    f(l->order);
+    f(l->ref);
+    f(l->count);
+    f(l->half);
+    f(l->rows);
+    f(l->flags);
+    f(l->scan_totals);
+    f(l->rows16);
+    f(l->tile_ranges);
+    f(l->tile_nr);
+    f(l->tile_order);
+    f(l->build_stats);
+}
+
 static void release_list_buffers(md_list* l) {
-    dev_free(l->cell_count);
-    dev_free(l->cell_start);
-    dev_free(l->scan_tmp);
-    dev_free(l->cell_of);
-    dev_free(l->cell_of_alt);
-    dev_free(l->order_raw);
-    dev_free(l->order);
-    dev_free(l->ref);
-    dev_free(l->count);
-    dev_free(l->half);
-    dev_free(l->rows);
-    dev_free(l->flags);
-    dev_free(l->scan_totals);
-    dev_free(l->rows16);
-    dev_free(l->tile_ranges);
-    dev_free(l->tile_nr);
-    dev_free(l->build_stats);
+    for_list_buffers(l, [](auto*& p) { dev_free(p); });
     l->rows16_cap = l->tiles_cap = l->stats_cap = 0;
 }
+
 
 void set_error(const std::string& m);
 
@@ -866,6 +876,8 @@ int md_list_copy(md_list* dst, const md_list* src) {
     md_ctx* const ctx = dst->ctx;
     md_list copy = *src;
     copy.ctx = ctx;
+    // no buffer of src may be reached through copy: a failed clone releases copy's buffers
+    for_list_buffers(&copy, [](auto*& p) { p = nullptr; });
     auto clone = [&](auto*& out, auto* in, size_t count) {
         using T = std::remove_const_t<std::remove_pointer_t<std::remove_reference_t<decltype(out)>>>;
         out = nullptr;
@@ -892,6 +904,7 @@ int md_list_copy(md_list* dst, const md_list* src) {
         clone(copy.rows16, src->rows16, src->rows16_cap);
         clone(copy.tile_ranges, src->tile_ranges, src->tiles_cap * TILE_RANGES);
         clone(copy.tile_nr, src->tile_nr, src->tiles_cap);
+        clone(copy.tile_order, src->tile_order, src->tiles_cap);
         MD_HIP(hipStreamSynchronize(st));
     } catch (...) {
         release_list_buffers(&copy);

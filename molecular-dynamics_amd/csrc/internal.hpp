@@ -165,6 +165,7 @@ struct md_list {
     size_t rows16_cap = 0;
     int4* tile_ranges = nullptr;     // tiles * TILE_RANGES (first cell, first particle, first slot, count)
     int* tile_nr = nullptr;          // tiles; number of ranges, -1 if the tile is not staged
+    int* tile_order = nullptr;       // tiles; the launch order of the tiles (cell columns in bands, see tiled.hip)
     size_t tiles_cap = 0;
     int hmax = 0;                    // LDS capacity of a tile's halo, in particles
     bool tiled = false;              // rows16 / tile_* describe the current rows
@@ -280,6 +281,9 @@ struct TileSystems {
     const int* gates;  // batch: per replica, nonzero when its list is rebuilt (the rebuild kernels skip the others)
     // batch rebuild kernels: active[0] tiles are rebuilt this step, active[1 + k] the k-th of them (replica order)
     const int* active;
+    // single system: the order in which launches take the tiles (order[k] = the k-th tile; nullptr = tile order);
+    // written by the tile setup of each rebuild
+    const int* order;
     size_t stride;
     int tiles;
 };

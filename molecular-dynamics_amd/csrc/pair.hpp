@@ -89,15 +89,17 @@ struct PairSumsF {
     int c = 0;
 };
 
-// One pair of a full row. Outside rc inv_r2 is zero, which makes the force and the energy terms zero; the cut-off
-// test is on the double r^2, as in mixed_pair_term.
-__device__ __forceinline__ void mixed_pair_term_f(double dx, double dy, double dz, const LJConst& lj, PairSumsF& a) {
+// One pair of a full row, without its inside count (returned: the caller counts it, per lane or per wavefront).
+// Outside rc inv_r2 is zero, which makes the force and the energy terms zero; the cut-off test is on the double r^2,
+// as in mixed_pair_term. UNIT (sigma = 1): s2 = inv_r2, the same value as sig2f * inv_r2 with sig2f = 1.
+template <bool UNIT = false>
+__device__ __forceinline__ bool mixed_pair_core(double dx, double dy, double dz, const LJConst& lj, PairSumsF& a) {
     const double r2d = fma(dx, dx, fma(dy, dy, dz * dz));
     const float r2 = static_cast<float>(r2d);
     const float q = __builtin_amdgcn_rcpf(r2);
     const bool inside = r2d < lj.rc2;
     const float inv_r2 = inside ? fmaf(q, fmaf(-r2, q, 1.0f), q) : 0.0f;
-    const float s2 = lj.sig2f * inv_r2;
+    const float s2 = UNIT ? inv_r2 : lj.sig2f * inv_r2;
     const float s6 = s2 * s2 * s2;
     const float s12 = s6 * s6;
     const float t = fmaf(2.0f, s12, -s6);
@@ -107,7 +109,12 @@ __device__ __forceinline__ void mixed_pair_term_f(double dx, double dy, double d
     a.gz = fma(g, dz, a.gz);
     a.t += t;
     a.s += s12;
-    a.c += inside ? 1 : 0;
+    return inside;
+}
+
+template <bool UNIT = false>
+__device__ __forceinline__ void mixed_pair_term_f(double dx, double dy, double dz, const LJConst& lj, PairSumsF& a) {
+    a.c += mixed_pair_core<UNIT>(dx, dy, dz, lj, a) ? 1 : 0;
 }
 
 // Mixed term of the gather kernels: double minimum image, then mixed_pair_term.
@@ -120,13 +127,14 @@ __device__ __forceinline__ void mixed_pair_gather(const double4& ri, const doubl
     mixed_pair_term(ex, ey, ez, lj, count_energy, a);
 }
 
+template <bool UNIT = false>
 __device__ __forceinline__ void mixed_pair_gather_f(const double4& ri, const double4& rj, double box, double inv_box,
                                                     const LJConst& lj, PairSumsF& a) {
     double ex = ri.x - rj.x, ey = ri.y - rj.y, ez = ri.z - rj.z;
     ex = fma(-box, rint(ex * inv_box), ex);
     ey = fma(-box, rint(ey * inv_box), ey);
     ez = fma(-box, rint(ez * inv_box), ez);
-    mixed_pair_term_f(ex, ey, ez, lj, a);
+    mixed_pair_term_f<UNIT>(ex, ey, ez, lj, a);
 }
 
 }  // namespace mdgpu

@@ -39,7 +39,11 @@ constexpr int CR_CAP = 192;  // cell ranges before merging (9 neighbour columns 
 constexpr int CMAX = 24;     // own cells of a tile covered by the stencil table
 constexpr int CHUNK = 32;    // slot-row entries per particle and chunk
 constexpr int CHUNK_STRIDE = TILE * CHUNK;  // entries between a tile's chunks
-constexpr int GHOST_SLOT = TILE_HALO_MAX - 1;  // LDS slot holding NaN coordinates: never a neighbour
+constexpr int GHOST_SLOT = TILE_HALO_MAX - 1;  // LDS slot past every halo: never a neighbour
+// The mixed force kernel's ghost slot coordinate: a frame coordinate is within half a box of 0, and boxes are far
+// shorter than 1e6 (a 1e6 box at liquid density would hold ~1e18 particles), so every separation from the ghost is
+// finite and beyond any cut-off. (The list build's ghost holds NaN or infinity: its tests reject them.)
+constexpr double GHOST_COORD = 1.0e6;
 constexpr int PRE_CHUNKS = 3;  // slot chunks the force kernel loads per lane ahead of its pair loop
 // The slot rows and the list build's candidate lists hold byte offsets of LDS slots (slot * 8, below 2^16), so that
 // the force kernel adds an entry to a fixed halo address folded into its LDS loads.
@@ -105,6 +109,17 @@ template <bool BATCH>
 __device__ __forceinline__ bool tile_gated_off(const TileSystems& ts, const TileSystem& sys) {
     if constexpr (!BATCH) return false;
     else return ts.gates[sys.gate] == 0;
+}
+
+// Launch order of a single system's tiles: cell columns (x, y) in bands of ORDER_BAND x-slabs; within a band the y
+// rows, each row's ORDER_BAND columns in x, each column's tiles in z order. A tile's halo spans its own column and
+// the eight around it. In cell order the columns at x +- 1 lie a whole slab (side^2 cells, ~500 tiles at 4 M atoms)
+// away, so an XCD's L2 has evicted their positions before the next slab stages them again; in band order they
+// return within a few columns. The blocks of a launch take tiles order[xcd_item(block)].
+constexpr int ORDER_BAND = 4;
+__device__ __forceinline__ int launch_tile(const TileSystems& ts, int block) {
+    const int k = xcd_item(block, ts.tiles);
+    return ts.order ? uniform_i32(ts.order[k]) : k;
 }
 
 static_assert(RMAX == WAVE, "one lane per halo range in k_tile_setup");
@@ -323,7 +338,7 @@ template <bool BATCH>
 __global__ void __launch_bounds__(SETUP_TILES * WAVE)
     k_tile_setup(const int* __restrict__ cell_of, const uint32_t* __restrict__ start_all, TileSystems ts,
                  int4* __restrict__ tile_ranges, int* __restrict__ tile_nr, const uint32_t* __restrict__ order,
-                 PermuteArgs pa, const int* gate, const int* halt) {
+                 PermuteArgs pa, int* __restrict__ tile_order, const int* gate, const int* halt) {
     if (gated_off(gate, halt)) return;
     __shared__ int2 cr_all[SETUP_TILES][CR_CAP];
     __shared__ int nr_all[SETUP_TILES];
@@ -373,6 +388,29 @@ __global__ void __launch_bounds__(SETUP_TILES * WAVE)
             nr_all[w] = 1;
             flat_all[w] = 0;
         }
+    }
+    if (tile_order && live) {
+        // the tile's place in band order (see ORDER_BAND): from the cell starts alone, a tile belonging to the
+        // column of its first particle; tiles(x) = tiles whose first particle lies before particle x
+        auto tiles_before = [](uint32_t x) { return static_cast<int>((x + TILE - 1) / TILE); };
+        int at = t;
+        if (use_cells && side >= 2 * ORDER_BAND) {
+            const int col = cell_of[static_cast<size_t>(t) * TILE] / side, cx = col / side, cy = col % side;
+            const int b0 = cx / ORDER_BAND * ORDER_BAND, bw = min(ORDER_BAND, side - b0), k = lane;
+            const size_t s2 = static_cast<size_t>(side) * side;
+            int v = 0;
+            if (k < bw) {
+                const size_t x = b0 + k, c = (x * side + cy) * side;
+                // tiles of slab x in the columns y < cy, and (columns x < cx of the band) of column (x, cy)
+                v = tiles_before(start[c]) - tiles_before(start[x * s2]);
+                if (x < static_cast<size_t>(cx)) v += tiles_before(start[c + side]) - tiles_before(start[c]);
+            }
+#pragma unroll
+            for (int off = 1; off < ORDER_BAND; off <<= 1) v += __shfl_xor(v, off, WAVE);
+            const size_t own = (static_cast<size_t>(cx) * side + cy) * side;
+            at = tiles_before(start[b0 * s2]) + v + (t - tiles_before(start[own]));
+        }
+        if (lane == 0) tile_order[at] = t;
     }
     __syncthreads();
     if (!live) return;
@@ -1026,6 +1064,10 @@ __device__ __forceinline__ void list_build_tile(
                 // (the wavefront's LDS accesses complete in issue order, so every lane's entries are in place)
                 __builtin_amdgcn_wave_barrier();
                 const int stored = cnt < padding ? cnt : padding;
+                // the last chunk ends in ghost entries (the force kernel's compact walk reads whole chunks)
+                if (stored + lane < (stored + CHUNK - 1) / CHUNK * CHUNK)
+                    rb[stored + lane] = static_cast<uint16_t>(GHOST_SLOT * LIST_BYTES);
+                __builtin_amdgcn_wave_barrier();
                 if (lane < (stored + CHUNK - 1) / CHUNK * LANES) {
                     const int c = lane >> 2, w = lane & 3;
                     const uint16_t* e = rb + c * CHUNK + w;
@@ -1129,6 +1171,11 @@ __device__ __forceinline__ void list_build_tile(
                 half[i] = hcnt < padding ? hcnt : padding;
                 ep_half = static_cast<unsigned>(hcnt);
             }
+            if (staged) {  // the slot row's last chunk ends in ghost entries, as on the walk above
+                const int stored = sc.cnt < padding ? sc.cnt : padding;
+                for (int k = stored + lane; k < (stored + CHUNK - 1) / CHUNK * CHUNK; k += LANES)
+                    sc.row16[slot_entry(static_cast<uint32_t>(k))] = static_cast<uint16_t>(GHOST_SLOT * SLOT_BYTES);
+            }
             ep_max = sc.cnt;
         }
     }
@@ -1165,7 +1212,7 @@ __global__ void __launch_bounds__(TILE_THREADS, 6)
         if (static_cast<int>(blockIdx.x) >= rebuilt) return;
         tile = uniform_i32(ts.active[1 + xcd_item(static_cast<int>(blockIdx.x), rebuilt)]);
     } else {
-        tile = xcd_item(static_cast<int>(blockIdx.x), ts.tiles);
+        tile = launch_tile(ts, static_cast<int>(blockIdx.x));
     }
     const TileSystem sys = tile_system<BATCH>(ts, tile);
     if (tile_gated_off<BATCH>(ts, sys) || static_cast<size_t>(tile) * TILE >= sys.end) return;  // not rebuilt, or empty
@@ -1234,14 +1281,14 @@ __device__ __forceinline__ double frame_coord(double x, double ref, const TileFr
 // drift of the next step, and the displacement test of the next list check (the arithmetic of k_kick_kick_drift).
 // The new positions go to a second buffer, since other tiles' halos still read this step's; the forces themselves
 // are not stored.
-template <bool STEP, bool BATCH>
-__global__ void __launch_bounds__(TILE_THREADS)
+// Six wavefronts per SIMD (three blocks per CU, as the halo's LDS allows) cap the registers at 80.
+template <bool STEP, bool BATCH, bool UNIT>
+__global__ void __launch_bounds__(TILE_THREADS, 6)
     k_forces_tiled_mixed(const double4* __restrict__ pos, TileSystems ts, const uint16_t* __restrict__ rows16,
                          const int32_t* __restrict__ rows, const int32_t* __restrict__ count,
                          const int4* __restrict__ tile_ranges, const int* __restrict__ tile_nr, int row_len,
                          LJConst lj, double* __restrict__ frc, double* __restrict__ partials,
                          int* __restrict__ clear_flag, ForceStep fs, const int* halt) {
-    if (halted(halt)) return;
     // The halo is a static array: its address is known when the kernel is compiled, so a slot's byte offset, as the
     // rows hold it, is the whole address computation.
     __shared__ double halo[3][TILE_HALO_MAX];
@@ -1253,17 +1300,15 @@ __global__ void __launch_bounds__(TILE_THREADS)
     constexpr int SUMS = STEP ? 4 : 3;
     double(*e_part)[TILE] = reinterpret_cast<double(*)[TILE]>(&halo[0][0]);
     static_assert(4 * TILE <= TILE_HALO_MAX, "the per-particle sums fit in the halo's x array");
-    __shared__ int ra[RMAX], rgs[RMAX], rofs[RMAX + 1];
     __shared__ int extent_bits;
-    const HaloRanges hr{ra, rgs, rofs};
-    const int tile = xcd_item(static_cast<int>(blockIdx.x), ts.tiles);
+    const int tile = launch_tile(ts, static_cast<int>(blockIdx.x));
     const TileSystem sys = tile_system<BATCH>(ts, tile);
     const TileFrame fr{sys.box, sys.inv_box, sys.reach};
     const size_t n = sys.end, pstride = ts.stride;  // pstride: column / component stride
     const int raw_nr = tile_nr[tile];
-    // the halo ranges, loaded together with the header (every tile owns RMAX entries; those past nr are unused)
-    const int4 rr = static_cast<int>(threadIdx.x) < RMAX ? tile_ranges[static_cast<size_t>(tile) * RMAX + threadIdx.x]
-                                                         : make_int4(0, 0, 0, 0);
+    // halo range `wave lane` of the tile, in every wavefront (the staging below reads the ranges from registers;
+    // ranges past nr are unused)
+    const int4 rr = tile_ranges[static_cast<size_t>(tile) * RMAX + threadIdx.x % WAVE];
     const int nr = tile_range_count(raw_nr);
     const bool staged = nr > 0;
     const bool flat = tile_is_flat(raw_nr);
@@ -1287,36 +1332,38 @@ __global__ void __launch_bounds__(TILE_THREADS)
             zi = frame_coord(ri.z, ref.z, fr);
         }
     }
-    if (threadIdx.x == 0) extent_bits = 0;
-    if (staged && static_cast<int>(threadIdx.x) < nr) {
-        ra[threadIdx.x] = rr.x;
-        rgs[threadIdx.x] = rr.y;
-        rofs[threadIdx.x] = rr.z;
-        if (static_cast<int>(threadIdx.x) == nr - 1) rofs[nr] = rr.z + rr.w;
+    if (threadIdx.x == 0) {
+        extent_bits = 0;
+        // the ghost slot, far from every frame coordinate: its pair term is exactly zero (see the compact walk)
+        xs[GHOST_SLOT] = GHOST_COORD;
+        ys[GHOST_SLOT] = GHOST_COORD;
+        zs[GHOST_SLOT] = GHOST_COORD;
     }
     __syncthreads();
     if (staged && flat) {
-        stage_halo(pos, hr, nr, rofs[nr], [&](int q, const double4& p) {
+        stage_halo_lanes<TILE_THREADS / WAVE>(pos, rr, nr, [&](int q, const double4& p) {
             xs[q] = p.x - ref.x;
             ys[q] = p.y - ref.y;
             zs[q] = p.z - ref.z;
         });
     } else if (staged) {
-        stage_halo(pos, hr, nr, rofs[nr], [&](int q, const double4& p) {
+        stage_halo_lanes<TILE_THREADS / WAVE>(pos, rr, nr, [&](int q, const double4& p) {
             xs[q] = frame_coord(p.x, ref.x, fr);
             ys[q] = frame_coord(p.y, ref.y, fr);
             zs[q] = frame_coord(p.z, ref.z, fr);
         });
     }
     // the lane's slot words of the row's first PRE_CHUNKS chunks, all issued before the barrier, so the pair loop
-    // below runs from registers (longer rows load their remaining chunks in the loop)
+    // below runs from registers (longer rows load their remaining chunks in the loop); chunks past the row are
+    // ghost words
     const int chunks = (m + CHUNK - 1) / CHUNK;
     const uint4* rp = reinterpret_cast<const uint4*>(rows16 + slot_row_base(i, row_len) + lane * 8);
     constexpr int stride = CHUNK_STRIDE / 8;  // uint4 words between chunks
+    constexpr unsigned GHOST_WORD = (GHOST_SLOT * SLOT_BYTES) * 0x10001u;  // two ghost entries
+    const uint4 ghost4 = make_uint4(GHOST_WORD, GHOST_WORD, GHOST_WORD, GHOST_WORD);
     uint4 pre[PRE_CHUNKS];
 #pragma unroll
-    for (int c = 0; c < PRE_CHUNKS; ++c)
-        pre[c] = staged && c < chunks ? rp[c * stride] : make_uint4(0u, 0u, 0u, 0u);
+    for (int c = 0; c < PRE_CHUNKS; ++c) pre[c] = staged && c < chunks ? rp[c * stride] : ghost4;
     if (active && lane == 0) {  // how far the tile's own particles lie from the reference particle
         const float e = static_cast<float>(fmax(fabs(xi), fmax(fabs(yi), fabs(zi))));
         atomicMax(&extent_bits, __float_as_int(e));
@@ -1325,36 +1372,59 @@ __global__ void __launch_bounds__(TILE_THREADS)
     const float e = __int_as_float(extent_bits);
     const bool compact = staged && (2.0f * e + static_cast<float>(fr.reach) < static_cast<float>(fr.box) - 1.0f);
     PairSumsF a;
-    if (active) {
+    // inside pairs counted per wavefront on the compact walk (a uniform sum of ballots)
+    unsigned wave_inside = 0;
+    if (compact) {
+        // Compact tile: every listed separation is already the minimum image. Each lane walks whole chunks: the
+        // entries past its row read the ghost slot (the build ends every row's last chunk with ghost entries, and
+        // chunks past the row are ghost words), whose separation is finite and beyond the cut-off, so its term
+        // adds exactly zero to every sum. The wavefront stops at its longest row, a uniform bound: no per-entry
+        // guard. Inactive lanes (past the system's end) walk ghost words only.
+        const int mw = wave_max_dpp(m);
+        const int cw = (mw + CHUNK - 1) / CHUNK;
+        auto chunk_pairs = [&](const uint4& cw4, int c) {
+            const unsigned words[4] = {cw4.x, cw4.y, cw4.z, cw4.w};
+            // entries two at a time (both halves of a word): two independent chains to interleave; past the
+            // wavefront's longest row both are ghosts for every lane
+#pragma unroll
+            for (int t = 0; t < 8; t += 2) {
+                if (c * CHUNK + t * LANES >= mw) break;  // no lane of the wavefront has entry c * CHUNK + t * LANES on
+                const unsigned w = words[t >> 1];
+                const double* xa = reinterpret_cast<const double*>(reinterpret_cast<const char*>(xs) + (w & 0xffffu));
+                const double* xb = reinterpret_cast<const double*>(reinterpret_cast<const char*>(xs) + (w >> 16));
+                const double ax = xi - xa[0], ay = yi - xa[TILE_HALO_MAX], az = zi - xa[2 * TILE_HALO_MAX];
+                const double bx = xi - xb[0], by = yi - xb[TILE_HALO_MAX], bz = zi - xb[2 * TILE_HALO_MAX];
+                const bool in_a = mixed_pair_core<UNIT>(ax, ay, az, lj, a);
+                const bool in_b = mixed_pair_core<UNIT>(bx, by, bz, lj, a);
+                wave_inside += static_cast<unsigned>(__popcll(__ballot(in_a)) + __popcll(__ballot(in_b)));
+            }
+        };
+#pragma unroll
+        for (int c = 0; c < PRE_CHUNKS; ++c)
+            if (c < cw) chunk_pairs(pre[c], c);
+        for (int c = PRE_CHUNKS; c < cw; ++c) chunk_pairs(c < chunks ? rp[c * stride] : ghost4, c);
+    } else if (active) {
         if (staged) {
-            auto run = [&](auto wrap) {
-                constexpr bool W = decltype(wrap)::value;
-                auto chunk_pairs = [&](const uint4& cw, int c) {
-                    const unsigned words[4] = {cw.x, cw.y, cw.z, cw.w};
-                    const int rem = m - c * CHUNK - lane;  // entry c * CHUNK + t * LANES + lane exists iff t * LANES < rem
+            // staged, not compact: the minimum image of each separation, entries guarded per lane
+            auto chunk_pairs = [&](const uint4& cw, int c) {
+                const unsigned words[4] = {cw.x, cw.y, cw.z, cw.w};
+                const int rem = m - c * CHUNK - lane;  // entry c * CHUNK + t * LANES + lane exists iff t * LANES < rem
 #pragma unroll
-                    for (int t = 0; t < 8; ++t) {
-                        if (t * LANES < rem) {
-                            const unsigned off = __builtin_amdgcn_ubfe(words[t >> 1], (t & 1) * 16, 16);
-                            const double* xj = reinterpret_cast<const double*>(reinterpret_cast<const char*>(xs) + off);
-                            double dx = xi - xj[0], dy = yi - xj[TILE_HALO_MAX], dz = zi - xj[2 * TILE_HALO_MAX];
-                            if constexpr (W) {
-                                dx = fma(-fr.box, rint(dx * fr.inv_box), dx);
-                                dy = fma(-fr.box, rint(dy * fr.inv_box), dy);
-                                dz = fma(-fr.box, rint(dz * fr.inv_box), dz);
-                            }
-                            mixed_pair_term_f(dx, dy, dz, lj, a);
-                        }
+                for (int t = 0; t < 8; ++t) {
+                    if (t * LANES < rem) {
+                        const unsigned off = __builtin_amdgcn_ubfe(words[t >> 1], (t & 1) * 16, 16);
+                        const double* xj = reinterpret_cast<const double*>(reinterpret_cast<const char*>(xs) + off);
+                        double dx = xi - xj[0], dy = yi - xj[TILE_HALO_MAX], dz = zi - xj[2 * TILE_HALO_MAX];
+                        dx = fma(-fr.box, rint(dx * fr.inv_box), dx);
+                        dy = fma(-fr.box, rint(dy * fr.inv_box), dy);
+                        dz = fma(-fr.box, rint(dz * fr.inv_box), dz);
+                        mixed_pair_term_f<UNIT>(dx, dy, dz, lj, a);
                     }
-                };
-#pragma unroll
-                for (int c = 0; c < PRE_CHUNKS; ++c) chunk_pairs(pre[c], c);
-                for (int c = PRE_CHUNKS; c < chunks; ++c) chunk_pairs(rp[c * stride], c);
+                }
             };
-            if (compact)
-                run(std::false_type{});
-            else
-                run(std::true_type{});
+#pragma unroll
+            for (int c = 0; c < PRE_CHUNKS; ++c) chunk_pairs(pre[c], c);
+            for (int c = PRE_CHUNKS; c < chunks; ++c) chunk_pairs(rp[c * stride], c);
         } else {
             // unstaged tile: neighbours gathered through the int32 rows, UNSTAGED_BATCH entries of the lane at a time
             // (their row words, then their positions, then their terms in entry order), so the two dependent loads
@@ -1379,10 +1449,15 @@ __global__ void __launch_bounds__(TILE_THREADS)
 #pragma unroll
                 for (int b = 0; b < UNSTAGED_BATCH; ++b)
                     if (j[b] >= 0)
-                        mixed_pair_gather_f(ri, make_double4(xj[b], yj[b], zj[b], 0.0), fr.box, fr.inv_box, lj, a);
+                        mixed_pair_gather_f<UNIT>(ri, make_double4(xj[b], yj[b], zj[b], 0.0), fr.box, fr.inv_box, lj,
+                                                  a);
             }
         }
     }
+    // A halted run (a list build of this step overflowed) stores nothing: the state stays at the last valid step.
+    // Tested here rather than first, so the halt word's load does not delay the tile's loads; the walk above
+    // read the truncated rows of such a build, which stay within the tile's halo and rows.
+    if (halted(halt)) return;
     // the particle's four lanes are adjacent in the wavefront
 #pragma unroll
     for (int off = 1; off < LANES; off <<= 1) {
@@ -1404,9 +1479,11 @@ __global__ void __launch_bounds__(TILE_THREADS)
         // gather the three components on lane 0 and add them in x, y, z order
         double m = 0.0, kick = 0.0, v = 0.0, d = 0.0, r = 0.0;
         const double* pi = reinterpret_cast<const double*>(pos + i);
+        if (active) m = fs.mass[i];
+        // x / 1 = x exactly: a wavefront of unit masses skips the division
+        const bool unit_mass = __ballot(active && m != 1.0) == 0;
         if (active) {
-            m = fs.mass[i];
-            kick = fs.half_dt * fc / m;
+            kick = unit_mass ? fs.half_dt * fc : fs.half_dt * fc / m;
             v = fs.vel[comp * pstride + i] + kick;  // closing kick of this step
         }
         const double vy = __shfl_xor(v, 1, WAVE), vz = __shfl_xor(v, 2, WAVE);
@@ -1431,24 +1508,28 @@ __global__ void __launch_bounds__(TILE_THREADS)
     if (active && !STEP && lane < 3) frc[lane * pstride + i] = fc;
     __syncthreads();  // every wavefront is done with the halo
     if (lane == 0) {
-        // every pair was met from both ends: half of each full-row sum (zero for the slots past n); the potential's
-        // shift once per inside pair
+        // every pair was met from both ends: half of each full-row sum (zero for the slots past n); the inside
+        // count of the particle's row (other walks) or of the wavefront's rows (the compact walk, on its first lane)
         const double u = static_cast<double>(lj.c4f) * (static_cast<double>(a.t) - static_cast<double>(a.s));
-        e_part[0][pl] = 0.5 * (u - static_cast<double>(lj.shiftf) * a.c);
+        const unsigned inside = static_cast<unsigned>(a.c) + (threadIdx.x % WAVE == 0 ? wave_inside : 0u);
+        e_part[0][pl] = 0.5 * u;
         e_part[1][pl] = 0.5 * c24 * static_cast<double>(a.t);
-        e_part[2][pl] = 0.5 * static_cast<double>(a.c);
+        e_part[2][pl] = 0.5 * static_cast<double>(inside);
         if constexpr (STEP) e_part[3][pl] = ke;
     }
     if (clear_flag && blockIdx.x == 0 && threadIdx.x == 0) *clear_flag = 0;
     __syncthreads();
     // the tile's U, W and inside count (and KE): the 128 particle sums added in double by one wavefront, in a fixed
-    // order
+    // order; the potential's shift once per inside pair (the halved count is exact)
     if (threadIdx.x < WAVE) {
+        double v[SUMS];
 #pragma unroll
-        for (int q = 0; q < SUMS; ++q) {
-            double v = e_part[q][threadIdx.x] + e_part[q][threadIdx.x + WAVE];
-            v = wave_sum(v);
-            if (threadIdx.x == 0) partials[static_cast<size_t>(tile) * SUMS + q] = v;
+        for (int q = 0; q < SUMS; ++q) v[q] = wave_sum(e_part[q][threadIdx.x] + e_part[q][threadIdx.x + WAVE]);
+        if (threadIdx.x == 0) {
+            double* out = partials + static_cast<size_t>(tile) * SUMS;
+            out[0] = v[0] - static_cast<double>(lj.shiftf) * v[2];
+#pragma unroll
+            for (int q = 1; q < SUMS; ++q) out[q] = v[q];
         }
     }
 }
@@ -1536,7 +1617,7 @@ __global__ void __launch_bounds__(PAIR_THREADS, 3)
     if (halted(halt)) return;
     __shared__ double halo[3][TILE_HALO_MAX];
     __shared__ double e_part[PAIR_THREADS / WAVE][3];
-    const int tile = xcd_item(static_cast<int>(blockIdx.x), ts.tiles);
+    const int tile = launch_tile(ts, static_cast<int>(blockIdx.x));
     const TileSystem sys = tile_system<BATCH>(ts, tile);
     const size_t n = sys.end, pstride = ts.stride;  // pstride: column / component stride
     const double box = sys.box, inv_box = sys.inv_box;
@@ -1683,6 +1764,12 @@ static TileSystems single_system(const md_state* s, const md_list* l, double rea
     // cells that a tile and its halo may span with every coordinate inside half a box (skin: the largest drift)
     o.flat_cells = l->side > 0 ? (0.5 * s->box - l->skin) / (s->box / l->side) : 0.0;
     o.reach = reach;
+    // MDGPU_TILE_ORDER=0: launches take the tiles in cell order (the band order is still written)
+    static const bool banded = [] {
+        const char* v = std::getenv("MDGPU_TILE_ORDER");
+        return !(v && v[0] == '0');
+    }();
+    ts.order = banded ? l->tile_order : nullptr;
     ts.stride = n;
     ts.tiles = tile_count(n ? n : 1);
     return ts;
@@ -1698,7 +1785,8 @@ void tiled_forces_fp64_enqueue(md_state* s, md_list* l, const LJConst& lj, doubl
     ensure_rows32(l);
     if (ev_start) MD_HIP(hipEventRecord(ev_start, st));
     if (n > 0) {
-        const TileSystems ts = single_system(s, l, 0.0);
+        TileSystems ts = single_system(s, l, 0.0);
+        ts.order = nullptr;  // the force kernel takes the tiles in cell order
         auto kernel = lj.sig2 == 1.0 ? k_forces_tiled_fp64_pair<true, false> : k_forces_tiled_fp64_pair<false, false>;
         hipLaunchKernelGGL(kernel, dim3(tiles), dim3(PAIR_THREADS), 0, st, s->pos, ts, l->rows16, l->rows, l->count,
                            l->half, l->tile_ranges, l->tile_nr, slot_row_len(l->padding), lj, s->frc, s->partials,
@@ -1775,7 +1863,8 @@ void tiled_list_enqueue(md_list* l, md_state* s, bool from_alt, const int* gate,
     const double4* build_pos = from_alt ? l->ref : s->pos;
     const int* cells = from_alt ? l->cell_of_alt : l->cell_of;
     hipLaunchKernelGGL(k_tile_setup<false>, dim3((tiles + SETUP_TILES - 1) / SETUP_TILES), dim3(SETUP_TILES * WAVE), 0, st,
-                       cells, l->cell_start, ts, l->tile_ranges, l->tile_nr, l->order, gather, gate, s->halt);
+                       cells, l->cell_start, ts, l->tile_ranges, l->tile_nr, l->order, gather, l->tile_order, gate,
+                       s->halt);
     // the float halo where the grid is wide enough for compact frames and the cells dense enough for the
     // candidate lists (MDGPU_BUILD_F64=1 keeps the double halo)
     const char* f64 = std::getenv("MDGPU_BUILD_F64");
@@ -1799,17 +1888,17 @@ static void tiled_forces_launch(md_state* s, md_list* l, const LJConst& lj, doub
     const int tiles = tile_count(n ? n : 1);
     const int sums = step ? 4 : 3;
     state_reserve_partials(s, static_cast<size_t>(tiles) * sums);
-    const TileSystems ts = single_system(s, l, cutoff + l->radius + l->skin);
+    TileSystems ts = single_system(s, l, cutoff + l->radius + l->skin);
+    ts.order = nullptr;  // the force kernel takes the tiles in cell order
     if (ev_start) MD_HIP(hipEventRecord(ev_start, st));
     if (n > 0) {
-        if (step)
-            hipLaunchKernelGGL((k_forces_tiled_mixed<true, false>), dim3(tiles), dim3(TILE_THREADS), 0, st, s->pos, ts,
-                               l->rows16, l->rows, l->count, l->tile_ranges, l->tile_nr, slot_row_len(l->padding),
-                               lj, s->frc, s->partials, clear_flag, *step, s->halt);
-        else
-            hipLaunchKernelGGL((k_forces_tiled_mixed<false, false>), dim3(tiles), dim3(TILE_THREADS), 0, st, s->pos, ts,
-                               l->rows16, l->rows, l->count, l->tile_ranges, l->tile_nr, slot_row_len(l->padding),
-                               lj, s->frc, s->partials, clear_flag, ForceStep{}, s->halt);
+        // sigma = 1 drops the sigma^2 factor (a product with 1.0f: the same values)
+        const bool unit = lj.sig2f == 1.0f;
+        auto kernel = step ? (unit ? k_forces_tiled_mixed<true, false, true> : k_forces_tiled_mixed<true, false, false>)
+                           : (unit ? k_forces_tiled_mixed<false, false, true> : k_forces_tiled_mixed<false, false, false>);
+        hipLaunchKernelGGL(kernel, dim3(tiles), dim3(TILE_THREADS), 0, st, s->pos, ts, l->rows16, l->rows, l->count,
+                           l->tile_ranges, l->tile_nr, slot_row_len(l->padding), lj, s->frc, s->partials, clear_flag,
+                           step ? *step : ForceStep{}, s->halt);
         MD_HIP(hipGetLastError());
     }
     if (ev_stop) MD_HIP(hipEventRecord(ev_stop, st));
@@ -1822,7 +1911,7 @@ void tiled_batch_build_enqueue(hipStream_t st, const BatchTiles& bt, const doubl
                                const uint32_t* start, double radius2, const TiledCopyBack& cb, const int* halt) {
     const int tiles = bt.ts.tiles;
     hipLaunchKernelGGL(k_tile_setup<true>, dim3((tiles + SETUP_TILES - 1) / SETUP_TILES), dim3(SETUP_TILES * WAVE), 0,
-                       st, cell_of, start, bt.ts, bt.tile_ranges, bt.tile_nr, nullptr, PermuteArgs{}, nullptr, halt);
+                       st, cell_of, start, bt.ts, bt.tile_ranges, bt.tile_nr, nullptr, PermuteArgs{}, nullptr, nullptr, halt);
     MD_HIP(hipGetLastError());
     // the float halo: three blocks per CU, and float distance tests (with the minimum image in small boxes)
     hipLaunchKernelGGL((k_list_build_tiled<true, true>), dim3(tiles), dim3(TILE_THREADS),
@@ -1837,7 +1926,8 @@ void tiled_batch_forces_enqueue(hipStream_t st, const BatchTiles& bt, const doub
     const int tiles = bt.ts.tiles;
     const int row_len = slot_row_len(bt.padding);
     if (precision == MD_MIXED) {
-        hipLaunchKernelGGL((k_forces_tiled_mixed<false, true>), dim3(tiles), dim3(TILE_THREADS), 0, st, pos, bt.ts,
+        auto kernel = lj.sig2f == 1.0f ? k_forces_tiled_mixed<false, true, true> : k_forces_tiled_mixed<false, true, false>;
+        hipLaunchKernelGGL(kernel, dim3(tiles), dim3(TILE_THREADS), 0, st, pos, bt.ts,
                            bt.rows16, bt.rows, bt.count, bt.tile_ranges, bt.tile_nr, row_len, lj, frc, partials,
                            nullptr, ForceStep{}, halt);
     } else {
