@@ -222,12 +222,70 @@ int md_step(md_state* s, md_list* l, const md_lj* lj, int precision, double dt, 
     MD_API_BEGIN
     if (!s || !l || !lj) fail(MD_ERR_ARG, "state/list/lj is NULL");
     if (precision != MD_FP64 && precision != MD_MIXED) fail(MD_ERR_ARG, "unknown precision");
-    kick_drift_enqueue(s, dt, nullptr);
-    eager_update(l, s);
+    if (!l->built || l->n != s->n || (tiling_enabled(s->n) && !l->tiled) || s->n == 0) {
+        // the first build (or a new particle count): the host-driven update
+        kick_drift_enqueue(s, dt, nullptr);
+        eager_update(l, s);
+        check_cutoff(s, *lj);
+        forces_enqueue(s, l, *lj, precision, s->scalars);
+        kick_enqueue(s, dt, nullptr);
+        read_force_result(s, out);
+        return MD_OK;
+    }
+    // One host synchronisation per step: the half-skin test rides on the opening kick and drift, the re-sort and
+    // rebuild run gated on the device (the launches md_run makes), and the force result comes back together with
+    // the list's flags. A rebuild whose rows overflow halts the step's force and kick kernels; the host then grows
+    // the rows, rebuilds at the drifted state and finishes the step.
     check_cutoff(s, *lj);
-    forces_enqueue(s, l, *lj, precision, s->scalars);
+    hipStream_t st = s->ctx->stream;
+    if (precision == MD_FP64) ensure_rows32(l);
+    struct Guard {
+        md_state* s;
+        md_list* l;
+        ~Guard() {
+            s->halt = nullptr;
+            l->slots_only = false;
+        }
+    } guard{s, l};
+    const bool slots = precision == MD_MIXED && tiling_enabled(s->n);
+    l->slots_only = slots;  // mixed: the gated rebuild writes only the slot rows
+    static_assert(FLAG_STALE == 0 && FLAG_OVERFLOW == 1 && FLAG_MAXCOUNT == 2 && FLAG_REBUILDS == 3,
+                  "one clear covers the stale, overflow, longest-row and rebuild words");
+    MD_HIP(hipMemsetAsync(l->flags, 0, 4 * sizeof(int), st));
+    s->halt = l->flags + FLAG_OVERFLOW;
+    kick_drift_enqueue(s, dt, l);
+    list_enqueue_conditional_rebuild(l, s);
+    forces_enqueue(s, l, *lj, precision, s->scalars, l->flags + FLAG_STALE, nullptr, nullptr);
     kick_enqueue(s, dt, nullptr);
-    read_force_result(s, out);
+    int* hf = s->ctx->host_flags;
+    MD_HIP(hipMemcpyAsync(s->ctx->host_scalars, s->scalars, 3 * sizeof(double), hipMemcpyDeviceToHost, st));
+    MD_HIP(hipMemcpyAsync(hf, l->flags, sizeof(int) * (FLAG_COUNT + 2), hipMemcpyDeviceToHost, st));
+    MD_HIP(hipStreamSynchronize(st));
+    if (hf[FLAG_OVERFLOW]) {
+        // halted after the re-sort of the overflowing rebuild: longer rows, the rebuild again (the re-sort is the
+        // identity now), then the step's forces and closing kick
+        s->halt = nullptr;
+        l->slots_only = false;
+        l->padding = (hf[FLAG_MAXCOUNT] + hf[FLAG_MAXCOUNT] / 4 + 16 + 7) / 8 * 8;
+        list_rebuild(l, s);
+        forces_enqueue(s, l, *lj, precision, s->scalars);
+        kick_enqueue(s, dt, nullptr);
+        read_force_result(s, out);
+        return MD_OK;
+    }
+    if (hf[FLAG_REBUILDS] > 0) {
+        l->rebuilds += hf[FLAG_REBUILDS];
+        unsigned long long total;
+        std::memcpy(&total, hf + FLAG_COUNT, sizeof(total));
+        l->half_pairs = static_cast<size_t>(total);
+        l->max_count = hf[FLAG_MAXCOUNT];
+        if (slots) l->rows32_valid = false;  // materialised again when a caller needs them
+    }
+    if (out) {
+        out->potential = s->ctx->host_scalars[0];
+        out->virial = s->ctx->host_scalars[1];
+        out->inside_pairs = s->ctx->host_scalars[2];
+    }
     MD_API_END
 }
 

@@ -125,3 +125,37 @@ def test_list_copy_carries_device_rows(ctx):
     assert (r1.potential, r1.virial, r1.inside_pairs) == (r0.potential, r0.virial, r0.inside_pairs)
     r2 = twin.forces(g, LJ, mdgpu.MD_MIXED)
     assert abs(r2.potential - r0.potential) <= 1e-6 * abs(r0.potential)
+
+
+@pytest.mark.parametrize("precision", [mdgpu.MD_FP64, mdgpu.MD_MIXED])
+def test_step_overflow_recovers_bit_equal(ctx, precision):
+    """md_step (VelocityVerletIntegrator::step): the rebuild runs gated on the device; rows that overflow halt the
+    step's force and kick kernels and the host finishes the step with longer rows. Started short, the per-call
+    steps end bit-equal to the same steps started with the automatic padding, and to md_run."""
+    print(f"md_step overflow: precision {precision}", flush=True)
+    steps = 150
+    res = []
+    for padding in (0, FCC_ROW):
+        g = fcc_gpu(ctx)
+        nl = mdgpu.NeighbourList(ctx, 2.5, 0.3, max_neighbours=padding)
+        nl.build(g)
+        nl.forces(g, LJ, precision)
+        us = [nl.step(g, LJ, DT, precision).potential for _ in range(steps)]
+        res.append((g, nl, np.array(us)))
+    (ga, la, ua), (gb, lb, ub) = res
+    assert lb.info().padding > FCC_ROW
+    assert la.info().rebuilds == lb.info().rebuilds > 3
+    for f in (mdgpu.FIELD_POS, mdgpu.FIELD_VEL, mdgpu.FIELD_FORCE, mdgpu.FIELD_ID):
+        assert np.array_equal(ga.download(f), gb.download(f)), f
+    assert np.array_equal(ua, ub)
+    assert la.info().half_pairs == lb.info().half_pairs
+    # the same steps by md_run (eager launches, the separate-kernel path for FP64): the same state
+    g = fcc_gpu(ctx)
+    nl = mdgpu.NeighbourList(ctx, 2.5, 0.3)
+    nl.build(g)
+    nl.forces(g, LJ, precision)
+    en = nl.run(g, LJ, DT, steps, precision=precision, use_graph=False)
+    assert nl.info().rebuilds == la.info().rebuilds
+    for f in (mdgpu.FIELD_POS, mdgpu.FIELD_VEL, mdgpu.FIELD_ID):
+        assert np.array_equal(g.download(f), ga.download(f)), f
+    assert np.array_equal(en[:, 0], ua)
