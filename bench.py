@@ -419,14 +419,15 @@ def main():
     k_full = 2.0 * info.half_pairs / n
     # force kernel: per atom it reads its position (32 B), its row length(s) (4 B, 8 B with the half-row length
     # of the FP64 and untiled kernels) and its row (4 B per full-list neighbour as int32, 2 B as the tiled kernel's 16-bit
-    # LDS slots), and writes its force (24 B); neighbour positions are re-reads of the same array. In a mixed NVE
-    # run the tiled kernel also carries the step for all but the last one: instead of writing the force it reads
-    # the velocity (24 B), mass (8 B) and list-build position (32 B) and writes the velocity (24 B) and the next
+    # LDS slots), and writes its force (24 B); neighbour positions are re-reads of the same array. In an NVE run
+    # the tiled kernels also carry the step for all but the last one: instead of writing the force they read the
+    # velocity (24 B), mass (8 B) and list-build position (32 B) and write the velocity (24 B) and the next
     # position (32 B)
     tiled = bool(info.tiled) and (prec == mdgpu.MD_MIXED or os.environ.get("MDGPU_FP64_GATHER") != "1")
-    plain = 32.0 + (4.0 if tiled and prec == mdgpu.MD_MIXED else 8.0) + (2.0 if tiled else 4.0) * k_full + 24.0
-    folded = 32.0 + 4.0 + 2.0 * k_full + 24.0 + 8.0 + 32.0 + 24.0 + 32.0
-    fold = tiled and prec == mdgpu.MD_MIXED and os.environ.get("MDGPU_UNFOLDED") != "1" and args.steps >= 2
+    lengths = 4.0 if tiled and prec == mdgpu.MD_MIXED else 8.0
+    plain = 32.0 + lengths + (2.0 if tiled else 4.0) * k_full + 24.0
+    folded = 32.0 + lengths + 2.0 * k_full + 24.0 + 8.0 + 32.0 + 24.0 + 32.0
+    fold = tiled and os.environ.get("MDGPU_UNFOLDED") != "1" and args.steps >= 2
     bytes_per_atom = ((args.steps - 1) * folded + plain) / args.steps if fold else plain
     roofline = None
     if timing:
@@ -443,7 +444,8 @@ def main():
                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                     "kernel": ("k_forces_tiled_mixed<true>" if fold else "k_forces_tiled_mixed" if tiled
                                else "k_forces_mixed") if prec == mdgpu.MD_MIXED
-                    else "k_forces_tiled_fp64_pair" if tiled else "k_forces_fp64",
+                    else ("k_forces_tiled_fp64_pair<STEP>" if fold else "k_forces_tiled_fp64_pair") if tiled
+                    else "k_forces_fp64",
                     "avg_launch_ms": round(avg_ms, 5), "algorithmic_bytes_per_launch": round(bytes_per_atom * n)}
         if valu:
             # the kernel's issue floor: its wavefront VALU instructions (rocprofv3 SQ_INSTS_VALU of the same

@@ -485,7 +485,7 @@ static void run_impl(md_state* s, md_list* l, const md_run_config* cfg, double* 
     };
 
     const bool fused = !cfg->thermostat && !exact_ke;
-    // Mixed NVE on the tiled rows goes further: the force kernel itself carries each step (closing kick, kinetic
+    // NVE on the tiled rows (mixed and FP64) goes further: the force kernel itself carries each step (closing kick, kinetic
     // energy, opening kick and drift of the next step) for its own particles, reading the positions from one
     // buffer and writing the next step's into the other. Step k (k = 1 .. steps - 1) reads buffer (k - 1) % 2 and
     // is gated by the stale word of that parity, which its force kernel clears while setting the other one. The
@@ -493,7 +493,11 @@ static void run_impl(md_state* s, md_list* l, const md_run_config* cfg, double* 
     const char* unfolded = std::getenv("MDGPU_UNFOLDED");  // 1: keep the separate kick-kick-drift kernel
     // (without a spare position buffer, which an allocation-free run cannot make, the separate kernels do the
     // same arithmetic)
-    const bool can_fold = fused && cfg->precision == MD_MIXED && tiling_enabled(s->n) && l->tiled &&
+    // (MDGPU_FP64_GATHER=1 keeps FP64 forces on the per-particle gather kernel, so FP64 runs then stay unfolded)
+    const char* gather = std::getenv("MDGPU_FP64_GATHER");
+    const bool fp64_gather = gather && gather[0] == '1';
+    const bool can_fold = fused && tiling_enabled(s->n) && l->tiled &&
+                          (cfg->precision == MD_MIXED || !fp64_gather) &&
                           !(unfolded && unfolded[0] == '1') && !(rb.no_alloc && !s->pos_spare);
     if (can_fold && steps >= 2 && !s->pos_spare) s->pos_spare = dev_alloc<double4>(s->n);
     double4* bufs[2] = {nullptr, nullptr};
@@ -512,7 +516,7 @@ static void run_impl(md_state* s, md_list* l, const md_run_config* cfg, double* 
         list_enqueue_conditional_rebuild(l, s, gate);
         hipEvent_t e0, e1;
         event_pair(e0, e1);
-        tiled_forces_step_enqueue(s, l, cfg->lj, cfg->dt, bufs[1 - p], gate, next, s->scalars,
+        tiled_forces_step_enqueue(s, l, cfg->lj, cfg->precision, cfg->dt, bufs[1 - p], gate, next, s->scalars,
                                   StepRecord{rb.energies, rb.counters}, e0, e1);
         enqueue_rdf(0);  // this step's positions, still in bufs[p]
     };

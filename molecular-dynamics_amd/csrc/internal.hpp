@@ -337,10 +337,11 @@ void tiled_forces_enqueue(md_state* s, md_list* l, const LJConst& lj, double cut
 // FP64 forces over the tiled rows, bit-equal to the per-particle gather kernel (k_forces_fp64)
 void tiled_forces_fp64_enqueue(md_state* s, md_list* l, const LJConst& lj, double* out_dev, int* clear_flag,
                                hipEvent_t ev_start, hipEvent_t ev_stop);
-// the same with the NVE step of the tile's particles folded in: U, W, inside count and KE to out_dev[0..3] (and the
-// step's record), new positions to pos_next, the next displacement test to *stale_next; the forces are not stored
-void tiled_forces_step_enqueue(md_state* s, md_list* l, const md_lj& p, double dt, double4* pos_next, int* clear_flag,
-                               int* stale_next, double* out_dev, StepRecord rec, hipEvent_t ev_start,
+// the mixed or FP64 tiled forces with the NVE step of the tile's particles folded in: U, W, inside count and KE to
+// out_dev[0..3] (and the step's record), new positions to pos_next, the next displacement test to *stale_next; the
+// forces are not stored
+void tiled_forces_step_enqueue(md_state* s, md_list* l, const md_lj& p, int precision, double dt, double4* pos_next,
+                               int* clear_flag, int* stale_next, double* out_dev, StepRecord rec, hipEvent_t ev_start,
                                hipEvent_t ev_stop);
 
 // integrate.hip

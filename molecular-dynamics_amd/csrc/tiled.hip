@@ -1607,16 +1607,22 @@ __device__ __forceinline__ void add_pair(PairLaneSums& a, double tx, double ty, 
     a.fz += pz;
 }
 
-template <bool UNIT, bool BATCH>
+// STEP: the velocity-Verlet step of the tile's particles folded in after their forces (NVE md_run, FP64), with
+// the arithmetic of k_kick_kick_drift on the particle's even lane: the closing kick and its kinetic energy, the
+// opening kick and drift of the next step into a second position buffer (other tiles' halos still read this
+// step's), and the next displacement test. The forces are not stored; the tile's fourth sum is the kinetic energy.
+template <bool UNIT, bool BATCH, bool STEP>
 __global__ void __launch_bounds__(PAIR_THREADS, 3)
     k_forces_tiled_fp64_pair(const double4* __restrict__ pos, TileSystems ts, const uint16_t* __restrict__ rows16,
                              const int32_t* __restrict__ rows, const int32_t* __restrict__ count,
                              const int32_t* __restrict__ half, const int4* __restrict__ tile_ranges,
                              const int* __restrict__ tile_nr, int row_len, LJConst lj, double* __restrict__ frc,
-                             double* __restrict__ partials, int* __restrict__ clear_flag, const int* halt) {
+                             double* __restrict__ partials, int* __restrict__ clear_flag, ForceStep fs,
+                             const int* halt) {
     if (halted(halt)) return;
+    constexpr int SUMS = STEP ? 4 : 3;
     __shared__ double halo[3][TILE_HALO_MAX];
-    __shared__ double e_part[PAIR_THREADS / WAVE][3];
+    __shared__ double e_part[PAIR_THREADS / WAVE][SUMS];
     const int tile = launch_tile(ts, static_cast<int>(blockIdx.x));
     const TileSystem sys = tile_system<BATCH>(ts, tile);
     const size_t n = sys.end, pstride = ts.stride;  // pstride: column / component stride
@@ -1721,26 +1727,60 @@ __global__ void __launch_bounds__(PAIR_THREADS, 3)
             }
         }
     }
-    if (active && par == 0) {
+    double ke = 0.0;
+    if constexpr (STEP) {
+        const double m = active && par == 0 ? fs.mass[i] : 1.0;
+        // x / 1 = x exactly: a wavefront of unit masses skips the divisions
+        const bool unit_mass = __ballot(m != 1.0) == 0;
+        if (active && par == 0) {
+            const double f[3] = {a.fx, a.fy, a.fz};
+            double v[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const double kick = fs.half_dt * f[c];
+                v[c] = fs.vel[c * pstride + i] + (unit_mass ? kick : kick / m);  // closing kick of this step
+            }
+            const double half_m = 0.5 * m;
+            ke = half_m * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const double kick = fs.half_dt * f[c];
+                v[c] = v[c] + (unit_mass ? kick : kick / m);  // opening kick of the next step
+                fs.vel[c * pstride + i] = v[c];
+            }
+            double4 r = ri;
+            r.x = wrap_coord(r.x + fs.dt * v[0], box, inv_box);
+            r.y = wrap_coord(r.y + fs.dt * v[1], box, inv_box);
+            r.z = wrap_coord(r.z + fs.dt * v[2], box, inv_box);
+            fs.pos_next[i] = r;
+            const double4 q = fs.ref[i];
+            const double dx = min_image(r.x - q.x, box, inv_box);
+            const double dy = min_image(r.y - q.y, box, inv_box);
+            const double dz = min_image(r.z - q.z, box, inv_box);
+            if (dx * dx + dy * dy + dz * dz > fs.max_disp2) *fs.stale_next = 1;
+        }
+    } else if (active && par == 0) {
         frc[i] = a.fx;
         frc[pstride + i] = a.fy;
         frc[2 * pstride + i] = a.fz;
     }
     const double cnt = wave_sum_dpp(static_cast<double>(a.c));
     const double s12 = wave_sum_dpp(a.s12), s6 = wave_sum_dpp(a.s6);
+    const double ke_wave = STEP ? wave_sum_dpp(ke) : 0.0;
     const int wave = threadIdx.x / WAVE;
     if (threadIdx.x % WAVE == 0) {
         e_part[wave][0] = lj.c4 * (s12 - s6) - lj.shift * cnt;
         e_part[wave][1] = lj.c24 * (2.0 * s12 - s6);
         e_part[wave][2] = cnt;
+        if constexpr (STEP) e_part[wave][3] = ke_wave;
     }
     if (clear_flag && blockIdx.x == 0 && threadIdx.x == 0) *clear_flag = 0;
     __syncthreads();
-    if (threadIdx.x < 3) {
+    if (threadIdx.x < SUMS) {
         double v = 0.0;
 #pragma unroll
         for (int q = 0; q < PAIR_THREADS / WAVE; ++q) v += e_part[q][threadIdx.x];
-        partials[static_cast<size_t>(tile) * 3 + threadIdx.x] = v;
+        partials[static_cast<size_t>(tile) * SUMS + threadIdx.x] = v;
     }
 }
 
@@ -1787,10 +1827,11 @@ void tiled_forces_fp64_enqueue(md_state* s, md_list* l, const LJConst& lj, doubl
     if (n > 0) {
         TileSystems ts = single_system(s, l, 0.0);
         ts.order = nullptr;  // the force kernel takes the tiles in cell order
-        auto kernel = lj.sig2 == 1.0 ? k_forces_tiled_fp64_pair<true, false> : k_forces_tiled_fp64_pair<false, false>;
+        auto kernel = lj.sig2 == 1.0 ? k_forces_tiled_fp64_pair<true, false, false>
+                                     : k_forces_tiled_fp64_pair<false, false, false>;
         hipLaunchKernelGGL(kernel, dim3(tiles), dim3(PAIR_THREADS), 0, st, s->pos, ts, l->rows16, l->rows, l->count,
                            l->half, l->tile_ranges, l->tile_nr, slot_row_len(l->padding), lj, s->frc, s->partials,
-                           clear_flag, s->halt);
+                           clear_flag, ForceStep{}, s->halt);
         MD_HIP(hipGetLastError());
     }
     if (ev_stop) MD_HIP(hipEventRecord(ev_stop, st));
@@ -1931,9 +1972,10 @@ void tiled_batch_forces_enqueue(hipStream_t st, const BatchTiles& bt, const doub
                            bt.rows16, bt.rows, bt.count, bt.tile_ranges, bt.tile_nr, row_len, lj, frc, partials,
                            nullptr, ForceStep{}, halt);
     } else {
-        auto kernel = lj.sig2 == 1.0 ? k_forces_tiled_fp64_pair<true, true> : k_forces_tiled_fp64_pair<false, true>;
+        auto kernel = lj.sig2 == 1.0 ? k_forces_tiled_fp64_pair<true, true, false>
+                                     : k_forces_tiled_fp64_pair<false, true, false>;
         hipLaunchKernelGGL(kernel, dim3(tiles), dim3(PAIR_THREADS), 0, st, pos, bt.ts, bt.rows16, bt.rows, bt.count,
-                           bt.half, bt.tile_ranges, bt.tile_nr, row_len, lj, frc, partials, nullptr, halt);
+                           bt.half, bt.tile_ranges, bt.tile_nr, row_len, lj, frc, partials, nullptr, ForceStep{}, halt);
     }
     MD_HIP(hipGetLastError());
 }
@@ -1943,8 +1985,8 @@ void tiled_forces_enqueue(md_state* s, md_list* l, const LJConst& lj, double cut
     tiled_forces_launch(s, l, lj, cutoff, out_dev, clear_flag, nullptr, StepRecord{}, ev_start, ev_stop);
 }
 
-void tiled_forces_step_enqueue(md_state* s, md_list* l, const md_lj& p, double dt, double4* pos_next, int* clear_flag,
-                               int* stale_next, double* out_dev, StepRecord rec, hipEvent_t ev_start,
+void tiled_forces_step_enqueue(md_state* s, md_list* l, const md_lj& p, int precision, double dt, double4* pos_next,
+                               int* clear_flag, int* stale_next, double* out_dev, StepRecord rec, hipEvent_t ev_start,
                                hipEvent_t ev_stop) {
     ForceStep fs;
     fs.pos_next = pos_next;
@@ -1955,7 +1997,31 @@ void tiled_forces_step_enqueue(md_state* s, md_list* l, const md_lj& p, double d
     fs.dt = dt;
     fs.max_disp2 = l->max_disp2;
     fs.stale_next = stale_next;
-    tiled_forces_launch(s, l, make_lj(p), p.cutoff, out_dev, clear_flag, &fs, rec, ev_start, ev_stop);
+    const LJConst lj = make_lj(p);
+    if (precision == MD_MIXED) {
+        tiled_forces_launch(s, l, lj, p.cutoff, out_dev, clear_flag, &fs, rec, ev_start, ev_stop);
+        return;
+    }
+    // FP64: the tiled FP64 kernel with the step folded in (unstaged tiles gather through the int32 rows, which the
+    // run's FP64 rebuilds write)
+    hipStream_t st = s->ctx->stream;
+    const size_t n = s->n;
+    const int tiles = tile_count(n ? n : 1);
+    state_reserve_partials(s, static_cast<size_t>(tiles) * 4);
+    ensure_rows32(l);
+    if (ev_start) MD_HIP(hipEventRecord(ev_start, st));
+    if (n > 0) {
+        TileSystems ts = single_system(s, l, 0.0);
+        ts.order = nullptr;  // the force kernels take the tiles in cell order
+        auto kernel = lj.sig2 == 1.0 ? k_forces_tiled_fp64_pair<true, false, true>
+                                     : k_forces_tiled_fp64_pair<false, false, true>;
+        hipLaunchKernelGGL(kernel, dim3(tiles), dim3(PAIR_THREADS), 0, st, s->pos, ts, l->rows16, l->rows, l->count,
+                           l->half, l->tile_ranges, l->tile_nr, slot_row_len(l->padding), lj, s->frc, s->partials,
+                           clear_flag, fs, s->halt);
+        MD_HIP(hipGetLastError());
+    }
+    if (ev_stop) MD_HIP(hipEventRecord(ev_stop, st));
+    reduce_partials_enqueue(s, n > 0 ? tiles : 0, 4, out_dev, rec);
 }
 
 }  // namespace mdgpu

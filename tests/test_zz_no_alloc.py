@@ -1,6 +1,9 @@
 """md_run_static, the allocation-free md_run (its counters, records and Bussi draws in the library's static device
 array), called through the C ABI, against md_run on the same inputs: bit-equal state and records, on two calls in
-a row, the second with a smaller system and fewer steps (the shared array keeps the first call's contents)."""
+a row, the second with a smaller system and fewer steps (the shared array keeps the first call's contents). An NVE
+md_run folds each step into the force kernel once the state has a spare position buffer, which md_run_static cannot
+allocate: before that it takes the separate kernels, with the same trajectory and the kinetic energy summed in
+another order."""
 import ctypes as C
 
 import numpy as np
@@ -60,8 +63,20 @@ def test_static_run_matches_md_run_twice(ctx):
     print("md_run_static: 2048 atoms FP64 NVE, 300 steps", flush=True)
     (a, la), (b, lb) = twin(ctx, 8, 0.722, 1, mdgpu.MD_FP64)
     ea = la.run(a, LJ, DT, 300, precision=mdgpu.MD_FP64, use_graph=False)
-    eb = static_run(b, lb, 300, mdgpu.MD_FP64)
+    eb = static_run(b, lb, 300, mdgpu.MD_FP64)  # no spare position buffer on b: the separate kernels
     assert la.info().rebuilds > 3
+    # md_run folds each NVE step into the force kernel (the kinetic energy summed per tile); the separate kernels
+    # give the same trajectory and records, the kinetic energy summed in another order
+    for f in (mdgpu.FIELD_POS, mdgpu.FIELD_VEL, mdgpu.FIELD_FORCE, mdgpu.FIELD_ID):
+        assert np.array_equal(a.download(f), b.download(f)), f
+    assert np.array_equal(ea[:, [0, 2, 3]], eb[:, [0, 2, 3]])
+    np.testing.assert_allclose(eb[:, 1], ea[:, 1], rtol=1e-12, atol=0)
+    assert la.info().rebuilds == lb.info().rebuilds and la.info().half_pairs == lb.info().half_pairs
+    # with a spare buffer on both (a short md_run each), md_run_static folds too: bit-equal records
+    la.run(a, LJ, DT, 2, precision=mdgpu.MD_FP64, use_graph=False, record=False)
+    lb.run(b, LJ, DT, 2, precision=mdgpu.MD_FP64, use_graph=False, record=False)
+    ea = la.run(a, LJ, DT, 100, precision=mdgpu.MD_FP64, use_graph=False)
+    eb = static_run(b, lb, 100, mdgpu.MD_FP64)
     assert_same(a, la, ea, b, lb, eb)
     # second call, smaller: 864 atoms, mixed with the Bussi thermostat, 120 steps
     print("md_run_static: 864 atoms mixed Bussi, 120 steps", flush=True)
