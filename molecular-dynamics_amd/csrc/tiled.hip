@@ -1374,6 +1374,21 @@ __global__ void __launch_bounds__(TILE_THREADS, 6)
     PairSumsF a;
     // inside pairs counted per wavefront on the compact walk (a uniform sum of ballots)
     unsigned wave_inside = 0;
+    // STEP: the particle's mass and its lane's component of velocity, position and list-build position. The compact
+    // walk issues these loads before its last preloaded chunk, when the slot words of the chunks already walked no
+    // longer hold registers, so they arrive while that chunk is evaluated; the other walks load them afterwards.
+    const int comp = lane < 3 ? lane : 2;
+    double st_m = 0.0, st_v = 0.0, st_r = 0.0, st_ref = 0.0;
+    auto load_step = [&]() {
+        if constexpr (STEP) {
+            if (active) {
+                st_m = fs.mass[i];
+                st_v = fs.vel[comp * pstride + i];
+                st_r = reinterpret_cast<const double*>(pos + i)[comp];
+                st_ref = reinterpret_cast<const double*>(fs.ref + i)[comp];
+            }
+        }
+    };
     if (compact) {
         // Compact tile: every listed separation is already the minimum image. Each lane walks whole chunks: the
         // entries past its row read the ghost slot (the build ends every row's last chunk with ghost entries, and
@@ -1400,8 +1415,10 @@ __global__ void __launch_bounds__(TILE_THREADS, 6)
             }
         };
 #pragma unroll
-        for (int c = 0; c < PRE_CHUNKS; ++c)
+        for (int c = 0; c < PRE_CHUNKS; ++c) {
+            if (c == PRE_CHUNKS - 1) load_step();
             if (c < cw) chunk_pairs(pre[c], c);
+        }
         for (int c = PRE_CHUNKS; c < cw; ++c) chunk_pairs(c < chunks ? rp[c * stride] : ghost4, c);
     } else if (active) {
         if (staged) {
@@ -1454,6 +1471,7 @@ __global__ void __launch_bounds__(TILE_THREADS, 6)
             }
         }
     }
+    if (!compact) load_step();
     // A halted run (a list build of this step overflowed) stores nothing: the state stays at the last valid step.
     // Tested here rather than first, so the halt word's load does not delay the tile's loads; the walk above
     // read the truncated rows of such a build, which stay within the tile's halo and rows.
@@ -1469,7 +1487,6 @@ __global__ void __launch_bounds__(TILE_THREADS, 6)
         a.c += __shfl_xor(a.c, off, WAVE);
     }
     // every lane now holds the particle's sums; lane c (c < 3) takes force component c (lane 3 repeats z)
-    const int comp = lane < 3 ? lane : 2;
     const double c24 = static_cast<double>(lj.c24f);
     const double fc = c24 * (comp == 0 ? a.gx : (comp == 1 ? a.gy : a.gz));
     const int pl = threadIdx.x >> 2;
@@ -1477,21 +1494,20 @@ __global__ void __launch_bounds__(TILE_THREADS, 6)
     if constexpr (STEP) {
         // the arithmetic of k_kick_kick_drift, one component per lane; the kinetic energy and the displacement test
         // gather the three components on lane 0 and add them in x, y, z order
-        double m = 0.0, kick = 0.0, v = 0.0, d = 0.0, r = 0.0;
-        const double* pi = reinterpret_cast<const double*>(pos + i);
-        if (active) m = fs.mass[i];
+        const double m = st_m;
+        double kick = 0.0, v = 0.0, d = 0.0, r = 0.0;
         // x / 1 = x exactly: a wavefront of unit masses skips the division
         const bool unit_mass = __ballot(active && m != 1.0) == 0;
         if (active) {
             kick = unit_mass ? fs.half_dt * fc : fs.half_dt * fc / m;
-            v = fs.vel[comp * pstride + i] + kick;  // closing kick of this step
+            v = st_v + kick;  // closing kick of this step
         }
         const double vy = __shfl_xor(v, 1, WAVE), vz = __shfl_xor(v, 2, WAVE);
         if (active) {
             ke = 0.5 * m * (v * v + vy * vy + vz * vz);  // lane 0's is the particle's
             v = v + kick;                                 // opening kick of the next step
-            r = wrap_coord(pi[comp] + fs.dt * v, fr.box, fr.inv_box);
-            d = min_image(r - reinterpret_cast<const double*>(fs.ref + i)[comp], fr.box, fr.inv_box);
+            r = wrap_coord(st_r + fs.dt * v, fr.box, fr.inv_box);
+            d = min_image(r - st_ref, fr.box, fr.inv_box);
         }
         const double dy = __shfl_xor(d, 1, WAVE), dz = __shfl_xor(d, 2, WAVE);
         if (active) {
@@ -1500,7 +1516,7 @@ __global__ void __launch_bounds__(TILE_THREADS, 6)
                 fs.vel[comp * pstride + i] = v;
                 next[lane] = r;
             } else {
-                next[3] = pi[3];
+                next[3] = reinterpret_cast<const double*>(pos + i)[3];
             }
             if (lane == 0 && d * d + dy * dy + dz * dz > fs.max_disp2) *fs.stale_next = 1;
         }
@@ -1639,6 +1655,18 @@ __global__ void __launch_bounds__(PAIR_THREADS, 3)
     const int m = active ? count[i] : 0;
     const int h = active ? half[i] : 0;
     const double4 ri = active ? pos[i] : make_double4(0.0, 0.0, 0.0, 0.0);
+    // STEP: the even lane's mass, velocity and list-build position, loaded with the header (the registers are there
+    // at three wavefronts per SIMD), so they have arrived when the pair loop ends
+    double st_m = 1.0, st_v[3] = {0.0, 0.0, 0.0};
+    double4 st_ref = make_double4(0.0, 0.0, 0.0, 0.0);
+    if constexpr (STEP) {
+        if (active && par == 0) {
+            st_m = fs.mass[i];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) st_v[c] = fs.vel[c * pstride + i];
+            st_ref = fs.ref[i];
+        }
+    }
     if (staged) {
         stage_halo_lanes<PAIR_THREADS / WAVE>(pos, rr, nr, [&](int q, const double4& p) {
             halo[0][q] = p.x;
@@ -1729,7 +1757,7 @@ __global__ void __launch_bounds__(PAIR_THREADS, 3)
     }
     double ke = 0.0;
     if constexpr (STEP) {
-        const double m = active && par == 0 ? fs.mass[i] : 1.0;
+        const double m = st_m;
         // x / 1 = x exactly: a wavefront of unit masses skips the divisions
         const bool unit_mass = __ballot(m != 1.0) == 0;
         if (active && par == 0) {
@@ -1738,7 +1766,7 @@ __global__ void __launch_bounds__(PAIR_THREADS, 3)
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
                 const double kick = fs.half_dt * f[c];
-                v[c] = fs.vel[c * pstride + i] + (unit_mass ? kick : kick / m);  // closing kick of this step
+                v[c] = st_v[c] + (unit_mass ? kick : kick / m);  // closing kick of this step
             }
             const double half_m = 0.5 * m;
             ke = half_m * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
@@ -1753,7 +1781,7 @@ __global__ void __launch_bounds__(PAIR_THREADS, 3)
             r.y = wrap_coord(r.y + fs.dt * v[1], box, inv_box);
             r.z = wrap_coord(r.z + fs.dt * v[2], box, inv_box);
             fs.pos_next[i] = r;
-            const double4 q = fs.ref[i];
+            const double4 q = st_ref;
             const double dx = min_image(r.x - q.x, box, inv_box);
             const double dy = min_image(r.y - q.y, box, inv_box);
             const double dz = min_image(r.z - q.z, box, inv_box);
