@@ -79,6 +79,10 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
+// DPP quad permutes: every lane reads lane ^ 1, or lane ^ 2, of its quad
+constexpr int QUAD_XOR1 = 0xB1;  // quad_perm [1,0,3,2]
+constexpr int QUAD_XOR2 = 0x4E;  // quad_perm [2,3,0,1]
+
 // Lanes' values moved by one DPP control (CTRL) within the rows of ROW_MASK; lanes not written read 0.
 template <int CTRL, int ROW_MASK = 0xF>
 __device__ __forceinline__ double dpp_f64(double v) {

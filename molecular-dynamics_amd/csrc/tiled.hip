@@ -1374,9 +1374,10 @@ __global__ void __launch_bounds__(TILE_THREADS, 6)
     PairSumsF a;
     // inside pairs counted per wavefront on the compact walk (a uniform sum of ballots)
     unsigned wave_inside = 0;
-    // STEP: the particle's mass and its lane's component of velocity, position and list-build position. The compact
-    // walk issues these loads before its last preloaded chunk, when the slot words of the chunks already walked no
-    // longer hold registers, so they arrive while that chunk is evaluated; the other walks load them afterwards.
+    // STEP: the particle's mass and its lane's component of velocity, position and list-build position (lane 3, whose
+    // step only repeats z's, takes the position's fourth word, which the drift copies). The compact walk issues these
+    // loads before its last preloaded chunk, when the slot words of the chunks already walked no longer hold
+    // registers, so they arrive while that chunk is evaluated; the other walks load them afterwards.
     const int comp = lane < 3 ? lane : 2;
     double st_m = 0.0, st_v = 0.0, st_r = 0.0, st_ref = 0.0;
     auto load_step = [&]() {
@@ -1384,7 +1385,7 @@ __global__ void __launch_bounds__(TILE_THREADS, 6)
             if (active) {
                 st_m = fs.mass[i];
                 st_v = fs.vel[comp * pstride + i];
-                st_r = reinterpret_cast<const double*>(pos + i)[comp];
+                st_r = reinterpret_cast<const double*>(pos + i)[lane];
                 st_ref = reinterpret_cast<const double*>(fs.ref + i)[comp];
             }
         }
@@ -1476,16 +1477,19 @@ __global__ void __launch_bounds__(TILE_THREADS, 6)
     // Tested here rather than first, so the halt word's load does not delay the tile's loads; the walk above
     // read the truncated rows of such a build, which stay within the tile's halo and rows.
     if (halted(halt)) return;
-    // the particle's four lanes are adjacent in the wavefront
-#pragma unroll
-    for (int off = 1; off < LANES; off <<= 1) {
-        a.gx += __shfl_xor(a.gx, off, WAVE);
-        a.gy += __shfl_xor(a.gy, off, WAVE);
-        a.gz += __shfl_xor(a.gz, off, WAVE);
-        a.t += __shfl_xor(a.t, off, WAVE);
-        a.s += __shfl_xor(a.s, off, WAVE);
-        a.c += __shfl_xor(a.c, off, WAVE);
-    }
+    // the particle's four lanes are a quad of the wavefront: its partner lanes' values (lane ^ 1, then lane ^ 2)
+    // arrive by DPP quad permutes, with no LDS round trip
+    auto quad_sums = [&](auto ctrl) {
+        constexpr int CTRL = decltype(ctrl)::value;
+        a.gx += dpp_f64<CTRL>(a.gx);
+        a.gy += dpp_f64<CTRL>(a.gy);
+        a.gz += dpp_f64<CTRL>(a.gz);
+        a.t += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(a.t), CTRL, 0xF, 0xF, false));
+        a.s += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(a.s), CTRL, 0xF, 0xF, false));
+        a.c += __builtin_amdgcn_update_dpp(0, a.c, CTRL, 0xF, 0xF, false);
+    };
+    quad_sums(std::integral_constant<int, QUAD_XOR1>{});
+    quad_sums(std::integral_constant<int, QUAD_XOR2>{});
     // every lane now holds the particle's sums; lane c (c < 3) takes force component c (lane 3 repeats z)
     const double c24 = static_cast<double>(lj.c24f);
     const double fc = c24 * (comp == 0 ? a.gx : (comp == 1 ? a.gy : a.gz));
@@ -1502,21 +1506,21 @@ __global__ void __launch_bounds__(TILE_THREADS, 6)
             kick = unit_mass ? fs.half_dt * fc : fs.half_dt * fc / m;
             v = st_v + kick;  // closing kick of this step
         }
-        const double vy = __shfl_xor(v, 1, WAVE), vz = __shfl_xor(v, 2, WAVE);
+        const double vy = dpp_f64<QUAD_XOR1>(v), vz = dpp_f64<QUAD_XOR2>(v);
         if (active) {
             ke = 0.5 * m * (v * v + vy * vy + vz * vz);  // lane 0's is the particle's
             v = v + kick;                                 // opening kick of the next step
             r = wrap_coord(st_r + fs.dt * v, fr.box, fr.inv_box);
             d = min_image(r - st_ref, fr.box, fr.inv_box);
         }
-        const double dy = __shfl_xor(d, 1, WAVE), dz = __shfl_xor(d, 2, WAVE);
+        const double dy = dpp_f64<QUAD_XOR1>(d), dz = dpp_f64<QUAD_XOR2>(d);
         if (active) {
             double* next = reinterpret_cast<double*>(fs.pos_next + i);
             if (lane < 3) {
                 fs.vel[comp * pstride + i] = v;
                 next[lane] = r;
             } else {
-                next[3] = reinterpret_cast<const double*>(pos + i)[3];
+                next[3] = st_r;
             }
             if (lane == 0 && d * d + dy * dy + dz * dz > fs.max_disp2) *fs.stale_next = 1;
         }
