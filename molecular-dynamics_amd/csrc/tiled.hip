@@ -1300,7 +1300,7 @@ __global__ void __launch_bounds__(TILE_THREADS, 6)
     constexpr int SUMS = STEP ? 4 : 3;
     double(*e_part)[TILE] = reinterpret_cast<double(*)[TILE]>(&halo[0][0]);
     static_assert(4 * TILE <= TILE_HALO_MAX, "the per-particle sums fit in the halo's x array");
-    __shared__ int extent_bits;
+    __shared__ int wave_extent[TILE_THREADS / WAVE];
     const int tile = launch_tile(ts, static_cast<int>(blockIdx.x));
     const TileSystem sys = tile_system<BATCH>(ts, tile);
     const TileFrame fr{sys.box, sys.inv_box, sys.reach};
@@ -1316,30 +1316,39 @@ __global__ void __launch_bounds__(TILE_THREADS, 6)
     const int lane = threadIdx.x & (LANES - 1);
     const size_t i = i0 + (threadIdx.x >> 2);
     const bool active = i < n;
-    const int m = active ? count[i] : 0;  // row length, loaded first: the slot loads below depend on it
+    // Header loads, unconditional so that no branch separates them: lanes past n read particle i0's count and
+    // position, which they do not use.
+    const size_t ic = active ? i : i0;
+    const int m_read = count[ic];  // row length, loaded first: the slot loads below depend on it
     const double4 ref = pos[i0];
-    // the particle's own frame coordinates, its position loaded with the header
-    double xi = 0.0, yi = 0.0, zi = 0.0;
-    if (active) {
-        const double4 ri = pos[i];
-        if (flat) {
-            xi = ri.x - ref.x;
-            yi = ri.y - ref.y;
-            zi = ri.z - ref.z;
-        } else {
-            xi = frame_coord(ri.x, ref.x, fr);
-            yi = frame_coord(ri.y, ref.y, fr);
-            zi = frame_coord(ri.z, ref.z, fr);
-        }
+    const double4 ri = pos[ic];  // used once the halo is staged
+    const int m = active ? m_read : 0;
+    // The lane's slot words of the row's first PRE_CHUNKS chunks, issued before the halo's loads, so both are in
+    // flight together and the pair loop runs from registers (longer rows load their remaining chunks in the loop);
+    // chunks past the row are ghost words. The loads are unconditional: a chunk index past the padded row (which
+    // every slot of the tile has, unstaged tiles included) is clamped to its last chunk, whose words go unused.
+    const int chunks = (m + CHUNK - 1) / CHUNK;
+    const uint4* rp = reinterpret_cast<const uint4*>(rows16 + slot_row_base(i, row_len) + lane * 8);
+    constexpr int stride = CHUNK_STRIDE / 8;  // uint4 words between chunks
+    constexpr unsigned GHOST_WORD = (GHOST_SLOT * SLOT_BYTES) * 0x10001u;  // two ghost entries
+    const uint4 ghost4 = make_uint4(GHOST_WORD, GHOST_WORD, GHOST_WORD, GHOST_WORD);
+    uint4 pre[PRE_CHUNKS];
+#pragma unroll
+    for (int c = 0; c < PRE_CHUNKS; ++c) pre[c] = rp[max(0, min(c, row_len / CHUNK - 1)) * stride];  // row_len >= CHUNK
+#pragma unroll
+    for (int c = 0; c < PRE_CHUNKS; ++c) {
+        const bool live = staged && c < chunks;
+        pre[c].x = live ? pre[c].x : GHOST_WORD;
+        pre[c].y = live ? pre[c].y : GHOST_WORD;
+        pre[c].z = live ? pre[c].z : GHOST_WORD;
+        pre[c].w = live ? pre[c].w : GHOST_WORD;
     }
     if (threadIdx.x == 0) {
-        extent_bits = 0;
         // the ghost slot, far from every frame coordinate: its pair term is exactly zero (see the compact walk)
         xs[GHOST_SLOT] = GHOST_COORD;
         ys[GHOST_SLOT] = GHOST_COORD;
         zs[GHOST_SLOT] = GHOST_COORD;
     }
-    __syncthreads();
     if (staged && flat) {
         stage_halo_lanes<TILE_THREADS / WAVE>(pos, rr, nr, [&](int q, const double4& p) {
             xs[q] = p.x - ref.x;
@@ -1353,23 +1362,29 @@ __global__ void __launch_bounds__(TILE_THREADS, 6)
             zs[q] = frame_coord(p.z, ref.z, fr);
         });
     }
-    // the lane's slot words of the row's first PRE_CHUNKS chunks, all issued before the barrier, so the pair loop
-    // below runs from registers (longer rows load their remaining chunks in the loop); chunks past the row are
-    // ghost words
-    const int chunks = (m + CHUNK - 1) / CHUNK;
-    const uint4* rp = reinterpret_cast<const uint4*>(rows16 + slot_row_base(i, row_len) + lane * 8);
-    constexpr int stride = CHUNK_STRIDE / 8;  // uint4 words between chunks
-    constexpr unsigned GHOST_WORD = (GHOST_SLOT * SLOT_BYTES) * 0x10001u;  // two ghost entries
-    const uint4 ghost4 = make_uint4(GHOST_WORD, GHOST_WORD, GHOST_WORD, GHOST_WORD);
-    uint4 pre[PRE_CHUNKS];
-#pragma unroll
-    for (int c = 0; c < PRE_CHUNKS; ++c) pre[c] = staged && c < chunks ? rp[c * stride] : ghost4;
-    if (active && lane == 0) {  // how far the tile's own particles lie from the reference particle
-        const float e = static_cast<float>(fmax(fabs(xi), fmax(fabs(yi), fabs(zi))));
-        atomicMax(&extent_bits, __float_as_int(e));
+    // the particle's own frame coordinates
+    double xi = 0.0, yi = 0.0, zi = 0.0;
+    if (active) {
+        if (flat) {
+            xi = ri.x - ref.x;
+            yi = ri.y - ref.y;
+            zi = ri.z - ref.z;
+        } else {
+            xi = frame_coord(ri.x, ref.x, fr);
+            yi = frame_coord(ri.y, ref.y, fr);
+            zi = frame_coord(ri.z, ref.z, fr);
+        }
+    }
+    {  // how far the tile's own particles lie from the reference particle: per wavefront, then over the block
+        const float e = active ? static_cast<float>(fmax(fabs(xi), fmax(fabs(yi), fabs(zi)))) : 0.0f;
+        const int e_wave = wave_max_dpp(__float_as_int(e));  // non-negative floats order as their bit patterns
+        if (threadIdx.x % WAVE == 0) wave_extent[threadIdx.x / WAVE] = e_wave;
     }
     __syncthreads();
-    const float e = __int_as_float(extent_bits);
+    int e_bits = 0;
+#pragma unroll
+    for (int w = 0; w < TILE_THREADS / WAVE; ++w) e_bits = max(e_bits, wave_extent[w]);
+    const float e = __int_as_float(e_bits);
     const bool compact = staged && (2.0f * e + static_cast<float>(fr.reach) < static_cast<float>(fr.box) - 1.0f);
     PairSumsF a;
     // inside pairs counted per wavefront on the compact walk (a uniform sum of ballots)
