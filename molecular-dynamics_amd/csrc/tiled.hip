@@ -895,7 +895,9 @@ __device__ __forceinline__ void list_build_tile(
     constexpr double EPS32 = 5.9604644775390625e-08;  // 2^-24
     const double band = compact ? 4.0 * (2.0 * 1.7320508075688772 * r_l * (2.0 * big_m + 3.0 + r_l) + 3.0 * radius2) * EPS32
                                 : (72.75 * r_l * box + 8.0 * radius2) * EPS32;
-    const float r2c = static_cast<float>(radius2), bandf = static_cast<float>(band);
+    // thresholds of the float r^2, rounded outwards: below r2_in a candidate is inside, above r2_out outside, in
+    // between it is decided by the exact test
+    const float r2_in = __double2float_rd(radius2 - band), r2_out = __double2float_ru(radius2 + band);
     const float box_f = static_cast<float>(box), inv_box_f = static_cast<float>(inv_box);
     const bool lists_ok = use_lists && s_lists_ok && padding <= ROWBUF_CAP;
     // fill: each lane copies its segment's slots; the list's lanes then pad it to a multiple of 64 with the ghost
@@ -950,9 +952,9 @@ __device__ __forceinline__ void list_build_tile(
             // A batch is 64 candidates: each lane tests one, the accepted ones are ranked over the ballot (mbcnt
             // counts the set bits below the lane, on top of the row's count so far) and stored. Two batches are
             // loaded and tested before either stores, so their LDS reads overlap.
-            // FH: d = float r^2 minus the list radius squared (infinite for the ghost); a candidate is inside when
-            // d < -band, unsure when |d| <= band (both from the same d, so every candidate is one of inside, unsure,
-            // outside), and an unsure one is decided by the exact test in double
+            // FH: the float r^2 (infinite for the ghost); a candidate is inside when r^2 < r2_in, unsure when
+            // r2_in <= r^2 <= r2_out (every candidate is one of inside, unsure, outside), and an unsure one is
+            // decided by the exact test in double
             // `flat_tag`: a compact frame (no image term); otherwise the float minimum image per component
             auto dist_f = [&](int sl, auto flat_tag) {
                 const float* const cj = coords_f(sl);
@@ -962,10 +964,10 @@ __device__ __forceinline__ void list_build_tile(
                     dy = fmaf(-box_f, __builtin_rintf(dy * inv_box_f), dy);
                     dz = fmaf(-box_f, __builtin_rintf(dz * inv_box_f), dz);
                 }
-                return fmaf(dx, dx, fmaf(dy, dy, dz * dz)) - r2c;
+                return fmaf(dx, dx, fmaf(dy, dy, dz * dz));
             };
-            auto unsure_f = [&](float d) { return fabsf(d) <= bandf; };
-            auto inside_f = [&](float d) { return d < -bandf; };
+            auto unsure_f = [&](float d) { return d <= r2_out && !(d < r2_in); };
+            auto inside_f = [&](float d) { return d < r2_in; };
             auto resolve_f = [&](int sl, float d, bool unsure) {
                 bool in = inside_f(d);
                 if (unsure) {
@@ -997,7 +999,8 @@ __device__ __forceinline__ void list_build_tile(
                         if (write32) row[static_cast<uint64_t>(at) * stride] = halo_particle(sl / LIST_BYTES, hr, nr);
                     }
                 }
-                cnt += __popcll(mask);
+                // the count is uniform: kept in a scalar register, the popcount and addition are scalar work
+                cnt = __builtin_amdgcn_readfirstlane(cnt + __popcll(mask));
             };
             // `upper_tag`: the U list (keep j > i) or the L list (keep j < i), a compile-time choice, so the order
             // test is a single comparison whose lane mask the ballots use directly
@@ -1010,17 +1013,18 @@ __device__ __forceinline__ void list_build_tile(
                     for (; q0 + 2 * WAVE <= len; q0 += 2 * WAVE) {
                         const int sa = lists[off + q0 + lane], sb = lists[off + q0 + WAVE + lane];
                         const float ra = dist_f(sa, interior_tag), rb2 = dist_f(sb, interior_tag);
-                        const bool ua = unsure_f(ra), ub = unsure_f(rb2);
                         const bool oa = upper ? sa > slot_i : sa < slot_i, ob = upper ? sb > slot_i : sb < slot_i;
                         const bool na = inside_f(ra), nb = inside_f(rb2);
                         // every ballot next to its comparison, where it is that comparison's own lane mask
-                        const unsigned long long ma = __ballot(na) & __ballot(oa);
-                        const unsigned long long mb = __ballot(nb) & __ballot(ob);
-                        if ((__ballot(ua) | __ballot(ub)) == 0) {
+                        const unsigned long long in_a = __ballot(na), in_b = __ballot(nb);
+                        const unsigned long long ma = in_a & __ballot(oa);
+                        const unsigned long long mb = in_b & __ballot(ob);
+                        // unsure lanes: up to r2_out and not inside
+                        if (((__ballot(ra <= r2_out) & ~in_a) | (__ballot(rb2 <= r2_out) & ~in_b)) == 0) {
                             commit(sa, na && oa, ma);
                             commit(sb, nb && ob, mb);
                         } else {
-                            const bool xa = resolve_f(sa, ra, ua), xb = resolve_f(sb, rb2, ub);
+                            const bool xa = resolve_f(sa, ra, unsure_f(ra)), xb = resolve_f(sb, rb2, unsure_f(rb2));
                             commit(sa, xa && oa, __ballot(xa && oa));
                             commit(sb, xb && ob, __ballot(xb && ob));
                         }
