@@ -626,9 +626,11 @@ __global__ void __launch_bounds__(BT)
     block_sum<3, BT>(acc, partials + static_cast<size_t>(blockIdx.x) * 3);
 }
 
-// Per replica, one wavefront: potential, virial, inside pairs -> scalars[0..2], from the force kernel's partial
-// rows, each covering `unit` particle slots (a tile, or a block of the per-particle kernel). Lane l adds rows
-// l, l + 64, ... of the replica, then a butterfly combines the lanes: a fixed order.
+// Per replica, one wavefront: potential, virial, inside pairs -> scalars[0..2] (and with SUMS = 4 the kicked
+// kinetic energy -> scalars[3]), from the force kernel's partial rows, each covering `unit` particle slots (a tile,
+// or a block of the per-particle kernel). Lane l adds rows l, l + 64, ... of the replica, then a butterfly
+// combines the lanes: a fixed order.
+template <int SUMS>
 __global__ void __launch_bounds__(REDUCE_WAVES * WAVE)
     k_b_reduce_forces(const Rep* __restrict__ reps, int count, int unit, const double* __restrict__ partials,
                       double* __restrict__ scalars, const int* halt) {
@@ -637,11 +639,12 @@ __global__ void __launch_bounds__(REDUCE_WAVES * WAVE)
     if (r >= count) return;
     const Rep rp = reps[r];
     const uint32_t b0 = rp.first / unit, b1 = (rp.first + rp.n + unit - 1) / unit;
-    double s[3] = {0.0, 0.0, 0.0};
+    double s[SUMS] = {};
     for (uint32_t b = b0 + lane; b < b1; b += WAVE)
-        for (int k = 0; k < 3; ++k) s[k] += partials[static_cast<size_t>(b) * 3 + k];
-    for (int k = 0; k < 3; ++k) s[k] = wave_sum(s[k]);
-    if (lane < 3) scalars[r * SCALARS + lane] = lane == 0 ? s[0] : (lane == 1 ? s[1] : s[2]);
+        for (int k = 0; k < SUMS; ++k) s[k] += partials[static_cast<size_t>(b) * SUMS + k];
+    for (int k = 0; k < SUMS; ++k) s[k] = wave_sum(s[k]);
+    if (lane == 0)
+        for (int k = 0; k < SUMS; ++k) scalars[r * SCALARS + k] = s[k];
 }
 
 // After a host-driven build: count it and clear the stale flags.
@@ -1001,15 +1004,24 @@ void enqueue_rebuild(md_batch* b) {
     MD_HIP(hipGetLastError());
 }
 
-void enqueue_forces(md_batch* b, const LJConst& lj, int precision, int lanes, hipEvent_t e0, hipEvent_t e1) {
+// kick_half_dt > 0 (tiled mixed batches only, see kick_folds): the force kernel also applies the closing kick
+// and the kicked kinetic energy goes to scalars[3], as k_b_kick_ke and k_b_ke_final would leave it
+void enqueue_forces(md_batch* b, const LJConst& lj, int precision, int lanes, hipEvent_t e0, hipEvent_t e1,
+                    double kick_half_dt) {
     hipStream_t st = b->ctx->stream;
     const size_t S = b->slots;
     if (b->tiled) {
+        const bool kick = kick_half_dt > 0.0;
         if (e0) MD_HIP(hipEventRecord(e0, st));
-        tiled_batch_forces_enqueue(st, batch_tiles(b), b->pos, lj, precision, b->frc, b->partials, b->halt);
+        tiled_batch_forces_enqueue(st, batch_tiles(b), b->pos, lj, precision, b->frc, b->partials, b->halt,
+                                   kick ? b->vel : nullptr, b->mass, kick_half_dt);
         if (e1) MD_HIP(hipEventRecord(e1, st));
-        k_b_reduce_forces<<<blocks_for(b->count, REDUCE_WAVES), REDUCE_WAVES * WAVE, 0, st>>>(
-            b->d_reps, b->count, TILE_PARTICLES, b->partials, b->scalars, b->halt);
+        if (kick)
+            k_b_reduce_forces<4><<<blocks_for(b->count, REDUCE_WAVES), REDUCE_WAVES * WAVE, 0, st>>>(
+                b->d_reps, b->count, TILE_PARTICLES, b->partials, b->scalars, b->halt);
+        else
+            k_b_reduce_forces<3><<<blocks_for(b->count, REDUCE_WAVES), REDUCE_WAVES * WAVE, 0, st>>>(
+                b->d_reps, b->count, TILE_PARTICLES, b->partials, b->scalars, b->halt);
         MD_HIP(hipGetLastError());
         return;
     }
@@ -1032,9 +1044,17 @@ void enqueue_forces(md_batch* b, const LJConst& lj, int precision, int lanes, hi
 #undef MD_B_FORCES
     MD_HIP(hipGetLastError());
     if (e1) MD_HIP(hipEventRecord(e1, st));
-    k_b_reduce_forces<<<blocks_for(b->count, REDUCE_WAVES), REDUCE_WAVES * WAVE, 0, st>>>(
+    k_b_reduce_forces<3><<<blocks_for(b->count, REDUCE_WAVES), REDUCE_WAVES * WAVE, 0, st>>>(
         b->d_reps, b->count, BT / lanes, b->partials, b->scalars, b->halt);
     MD_HIP(hipGetLastError());
+}
+
+// The closing kick and its kinetic energy ride on the force kernel for tiled mixed batches under the Bussi
+// thermostat (the kinetic energy is then summed per tile and per replica instead of in the reference's 1024-particle
+// blocks, a rounding-level difference that mixed precision allows); FP64 keeps k_b_kick_ke's order, so its replicas
+// stay bit-equal to md_run.
+bool kick_folds(const md_batch* b, const md_batch_config* cfg) {
+    return b->tiled && cfg->precision == MD_MIXED && cfg->thermostat;
 }
 
 void check_boxes(const md_batch* b, double cutoff) {
@@ -1374,7 +1394,7 @@ int md_batch_run(md_batch* b, const md_batch_config* cfg, double* energies) {
     const size_t S = b->slots;
     const int count = b->count;
     if (!b->forces_valid) {
-        enqueue_forces(b, lj, cfg->precision, lanes, nullptr, nullptr);
+        enqueue_forces(b, lj, cfg->precision, lanes, nullptr, nullptr, 0.0);
         b->forces_valid = true;
     }
     const int steps = cfg->steps;
@@ -1447,11 +1467,15 @@ int md_batch_run(md_batch* b, const md_batch_config* cfg, double* energies) {
             MD_HIP(hipEventCreate(&e1));
             events.push_back(e1);
         }
-        enqueue_forces(b, lj, cfg->precision, lanes, e0, e1);
+        const bool fold = kick_folds(b, cfg);
+        enqueue_forces(b, lj, cfg->precision, lanes, e0, e1, fold ? half_dt : 0.0);
         if (cfg->thermostat) {
-            k_b_kick_ke<<<b->ke_total, BT, 0, st>>>(b->d_reps, b->block_rep, b->ke_start, b->vel, b->frc, b->mass, S,
-                                                    half_dt, b->ke_part, b->halt);
-            k_b_ke_final<<<blocks_for(count, 256), 256, 0, st>>>(b->d_reps, count, b->ke_part, b->scalars, b->halt);
+            if (!fold) {
+                k_b_kick_ke<<<b->ke_total, BT, 0, st>>>(b->d_reps, b->block_rep, b->ke_start, b->vel, b->frc,
+                                                        b->mass, S, half_dt, b->ke_part, b->halt);
+                k_b_ke_final<<<blocks_for(count, 256), 256, 0, st>>>(b->d_reps, count, b->ke_part, b->scalars,
+                                                                     b->halt);
+            }
             k_b_bussi<<<blocks_for(count, 256), 256, 0, st>>>(count, b->scalars, rb.draws, steps, rb.draw_index,
                                                               rb.dof_kbar, c, b->halt);
             k_b_record<<<1, 1024, 0, st>>>(count, b->scalars, 6, rb.energies, b->stale, b->rebuilds, b->flags,

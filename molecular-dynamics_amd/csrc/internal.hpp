@@ -322,10 +322,12 @@ int tiled_row_len(int padding);                   // slot-row entries per partic
 // rebuild of the gated replicas: halo ranges from the re-sorted cells, then rows from build_pos (copied back by cb)
 void tiled_batch_build_enqueue(hipStream_t st, const BatchTiles& bt, const double4* build_pos, const int* cell_of,
                                const uint32_t* start, double radius2, const TiledCopyBack& cb, const int* halt);
-// forces of every replica: per tile U, W, inside count into partials[3 tile ..]
+// forces of every replica: per tile U, W, inside count into partials[3 tile ..]; with kick_vel (mixed only) also the
+// closing kick of the velocities and the kicked kinetic energy, the tile's fourth sum (partials[4 tile ..])
 struct LJConst;
 void tiled_batch_forces_enqueue(hipStream_t st, const BatchTiles& bt, const double4* pos, const LJConst& lj,
-                                int precision, double* frc, double* partials, const int* halt);
+                                int precision, double* frc, double* partials, const int* halt,
+                                double* kick_vel = nullptr, const double* mass = nullptr, double half_dt = 0.0);
 size_t tiled_rows16_entries(size_t n, int padding);
 void tiled_list_enqueue(md_list* l, md_state* s, bool from_alt, const int* gate, bool write32, bool carry_forces);
 void tiled_rows32_enqueue(md_list* l);                          // int32 rows from the slot rows

@@ -1404,8 +1404,10 @@ __global__ void __launch_bounds__(TILE_THREADS, 6)
             if (active) {
                 st_m = fs.mass[i];
                 st_v = fs.vel[comp * pstride + i];
-                st_r = reinterpret_cast<const double*>(pos + i)[lane];
-                st_ref = reinterpret_cast<const double*>(fs.ref + i)[comp];
+                if constexpr (!BATCH) {
+                    st_r = reinterpret_cast<const double*>(pos + i)[lane];
+                    st_ref = reinterpret_cast<const double*>(fs.ref + i)[comp];
+                }
             }
         }
     };
@@ -1526,8 +1528,13 @@ __global__ void __launch_bounds__(TILE_THREADS, 6)
             v = st_v + kick;  // closing kick of this step
         }
         const double vy = dpp_f64<QUAD_XOR1>(v), vz = dpp_f64<QUAD_XOR2>(v);
+        if (active) ke = 0.5 * m * (v * v + vy * vy + vz * vz);  // lane 0's is the particle's
+        if constexpr (BATCH) {
+            // batched replicas (Bussi): the closing kick only, since the thermostat scales each replica's velocities
+            // once its kinetic energy is summed; the next step's opening kick reads the stored force
+            if (active && lane < 3) fs.vel[comp * pstride + i] = v;
+        } else {
         if (active) {
-            ke = 0.5 * m * (v * v + vy * vy + vz * vz);  // lane 0's is the particle's
             v = v + kick;                                 // opening kick of the next step
             r = wrap_coord(st_r + fs.dt * v, fr.box, fr.inv_box);
             d = min_image(r - st_ref, fr.box, fr.inv_box);
@@ -1543,8 +1550,9 @@ __global__ void __launch_bounds__(TILE_THREADS, 6)
             }
             if (lane == 0 && d * d + dy * dy + dz * dz > fs.max_disp2) *fs.stale_next = 1;
         }
+        }
     }
-    if (active && !STEP && lane < 3) frc[lane * pstride + i] = fc;
+    if (active && (!STEP || BATCH) && lane < 3) frc[lane * pstride + i] = fc;
     __syncthreads();  // every wavefront is done with the halo
     if (lane == 0) {
         // every pair was met from both ends: half of each full-row sum (zero for the slots past n); the inside
@@ -2014,14 +2022,22 @@ void tiled_batch_build_enqueue(hipStream_t st, const BatchTiles& bt, const doubl
 }
 
 void tiled_batch_forces_enqueue(hipStream_t st, const BatchTiles& bt, const double4* pos, const LJConst& lj,
-                                int precision, double* frc, double* partials, const int* halt) {
+                                int precision, double* frc, double* partials, const int* halt, double* kick_vel,
+                                const double* mass, double half_dt) {
     const int tiles = bt.ts.tiles;
     const int row_len = slot_row_len(bt.padding);
     if (precision == MD_MIXED) {
-        auto kernel = lj.sig2f == 1.0f ? k_forces_tiled_mixed<false, true, true> : k_forces_tiled_mixed<false, true, false>;
+        // kick_vel: the closing kick of every particle and its kinetic energy as each tile's fourth sum
+        ForceStep fs{};
+        fs.vel = kick_vel;
+        fs.mass = mass;
+        fs.half_dt = half_dt;
+        const bool unit = lj.sig2f == 1.0f;
+        auto kernel = kick_vel ? (unit ? k_forces_tiled_mixed<true, true, true> : k_forces_tiled_mixed<true, true, false>)
+                               : (unit ? k_forces_tiled_mixed<false, true, true> : k_forces_tiled_mixed<false, true, false>);
         hipLaunchKernelGGL(kernel, dim3(tiles), dim3(TILE_THREADS), 0, st, pos, bt.ts,
                            bt.rows16, bt.rows, bt.count, bt.tile_ranges, bt.tile_nr, row_len, lj, frc, partials,
-                           nullptr, ForceStep{}, halt);
+                           nullptr, fs, halt);
     } else {
         auto kernel = lj.sig2 == 1.0 ? k_forces_tiled_fp64_pair<true, true, false>
                                      : k_forces_tiled_fp64_pair<false, true, false>;

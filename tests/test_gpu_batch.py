@@ -81,6 +81,25 @@ def test_fp64_bussi_replicas_match_single_runs(ctx):
         assert np.array_equal(eb[:, r, 1], en[:, 1])  # the thermostat's K' is computed in the same order
 
 
+def test_mixed_bussi_replicas_follow_single_runs(ctx):
+    """Mixed Bussi replicas on the tiled kernels, whose force kernel also applies the closing kick and sums its kinetic
+    energy per tile, against md_run of each System (separate kick, the kinetic energy in 1024-particle blocks): the
+    same trajectory up to the summation order of the kinetic energy, so over a short run the records agree to
+    rounding and the same pairs interact."""
+    specs = SPECS[1:]
+    steps = 20
+    draws = np.stack([mdgpu.BussiRng(200 + r).draws(4 * sp[0] ** 3, steps) for r, sp in enumerate(specs)])
+    temps = np.array([sp[2] for sp in specs])
+    batch = mdgpu.Batch(ctx, [gpu_state(ctx, sp) for sp in specs], 2.5, SKIN)
+    assert batch.info().tiled
+    eb = batch.run(LJ, DT, steps, precision=mdgpu.MD_MIXED, temperatures=temps, draws=draws, use_graph=True)
+    for r, sp in enumerate(specs):
+        _, _, en = single_run(ctx, sp, steps, mdgpu.MD_MIXED, draws=draws[r])
+        for k in (0, 1, 2):  # potential, the thermostat's K', virial
+            assert np.allclose(eb[:, r, k], en[:, k], rtol=1e-9, atol=1e-9 * np.abs(en[:, k]).max()), k
+        assert np.array_equal(eb[:, r, 3], en[:, 3])
+
+
 def test_fp64_replica_matches_oracle_hash(ctx):
     """One replica against the oracle's deterministic mode directly (40 steps, re-sorts included)."""
     sp = SPECS[2]
