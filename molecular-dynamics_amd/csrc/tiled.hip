@@ -673,7 +673,7 @@ __device__ __forceinline__ void list_build_tile(
     // current row in the same memory
     __shared__ union {
         Segment seg[CMAX * 27];
-        uint16_t rowbuf[TILE_THREADS / WAVE][ROWBUF_CAP];
+        uint16_t rowbuf[TILE_THREADS / WAVE][2][ROWBUF_CAP];  // two rows per wavefront (see `pair`)
     } sm;
     Segment* const seg = sm.seg;
     __shared__ uint16_t lists[LIST_CAP];  // slot byte offsets
@@ -918,17 +918,56 @@ __device__ __forceinline__ void list_build_tile(
     unsigned ep_half = 0;  // half-row lengths summed once per particle
     if (lists_ok) {
         const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
-        for (size_t p = wave; p < i1 - i0; p += TILE_THREADS / WAVE) {
+        // an entry is the byte offset of a slot: its x at xs + offset, y and z one and two arrays on (FH: half the
+        // offset into the float arrays)
+        auto coords = [&](int off) { return reinterpret_cast<const double*>(reinterpret_cast<const char*>(xs) + off); };
+        auto coords_f = [&](int off) {
+            return reinterpret_cast<const float*>(reinterpret_cast<const char*>(xf) + off);
+        };
+        // A row out of the wavefront's row buffer rb (cnt entries ranked, hcnt of them j > i): its slot chunks, count
+        // and half count, and the tile's row statistics.
+        auto write_row = [&](size_t i, uint16_t* rb, int cnt, int hcnt) {
+            uint16_t* const row16 = rows16 + slot_row_base(i, row_len);
+            {
+                // the row leaves LDS as 16-byte chunk words: lane 4 c + w packs entries 32 c + w + 4 t, t = 0..7
+                // (the wavefront's LDS accesses complete in issue order, so every lane's entries are in place)
+                __builtin_amdgcn_wave_barrier();
+                const int stored = cnt < padding ? cnt : padding;
+                // the last chunk ends in ghost entries (the force kernel's compact walk reads whole chunks)
+                if (stored + lane < (stored + CHUNK - 1) / CHUNK * CHUNK)
+                    rb[stored + lane] = static_cast<uint16_t>(GHOST_SLOT * LIST_BYTES);
+                __builtin_amdgcn_wave_barrier();
+                if (lane < (stored + CHUNK - 1) / CHUNK * LANES) {
+                    const int c = lane >> 2, w = lane & 3;
+                    const uint16_t* e = rb + c * CHUNK + w;
+                    uint4 v;
+                    v.x = static_cast<unsigned>(e[0]) | (static_cast<unsigned>(e[4]) << 16);
+                    v.y = static_cast<unsigned>(e[8]) | (static_cast<unsigned>(e[12]) << 16);
+                    v.z = static_cast<unsigned>(e[16]) | (static_cast<unsigned>(e[20]) << 16);
+                    v.w = static_cast<unsigned>(e[24]) | (static_cast<unsigned>(e[28]) << 16);
+                    if constexpr (FH) {  // float offsets to double offsets: both halves doubled (each < 2^15)
+                        static_assert(TILE_HALO_MAX * 4 < 0x8000, "a doubled float offset stays in its half");
+                        v.x <<= 1;
+                        v.y <<= 1;
+                        v.z <<= 1;
+                        v.w <<= 1;
+                    }
+                    *reinterpret_cast<uint4*>(row16 + c * CHUNK_STRIDE + 8 * w) = v;
+                }
+                __builtin_amdgcn_wave_barrier();
+            }
+            if (lane == 0) {
+                count[i] = cnt < padding ? cnt : padding;
+                half[i] = hcnt < padding ? hcnt : padding;
+            }
+            ep_max = cnt > ep_max ? cnt : ep_max;
+            if (lane == 0) ep_half += static_cast<unsigned>(hcnt);
+        };
+        auto single = [&](size_t p) {
             const size_t i = i0 + p;
             // uniform over the wavefront: one particle per wavefront
             const int cl = __builtin_amdgcn_readfirstlane(p_cl[p]);
             const int slot_i = __builtin_amdgcn_readfirstlane(p_slot[p]);
-            // an entry is the byte offset of a slot: its x at xs + offset, y and z one and two arrays on (FH: half the
-            // offset into the float arrays)
-            auto coords = [&](int off) { return reinterpret_cast<const double*>(reinterpret_cast<const char*>(xs) + off); };
-            auto coords_f = [&](int off) {
-                return reinterpret_cast<const float*>(reinterpret_cast<const char*>(xf) + off);
-            };
             double xi = 0.0, yi = 0.0, zi = 0.0;
             float xif = 0.0f, yif = 0.0f, zif = 0.0f;
             double4 ri = make_double4(0.0, 0.0, 0.0, 0.0);
@@ -945,9 +984,8 @@ __device__ __forceinline__ void list_build_tile(
                 zi = ci[2 * TILE_HALO_MAX];
             }
             const bool interior = cl_interior[cl] && !s_outside;
-            uint16_t* const row16 = rows16 + slot_row_base(i, row_len);
             int32_t* const row = rows + i;
-            uint16_t* const rb = sm.rowbuf[wave];
+            uint16_t* const rb = sm.rowbuf[wave][0];
             int cnt = 0;
             // A batch is 64 candidates: each lane tests one, the accepted ones are ranked over the ballot (mbcnt
             // counts the set bits below the lane, on top of the row's count so far) and stored. Two batches are
@@ -1063,40 +1101,127 @@ __device__ __forceinline__ void list_build_tile(
             };
             // FH: the interior tag marks a compact frame
             const int hcnt = (FH ? compact : interior) ? both(std::true_type{}) : both(std::false_type{});
-            {
-                // the row leaves LDS as 16-byte chunk words: lane 4 c + w packs entries 32 c + w + 4 t, t = 0..7
-                // (the wavefront's LDS accesses complete in issue order, so every lane's entries are in place)
-                __builtin_amdgcn_wave_barrier();
-                const int stored = cnt < padding ? cnt : padding;
-                // the last chunk ends in ghost entries (the force kernel's compact walk reads whole chunks)
-                if (stored + lane < (stored + CHUNK - 1) / CHUNK * CHUNK)
-                    rb[stored + lane] = static_cast<uint16_t>(GHOST_SLOT * LIST_BYTES);
-                __builtin_amdgcn_wave_barrier();
-                if (lane < (stored + CHUNK - 1) / CHUNK * LANES) {
-                    const int c = lane >> 2, w = lane & 3;
-                    const uint16_t* e = rb + c * CHUNK + w;
-                    uint4 v;
-                    v.x = static_cast<unsigned>(e[0]) | (static_cast<unsigned>(e[4]) << 16);
-                    v.y = static_cast<unsigned>(e[8]) | (static_cast<unsigned>(e[12]) << 16);
-                    v.z = static_cast<unsigned>(e[16]) | (static_cast<unsigned>(e[20]) << 16);
-                    v.w = static_cast<unsigned>(e[24]) | (static_cast<unsigned>(e[28]) << 16);
-                    if constexpr (FH) {  // float offsets to double offsets: both halves doubled (each < 2^15)
-                        static_assert(TILE_HALO_MAX * 4 < 0x8000, "a doubled float offset stays in its half");
-                        v.x <<= 1;
-                        v.y <<= 1;
-                        v.z <<= 1;
-                        v.w <<= 1;
+            write_row(i, rb, cnt, hcnt);
+        };
+        // Two particles of one own cell in one wavefront (FH, compact tiles): they share the cell's candidate lists,
+        // so each batch's slots and positions are read once and tested against both, and the two rows' rankings,
+        // independent of each other, interleave. Each row is the one `single` builds: the same candidates, tests and
+        // order.
+        auto pair = [&](size_t p) {
+            if constexpr (FH) {
+                const size_t iA = i0 + p, iB = iA + 1;
+                const int cl = __builtin_amdgcn_readfirstlane(p_cl[p]);
+                const int slotA = __builtin_amdgcn_readfirstlane(p_slot[p]);
+                const int slotB = __builtin_amdgcn_readfirstlane(p_slot[p + 1]);
+                const float* const qa = coords_f(slotA);
+                const float* const qb = coords_f(slotB);
+                const float xA = qa[0], yA = qa[TILE_HALO_MAX], zA = qa[2 * TILE_HALO_MAX];
+                const float xB = qb[0], yB = qb[TILE_HALO_MAX], zB = qb[2 * TILE_HALO_MAX];
+                uint16_t* const rbA = sm.rowbuf[wave][0];
+                uint16_t* const rbB = sm.rowbuf[wave][1];
+                int cntA = 0, cntB = 0;
+                auto inside_f = [&](float d) { return d < r2_in; };
+                auto unsure_f = [&](float d) { return d <= r2_out && !(d < r2_in); };
+                auto resolve_f = [&](size_t i, int sl, float d, bool unsure) {
+                    bool in = inside_f(d);
+                    if (unsure) {
+                        const double4 ri = pos[i];
+                        const double4 rj = pos[halo_particle(sl / LIST_BYTES, hr, nr)];
+                        const double ex = min_image(ri.x - rj.x, box, inv_box);
+                        const double ey = min_image(ri.y - rj.y, box, inv_box);
+                        const double ez = min_image(ri.z - rj.z, box, inv_box);
+                        in = ex * ex + ey * ey + ez * ez < radius2;
                     }
-                    *reinterpret_cast<uint4*>(row16 + c * CHUNK_STRIDE + 8 * w) = v;
-                }
-                __builtin_amdgcn_wave_barrier();
+                    return in;
+                };
+                auto commit = [&](size_t i, uint16_t* rb, int& cnt, int sl, bool accept, unsigned long long mask) {
+                    if (accept) {
+                        const unsigned at = __builtin_amdgcn_mbcnt_hi(
+                            static_cast<unsigned>(mask >> 32),
+                            __builtin_amdgcn_mbcnt_lo(static_cast<unsigned>(mask), static_cast<unsigned>(cnt)));
+                        if (at < static_cast<unsigned>(padding)) {
+                            rb[at] = static_cast<uint16_t>(sl);
+                            if (write32)
+                                rows[i + static_cast<uint64_t>(at) * stride] = halo_particle(sl / LIST_BYTES, hr, nr);
+                        }
+                    }
+                    cnt = __builtin_amdgcn_readfirstlane(cnt + __popcll(mask));
+                };
+                // the float r^2 of a candidate at (cx, cy, cz) in the compact frame
+                auto r2 = [](float x, float y, float z, float cx, float cy, float cz) {
+                    const float dx = x - cx, dy = y - cy, dz = z - cz;
+                    return fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+                };
+                struct Cand {
+                    int s;
+                    float x, y, z;
+                };
+                auto load = [&](int sl) {
+                    const float* const c = coords_f(sl);
+                    return Cand{sl, c[0], c[TILE_HALO_MAX], c[2 * TILE_HALO_MAX]};
+                };
+                // one particle's test of a pair of batches, as in `single`
+                auto test2 = [&](size_t i, uint16_t* rb, int& cnt, int slot, float x, float y, float z, const Cand& a,
+                                 const Cand& b, auto upper_tag) {
+                    constexpr bool upper = decltype(upper_tag)::value;
+                    const float ra = r2(x, y, z, a.x, a.y, a.z), rb2 = r2(x, y, z, b.x, b.y, b.z);
+                    const bool oa = upper ? a.s > slot : a.s < slot, ob = upper ? b.s > slot : b.s < slot;
+                    const bool na = inside_f(ra), nb = inside_f(rb2);
+                    const unsigned long long in_a = __ballot(na), in_b = __ballot(nb);
+                    const unsigned long long ma = in_a & __ballot(oa);
+                    const unsigned long long mb = in_b & __ballot(ob);
+                    if (((__ballot(ra <= r2_out) & ~in_a) | (__ballot(rb2 <= r2_out) & ~in_b)) == 0) {
+                        commit(i, rb, cnt, a.s, na && oa, ma);
+                        commit(i, rb, cnt, b.s, nb && ob, mb);
+                    } else {
+                        const bool xa = resolve_f(i, a.s, ra, unsure_f(ra)), xb = resolve_f(i, b.s, rb2, unsure_f(rb2));
+                        commit(i, rb, cnt, a.s, xa && oa, __ballot(xa && oa));
+                        commit(i, rb, cnt, b.s, xb && ob, __ballot(xb && ob));
+                    }
+                };
+                auto test1 = [&](size_t i, uint16_t* rb, int& cnt, int slot, float x, float y, float z, const Cand& a,
+                                 auto upper_tag) {
+                    constexpr bool upper = decltype(upper_tag)::value;
+                    const float ra = r2(x, y, z, a.x, a.y, a.z);
+                    const bool ua = unsure_f(ra), oa = upper ? a.s > slot : a.s < slot;
+                    const bool na = __ballot(ua) == 0 ? inside_f(ra) : resolve_f(i, a.s, ra, ua);
+                    commit(i, rb, cnt, a.s, na && oa, __ballot(na && oa));
+                };
+                auto walk2 = [&](int off, int len, auto upper_tag) {
+                    int q0 = 0;
+                    for (; q0 + 2 * WAVE <= len; q0 += 2 * WAVE) {
+                        const Cand a = load(lists[off + q0 + lane]), b = load(lists[off + q0 + WAVE + lane]);
+                        test2(iA, rbA, cntA, slotA, xA, yA, zA, a, b, upper_tag);
+                        test2(iB, rbB, cntB, slotB, xB, yB, zB, a, b, upper_tag);
+                    }
+                    if (q0 < len) {
+                        const Cand a = load(lists[off + q0 + lane]);
+                        test1(iA, rbA, cntA, slotA, xA, yA, zA, a, upper_tag);
+                        test1(iB, rbB, cntB, slotB, xB, yB, zB, a, upper_tag);
+                    }
+                };
+                walk2(__builtin_amdgcn_readfirstlane(lst[cl][0]), __builtin_amdgcn_readfirstlane(lst[cl][1]),
+                      std::true_type{});
+                const int hA = cntA, hB = cntB;
+                walk2(__builtin_amdgcn_readfirstlane(lst[cl][2]), __builtin_amdgcn_readfirstlane(lst[cl][3]),
+                      std::false_type{});
+                write_row(iA, rbA, cntA, hA);
+                write_row(iB, rbB, cntB, hB);
             }
-            if (lane == 0) {
-                count[i] = cnt < padding ? cnt : padding;
-                half[i] = hcnt < padding ? hcnt : padding;
+        };
+        // wavefront w takes particles 2 w, 2 w + 1, then 16 on: a pair of one cell together (FH, compact tiles),
+        // otherwise one after the other
+        const size_t np = i1 - i0;
+        for (size_t q = 2 * static_cast<size_t>(wave); q < np; q += 2 * (TILE_THREADS / WAVE)) {
+            bool two = false;
+            if constexpr (FH) two = compact && q + 1 < np && p_cl[q] == p_cl[q + 1];
+            if (__builtin_amdgcn_readfirstlane(two ? 1 : 0)) {
+                pair(q);
+            } else {
+                const size_t q1 = q + 2 < np ? q + 2 : np;
+#pragma unroll 1
+                for (size_t p = q; p < q1; ++p) single(p);  // one instance of the single walk's code
             }
-            ep_max = cnt > ep_max ? cnt : ep_max;
-            if (lane == 0) ep_half += static_cast<unsigned>(hcnt);
         }
     } else {
         const int lane = threadIdx.x & (LANES - 1);
