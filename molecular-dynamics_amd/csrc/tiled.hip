@@ -1103,11 +1103,11 @@ __device__ __forceinline__ void list_build_tile(
             const int hcnt = (FH ? compact : interior) ? both(std::true_type{}) : both(std::false_type{});
             write_row(i, rb, cnt, hcnt);
         };
-        // Two particles of one own cell in one wavefront (FH, compact tiles): they share the cell's candidate lists,
-        // so each batch's slots and positions are read once and tested against both, and the two rows' rankings,
-        // independent of each other, interleave. Each row is the one `single` builds: the same candidates, tests and
-        // order.
-        auto pair = [&](size_t p) {
+        // Two particles of one own cell in one wavefront (FH): they share the cell's candidate lists, so each batch's
+        // slots and positions are read once and tested against both, and the two rows' rankings, independent of each
+        // other, interleave. Each row is the one `single` builds: the same candidates, tests and order. `flat_tag`: a
+        // compact frame (no image term), as in `single`.
+        auto pair = [&](size_t p, auto flat_tag) {
             if constexpr (FH) {
                 const size_t iA = i0 + p, iB = iA + 1;
                 const int cl = __builtin_amdgcn_readfirstlane(p_cl[p]);
@@ -1147,9 +1147,15 @@ __device__ __forceinline__ void list_build_tile(
                     }
                     cnt = __builtin_amdgcn_readfirstlane(cnt + __popcll(mask));
                 };
-                // the float r^2 of a candidate at (cx, cy, cz) in the compact frame
-                auto r2 = [](float x, float y, float z, float cx, float cy, float cz) {
-                    const float dx = x - cx, dy = y - cy, dz = z - cz;
+                // the float r^2 of a candidate at (cx, cy, cz): in a compact frame the difference itself, otherwise its
+                // float minimum image per component (single's dist_f)
+                auto r2 = [&](float x, float y, float z, float cx, float cy, float cz) {
+                    float dx = x - cx, dy = y - cy, dz = z - cz;
+                    if constexpr (!decltype(flat_tag)::value) {
+                        dx = fmaf(-box_f, __builtin_rintf(dx * inv_box_f), dx);
+                        dy = fmaf(-box_f, __builtin_rintf(dy * inv_box_f), dy);
+                        dz = fmaf(-box_f, __builtin_rintf(dz * inv_box_f), dz);
+                    }
                     return fmaf(dx, dx, fmaf(dy, dy, dz * dz));
                 };
                 struct Cand {
@@ -1209,14 +1215,17 @@ __device__ __forceinline__ void list_build_tile(
                 write_row(iB, rbB, cntB, hB);
             }
         };
-        // wavefront w takes particles 2 w, 2 w + 1, then 16 on: a pair of one cell together (FH, compact tiles),
-        // otherwise one after the other
+        // wavefront w takes particles 2 w, 2 w + 1, then 16 on: a pair of one cell together (FH), otherwise one after
+        // the other
         const size_t np = i1 - i0;
         for (size_t q = 2 * static_cast<size_t>(wave); q < np; q += 2 * (TILE_THREADS / WAVE)) {
             bool two = false;
-            if constexpr (FH) two = compact && q + 1 < np && p_cl[q] == p_cl[q + 1];
+            if constexpr (FH) two = q + 1 < np && p_cl[q] == p_cl[q + 1];
             if (__builtin_amdgcn_readfirstlane(two ? 1 : 0)) {
-                pair(q);
+                if (compact)
+                    pair(q, std::true_type{});
+                else
+                    pair(q, std::false_type{});
             } else {
                 const size_t q1 = q + 2 < np ? q + 2 : np;
 #pragma unroll 1
