@@ -1701,11 +1701,11 @@ __global__ void __launch_bounds__(TILE_THREADS, 6)
     if (clear_flag && blockIdx.x == 0 && threadIdx.x == 0) *clear_flag = 0;
     __syncthreads();
     // the tile's U, W and inside count (and KE): the 128 particle sums added in double by one wavefront, in a fixed
-    // order; the potential's shift once per inside pair (the halved count is exact)
+    // order (DPP moves, no LDS round trips); the potential's shift once per inside pair (the halved count is exact)
     if (threadIdx.x < WAVE) {
         double v[SUMS];
 #pragma unroll
-        for (int q = 0; q < SUMS; ++q) v[q] = wave_sum(e_part[q][threadIdx.x] + e_part[q][threadIdx.x + WAVE]);
+        for (int q = 0; q < SUMS; ++q) v[q] = wave_sum_dpp(e_part[q][threadIdx.x] + e_part[q][threadIdx.x + WAVE]);
         if (threadIdx.x == 0) {
             double* out = partials + static_cast<size_t>(tile) * SUMS;
             out[0] = v[0] - static_cast<double>(lj.shiftf) * v[2];
