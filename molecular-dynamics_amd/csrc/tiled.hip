@@ -1482,16 +1482,18 @@ __global__ void __launch_bounds__(TILE_THREADS, 6)
         pre[c].w = live ? pre[c].w : GHOST_WORD;
     }
     if (threadIdx.x == 0) {
-        // the ghost slot, far from every frame coordinate: its pair term is exactly zero (see the compact walk)
+        // the ghost slot, far from every staged coordinate: its pair term is exactly zero (see the compact walk)
         xs[GHOST_SLOT] = GHOST_COORD;
         ys[GHOST_SLOT] = GHOST_COORD;
         zs[GHOST_SLOT] = GHOST_COORD;
     }
     if (staged && flat) {
+        // a flat tile: no periodic image anywhere in the tile and its halo, so the raw positions are staged and each
+        // separation is the difference of raw coordinates, the reference's minimum image exactly
         stage_halo_lanes<TILE_THREADS / WAVE>(pos, rr, nr, [&](int q, const double4& p) {
-            xs[q] = p.x - ref.x;
-            ys[q] = p.y - ref.y;
-            zs[q] = p.z - ref.z;
+            xs[q] = p.x;
+            ys[q] = p.y;
+            zs[q] = p.z;
         });
     } else if (staged) {
         stage_halo_lanes<TILE_THREADS / WAVE>(pos, rr, nr, [&](int q, const double4& p) {
@@ -1500,21 +1502,22 @@ __global__ void __launch_bounds__(TILE_THREADS, 6)
             zs[q] = frame_coord(p.z, ref.z, fr);
         });
     }
-    // the particle's own frame coordinates
+    // the particle's own coordinates: raw in a flat tile, in the tile's frame otherwise
     double xi = 0.0, yi = 0.0, zi = 0.0;
     if (active) {
         if (flat) {
-            xi = ri.x - ref.x;
-            yi = ri.y - ref.y;
-            zi = ri.z - ref.z;
+            xi = ri.x;
+            yi = ri.y;
+            zi = ri.z;
         } else {
             xi = frame_coord(ri.x, ref.x, fr);
             yi = frame_coord(ri.y, ref.y, fr);
             zi = frame_coord(ri.z, ref.z, fr);
         }
     }
-    {  // how far the tile's own particles lie from the reference particle: per wavefront, then over the block
-        const float e = active ? static_cast<float>(fmax(fabs(xi), fmax(fabs(yi), fabs(zi)))) : 0.0f;
+    {  // how far the tile's own particles lie from the reference particle (frames only): per wavefront, then over
+       // the block
+        const float e = active && !flat ? static_cast<float>(fmax(fabs(xi), fmax(fabs(yi), fabs(zi)))) : 0.0f;
         const int e_wave = wave_max_dpp(__float_as_int(e));  // non-negative floats order as their bit patterns
         if (threadIdx.x % WAVE == 0) wave_extent[threadIdx.x / WAVE] = e_wave;
     }
@@ -1523,7 +1526,8 @@ __global__ void __launch_bounds__(TILE_THREADS, 6)
 #pragma unroll
     for (int w = 0; w < TILE_THREADS / WAVE; ++w) e_bits = max(e_bits, wave_extent[w]);
     const float e = __int_as_float(e_bits);
-    const bool compact = staged && (2.0f * e + static_cast<float>(fr.reach) < static_cast<float>(fr.box) - 1.0f);
+    const bool compact =
+        staged && (flat || 2.0f * e + static_cast<float>(fr.reach) < static_cast<float>(fr.box) - 1.0f);
     PairSumsF a;
     // inside pairs counted per wavefront on the compact walk (a uniform sum of ballots)
     unsigned wave_inside = 0;
