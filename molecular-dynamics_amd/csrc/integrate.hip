@@ -14,6 +14,12 @@
 
 namespace mdgpu {
 
+// MDGPU_FP64_GATHER=1: FP64 forces on the per-particle gather kernel, which reads every particle's int32 row
+static bool fp64_on_int32_rows() {
+    const char* v = std::getenv("MDGPU_FP64_GATHER");
+    return v && v[0] == '1';
+}
+
 constexpr int VV_THREADS = 256;
 
 // v += (dt/2 f) / m; r = wrap(r + dt v) per component (Integrator.cpp:32-53). With `ref` set, also the list's
@@ -247,8 +253,10 @@ int md_step(md_state* s, md_list* l, const md_lj* lj, int precision, double dt, 
             l->slots_only = false;
         }
     } guard{s, l};
-    const bool slots = precision == MD_MIXED && tiling_enabled(s->n);
-    l->slots_only = slots;  // mixed: the gated rebuild writes only the slot rows
+    // the gated rebuild writes only the slot rows (and the int32 rows of the tiles without a staged halo, which
+    // both tiled force kernels read), unless FP64 forces stay on the int32-row gather kernel
+    const bool slots = (precision == MD_MIXED || !fp64_on_int32_rows()) && tiling_enabled(s->n);
+    l->slots_only = slots;
     static_assert(FLAG_STALE == 0 && FLAG_OVERFLOW == 1 && FLAG_MAXCOUNT == 2 && FLAG_REBUILDS == 3,
                   "one clear covers the stale, overflow, longest-row and rebuild words");
     MD_HIP(hipMemsetAsync(l->flags, 0, 4 * sizeof(int), st));
@@ -336,7 +344,7 @@ static void run_impl(md_state* s, md_list* l, const md_run_config* cfg, double* 
             l->slots_only = false;
         }
     } slots_guard{l};
-    l->slots_only = cfg->precision == MD_MIXED && tiling_enabled(s->n);
+    l->slots_only = (cfg->precision == MD_MIXED || !fp64_on_int32_rows()) && tiling_enabled(s->n);
     hipStream_t st = s->ctx->stream;
     const int steps = cfg->steps;
     MD_HIP(hipMemsetAsync(rb.counters, 0, 4 * sizeof(int), st));
